@@ -1,0 +1,176 @@
+/*
+ * wigner_dg.h -- C ABI of the MI355X Wigner transport library (libwdg.so).
+ *
+ * Drop-in boundary for the stationary 1D-x x 1D-k Wigner solve of the
+ * reference (jonasneu1207/real_space_DG, MATLAB).  The reference's host code
+ * reaches its operator only through a small plug-in surface:
+ *
+ *   [A,rhs]  = get_SysM(mat,p,EM)                   DG_code/core/get_SysM.m:1
+ *   sysInfo.apply(u) / assembleRhs() / getDiagonal() DG_code/core/get_SysM_full2D.m:80-86
+ *   pre.apply(r)                                    DG_code/core/get_LineTransversePreconditioner_full2D.m:113
+ *   gmres(A*M^-1, b, restart, tol, maxit)           DG_code/solve_transport_DG_full2D.m:462-484
+ *   rho_u = phi*rho ; n ; j                          DG_code/solve_transport_DG_2D.m:114-156
+ *
+ * Each entry point below replaces one of those and cites it.  All vectors
+ * are interleaved complex double (re,im) in the reference's permuted index
+ * order ((e*N_K_chi + a)*Nk + j), k fastest (DG_code/core/get_Diff.m:242-259).
+ * Host buffers are owned by the caller (read-only inputs, write-only
+ * outputs); the context owns every device allocation.  One HIP stream per
+ * context; every call is synchronous at its return.  Functions return
+ * WDG_OK (0) or a negative status; wdg_last_error() gives the message.
+ */
+#ifndef WIGNER_DG_H
+#define WIGNER_DG_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes.  The MATLAB shim maps them onto the reference identifiers. */
+enum {
+  WDG_OK = 0,
+  WDG_ERR_ARG = -1,          /* invalid argument (DG:Full2D:InvalidInitialRho and friends) */
+  WDG_ERR_HIP = -2,          /* HIP runtime failure */
+  WDG_ERR_MEMORY = -3,       /* DG:Full2D:GmresMemoryLimit / LineMemoryLimit */
+  WDG_ERR_STATE = -4,        /* call order: no potential set yet */
+  WDG_ERR_NUMERIC = -5,      /* singular factor / non-finite value */
+  WDG_ERR_UNSUPPORTED = -6   /* option outside the restated method */
+};
+
+enum { WDG_FERMI_MSA_3D = 0, WDG_FERMI_MSA_2D = 1 };     /* DG_code/core/verteilung.m:77,118 */
+enum { WDG_PREC_FP64 = 0, WDG_PREC_MIXED = 1 };
+
+/* Geometry: the fields of mat / mat.dg.params that initParams reads
+ * (DG_code/initParams.m:58-62,110-111,155-191). */
+typedef struct wdg_desc {
+  int32_t n_chi;       /* DG elements in chi: (mat.Nx-1)/(N_K_chi-1)          */
+  int32_t n_k_chi;     /* nodes per element N_K_chi (degree + 1), 2..8        */
+  int32_t n_xi;        /* FV cells in xi, mat.dg.params.N_xi (even)          */
+  int32_t nk;          /* plane waves, mat.dg.params.N (must equal n_xi)     */
+  double  l_chi;       /* device length [m] = mat.L_x*1e-9                    */
+  double  l_xi;        /* coherence length [m] = mat.dg.params.Ly             */
+} wdg_desc;
+
+/* Potential-dependent data: get_SysM(mat,p,EM) inputs
+ * (DG_code/solve_transport_DG_2D.m:44-49,91; verteilung.m:1-170). */
+typedef struct wdg_potential {
+  int32_t n_mat;        /* material grid points mat.Nx = n_chi*(n_k_chi-1)+1 */
+  int32_t fermi_model;  /* WDG_FERMI_*                                       */
+  const double* x_mat;  /* material grid [m] (mat.x*1e-9), ascending         */
+  const double* em;     /* subband energy EM on x_mat [eV]                   */
+  double m_left;        /* x-mass at the first material point [kg]           */
+  double m_right;       /* x-mass at the last material point [kg]            */
+  double ef_left;       /* EfL [eV]                                          */
+  double ef_right;      /* EfR [eV]                                          */
+  double temp;          /* mat.Temp [K]                                      */
+  double deg_factor;    /* valley degeneracy deg_factor(IV)                  */
+  double me_z;          /* mat.me_z_ch (DG_MSA_2D only)                      */
+} wdg_potential;
+
+typedef struct wdg_solve_opts {
+  double  tol;          /* true relative residual target (full2D_solverTol)        */
+  int32_t restart;      /* GMRES restart m (full2D_gmresRestart)                   */
+  int32_t max_iter;     /* cap on total inner iterations                           */
+  int32_t precision;    /* WDG_PREC_FP64 or WDG_PREC_MIXED                         */
+  double  inner_tol;    /* mixed: relative tolerance of each FP32 inner solve      */
+  int32_t max_outer;    /* mixed: cap on FP64 refinement steps                     */
+  int32_t use_precond;  /* 1: right line preconditioner, 0: none                   */
+} wdg_solve_opts;
+
+typedef struct wdg_solve_info {
+  int32_t flag;                 /* 0 converged (true residual <= tol), 1 not    */
+  int32_t iterations;           /* inner GMRES iterations (all refinement steps) */
+  int32_t outer_iterations;     /* FP64 refinement steps (1 for the FP64 path)   */
+  int32_t restarts;             /* restart cycles run                            */
+  double  true_relres;          /* ||b - A x|| / ||b|| in FP64                   */
+  double  reported_relres;      /* GMRES's own (Givens) estimate at exit         */
+  double  initial_true_relres;  /* of x0 (1 for the zero start)                  */
+  double  seconds;              /* device wall time of the solve                 */
+} wdg_solve_info;
+
+typedef struct wdg_ctx wdg_ctx;
+
+/* Message of the last failing call on ctx (or of the last failed create
+ * when ctx is NULL).  Never NULL. */
+const char* wdg_last_error(const wdg_ctx* ctx);
+
+/* Grid, basis phi, DG element matrices; device buffers for one instance.
+ * Replaces initParams + Basisfunktionen_Q_m (DG_code/initParams.m:58-221,
+ * DG_code/core/Basisfunktionen_Q_m.m:3-33). */
+int wdg_create(const wdg_desc* desc, wdg_ctx** out);
+int wdg_destroy(wdg_ctx* ctx);
+
+/* Number of unknowns N = n_chi*n_k_chi*nk. */
+int64_t wdg_size(const wdg_ctx* ctx);
+
+/* Build Theta[V]+CAP tables, inflow data and the line preconditioner for a
+ * potential.  Replaces get_SysM(mat,p,EM) (DG_code/core/get_SysM.m:1-16)
+ * and the inflow part of initParams (verteilung, DG_code/initParams.m:220). */
+int wdg_set_potential(wdg_ctx* ctx, const wdg_potential* pot);
+
+/* rhs of get_Diff (DG_code/core/get_Diff.m:253-259); sysInfo.assembleRhs. */
+int wdg_rhs(wdg_ctx* ctx, double* rhs);
+
+/* y = A u with A = A_diff + G (sysInfo.apply, DG_code/core/get_SysM_full2D.m:124-127
+ * for the 1D operator of get_SysM.m:3-8).  precision WDG_PREC_FP64 gives the
+ * FP64 operator; WDG_PREC_MIXED gives the inner operator of the mixed
+ * solver (FP32 transport, split-BF16 MFMA Theta with FP32 accumulation). */
+int wdg_apply(wdg_ctx* ctx, const double* u, double* y, int32_t precision);
+
+/* z = M^-1 r for the line preconditioner (pre.apply,
+ * DG_code/core/get_LineTransversePreconditioner_full2D.m:199-239, restated
+ * for the k basis: exact per-k block-bidiagonal upwind solve along x of
+ * A_diff + diag_k(G)). */
+int wdg_precond(wdg_ctx* ctx, const double* r, double* z);
+
+/* diag(A) (sysInfo.getDiagonal, DG_code/core/get_SysM_full2D.m:83). */
+int wdg_diagonal(wdg_ctx* ctx, double* d);
+
+/* GMRES solve of A x = rhs (solveStationarySystem gmres branch,
+ * DG_code/solve_transport_DG_full2D.m:276-334 with runGmresWithProgress
+ * :462-484): RHS normalisation, right preconditioning, warm start x0 as a
+ * correction with the absolute target tol*||b||, true residual check that
+ * overrides the flag.  x0 may be NULL (zero start); x may be NULL (the
+ * solution stays on the device for wdg_observables). */
+int wdg_solve(wdg_ctx* ctx, const wdg_solve_opts* opts, const double* x0, double* x,
+              wdg_solve_info* info);
+
+/* Observables of rho (or of the last solution when rho is NULL):
+ * rho_u = phi*rho per node, n = Re rho_u at xi=0, j = q*hbar/m*Im d rho_u/dxi
+ * (DG_code/solve_transport_DG_2D.m:114-156).  m_nodes: x-mass per DG node
+ * [kg] (n_chi*n_k_chi).  n, j: n_chi*n_k_chi doubles.  rho_u may be NULL,
+ * otherwise n_xi x (n_chi*n_k_chi) interleaved complex, column-major
+ * (xi fastest) like RHO_DG's transpose. */
+int wdg_observables(wdg_ctx* ctx, const double* rho, const double* m_nodes, double* n,
+                    double* j, double* rho_u);
+
+/* ---- device-resident variants (for callers that keep data in HBM) ---- */
+/* u, y: device pointers, interleaved complex of the precision's type
+ * (double for FP64, float for mixed).  Enqueued on the context stream and
+ * synchronised before return. */
+int wdg_apply_device(wdg_ctx* ctx, const void* d_u, void* d_y, int32_t precision);
+int wdg_precond_device(wdg_ctx* ctx, const void* d_r, void* d_z, int32_t precision);
+
+/* ---- host-only helpers (no GPU needed) ---- */
+/* The host-side setup of wdg_create + wdg_set_potential without a device:
+ * rhs (2N doubles) and the per-element transport matrices K1,K2L,K2R,K3,K5
+ * (5*n*n doubles, row-major, xi-scaled inverse mass, get_Diff.m:198-221).
+ * Either output may be NULL. */
+int wdg_host_setup(const wdg_desc* desc, const wdg_potential* pot, double* rhs, double* kmats);
+
+/* Enable per-kernel timing with HIP events on the context stream for the
+ * kernels selected by mask (bit i = entry i of wdg_profile's list); 0 turns
+ * it off.  Resets the accumulated statistics. */
+int wdg_set_profiling(wdg_ctx* ctx, uint32_t mask);
+
+/* Kernel timing of the last wdg_solve: average device time [ms] of each
+ * hot kernel per launch, and launches counted.  names: comma-separated. */
+int wdg_profile(wdg_ctx* ctx, char* names, int32_t names_len, double* avg_ms, int64_t* launches,
+                int32_t max_entries, int32_t* n_entries);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* WIGNER_DG_H */

@@ -1,0 +1,120 @@
+"""ctypes binding of libwdg.so (include/wigner_dg.h).
+
+The library is built in-tree by ``real_space_dg_amd.build``.  There is no CPU
+fallback: if the shared library is missing or cannot be loaded, every entry
+point raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "libwdg.so"
+
+WDG_OK = 0
+WDG_FERMI_MSA_3D = 0
+WDG_FERMI_MSA_2D = 1
+WDG_PREC_FP64 = 0
+WDG_PREC_MIXED = 1
+
+# Status code -> reference error identifier (DG:Full2D:* in the MATLAB code).
+ERROR_IDS = {
+    -1: "DG:Full2D:InvalidArgument",
+    -2: "DG:Full2D:DeviceError",
+    -3: "DG:Full2D:GmresMemoryLimit",
+    -4: "DG:Full2D:OperatorNotReady",
+    -5: "DG:Full2D:NumericalFailure",
+    -6: "DG:Full2D:Unsupported",
+}
+
+
+class WdgError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        self.code = code
+        self.identifier = ERROR_IDS.get(code, "DG:Full2D:Error")
+        super().__init__(f"{self.identifier} ({code}): {msg}")
+
+
+class wdg_desc(C.Structure):
+    _fields_ = [("n_chi", C.c_int32), ("n_k_chi", C.c_int32), ("n_xi", C.c_int32), ("nk", C.c_int32),
+                ("l_chi", C.c_double), ("l_xi", C.c_double)]
+
+
+class wdg_potential(C.Structure):
+    _fields_ = [("n_mat", C.c_int32), ("fermi_model", C.c_int32),
+                ("x_mat", C.POINTER(C.c_double)), ("em", C.POINTER(C.c_double)),
+                ("m_left", C.c_double), ("m_right", C.c_double), ("ef_left", C.c_double),
+                ("ef_right", C.c_double), ("temp", C.c_double), ("deg_factor", C.c_double),
+                ("me_z", C.c_double)]
+
+
+class wdg_solve_opts(C.Structure):
+    _fields_ = [("tol", C.c_double), ("restart", C.c_int32), ("max_iter", C.c_int32),
+                ("precision", C.c_int32), ("inner_tol", C.c_double), ("max_outer", C.c_int32),
+                ("use_precond", C.c_int32)]
+
+
+class wdg_solve_info(C.Structure):
+    _fields_ = [("flag", C.c_int32), ("iterations", C.c_int32), ("outer_iterations", C.c_int32),
+                ("restarts", C.c_int32), ("true_relres", C.c_double), ("reported_relres", C.c_double),
+                ("initial_true_relres", C.c_double), ("seconds", C.c_double)]
+
+
+# name -> (restype, argtypes)
+_P = C.c_void_p
+_D = C.POINTER(C.c_double)
+SIGNATURES = {
+    "wdg_last_error": (C.c_char_p, [_P]),
+    "wdg_create": (C.c_int, [C.POINTER(wdg_desc), C.POINTER(_P)]),
+    "wdg_destroy": (C.c_int, [_P]),
+    "wdg_size": (C.c_int64, [_P]),
+    "wdg_set_potential": (C.c_int, [_P, C.POINTER(wdg_potential)]),
+    "wdg_rhs": (C.c_int, [_P, _D]),
+    "wdg_apply": (C.c_int, [_P, _D, _D, C.c_int32]),
+    "wdg_precond": (C.c_int, [_P, _D, _D]),
+    "wdg_diagonal": (C.c_int, [_P, _D]),
+    "wdg_solve": (C.c_int, [_P, C.POINTER(wdg_solve_opts), _D, _D, C.POINTER(wdg_solve_info)]),
+    "wdg_observables": (C.c_int, [_P, _D, _D, _D, _D, _D]),
+    "wdg_apply_device": (C.c_int, [_P, _P, _P, C.c_int32]),
+    "wdg_precond_device": (C.c_int, [_P, _P, _P, C.c_int32]),
+    "wdg_host_setup": (C.c_int, [C.POINTER(wdg_desc), C.POINTER(wdg_potential), _D, _D]),
+    "wdg_set_profiling": (C.c_int, [_P, C.c_uint32]),
+    "wdg_profile": (C.c_int, [_P, C.c_char_p, C.c_int32, _D, C.POINTER(C.c_int64), C.c_int32,
+                              C.POINTER(C.c_int32)]),
+}
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    """Load libwdg.so (raises if it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise ImportError(f"{LIB_PATH} is not built; run python -m real_space_dg_amd.build")
+    lib = C.CDLL(str(LIB_PATH), mode=os.RTLD_NOW | C.RTLD_GLOBAL)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(rc: int, ctx=None):
+    if rc != WDG_OK:
+        lib = load()
+        msg = lib.wdg_last_error(ctx).decode(errors="replace")
+        raise WdgError(rc, msg)
+
+
+def dptr(a: np.ndarray):
+    """Pointer to a C-contiguous float64 (or complex128 viewed as float64) array."""
+    if a is None:
+        return None
+    assert a.flags.c_contiguous and a.dtype in (np.float64, np.complex128)
+    return a.ctypes.data_as(_D)

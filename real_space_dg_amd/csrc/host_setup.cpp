@@ -1,0 +1,374 @@
+// Host-side setup: DG element matrices on Gauss-Lobatto nodes, the
+// plane-wave basis and the inflow data of the FV-in-xi Wigner operator.
+//
+// Method (restated from the reference, computed independently here):
+//  * GLL nodes by Newton iteration on (1-x^2) P'_N, weights 2/(N(N+1)P_N^2)
+//    (DG_code/core/JacobiGL.m:44-50).
+//  * Nodal Lagrange basis l_a on the element nodes r.  The reference builds
+//    Mref = invV'*invV and Dr = Mref*Dmatrix1D from orthonormal Vandermonde
+//    matrices (DG_code/initParams.m:188-191, DG_code/core/get_Diff.m:198);
+//    both are the exact integrals int l_a l_b and int l_a l_b', computed here
+//    by the 10-point GLL rule (exact to degree 17, so for n <= 9).
+//  * W = V_tilde/V is Lagrange interpolation W_qa = l_a(x_q)
+//    (get_Drift_accel_trial_permutedShell.m:183).
+#include "host_setup.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <stdexcept>
+
+namespace wdg {
+
+namespace {
+
+double legendre(int N, double x, double* dP = nullptr) {
+  double p0 = 1.0, p1 = x;
+  if (N == 0) {
+    if (dP) *dP = 0;
+    return 1.0;
+  }
+  for (int k = 1; k < N; ++k) {
+    double p2 = ((2.0 * k + 1.0) * x * p1 - k * p0) / (k + 1.0);
+    p0 = p1;
+    p1 = p2;
+  }
+  if (dP) {
+    if (std::abs(x) < 1.0)
+      *dP = N * (x * p1 - p0) / (x * x - 1.0);
+    else
+      *dP = (x > 0 ? 1.0 : (N % 2 ? 1.0 : -1.0)) * 0.5 * N * (N + 1.0);
+  }
+  return p1;
+}
+
+// n GLL points on [-1,1] (degree N = n-1), ascending.
+void gll(int n, std::vector<double>& x, std::vector<double>& w) {
+  x.assign(n, 0.0);
+  w.assign(n, 0.0);
+  if (n == 1) {
+    w[0] = 2.0;
+    return;
+  }
+  const int N = n - 1;
+  for (int i = 0; i < n; ++i) {
+    double xi = -std::cos(M_PI * i / N);
+    if (i > 0 && i < N) {
+      for (int it = 0; it < 100; ++it) {
+        // Newton on (1-x^2) P'_N(x) via the Hesthaven-Warburton update
+        // x <- x - (x P_N - P_{N-1}) / (n P_N).
+        double pN = legendre(N, xi);
+        double pNm1 = legendre(N - 1, xi);
+        double dx = (xi * pN - pNm1) / (n * pN);
+        xi -= dx;
+        if (std::abs(dx) < 1e-16) break;
+      }
+    }
+    x[i] = xi;
+  }
+  x[0] = -1.0;
+  x[N] = 1.0;
+  for (int i = 0; i < n; ++i) {
+    double p = legendre(N, x[i]);
+    w[i] = 2.0 / (N * (N + 1.0) * p * p);
+  }
+}
+
+double lagrange(const std::vector<double>& r, int a, double x) {
+  double v = 1.0;
+  for (size_t b = 0; b < r.size(); ++b)
+    if ((int)b != a) v *= (x - r[b]) / (r[a] - r[b]);
+  return v;
+}
+
+double lagrange_d(const std::vector<double>& r, int a, double x) {
+  double s = 0.0;
+  for (size_t m = 0; m < r.size(); ++m) {
+    if ((int)m == a) continue;
+    double t = 1.0 / (r[a] - r[m]);
+    for (size_t b = 0; b < r.size(); ++b)
+      if ((int)b != a && b != m) t *= (x - r[b]) / (r[a] - r[b]);
+    s += t;
+  }
+  return s;
+}
+
+// Gauss-Jordan inverse with partial pivoting (small real matrices).
+std::vector<double> inverse(const std::vector<double>& A, int n) {
+  std::vector<double> a(A), inv(n * n, 0.0);
+  for (int i = 0; i < n; ++i) inv[i * n + i] = 1.0;
+  for (int c = 0; c < n; ++c) {
+    int p = c;
+    for (int r = c + 1; r < n; ++r)
+      if (std::abs(a[r * n + c]) > std::abs(a[p * n + c])) p = r;
+    if (a[p * n + c] == 0.0) throw std::runtime_error("singular element mass matrix");
+    if (p != c)
+      for (int k = 0; k < n; ++k) {
+        std::swap(a[p * n + k], a[c * n + k]);
+        std::swap(inv[p * n + k], inv[c * n + k]);
+      }
+    double d = 1.0 / a[c * n + c];
+    for (int k = 0; k < n; ++k) {
+      a[c * n + k] *= d;
+      inv[c * n + k] *= d;
+    }
+    for (int r = 0; r < n; ++r) {
+      if (r == c) continue;
+      double f = a[r * n + c];
+      if (f == 0.0) continue;
+      for (int k = 0; k < n; ++k) {
+        a[r * n + k] -= f * a[c * n + k];
+        inv[r * n + k] -= f * inv[c * n + k];
+      }
+    }
+  }
+  return inv;
+}
+
+std::vector<double> matmul(const std::vector<double>& A, const std::vector<double>& B, int n) {
+  std::vector<double> C(n * n, 0.0);
+  for (int i = 0; i < n; ++i)
+    for (int k = 0; k < n; ++k)
+      for (int j = 0; j < n; ++j) C[i * n + j] += A[i * n + k] * B[k * n + j];
+  return C;
+}
+
+std::vector<double> linspace(double a, double b, int n) {
+  std::vector<double> v(n);
+  if (n == 1) {
+    v[0] = b;
+    return v;
+  }
+  double step = (b - a) / (n - 1);
+  for (int i = 0; i < n; ++i) v[i] = a + i * step;
+  v[n - 1] = b;
+  return v;
+}
+
+// Complex LU with partial pivoting, row-major, in place.
+void lu_factor(std::vector<cplx>& a, std::vector<int>& piv, int n) {
+  piv.resize(n);
+  for (int c = 0; c < n; ++c) {
+    int p = c;
+    double best = std::abs(a[(size_t)c * n + c]);
+    for (int r = c + 1; r < n; ++r) {
+      double v = std::abs(a[(size_t)r * n + c]);
+      if (v > best) {
+        best = v;
+        p = r;
+      }
+    }
+    if (best == 0.0) throw std::runtime_error("plane-wave basis phi is singular");
+    piv[c] = p;
+    if (p != c)
+      for (int k = 0; k < n; ++k) std::swap(a[(size_t)p * n + k], a[(size_t)c * n + k]);
+    const cplx d = 1.0 / a[(size_t)c * n + c];
+    const cplx* rowc = &a[(size_t)c * n];
+#pragma omp parallel for schedule(static)
+    for (int r = c + 1; r < n; ++r) {
+      cplx* row = &a[(size_t)r * n];
+      cplx f = row[c] * d;
+      row[c] = f;
+      for (int k = c + 1; k < n; ++k) row[k] -= f * rowc[k];
+    }
+  }
+}
+
+std::vector<cplx> lu_solve(const std::vector<cplx>& lu, const std::vector<int>& piv, int n,
+                           std::vector<cplx> b) {
+  for (int c = 0; c < n; ++c)
+    if (piv[c] != c) std::swap(b[c], b[piv[c]]);
+  for (int r = 1; r < n; ++r) {
+    cplx s = b[r];
+    for (int k = 0; k < r; ++k) s -= lu[(size_t)r * n + k] * b[k];
+    b[r] = s;
+  }
+  for (int r = n - 1; r >= 0; --r) {
+    cplx s = b[r];
+    for (int k = r + 1; k < n; ++k) s -= lu[(size_t)r * n + k] * b[k];
+    b[r] = s / lu[(size_t)r * n + r];
+  }
+  return b;
+}
+
+double cap_profile(double q, double l_xi, double delta_xi) {
+  // DG_code/core/get_CAP.m:11-18 (W0 = 1, n = 1, width 0.4 L_xi)
+  const double W0 = 1.0, nn = 1.0;
+  const double L0 = l_xi / 2 - delta_xi;
+  const double weite = l_xi * 0.40;
+  double v = 0.0;
+  if (q >= L0 - weite) v += std::pow(W0 / weite * (q - L0 + weite), 2 * nn);
+  if (q <= -L0 + weite) v += std::pow(W0 / weite * (q + L0 - weite), 2 * nn);
+  return v;
+}
+
+}  // namespace
+
+void build_geometry(const wdg_desc& d, Geometry& g) {
+  if (d.n_chi < 2) throw std::runtime_error("n_chi must be >= 2");
+  if (d.n_k_chi < 2 || d.n_k_chi > 8) throw std::runtime_error("n_k_chi must be in 2..8");
+  if (d.nk != d.n_xi) throw std::runtime_error("nk must equal n_xi (FV plane-wave basis is square)");
+  if (d.nk < 32 || d.nk % 32 != 0) throw std::runtime_error("nk must be a positive multiple of 32");
+  if (!(d.l_chi > 0) || !(d.l_xi > 0)) throw std::runtime_error("lengths must be positive");
+  g.n_chi = d.n_chi;
+  g.n = d.n_k_chi;
+  g.n_xi = d.n_xi;
+  g.nk = d.nk;
+  g.l_chi = d.l_chi;
+  g.l_xi = d.l_xi;
+  const int n = g.n;
+  g.q_diff = kHbarIP / kMElektronIP;  // initParams.m:106
+  g.q_drift = kQIP / kHbarIP;         // initParams.m:107
+  g.delta_chi = g.l_chi / g.n_chi;    // initParams.m:155
+  g.jac = g.delta_chi / 2;            // get_Diff.m:199
+  {
+    auto xr = linspace(-g.l_xi / 2, g.l_xi / 2, g.n_xi);  // initParams.m:179-181
+    g.delta_xi = xr[1] - xr[0];
+  }
+  g.y = linspace(-(g.l_xi - g.delta_xi) / 2, (g.l_xi - g.delta_xi) / 2, g.n_xi);  // :182
+  gll(kNq, g.xq, g.wq);                                                         // :186
+  std::vector<double> wr;
+  gll(n, g.r, wr);                                                              // :187
+  // Exact mass / stiffness of the nodal basis (see file header).
+  g.Mref.assign(n * n, 0.0);
+  g.S.assign(n * n, 0.0);
+  g.Wq.assign(kNq * n, 0.0);
+  for (int q = 0; q < kNq; ++q)
+    for (int a = 0; a < n; ++a) g.Wq[q * n + a] = lagrange(g.r, a, g.xq[q]);
+  for (int a = 0; a < n; ++a)
+    for (int b = 0; b < n; ++b) {
+      double m = 0, s = 0;
+      for (int q = 0; q < kNq; ++q) {
+        m += g.wq[q] * g.Wq[q * n + a] * g.Wq[q * n + b];
+        s += g.wq[q] * g.Wq[q * n + a] * lagrange_d(g.r, b, g.xq[q]);
+      }
+      g.Mref[a * n + b] = m;
+      g.S[a * n + b] = (std::abs(s) < 1e-15) ? 0.0 : s;  // get_Diff.m:203-204
+    }
+  std::vector<double> Mx(n * n);
+  for (int i = 0; i < n * n; ++i) Mx[i] = g.delta_xi / 2 * g.Mref[i];  // get_Diff.m:200
+  g.invM = inverse(Mx, n);
+  std::vector<double> P1(n * n, 0.0), P3(n * n, 0.0);
+  P1[0] = 1.0;
+  P3[(n - 1) * n + (n - 1)] = 1.0;
+  std::vector<double> t1(n * n), t2L(n * n), t2R(n * n);
+  for (int i = 0; i < n * n; ++i) {
+    t1[i] = g.S[i] + 0.5 * P1[i] - 0.5 * P3[i];
+    t2L[i] = 0.5 * P1[i];
+    t2R[i] = 0.5 * P3[i];
+  }
+  g.K1 = matmul(g.invM, t1, n);   // get_Diff.m:214
+  g.K2L = matmul(g.invM, t2L, n); // :216
+  g.K2R = matmul(g.invM, t2R, n); // :217
+  g.K2.assign(n * n, 0.0);
+  for (int i = 0; i < n * n; ++i) g.K2[i] = g.K2L[i] + g.K2R[i];
+  g.K3.assign(n * n, 0.0);
+  g.K5.assign(n * n, 0.0);
+  for (int a = 0; a < n; ++a) {
+    g.K3[a * n + (n - 1)] = -0.5 * g.invM[a * n + 0];  // -0.5 invM P2, P2 = e1 en' (:218)
+    g.K5[a * n + 0] = 0.5 * g.invM[a * n + (n - 1)];   // 0.5 invM P4, P4 = en e1' (:220)
+  }
+  // Wave numbers and basis (Basisfunktionen_Q_m.m:22-32).
+  g.k.resize(g.nk);
+  for (int j = 0; j < g.nk; ++j) g.k[j] = -2.0 * M_PI / g.l_xi * ((j + 1) - 0.5 * (g.nk + 1));
+  g.phi.resize((size_t)g.n_xi * g.nk);
+  const double isl = 1.0 / std::sqrt(g.l_xi);
+  for (int l = 0; l < g.n_xi; ++l)
+    for (int j = 0; j < g.nk; ++j) {
+      double ph = g.k[j] * g.y[l];
+      g.phi[(size_t)l * g.nk + j] = cplx(isl * std::cos(ph), isl * std::sin(ph));
+    }
+  g.phi_lu = g.phi;
+  lu_factor(g.phi_lu, g.phi_piv, g.nk);
+  // Theta sample points (get_Drift_accel_trial_permutedShell.m:15-55).
+  g.edges = linspace(-g.l_xi / 2, g.l_xi / 2, g.n_xi + 1);
+  g.cap.resize(g.n_xi + 1);
+  for (int l = 0; l <= g.n_xi; ++l) g.cap[l] = cap_profile(g.edges[l], g.l_xi, g.delta_xi);
+  std::vector<double> chi_gl(kNq, 0.0);
+  for (int i = 0; i + 1 < kNq; ++i) chi_gl[i + 1] = chi_gl[i] + (g.xq[i + 1] - g.xq[i]) / 2 * g.delta_chi;
+  auto ends = linspace(0.0, g.l_chi, g.n_chi + 1);
+  g.xquad.resize((size_t)g.n_chi * kNq);
+  for (int e = 0; e < g.n_chi; ++e)
+    for (int q = 0; q < kNq; ++q) g.xquad[(size_t)e * kNq + q] = ends[e] + chi_gl[q];
+  g.chi.resize((size_t)g.n_chi * n);
+  for (int e = 0; e < g.n_chi; ++e) {
+    auto c = linspace(ends[e], ends[e + 1], n);  // initParams.m:157-159 (equispaced)
+    for (int a = 0; a < n; ++a) g.chi[(size_t)e * n + a] = c[a];
+  }
+  // Device scale: the largest transport coefficient.
+  double kmax = 0.0, k1max = 0.0;
+  for (double kv : g.k) kmax = std::max(kmax, std::abs(kv));
+  for (int i = 0; i < n * n; ++i) k1max = std::max(k1max, std::abs(g.K1[i]) + std::abs(g.K2[i]));
+  g.scale = g.q_diff / g.jac * kmax * k1max;
+}
+
+void inflow(const Geometry& g, const wdg_potential& pot, std::vector<cplx>& fnl,
+            std::vector<cplx>& fnr) {
+  const double hb = kHVT / 2.0 / M_PI;
+  const double q = kQVT, kB = kKBVT, T = pot.temp;
+  const int nk = g.nk;
+  const double dk = std::abs(g.k[1] - g.k[0]);
+  const double V1 = pot.em[0], VN = pot.em[pot.n_mat - 1];
+  std::vector<double> fL(nk), fR(nk);
+  for (int j = 0; j < nk; ++j) {
+    const double kk = g.k[j];
+    const double EL = V1 + hb * hb * kk * kk / 2.0 / pot.m_left / q;   // verteilung.m:92 / :135
+    const double ER = VN + hb * hb * kk * kk / 2.0 / pot.m_right / q;  // :93 / :136
+    if (pot.fermi_model == WDG_FERMI_MSA_3D) {                          // :98-99
+      fL[j] = 2.0 * pot.deg_factor / (1.0 + std::exp(q * (EL - pot.ef_left) / (kB * T)));
+      fR[j] = 2.0 * pot.deg_factor / (1.0 + std::exp(q * (ER - pot.ef_right) / (kB * T)));
+    } else {                                                            // :139-152
+      double vmax = pot.em[0];
+      for (int i = 0; i < pot.n_mat; ++i) vmax = std::max(vmax, pot.em[i]);
+      const double kzmax = std::sqrt(2.0 * 2.0 * kM0VT * pot.me_z * q * (vmax + 0.3)) / hb;
+      const int nkz = 1001;
+      const double dkz = kzmax / (nkz - 1);
+      double sl = 0.0, sr = 0.0;
+      for (int z = 0; z < nkz; ++z) {
+        const double kz = (z == nkz - 1) ? kzmax : z * dkz;
+        const double Ekz = hb * hb * kz * kz / 2.0 / kM0VT / pot.me_z / q;
+        sl += 1.0 / (1.0 + std::exp(q * (EL + Ekz - pot.ef_left) / (kB * T)));
+        sr += 1.0 / (1.0 + std::exp(q * (ER + Ekz - pot.ef_right) / (kB * T)));
+      }
+      fL[j] = 2.0 / M_PI * dkz * sl;
+      fR[j] = 2.0 / M_PI * dkz * sr;
+    }
+  }
+  // Cosine transform to xi, then phi \ f (verteilung.m:109-114).
+  std::vector<cplx> bl(g.n_xi), br(g.n_xi);
+  for (int l = 0; l < g.n_xi; ++l) {
+    double sl = 0.0, sr = 0.0;
+    for (int j = 0; j < nk; ++j) {
+      const double c = std::cos(g.y[l] * g.k[j]);
+      sl += c * fL[j];
+      sr += c * fR[j];
+    }
+    bl[l] = sl * dk / 2.0 / M_PI;
+    br[l] = sr * dk / 2.0 / M_PI;
+  }
+  fnl = lu_solve(g.phi_lu, g.phi_piv, nk, bl);
+  fnr = lu_solve(g.phi_lu, g.phi_piv, nk, br);
+}
+
+void build_rhs(const Geometry& g, const std::vector<cplx>& fnl, const std::vector<cplx>& fnr,
+               std::vector<double>& rhs) {
+  const int n = g.n, nk = g.nk, ne = g.n_chi;
+  const size_t N = (size_t)ne * n * nk;
+  rhs.assign(2 * N, 0.0);
+  for (int a = 0; a < n; ++a)
+    for (int j = 0; j < nk; ++j) {
+      const double kk = g.k[j];
+      if (kk > 0) {  // Q1 = invM(:,1)/detJx on element 1 (get_Diff.m:253-254,257)
+        cplx v = g.q_diff * g.invM[a * n + 0] / g.jac * std::abs(kk) * fnl[j];
+        size_t i = ((size_t)0 * n + a) * nk + j;
+        rhs[2 * i] = v.real();
+        rhs[2 * i + 1] = v.imag();
+      } else if (kk < 0) {  // Q2 = invM(:,n)/detJx on the last element (:255-256,258)
+        cplx v = g.q_diff * g.invM[a * n + (n - 1)] / g.jac * std::abs(kk) * fnr[j];
+        size_t i = ((size_t)(ne - 1) * n + a) * nk + j;
+        rhs[2 * i] = v.real();
+        rhs[2 * i + 1] = v.imag();
+      }
+    }
+}
+
+}  // namespace wdg

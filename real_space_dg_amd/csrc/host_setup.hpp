@@ -1,0 +1,64 @@
+// Host-side setup of the 1D-x x 1D-k Wigner operator: grids, DG element
+// matrices, plane-wave basis, inflow data.  Pure C++ (no HIP), so that it is
+// usable and testable without a GPU.
+#pragma once
+
+#include <complex>
+#include <string>
+#include <vector>
+
+#include "../../include/wigner_dg.h"
+
+namespace wdg {
+
+using cplx = std::complex<double>;
+
+// Reference constants, as each reference file spells them.
+constexpr double kHbarIP = 1.054571817e-34;           // DG_code/initParams.m:101
+constexpr double kMElektronIP = 0.041 * 9.1094e-31;   // initParams.m:102 (hard-coded 0.041 m0)
+constexpr double kQIP = 1.602176634e-19;              // initParams.m:103
+constexpr double kHVT = 6.626e-34;                    // verteilung.m:6, solve_transport_DG_2D.m:6
+constexpr double kM0VT = 9.109e-31;                   // verteilung.m:7
+constexpr double kKBVT = 1.38e-23;                    // verteilung.m:9
+constexpr double kQVT = 1.602e-19;                    // verteilung.m:10
+constexpr int kNq = 10;                               // initParams.m:61 (Nqx)
+
+struct Geometry {
+  int n_chi = 0, n = 0, n_xi = 0, nk = 0;
+  double l_chi = 0, l_xi = 0;
+  double delta_chi = 0, delta_xi = 0, jac = 0;  // jac = delta_chi/2 (detJx)
+  double q_diff = 0, q_drift = 0;
+  double scale = 1;                             // s_A: device operator is A/s_A
+  std::vector<double> r;                        // element GLL nodes (n)
+  std::vector<double> xq, wq;                   // 10 GLL quadrature nodes/weights
+  std::vector<double> Mref;                     // n*n mass (reference element)
+  std::vector<double> S;                        // n*n stiffness (M*Dmatrix)
+  std::vector<double> invM;                     // n*n inverse of (delta_xi/2)*Mref
+  std::vector<double> K1, K2;                   // n*n, K2 = K2L + K2R
+  std::vector<double> K2L, K2R, K3, K5;         // n*n
+  std::vector<double> Wq;                       // kNq*n Lagrange interpolation W_qa
+  std::vector<double> k;                        // nk wave numbers
+  std::vector<double> y;                        // n_xi phi sample points (xi_Rechengebiet_Elem)
+  std::vector<double> edges;                    // n_xi+1 B sample edges
+  std::vector<double> cap;                      // CAP at the edges
+  std::vector<double> xquad;                    // n_chi*kNq chi quadrature points
+  std::vector<double> chi;                      // n_chi*n node coordinates (equispaced, as reference)
+  std::vector<cplx> phi;                        // n_xi x nk, row-major phi[l*nk + j]
+  // LU factors of phi (row-major, partial pivoting) for the phi\f solves.
+  std::vector<cplx> phi_lu;
+  std::vector<int> phi_piv;
+};
+
+// Throws std::runtime_error on invalid input.
+void build_geometry(const wdg_desc& d, Geometry& g);
+
+// Inflow distributions fnl, fnr (k basis), DG_code/core/verteilung.m:77-170.
+void inflow(const Geometry& g, const wdg_potential& pot, std::vector<cplx>& fnl,
+            std::vector<cplx>& fnr);
+
+// Full right-hand side (interleaved complex, length 2N) in UNSCALED units
+// (get_Diff.m:253-259).
+void build_rhs(const Geometry& g, const std::vector<cplx>& fnl, const std::vector<cplx>& fnr,
+               std::vector<double>& rhs);
+
+}  // namespace wdg

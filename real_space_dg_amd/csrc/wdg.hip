@@ -1,0 +1,871 @@
+// libwdg: C ABI of the MI355X Wigner transport solver (see include/wigner_dg.h).
+//
+// One context = one stationary instance (grid + potential) with its own HIP
+// stream and device buffers.  The solve runs restarted right-preconditioned
+// GMRES entirely on the device: each restart cycle (Arnoldi steps, Givens
+// rotations, Hessenberg solve, update) is enqueued without host round trips;
+// kernels return immediately once the device-side convergence flag is set.
+// The host synchronises once per cycle to read the FP64 true residual.
+//
+// Precision modes share that loop:
+//   FP64  - every cycle in FP64 (FP64 MFMA GEMMs for Theta).
+//   mixed - the residual r = b - A x and the update x += M^-1 V y are FP64;
+//           each cycle's Krylov basis, preconditioner I/O and operator are
+//           FP32 with Theta on split-BF16 MFMA (FP32 accumulation).  This is
+//           GMRES-based iterative refinement with one cycle per refinement.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "host_setup.hpp"
+#include "kernels.hpp"
+
+using namespace wdg;
+
+namespace {
+
+thread_local std::string g_create_error;
+
+struct HipError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+#define HIPCHK(x)                                                                          \
+  do {                                                                                     \
+    hipError_t e_ = (x);                                                                   \
+    if (e_ != hipSuccess)                                                                  \
+      throw HipError(std::string(#x) + ": " + hipGetErrorString(e_));                      \
+  } while (0)
+
+template <typename T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  void alloc(size_t count) {
+    if (count <= n && p) return;
+    release();
+    HIPCHK(hipMalloc((void**)&p, std::max<size_t>(count, 1) * sizeof(T)));
+    n = count;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  ~DevBuf() { release(); }
+};
+
+enum ProfId { P_TRANSPORT = 0, P_GEMM1, P_COMBINE, P_GEMM2, P_LINE_AGG, P_LINE_APPLY, P_DOT, P_UPDATE, P_COUNT };
+const char* kProfNames[P_COUNT] = {"transport", "theta_gemm1", "theta_combine", "theta_gemm2",
+                                   "line_aggregate", "line_apply", "arnoldi_dot", "arnoldi_update"};
+
+}  // namespace
+
+struct wdg_ctx {
+  Geometry g;
+  std::string err;
+  hipStream_t stream = nullptr;
+  OpConst oc{};
+  DevBuf<OpConst> d_oc;
+  DevBuf<double> d_k, d_xquad, d_edges, d_cap, d_xg, d_ug;
+  DevBuf<double> d_sbar64;
+  DevBuf<float> d_sbar32;
+  DevBuf<double2> d_s0s1, d_agg;
+  DevBuf<double> d_A1, d_A2;          // realified phi / i*phi^H, FP64
+  DevBuf<uint16_t> d_A1b, d_A2b;      // same in BF16
+  DevBuf<double2> d_T64, d_Z64;       // Theta intermediates
+  DevBuf<float2> d_T32, d_Z32;
+  DevBuf<double2> d_b, d_x, d_r, d_y; // FP64 solve vectors (normalised units)
+  DevBuf<double2> d_u64, d_v64;       // scratch for host-buffer entry points
+  DevBuf<float2> d_u32, d_v32;
+  DevBuf<double2> d_phic;             // two phi rows around xi = 0
+  DevBuf<double2> d_obs;
+  // Krylov storage (allocated per precision on demand)
+  DevBuf<double2> d_V64, d_w64, d_z64, d_s64;
+  DevBuf<float2> d_V32, d_w32, d_z32, d_s32;
+  DevBuf<double2> d_H, d_cs, d_sn, d_g, d_h1, d_h2, d_yv, d_part;
+  DevBuf<GmresState> d_st;
+  std::vector<double> rhs;           // physical rhs, interleaved
+  double rhs_norm = 0;
+  bool have_potential = false;
+  bool have_solution = false;
+  double sol_scale = 1;              // physical x = sol_scale * x_device
+  long N = 0;
+  int ncols = 0;
+  int red_blocks = 0;
+  int m_alloc = -1;
+  // profiling
+  unsigned prof_mask = 0;
+  std::vector<hipEvent_t> ev_pool;
+  size_t ev_used = 0;
+  std::vector<std::pair<int, size_t>> ev_pending;  // (kernel id, start event index)
+  double prof_ms[P_COUNT] = {0};
+  long prof_n[P_COUNT] = {0};
+  GmresBufs gb() {
+    GmresBufs b;
+    b.H = d_H.p; b.cs = d_cs.p; b.sn = d_sn.p; b.g = d_g.p; b.h1 = d_h1.p; b.h2 = d_h2.p;
+    b.y = d_yv.p; b.part = d_part.p; b.st = d_st.p; b.m = m_alloc;
+    return b;
+  }
+};
+
+namespace {
+
+int fail(wdg_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  else g_create_error = msg;
+  return code;
+}
+
+template <typename F>
+int guarded(wdg_ctx* c, F&& f) {
+  try {
+    return f();
+  } catch (const HipError& e) {
+    return fail(c, WDG_ERR_HIP, e.what());
+  } catch (const std::bad_alloc&) {
+    return fail(c, WDG_ERR_MEMORY, "host allocation failed");
+  } catch (const std::invalid_argument& e) {
+    return fail(c, WDG_ERR_ARG, e.what());
+  } catch (const std::runtime_error& e) {
+    return fail(c, WDG_ERR_ARG, e.what());
+  }
+}
+
+inline unsigned grid1(long work, int nt) { return (unsigned)std::max<long>(1, (work + nt - 1) / nt); }
+
+// -------- profiling helpers --------
+struct ProfScope {
+  wdg_ctx* c;
+  int id;
+  size_t i0 = (size_t)-1;
+  ProfScope(wdg_ctx* c_, int id_) : c(c_), id(id_) {
+    if (!(c->prof_mask & (1u << id))) return;
+    if (c->ev_used + 2 > c->ev_pool.size()) {
+      for (int i = 0; i < 64; ++i) {
+        hipEvent_t e;
+        HIPCHK(hipEventCreate(&e));
+        c->ev_pool.push_back(e);
+      }
+    }
+    i0 = c->ev_used;
+    c->ev_used += 2;
+    HIPCHK(hipEventRecord(c->ev_pool[i0], c->stream));
+  }
+  ~ProfScope() {
+    if (i0 == (size_t)-1) return;
+    (void)hipEventRecord(c->ev_pool[i0 + 1], c->stream);
+    c->ev_pending.push_back({id, i0});
+  }
+};
+
+void prof_collect(wdg_ctx* c) {
+  if (c->ev_pending.empty()) {
+    c->ev_used = 0;
+    return;
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (auto& pe : c->ev_pending) {
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, c->ev_pool[pe.second], c->ev_pool[pe.second + 1]));
+    c->prof_ms[pe.first] += ms;
+    c->prof_n[pe.first] += 1;
+  }
+  c->ev_pending.clear();
+  c->ev_used = 0;
+}
+
+}  // namespace
+
+namespace {
+
+template <typename T>
+void op_transport(wdg_ctx* c, const typename cvec<T>::type* u, typename cvec<T>::type* y, const GmresState* st) {
+  ProfScope ps(c, P_TRANSPORT);
+  const long work = (long)c->g.n_chi * c->g.nk;
+  hipLaunchKernelGGL((transport_kernel<T>), dim3(grid1(work, 256)), dim3(256), 0, c->stream, c->d_oc.p, c->d_k.p,
+                     u, y, st);
+}
+
+void op_gemm64(wdg_ctx* c, int which, const double* A, const double* B, double* C, int M, int K, bool accum,
+               const GmresState* st) {
+  ProfScope ps(c, which);
+  constexpr int BM = 64, BN = 64;
+  dim3 grid(M / BM, (c->ncols + BN - 1) / BN);
+  if (accum)
+    hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, true>), grid, dim3(256), 0, c->stream, A, B, C, M, c->ncols, K, st);
+  else
+    hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, false>), grid, dim3(256), 0, c->stream, A, B, C, M, c->ncols, K, st);
+}
+
+void op_gemm32(wdg_ctx* c, int which, const uint16_t* A, const float* B, float* C, int M, int K, bool accum,
+               const GmresState* st) {
+  ProfScope ps(c, which);
+  constexpr int BM = 64, BN = 64;
+  dim3 grid(M / BM, (c->ncols + BN - 1) / BN);
+  if (accum)
+    hipLaunchKernelGGL((gemm_bf16x2_kernel<BM, BN, true>), grid, dim3(256), 0, c->stream, A, B, C, M, c->ncols, K, st);
+  else
+    hipLaunchKernelGGL((gemm_bf16x2_kernel<BM, BN, false>), grid, dim3(256), 0, c->stream, A, B, C, M, c->ncols, K, st);
+}
+
+// y = A_hat u in precision T.
+template <typename T>
+void op_apply(wdg_ctx* c, const typename cvec<T>::type* u, typename cvec<T>::type* y, const GmresState* st) {
+  op_transport<T>(c, u, y, st);
+  const int M1 = 2 * c->g.n_xi, K1 = 2 * c->g.nk;
+  const long work = (long)c->g.n_chi * c->g.n_xi;
+  if constexpr (std::is_same<T, double>::value) {
+    op_gemm64(c, P_GEMM1, c->d_A1.p, (const double*)u, (double*)c->d_T64.p, M1, K1, false, st);
+    {
+      ProfScope ps(c, P_COMBINE);
+      hipLaunchKernelGGL((combine_kernel<double, double>), dim3(grid1(work, 256)), dim3(256), 0, c->stream,
+                         c->d_oc.p, c->d_sbar64.p, c->d_cap.p, c->d_T64.p, c->d_Z64.p, st);
+    }
+    op_gemm64(c, P_GEMM2, c->d_A2.p, (const double*)c->d_Z64.p, (double*)y, 2 * c->g.nk, 2 * c->g.n_xi, true, st);
+  } else {
+    op_gemm32(c, P_GEMM1, c->d_A1b.p, (const float*)u, (float*)c->d_T32.p, M1, K1, false, st);
+    {
+      ProfScope ps(c, P_COMBINE);
+      hipLaunchKernelGGL((combine_kernel<float, float>), dim3(grid1(work, 256)), dim3(256), 0, c->stream,
+                         c->d_oc.p, c->d_sbar32.p, c->d_cap.p, c->d_T32.p, c->d_Z32.p, st);
+    }
+    op_gemm32(c, P_GEMM2, c->d_A2b.p, (const float*)c->d_Z32.p, (float*)y, 2 * c->g.nk, 2 * c->g.n_xi, true, st);
+  }
+}
+
+template <typename T>
+void op_precond(wdg_ctx* c, const typename cvec<T>::type* r, typename cvec<T>::type* z, const GmresState* st) {
+  const long work = (long)c->oc.n_chunks * c->g.nk;
+  {
+    ProfScope ps(c, P_LINE_AGG);
+    hipLaunchKernelGGL((line_aggregate_kernel<T>), dim3(grid1(work, 128)), dim3(128), 0, c->stream, c->d_oc.p,
+                       c->d_k.p, c->d_s0s1.p, r, c->d_agg.p, st);
+  }
+  {
+    ProfScope ps(c, P_LINE_APPLY);
+    hipLaunchKernelGGL((line_apply_kernel<T>), dim3(grid1(work, 128)), dim3(128), 0, c->stream, c->d_oc.p,
+                       c->d_k.p, c->d_s0s1.p, c->d_agg.p, r, z, st);
+  }
+}
+
+void ensure_krylov(wdg_ctx* c, int m, bool fp64) {
+  if (m > kMaxRestart) throw std::invalid_argument("restart exceeds the supported maximum of 64");
+  if (c->m_alloc != m) {
+    c->d_H.alloc((size_t)(m + 1) * m);
+    c->d_cs.alloc(m);
+    c->d_sn.alloc(m);
+    c->d_g.alloc(m + 1);
+    c->d_h1.alloc(m + 2);
+    c->d_h2.alloc(m + 2);
+    c->d_yv.alloc(m);
+    c->m_alloc = m;
+  }
+  const int ngrp = (m + kDotGroup) / kDotGroup + 1;
+  c->d_part.alloc((size_t)c->red_blocks * ngrp * (kDotGroup + 1));
+  if (!c->d_st.p) {
+    c->d_st.alloc(1);
+    HIPCHK(hipMemset(c->d_st.p, 0, sizeof(GmresState)));
+  }
+  const size_t N = c->N;
+  if (fp64) {
+    c->d_V64.alloc((size_t)(m + 1) * N);
+    c->d_w64.alloc(N);
+    c->d_z64.alloc(N);
+    c->d_s64.alloc(N);
+  } else {
+    c->d_V32.alloc((size_t)(m + 1) * N);
+    c->d_w32.alloc(N);
+    c->d_z32.alloc(N);
+    c->d_s32.alloc(N);
+  }
+}
+
+// r = b - A x in FP64; ||r|| into st->resnorm.
+void residual64(wdg_ctx* c) {
+  op_apply<double>(c, c->d_x.p, c->d_y.p, nullptr);
+  hipLaunchKernelGGL((residual_kernel<256>), dim3(c->red_blocks), dim3(256), 0, c->stream, c->d_b.p, c->d_y.p,
+                     c->d_r.p, c->N, c->d_part.p, c->d_st.p);
+}
+
+// One restart cycle in precision T (no host synchronisation).
+template <typename T>
+void gmres_cycle(wdg_ctx* c, int m, double tol_abs, double inner_tol, bool precond) {
+  using T2 = typename cvec<T>::type;
+  T2 *V, *w, *z, *s;
+  if constexpr (std::is_same<T, double>::value) {
+    V = c->d_V64.p; w = c->d_w64.p; z = c->d_z64.p; s = c->d_s64.p;
+  } else {
+    V = c->d_V32.p; w = c->d_w32.p; z = c->d_z32.p; s = c->d_s32.p;
+  }
+  const long N = c->N;
+  GmresBufs gb = c->gb();
+  const GmresState* st = c->d_st.p;
+  (void)inner_tol;
+  hipLaunchKernelGGL(cycle_init_kernel, dim3(1), dim3(1), 0, c->stream, gb, tol_abs, inner_tol);
+  hipLaunchKernelGGL((residual_normalize_kernel<T>), dim3(c->red_blocks), dim3(256), 0, c->stream, c->d_r.p, V, N,
+                     st);
+  for (int j = 0; j < m; ++j) {
+    T2* vj = V + (size_t)j * N;
+    const T2* opin = vj;
+    if (precond) {
+      op_precond<T>(c, vj, z, st);
+      opin = z;
+    }
+    op_apply<T>(c, opin, w, st);
+    const int nv = j + 1;
+    const dim3 dg(c->red_blocks, (nv + kDotGroup - 1) / kDotGroup);
+    {
+      ProfScope ps(c, P_DOT);
+      hipLaunchKernelGGL((arnoldi_dot_kernel<T, 256>), dg, dim3(256), 0, c->stream, gb, V, w, N, nv, 0);
+    }
+    {
+      ProfScope ps(c, P_UPDATE);
+      hipLaunchKernelGGL((arnoldi_update_kernel<T, 256>), dim3(c->red_blocks), dim3(256), 0, c->stream, gb, V, w, N,
+                         j, 0);
+    }
+    hipLaunchKernelGGL((arnoldi_dot_kernel<T, 256>), dg, dim3(256), 0, c->stream, gb, V, w, N, nv, 1);
+    hipLaunchKernelGGL((arnoldi_update_kernel<T, 256>), dim3(c->red_blocks), dim3(256), 0, c->stream, gb, V, w, N, j,
+                       1);
+    hipLaunchKernelGGL((arnoldi_normalize_kernel<T>), dim3(c->red_blocks), dim3(256), 0, c->stream, gb, w,
+                       V + (size_t)(j + 1) * N, N, j);
+  }
+  hipLaunchKernelGGL(hess_solve_kernel, dim3(1), dim3(1), 0, c->stream, gb);
+  hipLaunchKernelGGL((lincomb_kernel<T>), dim3(c->red_blocks), dim3(256), 0, c->stream, gb, V, s, N);
+  const T2* corr = s;
+  if (precond) {
+    op_precond<T>(c, s, z, nullptr);
+    corr = z;
+  }
+  hipLaunchKernelGGL((axpy_to64_kernel<T>), dim3(c->red_blocks), dim3(256), 0, c->stream, c->d_x.p, corr, 1.0, N);
+}
+
+void upload_const(wdg_ctx* c) {
+  HIPCHK(hipMemcpyAsync(c->d_oc.p, &c->oc, sizeof(OpConst), hipMemcpyHostToDevice, c->stream));
+}
+
+uint16_t host_bf16(double v) {
+  float f = (float)v;
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+void create_impl(wdg_ctx* c, const wdg_desc& d) {
+  build_geometry(d, c->g);
+  const Geometry& g = c->g;
+  const int n = g.n;
+  c->N = (long)g.n_chi * n * g.nk;
+  c->ncols = g.n_chi * n;
+  c->red_blocks = (int)std::min<long>(512, std::max<long>(1, c->N / (256 * 8)));
+  HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  OpConst& oc = c->oc;
+  std::memset(&oc, 0, sizeof(oc));
+  oc.n_chi = g.n_chi;
+  oc.n = n;
+  oc.nk = g.nk;
+  oc.n_xi = g.n_xi;
+  oc.n_pairs = n * (n + 1) / 2;
+  oc.chunk = std::max(1, std::min(16, g.n_chi));
+  oc.n_chunks = (g.n_chi + oc.chunk - 1) / oc.chunk;
+  oc.tq = g.q_diff / g.jac / g.scale;
+  oc.inv_len = 1.0 / g.l_xi;
+  oc.hprime = g.y[1] - g.y[0];
+  for (int a = 0; a < n; ++a) {
+    for (int b = 0; b < n; ++b) {
+      oc.K1[a * kMaxN + b] = g.K1[a * n + b];
+      oc.K2[a * kMaxN + b] = g.K2[a * n + b];
+      oc.cab[a * kMaxN + b] = g.q_drift * g.invM[a * n + b] * g.delta_xi / 4.0 / g.scale;
+      double mq = 0;
+      for (int q = 0; q < kNq; ++q) mq += g.wq[q] * g.Wq[q * n + a] * g.Wq[q * n + b];
+      oc.Mq[a * kMaxN + b] = mq;
+    }
+    oc.invM0[a] = g.invM[a * n + 0];
+    oc.invMn[a] = g.invM[a * n + (n - 1)];
+  }
+  int p = 0;
+  for (int a = 0; a < n; ++a)
+    for (int b = a; b < n; ++b) {
+      oc.pa[p] = a;
+      oc.pb[p] = b;
+      oc.pidx[a * kMaxN + b] = p;
+      oc.pidx[b * kMaxN + a] = p;
+      for (int q = 0; q < kNq; ++q) oc.Sq[q * kMaxPairs + p] = g.wq[q] * g.Wq[q * n + a] * g.Wq[q * n + b];
+      ++p;
+    }
+  c->d_oc.alloc(1);
+  upload_const(c);
+  auto up = [&](DevBuf<double>& b, const std::vector<double>& v) {
+    b.alloc(v.size());
+    HIPCHK(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  };
+  up(c->d_k, g.k);
+  up(c->d_xquad, g.xquad);
+  up(c->d_edges, g.edges);
+  up(c->d_cap, g.cap);
+  // Realified phi for GEMM1 (T = phi U) and i*phi^H for GEMM2 (Y = i phi^H Z).
+  const int nx = g.n_xi, nk = g.nk;
+  std::vector<double> A1((size_t)2 * nx * 2 * nk), A2((size_t)2 * nk * 2 * nx);
+  for (int l = 0; l < nx; ++l)
+    for (int j = 0; j < nk; ++j) {
+      const double pr = g.phi[(size_t)l * nk + j].real(), pi = g.phi[(size_t)l * nk + j].imag();
+      const size_t K1 = 2 * nk, K2 = 2 * nx;
+      A1[(size_t)(2 * l) * K1 + 2 * j] = pr;
+      A1[(size_t)(2 * l) * K1 + 2 * j + 1] = -pi;
+      A1[(size_t)(2 * l + 1) * K1 + 2 * j] = pi;
+      A1[(size_t)(2 * l + 1) * K1 + 2 * j + 1] = pr;
+      // psi(j,l) = i conj(phi(l,j)) = pi + i pr
+      A2[(size_t)(2 * j) * K2 + 2 * l] = pi;
+      A2[(size_t)(2 * j) * K2 + 2 * l + 1] = -pr;
+      A2[(size_t)(2 * j + 1) * K2 + 2 * l] = pr;
+      A2[(size_t)(2 * j + 1) * K2 + 2 * l + 1] = pi;
+    }
+  up(c->d_A1, A1);
+  up(c->d_A2, A2);
+  std::vector<uint16_t> A1b(A1.size()), A2b(A2.size());
+  for (size_t i = 0; i < A1.size(); ++i) A1b[i] = host_bf16(A1[i]);
+  for (size_t i = 0; i < A2.size(); ++i) A2b[i] = host_bf16(A2[i]);
+  c->d_A1b.alloc(A1b.size());
+  c->d_A2b.alloc(A2b.size());
+  HIPCHK(hipMemcpyAsync(c->d_A1b.p, A1b.data(), A1b.size() * 2, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->d_A2b.p, A2b.data(), A2b.size() * 2, hipMemcpyHostToDevice, c->stream));
+  // Centre phi rows for the observables.
+  const int lc = nx / 2;
+  std::vector<double2> ph(2 * nk);
+  for (int j = 0; j < nk; ++j) {
+    ph[j] = make_double2(g.phi[(size_t)(lc - 1) * nk + j].real(), g.phi[(size_t)(lc - 1) * nk + j].imag());
+    ph[nk + j] = make_double2(g.phi[(size_t)lc * nk + j].real(), g.phi[(size_t)lc * nk + j].imag());
+  }
+  c->d_phic.alloc(2 * nk);
+  HIPCHK(hipMemcpyAsync(c->d_phic.p, ph.data(), ph.size() * sizeof(double2), hipMemcpyHostToDevice, c->stream));
+  const size_t tz = (size_t)c->ncols * nx;
+  c->d_T64.alloc(tz);
+  c->d_Z64.alloc(tz);
+  c->d_T32.alloc(tz);
+  c->d_Z32.alloc(tz);
+  c->d_s0s1.alloc((size_t)g.n_chi * n * n * 2);
+  c->d_agg.alloc((size_t)oc.n_chunks * nk * 2);
+  c->d_sbar64.alloc((size_t)g.n_chi * oc.n_pairs * (nx + 1));
+  c->d_sbar32.alloc((size_t)g.n_chi * oc.n_pairs * (nx + 1));
+  c->d_b.alloc(c->N);
+  c->d_x.alloc(c->N);
+  c->d_r.alloc(c->N);
+  c->d_y.alloc(c->N);
+  c->d_obs.alloc((size_t)2 * c->ncols);
+  c->d_st.alloc(1);
+  HIPCHK(hipMemsetAsync(c->d_st.p, 0, sizeof(GmresState), c->stream));
+  c->d_part.alloc((size_t)c->red_blocks * 2 * (kDotGroup + 1));
+  HIPCHK(hipStreamSynchronize(c->stream));
+}
+
+void set_potential_impl(wdg_ctx* c, const wdg_potential& pot) {
+  const Geometry& g = c->g;
+  if (pot.n_mat != g.n_chi * (g.n - 1) + 1)
+    throw std::invalid_argument("n_mat must equal n_chi*(n_k_chi-1)+1 (mat.Nx)");
+  if (!pot.x_mat || !pot.em) throw std::invalid_argument("x_mat and em are required");
+  for (int i = 1; i < pot.n_mat; ++i)
+    if (!(pot.x_mat[i] > pot.x_mat[i - 1])) throw std::invalid_argument("x_mat must be strictly ascending");
+  for (int i = 0; i < pot.n_mat; ++i)
+    if (!std::isfinite(pot.em[i])) throw std::invalid_argument("em must be finite");
+  if (!(pot.m_left > 0) || !(pot.m_right > 0) || !(pot.temp > 0))
+    throw std::invalid_argument("masses and temperature must be positive");
+  if (pot.fermi_model != WDG_FERMI_MSA_3D && pot.fermi_model != WDG_FERMI_MSA_2D)
+    throw std::invalid_argument("unknown fermi_model");
+  if (pot.fermi_model == WDG_FERMI_MSA_2D && !(pot.me_z > 0)) throw std::invalid_argument("me_z must be positive");
+  // inflow and rhs (host, FP64)
+  std::vector<cplx> fnl, fnr;
+  inflow(g, pot, fnl, fnr);
+  build_rhs(g, fnl, fnr, c->rhs);
+  double s = 0;
+  for (double v : c->rhs) s += v * v;
+  c->rhs_norm = std::sqrt(s);
+  std::vector<double> bn(c->rhs.size());
+  const double inv = (c->rhs_norm > 0) ? 1.0 / c->rhs_norm : 0.0;
+  for (size_t i = 0; i < bn.size(); ++i) bn[i] = c->rhs[i] * inv;
+  HIPCHK(hipMemcpyAsync(c->d_b.p, bn.data(), bn.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  // potential tables on the device
+  std::vector<double> xg(pot.x_mat, pot.x_mat + pot.n_mat), ug(pot.em, pot.em + pot.n_mat);
+  c->d_xg.alloc(xg.size());
+  c->d_ug.alloc(ug.size());
+  HIPCHK(hipMemcpyAsync(c->d_xg.p, xg.data(), xg.size() * 8, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->d_ug.p, ug.data(), ug.size() * 8, hipMemcpyHostToDevice, c->stream));
+  const double umin = *std::min_element(ug.begin(), ug.end());
+  const long work = (long)g.n_chi * (g.n_xi + 1);
+  hipLaunchKernelGGL(drift_table_kernel, dim3(grid1(work, 256)), dim3(256), 0, c->stream, c->d_oc.p, c->d_xquad.p,
+                     c->d_edges.p, c->d_xg.p, c->d_ug.p, pot.n_mat, g.l_chi, umin, c->d_sbar64.p, c->d_sbar32.p);
+  hipLaunchKernelGGL(precond_sums_kernel, dim3(g.n_chi * g.n * g.n), dim3(256), 0, c->stream, c->d_oc.p,
+                     c->d_sbar64.p, c->d_cap.p, c->d_s0s1.p);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->have_potential = true;
+  c->have_solution = false;
+}
+
+void check_ready(wdg_ctx* c) {
+  if (!c->have_potential) throw std::logic_error("wdg_set_potential has not been called");
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+const char* wdg_last_error(const wdg_ctx* ctx) {
+  if (ctx) return ctx->err.c_str();
+  return g_create_error.c_str();
+}
+
+int wdg_create(const wdg_desc* desc, wdg_ctx** out) {
+  if (!desc || !out) return fail(nullptr, WDG_ERR_ARG, "null argument");
+  *out = nullptr;
+  auto c = std::make_unique<wdg_ctx>();
+  int rc = guarded(nullptr, [&]() {
+    create_impl(c.get(), *desc);
+    return WDG_OK;
+  });
+  if (rc != WDG_OK) {
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    c->stream = nullptr;
+    return rc;
+  }
+  *out = c.release();
+  return WDG_OK;
+}
+
+int wdg_destroy(wdg_ctx* ctx) {
+  if (!ctx) return WDG_OK;
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  for (auto e : ctx->ev_pool) (void)hipEventDestroy(e);
+  hipStream_t s = ctx->stream;
+  delete ctx;
+  if (s) (void)hipStreamDestroy(s);
+  return WDG_OK;
+}
+
+int64_t wdg_size(const wdg_ctx* ctx) { return ctx ? ctx->N : 0; }
+
+int wdg_set_potential(wdg_ctx* ctx, const wdg_potential* pot) {
+  if (!ctx || !pot) return fail(ctx, WDG_ERR_ARG, "null argument");
+  return guarded(ctx, [&]() {
+    set_potential_impl(ctx, *pot);
+    return WDG_OK;
+  });
+}
+
+int wdg_rhs(wdg_ctx* ctx, double* rhs) {
+  if (!ctx || !rhs) return fail(ctx, WDG_ERR_ARG, "null argument");
+  if (!ctx->have_potential) return fail(ctx, WDG_ERR_STATE, "wdg_set_potential has not been called");
+  std::memcpy(rhs, ctx->rhs.data(), ctx->rhs.size() * sizeof(double));
+  return WDG_OK;
+}
+
+int wdg_apply(wdg_ctx* ctx, const double* u, double* y, int32_t precision) {
+  if (!ctx || !u || !y) return fail(ctx, WDG_ERR_ARG, "null argument");
+  if (!ctx->have_potential) return fail(ctx, WDG_ERR_STATE, "wdg_set_potential has not been called");
+  if (precision != WDG_PREC_FP64 && precision != WDG_PREC_MIXED) return fail(ctx, WDG_ERR_ARG, "unknown precision");
+  return guarded(ctx, [&]() {
+    const long N = ctx->N;
+    const double sA = ctx->g.scale;
+    if (precision == WDG_PREC_FP64) {
+      ctx->d_u64.alloc(N);
+      ctx->d_v64.alloc(N);
+      HIPCHK(hipMemcpyAsync(ctx->d_u64.p, u, N * 16, hipMemcpyHostToDevice, ctx->stream));
+      op_apply<double>(ctx, ctx->d_u64.p, ctx->d_v64.p, nullptr);
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipMemcpyAsync(y, ctx->d_v64.p, N * 16, hipMemcpyDeviceToHost, ctx->stream));
+      HIPCHK(hipStreamSynchronize(ctx->stream));
+      for (long i = 0; i < 2 * N; ++i) y[i] *= sA;
+    } else {
+      std::vector<float> u32(2 * N), y32(2 * N);
+      for (long i = 0; i < 2 * N; ++i) u32[i] = (float)u[i];
+      ctx->d_u32.alloc(N);
+      ctx->d_v32.alloc(N);
+      HIPCHK(hipMemcpyAsync(ctx->d_u32.p, u32.data(), N * 8, hipMemcpyHostToDevice, ctx->stream));
+      op_apply<float>(ctx, ctx->d_u32.p, ctx->d_v32.p, nullptr);
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipMemcpyAsync(y32.data(), ctx->d_v32.p, N * 8, hipMemcpyDeviceToHost, ctx->stream));
+      HIPCHK(hipStreamSynchronize(ctx->stream));
+      for (long i = 0; i < 2 * N; ++i) y[i] = sA * (double)y32[i];
+    }
+    prof_collect(ctx);
+    return WDG_OK;
+  });
+}
+
+int wdg_apply_device(wdg_ctx* ctx, const void* d_u, void* d_y, int32_t precision) {
+  if (!ctx || !d_u || !d_y) return fail(ctx, WDG_ERR_ARG, "null argument");
+  if (!ctx->have_potential) return fail(ctx, WDG_ERR_STATE, "wdg_set_potential has not been called");
+  return guarded(ctx, [&]() {
+    if (precision == WDG_PREC_FP64)
+      op_apply<double>(ctx, (const double2*)d_u, (double2*)d_y, nullptr);
+    else
+      op_apply<float>(ctx, (const float2*)d_u, (float2*)d_y, nullptr);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    prof_collect(ctx);
+    return WDG_OK;
+  });
+}
+
+int wdg_precond(wdg_ctx* ctx, const double* r, double* z) {
+  if (!ctx || !r || !z) return fail(ctx, WDG_ERR_ARG, "null argument");
+  if (!ctx->have_potential) return fail(ctx, WDG_ERR_STATE, "wdg_set_potential has not been called");
+  return guarded(ctx, [&]() {
+    const long N = ctx->N;
+    ctx->d_u64.alloc(N);
+    ctx->d_v64.alloc(N);
+    HIPCHK(hipMemcpyAsync(ctx->d_u64.p, r, N * 16, hipMemcpyHostToDevice, ctx->stream));
+    op_precond<double>(ctx, ctx->d_u64.p, ctx->d_v64.p, nullptr);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(z, ctx->d_v64.p, N * 16, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    const double inv = 1.0 / ctx->g.scale;
+    for (long i = 0; i < 2 * N; ++i) z[i] *= inv;
+    prof_collect(ctx);
+    return WDG_OK;
+  });
+}
+
+int wdg_precond_device(wdg_ctx* ctx, const void* d_r, void* d_z, int32_t precision) {
+  if (!ctx || !d_r || !d_z) return fail(ctx, WDG_ERR_ARG, "null argument");
+  if (!ctx->have_potential) return fail(ctx, WDG_ERR_STATE, "wdg_set_potential has not been called");
+  return guarded(ctx, [&]() {
+    if (precision == WDG_PREC_FP64)
+      op_precond<double>(ctx, (const double2*)d_r, (double2*)d_z, nullptr);
+    else
+      op_precond<float>(ctx, (const float2*)d_r, (float2*)d_z, nullptr);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    prof_collect(ctx);
+    return WDG_OK;
+  });
+}
+
+int wdg_diagonal(wdg_ctx* ctx, double* d) {
+  if (!ctx || !d) return fail(ctx, WDG_ERR_ARG, "null argument");
+  if (!ctx->have_potential) return fail(ctx, WDG_ERR_STATE, "wdg_set_potential has not been called");
+  return guarded(ctx, [&]() {
+    const Geometry& g = ctx->g;
+    const int n = g.n, nk = g.nk;
+    std::vector<double2> s0s1((size_t)g.n_chi * n * n * 2);
+    HIPCHK(hipMemcpyAsync(s0s1.data(), ctx->d_s0s1.p, s0s1.size() * sizeof(double2), hipMemcpyDeviceToHost,
+                          ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    const double sA = g.scale, tq = ctx->oc.tq, il = ctx->oc.inv_len, hp = ctx->oc.hprime;
+    for (int e = 0; e < g.n_chi; ++e)
+      for (int a = 0; a < n; ++a)
+        for (int j = 0; j < nk; ++j) {
+          const double k = g.k[j];
+          const double2 s0 = s0s1[((size_t)e * n * n + a * n + a) * 2];
+          const double2 s1 = s0s1[((size_t)e * n * n + a * n + a) * 2 + 1];
+          const double ck = std::cos(k * hp);
+          const double gr = s0.x + 2 * ck * s1.x, gi = s0.y + 2 * ck * s1.y;
+          const double re = tq * (k * g.K1[a * n + a] + std::abs(k) * g.K2[a * n + a]) - il * gi;
+          const double im = il * gr;
+          const size_t i = ((size_t)e * n + a) * nk + j;
+          d[2 * i] = sA * re;
+          d[2 * i + 1] = sA * im;
+        }
+    return WDG_OK;
+  });
+}
+
+int wdg_solve(wdg_ctx* ctx, const wdg_solve_opts* opts, const double* x0, double* x, wdg_solve_info* info) {
+  if (!ctx || !opts || !info) return fail(ctx, WDG_ERR_ARG, "null argument");
+  if (!ctx->have_potential) return fail(ctx, WDG_ERR_STATE, "wdg_set_potential has not been called");
+  if (!(opts->tol > 0) || opts->restart < 1 || opts->max_iter < 1)
+    return fail(ctx, WDG_ERR_ARG, "tol must be positive, restart and max_iter at least 1");
+  if (opts->precision != WDG_PREC_FP64 && opts->precision != WDG_PREC_MIXED)
+    return fail(ctx, WDG_ERR_ARG, "unknown precision");
+  return guarded(ctx, [&]() {
+    std::memset(info, 0, sizeof(*info));
+    const long N = ctx->N;
+    const bool fp64 = opts->precision == WDG_PREC_FP64;
+    const int m = std::min<long>(opts->restart, N);
+    ensure_krylov(ctx, m, fp64);
+    const double sA = ctx->g.scale;
+    // physical x = (||b|| / s_A) * x_device
+    const double xs = (ctx->rhs_norm > 0) ? ctx->rhs_norm / sA : 1.0;
+    ctx->sol_scale = xs;
+    if (x0) {
+      std::vector<double> xd(2 * N);
+      for (long i = 0; i < 2 * N; ++i) {
+        if (!std::isfinite(x0[i])) throw std::invalid_argument("x0 must contain only finite values");
+        xd[i] = x0[i] / xs;
+      }
+      HIPCHK(hipMemcpyAsync(ctx->d_x.p, xd.data(), 2 * N * 8, hipMemcpyHostToDevice, ctx->stream));
+    } else {
+      HIPCHK(hipMemsetAsync(ctx->d_x.p, 0, N * 16, ctx->stream));
+    }
+    if (ctx->rhs_norm == 0 && !x0) {
+      info->flag = 0;
+      info->true_relres = 0;
+      ctx->have_solution = true;
+      if (x) std::memset(x, 0, 2 * N * sizeof(double));
+      return WDG_OK;
+    }
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventRecord(e0, ctx->stream));
+    residual64(ctx);
+    GmresState hs{};
+    HIPCHK(hipMemcpyAsync(&hs, ctx->d_st.p, sizeof(GmresState), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    info->initial_true_relres = hs.resnorm;
+    double relres = hs.resnorm;
+    int its = 0, cycles = 0;
+    const double tol = opts->tol;
+    const double inner = (opts->inner_tol > 0) ? opts->inner_tol : 1e-4;
+    const int max_outer = (opts->max_outer > 0) ? opts->max_outer : 1000000;
+    double reported = relres;
+    while (relres > tol && its < opts->max_iter && cycles < max_outer) {
+      const int mc = std::min<int>(m, opts->max_iter - its);
+      if (fp64)
+        gmres_cycle<double>(ctx, mc, tol, 0.0, opts->use_precond != 0);
+      else
+        gmres_cycle<float>(ctx, mc, tol, inner, opts->use_precond != 0);
+      residual64(ctx);
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipMemcpyAsync(&hs, ctx->d_st.p, sizeof(GmresState), hipMemcpyDeviceToHost, ctx->stream));
+      HIPCHK(hipStreamSynchronize(ctx->stream));
+      prof_collect(ctx);
+      its += hs.jlast;
+      cycles += 1;
+      reported = hs.resid;
+      relres = hs.resnorm;
+      if (!std::isfinite(relres)) break;
+      if (hs.jlast == 0) break;  // no progress possible (breakdown at start)
+    }
+    HIPCHK(hipEventRecord(e1, ctx->stream));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    info->seconds = ms * 1e-3;
+    info->iterations = its;
+    info->restarts = cycles;
+    info->outer_iterations = fp64 ? 1 : cycles;
+    info->true_relres = relres;
+    info->reported_relres = reported;
+    info->flag = (std::isfinite(relres) && relres <= tol) ? 0 : 1;
+    ctx->have_solution = true;
+    if (x) {
+      HIPCHK(hipMemcpyAsync(x, ctx->d_x.p, 2 * N * 8, hipMemcpyDeviceToHost, ctx->stream));
+      HIPCHK(hipStreamSynchronize(ctx->stream));
+      for (long i = 0; i < 2 * N; ++i) x[i] *= xs;
+    }
+    return WDG_OK;
+  });
+}
+
+int wdg_observables(wdg_ctx* ctx, const double* rho, const double* m_nodes, double* n_out, double* j_out,
+                    double* rho_u) {
+  if (!ctx || !m_nodes || !n_out || !j_out) return fail(ctx, WDG_ERR_ARG, "null argument");
+  return guarded(ctx, [&]() {
+    const Geometry& g = ctx->g;
+    const long N = ctx->N;
+    const double2* src;
+    double scale;
+    if (rho) {
+      ctx->d_u64.alloc(N);
+      HIPCHK(hipMemcpyAsync(ctx->d_u64.p, rho, N * 16, hipMemcpyHostToDevice, ctx->stream));
+      src = ctx->d_u64.p;
+      scale = 1.0;
+    } else {
+      if (!ctx->have_solution) throw std::logic_error("no solution available: call wdg_solve first");
+      src = ctx->d_x.p;
+      scale = ctx->sol_scale;
+    }
+    const int nodes = ctx->ncols;
+    hipLaunchKernelGGL(observables_kernel, dim3(nodes), dim3(256), 0, ctx->stream, src, ctx->d_phic.p, g.nk,
+                       ctx->d_obs.p);
+    std::vector<double2> obs(2 * nodes);
+    HIPCHK(hipMemcpyAsync(obs.data(), ctx->d_obs.p, obs.size() * sizeof(double2), hipMemcpyDeviceToHost,
+                          ctx->stream));
+    if (rho_u) {
+      op_gemm64(ctx, P_GEMM1, ctx->d_A1.p, (const double*)src, (double*)ctx->d_T64.p, 2 * g.n_xi, 2 * g.nk, false,
+                nullptr);
+      HIPCHK(hipMemcpyAsync(rho_u, ctx->d_T64.p, (size_t)nodes * g.n_xi * 16, hipMemcpyDeviceToHost, ctx->stream));
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    prof_collect(ctx);
+    // n = (Re rho_u(c-1) + Re rho_u(c))/2, j = q*hbar/m * Im(rho_u(c) - rho_u(c-1)) / delta_xi
+    const double hb = kHVT / 2.0 / M_PI;
+    for (int i = 0; i < nodes; ++i) {
+      const double2 a = obs[2 * i], b = obs[2 * i + 1];
+      n_out[i] = scale * 0.5 * (a.x + b.x);
+      j_out[i] = scale * kQIP * hb / m_nodes[i] * (b.y - a.y) / g.delta_xi;
+    }
+    if (rho_u)
+      for (long i = 0; i < (long)nodes * g.n_xi * 2; ++i) rho_u[i] *= scale;
+    return WDG_OK;
+  });
+}
+
+int wdg_host_setup(const wdg_desc* desc, const wdg_potential* pot, double* rhs, double* kmats) {
+  if (!desc) return fail(nullptr, WDG_ERR_ARG, "null argument");
+  return guarded(nullptr, [&]() {
+    Geometry g;
+    build_geometry(*desc, g);
+    const int n = g.n;
+    if (kmats) {
+      const std::vector<double>* mats[5] = {&g.K1, &g.K2L, &g.K2R, &g.K3, &g.K5};
+      for (int i = 0; i < 5; ++i) std::memcpy(kmats + (size_t)i * n * n, mats[i]->data(), n * n * sizeof(double));
+    }
+    if (rhs) {
+      if (!pot) throw std::invalid_argument("rhs requires a potential");
+      if (pot->n_mat != g.n_chi * (g.n - 1) + 1) throw std::invalid_argument("n_mat must equal mat.Nx");
+      std::vector<cplx> fnl, fnr;
+      inflow(g, *pot, fnl, fnr);
+      std::vector<double> r;
+      build_rhs(g, fnl, fnr, r);
+      std::memcpy(rhs, r.data(), r.size() * sizeof(double));
+    }
+    return WDG_OK;
+  });
+}
+
+int wdg_set_profiling(wdg_ctx* ctx, uint32_t mask) {
+  if (!ctx) return fail(nullptr, WDG_ERR_ARG, "null argument");
+  ctx->prof_mask = mask;
+  for (int i = 0; i < P_COUNT; ++i) {
+    ctx->prof_ms[i] = 0;
+    ctx->prof_n[i] = 0;
+  }
+  return WDG_OK;
+}
+
+int wdg_profile(wdg_ctx* ctx, char* names, int32_t names_len, double* avg_ms, int64_t* launches, int32_t max_entries,
+                int32_t* n_entries) {
+  if (!ctx) return fail(nullptr, WDG_ERR_ARG, "null argument");
+  std::string s;
+  int k = 0;
+  for (int i = 0; i < P_COUNT && k < max_entries; ++i, ++k) {
+    if (i) s += ",";
+    s += kProfNames[i];
+    if (avg_ms) avg_ms[k] = ctx->prof_n[i] ? ctx->prof_ms[i] / ctx->prof_n[i] : 0.0;
+    if (launches) launches[k] = ctx->prof_n[i];
+  }
+  if (n_entries) *n_entries = k;
+  if (names && names_len > 0) {
+    std::strncpy(names, s.c_str(), names_len - 1);
+    names[names_len - 1] = 0;
+  }
+  return WDG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,66 @@
+"""Device descriptions (the reference's ``mat`` struct) for resonant-tunnelling
+diodes on the 1D-x x 1D-k path.
+
+The reference builds ``mat`` in RTD/buildDevice2D_rtd.m (uniform material
+grid ``mat.x = (0:Nx-1)*dx`` in nm, :640-648) and hands the subband energy
+``Esub(valley, x, mode)`` and transverse wavefunctions ``Vsub`` to
+``solve_transport_DG_2D(mat, Esub, Vsub, EfL, EfR)``
+(DG_code/solve_transport_DG_2D.m:1).  This module produces exactly those
+inputs for the double-barrier RTD workloads: a 60 nm device with two 3 nm
+barriers around a 5 nm well, effective mass 0.067 m0, T = 300 K.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+
+def rtd_potential(x_nm, length_nm=60.0, barrier_ev=0.3, barrier_nm=3.0, well_nm=5.0, bias_v=0.0):
+    """Conduction-band edge [eV] of a centred double barrier with a linear bias drop."""
+    x = np.asarray(x_nm, float)
+    c = length_nm / 2
+    lo1, hi1 = c - well_nm / 2 - barrier_nm, c - well_nm / 2
+    lo2, hi2 = c + well_nm / 2, c + well_nm / 2 + barrier_nm
+    v = np.where(((x >= lo1) & (x <= hi1)) | ((x >= lo2) & (x <= hi2)), barrier_ev, 0.0)
+    return v - bias_v * x / length_nm
+
+
+def rtd_1d(n_elem: int, degree: int, nk: int, bias_v: float = 0.0, barrier_ev: float = 0.3,
+           length_nm: float = 60.0, fermi_ev: float = 0.1, mass: float = 0.067, temp: float = 300.0,
+           fermi_model: str = "DG_MSA_3D", l_xi: float | None = None):
+    """Return (mat, Esub, Vsub, EfL, EfR) for a 1D RTD.
+
+    Discretisation: ``n_elem`` DG elements of degree ``degree`` in x (the
+    material grid has n_elem*degree+1 points, so N_chi = n_elem), ``nk``
+    FV cells in xi / plane waves in k.  Unless given, L_xi = (nk-1)*dx so
+    that k spans [-pi/dx, pi/dx] with dx the material spacing.
+    """
+    nloc = degree + 1
+    nx = n_elem * degree + 1
+    dx = length_nm / (n_elem * degree)
+    x = np.arange(nx) * dx
+    if l_xi is None:
+        l_xi = (nk - 1) * dx * 1e-9
+    mat = {
+        "type": "RTD",
+        "fermi_model": fermi_model,
+        "Nx": nx,
+        "L_x": length_nm,
+        "x": x,
+        "dx": dx,
+        "Ny": 1,
+        "dy": 1.0,
+        "Temp": temp,
+        "n_of_valleys": 1,
+        "n_of_modes": 1,
+        "deg_factor": np.array([1.0]),
+        "me_x": np.full((1, nx, 1), mass),
+        "me_z_ch": mass,
+        "dg": {"params": {
+            "N_K_chi": nloc, "N_xi": nk, "N": nk, "Ly": l_xi,
+            "Xi_solver": "FV", "Q": "EXP", "permute_shell": True,
+            "MPC": 1.0, "st_init": False, "sl": 0,
+        }},
+    }
+    esub = rtd_potential(x, length_nm, barrier_ev, bias_v=bias_v)[None, :, None]
+    vsub = np.ones((1, nx, 1, 1))
+    return mat, esub, vsub, fermi_ev, fermi_ev - bias_v

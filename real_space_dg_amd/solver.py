@@ -1,0 +1,259 @@
+"""Host-side mirror of the reference's 1D-x x 1D-k Wigner transport API.
+
+``solve_transport_DG_2D(mat, Esub, Vsub, EfL, EfR)`` keeps the signature and
+output fields of DG_code/solve_transport_DG_2D.m:1-225 (``n``, ``j``, ``RHO``,
+``p``, ``chi``, ``xi``).  The operator itself lives on the GPU behind the C ABI
+of libwdg.so; :class:`WignerOperator` is the ``sysInfo``/preconditioner handle
+surface of the reference (``apply``, ``assembleRhs``, ``getDiagonal``,
+``pre.apply``, DG_code/core/get_SysM_full2D.m:80-86,
+DG_code/core/get_LineTransversePreconditioner_full2D.m:113) for that operator.
+
+The reference solves each (valley, mode) system with a sparse direct solve
+(``rho = A\\rhs``, DG_code/solve_transport_DG_2D.m:101).  Here the solve is
+right-preconditioned restarted GMRES on the device with a true-residual
+check (the Krylov path of DG_code/solve_transport_DG_full2D.m:276-334), in
+FP64 or in mixed precision with FP64 refinement.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, dptr
+
+M0 = 9.109e-31          # DG_code/solve_transport_DG_2D.m:7
+FERMI_MODELS = {"DG_MSA_3D": _lib.WDG_FERMI_MSA_3D, "DG_MSA_2D": _lib.WDG_FERMI_MSA_2D}
+PRECISIONS = {"fp64": _lib.WDG_PREC_FP64, "mixed": _lib.WDG_PREC_MIXED}
+
+
+@dataclass
+class SolveOptions:
+    """GMRES controls (names follow the reference's full2D_* parameters)."""
+    tol: float = 1e-10              # full2D_solverTol
+    restart: int = 30               # full2D_gmresRestart
+    max_iter: int = 20000           # cap on inner iterations
+    precision: str = "fp64"         # 'fp64' or 'mixed'
+    inner_tol: float = 1e-4         # mixed: per-cycle relative reduction target
+    max_outer: int = 0              # mixed: cap on refinement cycles (0 = none)
+    precondition: bool = True       # right line preconditioner
+
+    def to_c(self) -> _lib.wdg_solve_opts:
+        if self.precision not in PRECISIONS:
+            raise ValueError(f"precision must be one of {sorted(PRECISIONS)}")
+        return _lib.wdg_solve_opts(float(self.tol), int(self.restart), int(self.max_iter),
+                                   PRECISIONS[self.precision], float(self.inner_tol),
+                                   int(self.max_outer), 1 if self.precondition else 0)
+
+
+def _as_c128(a, n):
+    a = np.ascontiguousarray(np.asarray(a, dtype=np.complex128).reshape(-1))
+    if a.size != n:
+        raise ValueError(f"expected {n} entries, got {a.size}")
+    return a
+
+
+class WignerOperator:
+    """One stationary 1D-x x 1D-k instance on the GPU (grid + potential).
+
+    Geometry follows initParams (DG_code/initParams.m:58-191): ``n_chi``
+    elements with ``n_k_chi`` nodes, ``nk`` plane waves on an ``n_xi``-cell
+    FV grid of length ``l_xi`` [m], device length ``l_chi`` [m].
+    """
+
+    def __init__(self, n_chi: int, n_k_chi: int, nk: int, l_chi: float, l_xi: float, n_xi: int | None = None):
+        lib = _lib.load()
+        self._lib = lib
+        self.desc = _lib.wdg_desc(int(n_chi), int(n_k_chi), int(n_xi or nk), int(nk), float(l_chi), float(l_xi))
+        h = C.c_void_p()
+        check(lib.wdg_create(C.byref(self.desc), C.byref(h)), None)
+        self._h = h
+        self.n_chi, self.n_k_chi, self.nk = int(n_chi), int(n_k_chi), int(nk)
+        self.n_xi = int(n_xi or nk)
+        self.N = int(lib.wdg_size(h))
+        self._pot_keep = None
+
+    @classmethod
+    def from_mat(cls, mat: dict) -> "WignerOperator":
+        dgp = mat["dg"]["params"]
+        if str(dgp.get("Xi_solver", "FV")).upper() != "FV":
+            raise _lib.WdgError(-6, "only the FV xi solver is supported")
+        if not bool(dgp.get("permute_shell", True)):
+            raise _lib.WdgError(-6, "only permute_shell = true ordering is supported")
+        n = int(dgp["N_K_chi"])
+        nx = int(mat["Nx"])
+        if (nx - 1) % (n - 1):
+            raise _lib.WdgError(-1, "(mat.Nx-1) must be a multiple of N_K_chi-1")
+        return cls((nx - 1) // (n - 1), n, int(dgp["N"]), float(mat["L_x"]) * 1e-9, float(dgp["Ly"]),
+                   n_xi=int(dgp["N_xi"]))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.wdg_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ---- potential / system (get_SysM) ----
+    def set_potential(self, x_mat_m, em, ef_left: float, ef_right: float, m_left: float, m_right: float,
+                      temp: float = 300.0, deg_factor: float = 1.0, fermi_model: str = "DG_MSA_3D",
+                      me_z: float = 0.0):
+        x = np.ascontiguousarray(np.asarray(x_mat_m, float))
+        v = np.ascontiguousarray(np.asarray(em, float))
+        if x.shape != v.shape:
+            raise ValueError("x_mat and em must have the same length")
+        if fermi_model not in FERMI_MODELS:
+            raise ValueError(f"fermi_model must be one of {sorted(FERMI_MODELS)}")
+        self._pot_keep = (x, v)
+        pot = _lib.wdg_potential(x.size, FERMI_MODELS[fermi_model], dptr(x), dptr(v), float(m_left),
+                                 float(m_right), float(ef_left), float(ef_right), float(temp),
+                                 float(deg_factor), float(me_z))
+        check(self._lib.wdg_set_potential(self._h, C.byref(pot)), self._h)
+
+    def rhs(self) -> np.ndarray:
+        out = np.zeros(self.N, np.complex128)
+        check(self._lib.wdg_rhs(self._h, dptr(out)), self._h)
+        return out
+
+    assembleRhs = rhs
+
+    def apply(self, u, precision: str = "fp64") -> np.ndarray:
+        u = _as_c128(u, self.N)
+        y = np.empty(self.N, np.complex128)
+        check(self._lib.wdg_apply(self._h, dptr(u), dptr(y), PRECISIONS[precision]), self._h)
+        return y
+
+    def precond(self, r) -> np.ndarray:
+        r = _as_c128(r, self.N)
+        z = np.empty(self.N, np.complex128)
+        check(self._lib.wdg_precond(self._h, dptr(r), dptr(z)), self._h)
+        return z
+
+    def diagonal(self) -> np.ndarray:
+        d = np.empty(self.N, np.complex128)
+        check(self._lib.wdg_diagonal(self._h, dptr(d)), self._h)
+        return d
+
+    getDiagonal = diagonal
+
+    def solve(self, opts: SolveOptions | None = None, x0=None, return_x: bool = True):
+        opts = opts or SolveOptions()
+        co = opts.to_c()
+        x0a = None if x0 is None else _as_c128(x0, self.N)
+        x = np.empty(self.N, np.complex128) if return_x else None
+        info = _lib.wdg_solve_info()
+        check(self._lib.wdg_solve(self._h, C.byref(co), dptr(x0a), dptr(x), C.byref(info)), self._h)
+        d = {f: getattr(info, f) for f, _ in info._fields_}
+        d["status"] = "solved-gmres" if info.flag == 0 else "iterative-gmres-not-converged"
+        d["precision"] = opts.precision
+        return x, d
+
+    def observables(self, m_nodes, rho=None, want_rho_u: bool = False):
+        """n, j per node (and rho_u, n_xi x nodes) of rho or of the last solution."""
+        nodes = self.n_chi * self.n_k_chi
+        m = np.ascontiguousarray(np.broadcast_to(np.asarray(m_nodes, float), (nodes,)))
+        n = np.empty(nodes)
+        j = np.empty(nodes)
+        r = None if rho is None else _as_c128(rho, self.N)
+        ru = np.empty((nodes, self.n_xi), np.complex128) if want_rho_u else None
+        check(self._lib.wdg_observables(self._h, dptr(r), dptr(m), dptr(n), dptr(j), dptr(ru)), self._h)
+        return n, j, (None if ru is None else ru.T)
+
+    # ---- profiling ----
+    def set_profiling(self, mask: int):
+        check(self._lib.wdg_set_profiling(self._h, C.c_uint32(mask)), self._h)
+
+    def profile(self) -> dict:
+        names = C.create_string_buffer(512)
+        avg = (C.c_double * 32)()
+        cnt = (C.c_int64 * 32)()
+        ne = C.c_int32()
+        check(self._lib.wdg_profile(self._h, names, 512, avg, cnt, 32, C.byref(ne)), self._h)
+        keys = names.value.decode().split(",")
+        return {k: {"avg_ms": avg[i], "launches": cnt[i]} for i, k in enumerate(keys[:ne.value])}
+
+
+PROFILE_BITS = {"transport": 0, "theta_gemm1": 1, "theta_combine": 2, "theta_gemm2": 3,
+                "line_aggregate": 4, "line_apply": 5, "arnoldi_dot": 6, "arnoldi_update": 7}
+
+
+# ---------------------------------------------------------------------------
+# solve_transport_DG_2D mirror
+# ---------------------------------------------------------------------------
+
+def _node_values(n_chi: int, n: int, f):
+    """Material grid -> DG nodes, DG_code/solve_transport_DG_2D.m:81-87."""
+    f = np.asarray(f)
+    out = np.zeros((n_chi * n,) + f.shape[1:], dtype=f.dtype)
+    for e in range(n_chi):
+        out[e * n:(e + 1) * n] = f[e * (n - 1):e * (n - 1) + n]
+    return out
+
+
+def solve_transport_DG_2D(mat: dict, Esub, Vsub, EfL: float, EfR: float, solver: SolveOptions | None = None,
+                          rho0=None):
+    """Stationary solve per valley/mode (DG_code/solve_transport_DG_2D.m:1-225).
+
+    Returns a dict with the reference's fields ``n`` (nodes x Ny), ``j``
+    (nodes), ``RHO`` (valleys x modes x nodes x N_xi), ``p``, ``chi``, ``xi``
+    and, per (valley, mode), the GMRES ``info`` and the solution ``rho``.
+    ``rho0`` optionally maps (valley, mode) -> warm start vector.
+    """
+    solver = solver or SolveOptions()
+    Esub = np.asarray(Esub, float)
+    Vsub = np.asarray(Vsub)
+    nv, nm = int(mat["n_of_valleys"]), int(mat["n_of_modes"])
+    deg = np.atleast_1d(np.asarray(mat["deg_factor"], float))
+    me_x = np.asarray(mat["me_x"], float)
+    Ny = int(mat["Ny"])
+    mid = int(math.ceil(Ny / 2)) - 1
+    dgp = mat["dg"]["params"]
+    x_mat = np.asarray(mat["x"], float) * 1e-9
+    fermi_model = mat.get("fermi_model", "DG_MSA_3D")
+    op = WignerOperator.from_mat(mat)
+    n_loc, n_chi, n_xi = op.n_k_chi, op.n_chi, op.n_xi
+    nodes = n_chi * n_loc
+    n_tot = np.zeros((nodes, Ny))
+    j_tot = np.zeros(nodes)
+    RHO = np.zeros((nv, nm, nodes, n_xi), np.complex128)
+    infos, rhos = {}, {}
+    try:
+        for iv in range(nv):
+            for im in range(nm):
+                m = me_x[iv, :, mid] * M0
+                op.set_potential(x_mat, Esub[iv, :, im], EfL, EfR, m[0], m[-1], temp=float(mat["Temp"]),
+                                 deg_factor=float(deg[iv]), fermi_model=fermi_model,
+                                 me_z=float(mat.get("me_z_ch", 0.0)))
+                x0 = None if rho0 is None else rho0.get((iv, im))
+                rho, info = op.solve(solver, x0=x0)
+                m_nodes = _node_values(n_chi, n_loc, m)
+                n_dg, j_dg, rho_u = op.observables(m_nodes, want_rho_u=True)
+                vs = _node_values(n_chi, n_loc, Vsub[iv, :, im, :])
+                n_tot += n_dg[:, None] * np.abs(vs) ** 2 / float(mat["dy"]) / 1e-9   # :203-205
+                j_tot += j_dg
+                RHO[iv, im] = rho_u.T
+                infos[(iv, im)] = info
+                rhos[(iv, im)] = rho
+    finally:
+        op.close()
+    L = float(dgp["Ly"])
+    dxi = L / (n_xi - 1)
+    chi_ends = np.linspace(0.0, float(mat["L_x"]) * 1e-9, n_chi + 1)
+    chi = np.concatenate([np.linspace(chi_ends[e], chi_ends[e + 1], n_loc) for e in range(n_chi)])
+    p = {"N_chi": n_chi, "N_K_chi": n_loc, "Nk": op.nk, "N_xi": n_xi, "L_xi": L, "delta_xi": dxi,
+         "delta_chi": float(mat["L_x"]) * 1e-9 / n_chi, "chi_Koord": chi}
+    return {"n": n_tot, "j": j_tot, "RHO": RHO, "p": p, "chi": chi,
+            "xi": np.linspace(-(L - dxi) / 2, (L - dxi) / 2, n_xi), "info": infos, "rho": rhos}

@@ -1,0 +1,82 @@
+"""End-to-end GPU solves against the oracle's direct sparse solve."""
+import numpy as np
+import pytest
+import scipy.sparse.linalg as spla
+
+from oracle import wigner1d as W
+from real_space_dg_amd.solver import SolveOptions, solve_transport_DG_2D
+from tests._cases import build_case, gpu_operator, oracle_system
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", params=[(64, 2, 128, 0.0), (32, 3, 128, 0.2)],
+                ids=lambda c: "ne%d_p%d_nk%d_b%.1f" % c)
+def solved(request):
+    ne, deg, nk, bias = request.param
+    print(f"\n[case] ne={ne} degree={deg} nk={nk} bias={bias}", flush=True)
+    c = build_case(ne, deg, nk, bias)
+    A, G, rhs = oracle_system(c)
+    c["K"] = (A + G).tocsc()
+    c["rhs"] = rhs
+    c["x_ref"] = spla.spsolve(c["K"], rhs)
+    c["op"] = gpu_operator(c)
+    yield c
+    c["op"].close()
+
+
+def test_fp64_matches_direct(solved):
+    x, info = solved["op"].solve(SolveOptions(tol=1e-13, restart=30, precision="fp64"))
+    print(info)
+    assert info["flag"] == 0 and info["true_relres"] <= 1e-13
+    err = np.linalg.norm(x - solved["x_ref"]) / np.linalg.norm(solved["x_ref"])
+    print("fp64 solution rel err vs direct", err)
+    assert err < 1e-10
+    # observables
+    m_nodes = np.full(solved["p"]["N_chi"] * solved["p"]["N_K_chi"], 0.067 * 9.109e-31)
+    n, j, _ = solved["op"].observables(m_nodes)
+    n_ref, j_ref, _ = W.observables(solved["p"], solved["x_ref"], m_nodes)
+    en = np.max(np.abs(n - n_ref)) / np.max(np.abs(n_ref))
+    ej = np.max(np.abs(j - j_ref)) / max(np.max(np.abs(j_ref)), 1e-300)
+    print("density err", en, "current err", ej)
+    assert en < 1e-10
+    assert ej < 1e-8
+
+
+def test_mixed_reaches_tolerance(solved):
+    x, info = solved["op"].solve(SolveOptions(tol=1e-10, restart=30, precision="mixed"))
+    print(info)
+    assert info["flag"] == 0 and info["true_relres"] <= 1e-10
+    # refined to 1e-12, the mixed path lands on the FP64 oracle solution
+    x, info = solved["op"].solve(SolveOptions(tol=1e-12, restart=30, precision="mixed"))
+    assert info["flag"] == 0 and info["true_relres"] <= 1e-12
+    err = np.linalg.norm(x - solved["x_ref"]) / np.linalg.norm(solved["x_ref"])
+    print("mixed solution rel err vs direct", err)
+    m_nodes = np.full(solved["p"]["N_chi"] * solved["p"]["N_K_chi"], 0.067 * 9.109e-31)
+    n, j, _ = solved["op"].observables(m_nodes)
+    n_ref, j_ref, _ = W.observables(solved["p"], solved["x_ref"], m_nodes)
+    en = np.max(np.abs(n - n_ref)) / np.max(np.abs(n_ref))
+    ej = np.max(np.abs(j - j_ref)) / max(np.max(np.abs(j_ref)), 1e-300)
+    print("mixed density err", en, "current err", ej)
+    assert en < 1e-9
+    assert ej < 1e-7
+
+
+def test_warm_start_and_flag(solved):
+    op = solved["op"]
+    x, info = op.solve(SolveOptions(tol=1e-10))
+    x2, info2 = op.solve(SolveOptions(tol=1e-10), x0=x)
+    assert info2["iterations"] == 0 and info2["flag"] == 0
+    _, info3 = op.solve(SolveOptions(tol=1e-10, max_iter=3))
+    assert info3["flag"] == 1 and info3["status"] == "iterative-gmres-not-converged"
+
+
+def test_solve_transport_dg_2d_mirror(solved):
+    c = solved
+    DG = solve_transport_DG_2D(c["mat"], c["E"], c["Vs"], c["efl"], c["efr"], SolveOptions(tol=1e-13))
+    ref = W.solve_transport_dg_2d(c["mat"], c["E"], c["Vs"], c["efl"], c["efr"])
+    en = np.max(np.abs(DG["n"] - ref["n"])) / np.max(np.abs(ref["n"]))
+    er = np.max(np.abs(DG["RHO"] - ref["RHO"])) / np.max(np.abs(ref["RHO"]))
+    print("mirror n err", en, "RHO err", er)
+    assert en < 1e-10 and er < 1e-10
+    assert DG["RHO"].shape == ref["RHO"].shape
