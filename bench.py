@@ -1,0 +1,214 @@
+#!/usr/bin/env python3
+"""Bench: stationary Wigner solves/s to true relative residual 1e-10 on one MI355X.
+
+Workload (RTD with bias): 60 nm double-barrier RTD (0.3 eV x 3 nm barriers
+around a 5 nm well), 0.2 V linear drop, 512 DG elements of degree 3 in x,
+1024 plane waves / FV cells in k (N = 2,097,152 complex unknowns), GMRES(30)
+with the right line preconditioner, tolerance 1e-10.
+
+A step = one stationary solve of that instance from the zero start: build the
+potential tables, inflow RHS and preconditioner data on the device
+(wdg_set_potential), GMRES to true relres <= 1e-10 (wdg_solve) and the
+observables n, j (wdg_observables).  All solver state stays in HBM; only
+n and j return to the host.  Timed: the mixed-precision solver (FP32/split-BF16
+inner cycles, FP64 residual refinement) and, for comparison, the plain FP64
+solver.  Kernel durations are HIP events on the context stream recorded
+inside the timed region.  Every library call synchronises its stream before
+returning, so the wall clock brackets finished GPU work.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "GMRES solves/s to 1e-10 (mixed vs FP64, 1 GPU); Wigner apply % HBM roofline"
+HBM_PEAK_GBS = 8000.0
+MFMA_PEAK_TFS = {"bf16": 2500.0, "fp64": 78.6, "fp32": 157.3}
+M0 = 9.109e-31
+
+WORKLOAD = dict(n_elem=512, degree=3, nk=1024, bias_v=0.2)
+MFMA_KERNELS = {"theta_gemm1", "theta_gemm2"}
+
+
+def parse():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--fp64-steps", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-sample-iters", type=int, default=6)
+    return ap.parse_args()
+
+
+def kernel_roofline(prof: dict, name: str, peak_kind: str) -> dict:
+    st = prof[name]
+    t_ms = st["avg_ms"]
+    per_launch = st["units"] / max(st["launches"], 1)
+    if name in MFMA_KERNELS:
+        ach = per_launch / (t_ms * 1e-3) / 1e12
+        peak = MFMA_PEAK_TFS[peak_kind]
+        out = {"bound": "mfma", "achieved": ach, "peak": peak, "unit": "TFLOP/s", "frac": ach / peak}
+    else:
+        ach = per_launch / (t_ms * 1e-3) / 1e9
+        out = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS}
+    out.update({"kernel": name, "avg_launch_ms": t_ms, "launches": st["launches"],
+                "algorithmic_per_launch": per_launch, "traffic": traffic_for(name)})
+    return out
+
+
+def traffic_for(name: str):
+    """HBM bytes per launch measured with rocprofv3 --pmc (profiles/traffic.json), if recorded."""
+    f = ROOT / "profiles" / "traffic.json"
+    if not f.exists():
+        return None
+    try:
+        d = json.loads(f.read_text())
+        v = d.get("kernels", {}).get(name)
+        return None if v is None else v.get("bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
+def cpu_baseline(mat, E, efl, efr, its_per_solve: int, sample_iters: int) -> dict:
+    """The oracle's FP64 matrix-free operator + line preconditioner + GMRES on
+    the host: setup of one instance plus a bounded number of GMRES iterations,
+    scaled to the iteration count of a full solve."""
+    from oracle import krylov as K
+    from oracle import wigner1d as W
+    try:
+        from threadpoolctl import threadpool_info
+        threads = max([i.get("num_threads", 1) for i in threadpool_info()] or [1])
+    except Exception:
+        threads = os.cpu_count() or 1
+    m = mat["me_x"][0, :, 0] * M0
+    t0 = time.perf_counter()
+    p = W.init_params(mat, efl, efr, E[0, :, 0], m, 1.0)
+    op = W.MatrixFreeOperator.build(mat, p, E[0, :, 0])
+    Bbar = op.edge_coefficients()
+    pre = K.LinePreconditioner(op, Bbar)
+    _, rhs = W.get_diff(p)
+    t_setup = time.perf_counter() - t0
+    t1 = time.perf_counter()
+    _, info = K.gmres(lambda u: op.apply(u, Bbar), pre.apply, rhs, restart=30, tol=1e-10,
+                      maxit=sample_iters)
+    t_iter = (time.perf_counter() - t1) / max(info["iterations"], 1)
+    t_solve = t_setup + t_iter * its_per_solve
+    return {"value": 1.0 / t_solve, "unit": "solves/s", "cores": int(threads), "kind": "port",
+            "sample": (f"oracle FP64 restatement (numpy, {threads} BLAS threads): instance setup "
+                       f"{t_setup:.2f} s + {info['iterations']} GMRES(30) iterations at "
+                       f"{t_iter:.3f} s each on the same workload, scaled to {its_per_solve} "
+                       f"iterations per solve (the FP64 GPU solver's count)"),
+            "seconds_per_iteration": t_iter, "setup_seconds": t_setup}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (--gpus {args.gpus} requested); "
+              "multi-GPU sharding is design-only.", file=sys.stderr)
+        sys.exit(2)
+    from real_space_dg_amd.devices import rtd_1d
+    from real_space_dg_amd.solver import PROFILE_BITS, SolveOptions, WignerOperator
+
+    mat, E, Vs, efl, efr = rtd_1d(WORKLOAD["n_elem"], WORKLOAD["degree"], WORKLOAD["nk"],
+                                  bias_v=WORKLOAD["bias_v"])
+    m = mat["me_x"][0, :, 0] * M0
+    m_nodes = np.concatenate([m[e * WORKLOAD["degree"]: e * WORKLOAD["degree"] + WORKLOAD["degree"] + 1]
+                              for e in range(WORKLOAD["n_elem"])])
+    op = WignerOperator.from_mat(mat)
+
+    def step(prec: str, tol: float = 1e-10):
+        op.set_potential(mat["x"] * 1e-9, E[0, :, 0], efl, efr, m[0], m[-1], temp=mat["Temp"])
+        _, info = op.solve(SolveOptions(tol=tol, restart=30, precision=prec, max_iter=20000), return_x=False)
+        n, j, _ = op.observables(m_nodes)
+        return info, n, j
+
+    print(f"[bench] N={op.N} reference FP64 solve (tol 1e-12)", file=sys.stderr, flush=True)
+    ref_info, n_ref, j_ref = step("fp64", 1e-12)
+    for _ in range(args.warmup):
+        step("mixed")
+        step("fp64")
+    all_mask = sum(1 << b for b in PROFILE_BITS.values())
+
+    op.set_profiling(all_mask)
+    t0 = time.perf_counter()
+    mixed = [step("mixed") for _ in range(args.steps)]
+    t_mixed = time.perf_counter() - t0
+    prof_mixed = op.profile()
+    print(f"[bench] mixed: {args.steps} solves in {t_mixed:.3f} s", file=sys.stderr, flush=True)
+
+    op.set_profiling(all_mask)
+    t0 = time.perf_counter()
+    fp64 = [step("fp64") for _ in range(args.fp64_steps)]
+    t_fp64 = time.perf_counter() - t0
+    prof_fp64 = op.profile()
+    op.set_profiling(0)
+    print(f"[bench] fp64: {args.fp64_steps} solves in {t_fp64:.3f} s", file=sys.stderr, flush=True)
+
+    v_mixed = args.steps / t_mixed
+    v_fp64 = args.fp64_steps / t_fp64
+    info_m, n_m, j_m = mixed[-1]
+    info_d, n_d, j_d = fp64[-1]
+    scale_n = np.max(np.abs(n_ref))
+    scale_j = np.max(np.abs(j_ref))
+    err = {
+        "mixed_density_rel_err_vs_fp64_1e-12": float(np.max(np.abs(n_m - n_ref)) / scale_n),
+        "mixed_current_rel_err_vs_fp64_1e-12": float(np.max(np.abs(j_m - j_ref)) / scale_j),
+        "fp64_density_rel_err_vs_fp64_1e-12": float(np.max(np.abs(n_d - n_ref)) / scale_n),
+        "fp64_current_rel_err_vs_fp64_1e-12": float(np.max(np.abs(j_d - j_ref)) / scale_j),
+    }
+    tot = {k: v["avg_ms"] * v["launches"] for k, v in prof_mixed.items() if v["launches"]}
+    dominant = max(tot, key=tot.get)
+    roof = kernel_roofline(prof_mixed, dominant, "bf16")
+    kernels = {k: {"avg_ms": v["avg_ms"], "launches": v["launches"],
+                   "share": tot.get(k, 0.0) / max(sum(tot.values()), 1e-30)} for k, v in prof_mixed.items()}
+    line = {
+        "metric": METRIC,
+        "value": v_mixed,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * t_mixed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "mixed: split-bf16 MFMA + fp32 inner, fp64 residual/refinement",
+        "data": "synthetic RTD potential (deterministic), no learned weights",
+        "config": {"workload": "RTD 60 nm, 0.2 V bias, 512 DG elements p=3 x 1024 k, GMRES(30)+line "
+                               "preconditioner to true relres 1e-10",
+                   "unknowns": op.N, "restart": 30, "tol": 1e-10},
+        "fp64_solves_per_s": v_fp64,
+        "fp64_ms_per_step": 1e3 * t_fp64 / args.fp64_steps,
+        "mixed_over_fp64": v_mixed / v_fp64,
+        "iterations": {"mixed": info_m["iterations"], "mixed_refinements": info_m["outer_iterations"],
+                       "fp64": info_d["iterations"], "fp64_ref_1e-12": ref_info["iterations"]},
+        "true_relres": {"mixed": info_m["true_relres"], "fp64": info_d["true_relres"]},
+        "errors": err,
+        "roofline": roof,
+        "wigner_apply_roofline": kernel_roofline(prof_mixed, "transport", "fp32"),
+        "theta_gemm_roofline_mixed": kernel_roofline(prof_mixed, "theta_gemm1", "bf16"),
+        "theta_gemm_roofline_fp64": kernel_roofline(prof_fp64, "theta_gemm1", "fp64"),
+        "kernels_mixed": kernels,
+    }
+    if not args.no_cpu_baseline:
+        print("[bench] CPU baseline sample", file=sys.stderr, flush=True)
+        line["cpu_baseline"] = cpu_baseline(mat, E, efl, efr, info_d["iterations"], args.cpu_sample_iters)
+    else:
+        line["cpu_baseline"] = None
+    op.close()
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -165,10 +165,12 @@ int wdg_host_setup(const wdg_desc* desc, const wdg_potential* pot, double* rhs, 
  * it off.  Resets the accumulated statistics. */
 int wdg_set_profiling(wdg_ctx* ctx, uint32_t mask);
 
-/* Kernel timing of the last wdg_solve: average device time [ms] of each
- * hot kernel per launch, and launches counted.  names: comma-separated. */
+/* Per-kernel statistics collected while profiling is enabled: average
+ * device time [ms] per launch, launches, and the total algorithmic work
+ * (bytes for memory-bound kernels, flops for the Theta GEMMs).  names gets
+ * the comma-separated kernel list; each array holds max_entries values. */
 int wdg_profile(wdg_ctx* ctx, char* names, int32_t names_len, double* avg_ms, int64_t* launches,
-                int32_t max_entries, int32_t* n_entries);
+                double* units, int32_t max_entries, int32_t* n_entries);
 
 #ifdef __cplusplus
 }

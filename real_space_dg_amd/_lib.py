@@ -82,7 +82,7 @@ SIGNATURES = {
     "wdg_precond_device": (C.c_int, [_P, _P, _P, C.c_int32]),
     "wdg_host_setup": (C.c_int, [C.POINTER(wdg_desc), C.POINTER(wdg_potential), _D, _D]),
     "wdg_set_profiling": (C.c_int, [_P, C.c_uint32]),
-    "wdg_profile": (C.c_int, [_P, C.c_char_p, C.c_int32, _D, C.POINTER(C.c_int64), C.c_int32,
+    "wdg_profile": (C.c_int, [_P, C.c_char_p, C.c_int32, _D, C.POINTER(C.c_int64), _D, C.c_int32,
                               C.POINTER(C.c_int32)]),
 }
 

@@ -106,9 +106,11 @@ struct wdg_ctx {
   unsigned prof_mask = 0;
   std::vector<hipEvent_t> ev_pool;
   size_t ev_used = 0;
-  std::vector<std::pair<int, size_t>> ev_pending;  // (kernel id, start event index)
+  struct Pending { int id; size_t ev; double units; };
+  std::vector<Pending> ev_pending;
   double prof_ms[P_COUNT] = {0};
   long prof_n[P_COUNT] = {0};
+  double prof_units[P_COUNT] = {0};   // algorithmic bytes (HBM kernels) or flops (GEMMs)
   GmresBufs gb() {
     GmresBufs b;
     b.H = d_H.p; b.cs = d_cs.p; b.sn = d_sn.p; b.g = d_g.p; b.h1 = d_h1.p; b.h2 = d_h2.p;
@@ -146,8 +148,9 @@ inline unsigned grid1(long work, int nt) { return (unsigned)std::max<long>(1, (w
 struct ProfScope {
   wdg_ctx* c;
   int id;
+  double units;
   size_t i0 = (size_t)-1;
-  ProfScope(wdg_ctx* c_, int id_) : c(c_), id(id_) {
+  ProfScope(wdg_ctx* c_, int id_, double units_) : c(c_), id(id_), units(units_) {
     if (!(c->prof_mask & (1u << id))) return;
     if (c->ev_used + 2 > c->ev_pool.size()) {
       for (int i = 0; i < 64; ++i) {
@@ -163,7 +166,7 @@ struct ProfScope {
   ~ProfScope() {
     if (i0 == (size_t)-1) return;
     (void)hipEventRecord(c->ev_pool[i0 + 1], c->stream);
-    c->ev_pending.push_back({id, i0});
+    c->ev_pending.push_back({id, i0, units});
   }
 };
 
@@ -175,9 +178,10 @@ void prof_collect(wdg_ctx* c) {
   HIPCHK(hipStreamSynchronize(c->stream));
   for (auto& pe : c->ev_pending) {
     float ms = 0;
-    HIPCHK(hipEventElapsedTime(&ms, c->ev_pool[pe.second], c->ev_pool[pe.second + 1]));
-    c->prof_ms[pe.first] += ms;
-    c->prof_n[pe.first] += 1;
+    HIPCHK(hipEventElapsedTime(&ms, c->ev_pool[pe.ev], c->ev_pool[pe.ev + 1]));
+    c->prof_ms[pe.id] += ms;
+    c->prof_n[pe.id] += 1;
+    c->prof_units[pe.id] += pe.units;
   }
   c->ev_pending.clear();
   c->ev_used = 0;
@@ -189,15 +193,18 @@ namespace {
 
 template <typename T>
 void op_transport(wdg_ctx* c, const typename cvec<T>::type* u, typename cvec<T>::type* y, const GmresState* st) {
-  ProfScope ps(c, P_TRANSPORT);
+  ProfScope ps(c, P_TRANSPORT, 2.0 * c->N * sizeof(typename cvec<T>::type));
   const long work = (long)c->g.n_chi * c->g.nk;
   hipLaunchKernelGGL((transport_kernel<T>), dim3(grid1(work, 256)), dim3(256), 0, c->stream, c->d_oc.p, c->d_k.p,
                      u, y, st);
 }
 
+// Algorithmic flops of one complex GEMM: 8 real flops per complex MAC.
+double gemm_flops(wdg_ctx* c, int M, int K) { return 8.0 * (M / 2) * (double)(K / 2) * c->ncols; }
+
 void op_gemm64(wdg_ctx* c, int which, const double* A, const double* B, double* C, int M, int K, bool accum,
                const GmresState* st) {
-  ProfScope ps(c, which);
+  ProfScope ps(c, which, gemm_flops(c, M, K));
   constexpr int BM = 64, BN = 64;
   dim3 grid(M / BM, (c->ncols + BN - 1) / BN);
   if (accum)
@@ -208,13 +215,18 @@ void op_gemm64(wdg_ctx* c, int which, const double* A, const double* B, double* 
 
 void op_gemm32(wdg_ctx* c, int which, const uint16_t* A, const float* B, float* C, int M, int K, bool accum,
                const GmresState* st) {
-  ProfScope ps(c, which);
+  ProfScope ps(c, which, gemm_flops(c, M, K));
   constexpr int BM = 64, BN = 64;
   dim3 grid(M / BM, (c->ncols + BN - 1) / BN);
   if (accum)
     hipLaunchKernelGGL((gemm_bf16x2_kernel<BM, BN, true>), grid, dim3(256), 0, c->stream, A, B, C, M, c->ncols, K, st);
   else
     hipLaunchKernelGGL((gemm_bf16x2_kernel<BM, BN, false>), grid, dim3(256), 0, c->stream, A, B, C, M, c->ncols, K, st);
+}
+
+// Theta combine: read T, write Z (complex of size s), read the edge table.
+double combine_bytes(wdg_ctx* c, int s, int ts) {
+  return 2.0 * c->ncols * c->g.n_xi * s + (double)c->g.n_chi * c->oc.n_pairs * (c->g.n_xi + 1) * ts;
 }
 
 // y = A_hat u in precision T.
@@ -226,7 +238,7 @@ void op_apply(wdg_ctx* c, const typename cvec<T>::type* u, typename cvec<T>::typ
   if constexpr (std::is_same<T, double>::value) {
     op_gemm64(c, P_GEMM1, c->d_A1.p, (const double*)u, (double*)c->d_T64.p, M1, K1, false, st);
     {
-      ProfScope ps(c, P_COMBINE);
+      ProfScope ps(c, P_COMBINE, combine_bytes(c, 16, 8));
       hipLaunchKernelGGL((combine_kernel<double, double>), dim3(grid1(work, 256)), dim3(256), 0, c->stream,
                          c->d_oc.p, c->d_sbar64.p, c->d_cap.p, c->d_T64.p, c->d_Z64.p, st);
     }
@@ -234,7 +246,7 @@ void op_apply(wdg_ctx* c, const typename cvec<T>::type* u, typename cvec<T>::typ
   } else {
     op_gemm32(c, P_GEMM1, c->d_A1b.p, (const float*)u, (float*)c->d_T32.p, M1, K1, false, st);
     {
-      ProfScope ps(c, P_COMBINE);
+      ProfScope ps(c, P_COMBINE, combine_bytes(c, 8, 4));
       hipLaunchKernelGGL((combine_kernel<float, float>), dim3(grid1(work, 256)), dim3(256), 0, c->stream,
                          c->d_oc.p, c->d_sbar32.p, c->d_cap.p, c->d_T32.p, c->d_Z32.p, st);
     }
@@ -246,12 +258,12 @@ template <typename T>
 void op_precond(wdg_ctx* c, const typename cvec<T>::type* r, typename cvec<T>::type* z, const GmresState* st) {
   const long work = (long)c->oc.n_chunks * c->g.nk;
   {
-    ProfScope ps(c, P_LINE_AGG);
+    ProfScope ps(c, P_LINE_AGG, 1.0 * c->N * sizeof(typename cvec<T>::type));
     hipLaunchKernelGGL((line_aggregate_kernel<T>), dim3(grid1(work, 128)), dim3(128), 0, c->stream, c->d_oc.p,
                        c->d_k.p, c->d_s0s1.p, r, c->d_agg.p, st);
   }
   {
-    ProfScope ps(c, P_LINE_APPLY);
+    ProfScope ps(c, P_LINE_APPLY, 2.0 * c->N * sizeof(typename cvec<T>::type));
     hipLaunchKernelGGL((line_apply_kernel<T>), dim3(grid1(work, 128)), dim3(128), 0, c->stream, c->d_oc.p,
                        c->d_k.p, c->d_s0s1.p, c->d_agg.p, r, z, st);
   }
@@ -324,11 +336,11 @@ void gmres_cycle(wdg_ctx* c, int m, double tol_abs, double inner_tol, bool preco
     const int nv = j + 1;
     const dim3 dg(c->red_blocks, (nv + kDotGroup - 1) / kDotGroup);
     {
-      ProfScope ps(c, P_DOT);
+      ProfScope ps(c, P_DOT, (nv + 1.0) * N * sizeof(T2));
       hipLaunchKernelGGL((arnoldi_dot_kernel<T, 256>), dg, dim3(256), 0, c->stream, gb, V, w, N, nv, 0);
     }
     {
-      ProfScope ps(c, P_UPDATE);
+      ProfScope ps(c, P_UPDATE, (nv + 2.0) * N * sizeof(T2));
       hipLaunchKernelGGL((arnoldi_update_kernel<T, 256>), dim3(c->red_blocks), dim3(256), 0, c->stream, gb, V, w, N,
                          j, 0);
     }
@@ -845,12 +857,13 @@ int wdg_set_profiling(wdg_ctx* ctx, uint32_t mask) {
   for (int i = 0; i < P_COUNT; ++i) {
     ctx->prof_ms[i] = 0;
     ctx->prof_n[i] = 0;
+    ctx->prof_units[i] = 0;
   }
   return WDG_OK;
 }
 
-int wdg_profile(wdg_ctx* ctx, char* names, int32_t names_len, double* avg_ms, int64_t* launches, int32_t max_entries,
-                int32_t* n_entries) {
+int wdg_profile(wdg_ctx* ctx, char* names, int32_t names_len, double* avg_ms, int64_t* launches, double* units,
+                int32_t max_entries, int32_t* n_entries) {
   if (!ctx) return fail(nullptr, WDG_ERR_ARG, "null argument");
   std::string s;
   int k = 0;
@@ -859,6 +872,7 @@ int wdg_profile(wdg_ctx* ctx, char* names, int32_t names_len, double* avg_ms, in
     s += kProfNames[i];
     if (avg_ms) avg_ms[k] = ctx->prof_n[i] ? ctx->prof_ms[i] / ctx->prof_n[i] : 0.0;
     if (launches) launches[k] = ctx->prof_n[i];
+    if (units) units[k] = ctx->prof_units[i];
   }
   if (n_entries) *n_entries = k;
   if (names && names_len > 0) {

@@ -180,10 +180,12 @@ class WignerOperator:
         names = C.create_string_buffer(512)
         avg = (C.c_double * 32)()
         cnt = (C.c_int64 * 32)()
+        units = (C.c_double * 32)()
         ne = C.c_int32()
-        check(self._lib.wdg_profile(self._h, names, 512, avg, cnt, 32, C.byref(ne)), self._h)
+        check(self._lib.wdg_profile(self._h, names, 512, avg, cnt, units, 32, C.byref(ne)), self._h)
         keys = names.value.decode().split(",")
-        return {k: {"avg_ms": avg[i], "launches": cnt[i]} for i, k in enumerate(keys[:ne.value])}
+        return {k: {"avg_ms": avg[i], "launches": cnt[i], "units": units[i]}
+                for i, k in enumerate(keys[:ne.value])}
 
 
 PROFILE_BITS = {"transport": 0, "theta_gemm1": 1, "theta_combine": 2, "theta_gemm2": 3,
