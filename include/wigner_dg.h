@@ -88,6 +88,7 @@ typedef struct wdg_solve_info {
   double  reported_relres;      /* GMRES's own (Givens) estimate at exit         */
   double  initial_true_relres;  /* of x0 (1 for the zero start)                  */
   double  seconds;              /* device wall time of the solve                 */
+  int32_t reorth_steps;         /* Arnoldi steps that took a second CGS pass     */
 } wdg_solve_info;
 
 typedef struct wdg_ctx wdg_ctx;

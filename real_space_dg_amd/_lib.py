@@ -60,7 +60,8 @@ class wdg_solve_opts(C.Structure):
 class wdg_solve_info(C.Structure):
     _fields_ = [("flag", C.c_int32), ("iterations", C.c_int32), ("outer_iterations", C.c_int32),
                 ("restarts", C.c_int32), ("true_relres", C.c_double), ("reported_relres", C.c_double),
-                ("initial_true_relres", C.c_double), ("seconds", C.c_double)]
+                ("initial_true_relres", C.c_double), ("seconds", C.c_double),
+                ("reorth_steps", C.c_int32)]
 
 
 # name -> (restype, argtypes)

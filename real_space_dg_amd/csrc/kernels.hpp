@@ -33,11 +33,12 @@ struct GmresState {
   double nrm0;       // ||w|| before orthogonalisation
   double resnorm;    // output of residual_kernel
   double hnext;      // ||w|| after orthogonalisation = H[j+1,j]
+  int reorth_count;  // second Gram-Schmidt passes taken (diagnostic)
   int done;          // cycle converged
   int jlast;         // number of completed Arnoldi steps in this cycle
   int reorth;        // second CGS pass needed for the current step
   int breakdown;
-  unsigned int ticket;
+  unsigned int ticket;   // unused (kept for layout stability)
 };
 
 __device__ __forceinline__ bool skip(const GmresState* st) { return st && st->done; }
@@ -404,9 +405,11 @@ __global__ __launch_bounds__(256) void gemm_bf16x2_kernel(const uint16_t* __rest
                                                           int K, const GmresState* st) {
   if (skip(st)) return;
   constexpr int BK = 64;
-  constexpr int LDK = BK + 8;                   // bf16 elements per LDS row (pad 16 B)
-  constexpr int WM = BM / 2, WN = BN / 2;
-  constexpr int TM = WM / 16, TN = WN / 16;
+  constexpr int LDK = BK + 8;                   // bf16 elements per LDS row (16 B pad)
+  constexpr int WM = BM / 2, WN = BN / 2;       // per-wave tile (2 x 2 waves)
+  constexpr int TM = WM / 16, TN = WN / 16;     // 16x16 MFMA tiles per wave
+  constexpr int ACH = BM * BK / 8 / 256;        // 16-B chunks of A per thread
+  constexpr int BCH = BN * BK / 4 / 256;        // 16-B chunks of B per thread
   __shared__ __attribute__((aligned(16))) uint16_t As[BM * LDK];
   __shared__ __attribute__((aligned(16))) uint16_t Bh[BN * LDK];
   __shared__ __attribute__((aligned(16))) uint16_t Bl[BN * LDK];
@@ -418,27 +421,24 @@ __global__ __launch_bounds__(256) void gemm_bf16x2_kernel(const uint16_t* __rest
   for (int i = 0; i < TM; ++i)
 #pragma unroll
     for (int j = 0; j < TN; ++j) acc[i][j] = flt4{0, 0, 0, 0};
-  // A loads: 8 bf16 (16 B) per chunk; B loads: 4 floats (16 B) per chunk.
-  constexpr int ACH = BM * BK / 8 / 256;   // 16-B chunks of A per thread
-  constexpr int BCH = BN * BK / 4 / 256;   // 16-B chunks of B per thread
   uint4 ra[ACH];
   float4 rb[BCH];
-  auto gload = [&](int k0) {
+  // prefetch of k-step 0
 #pragma unroll
-    for (int i = 0; i < ACH; ++i) {
-      const int idx = tid + i * 256;
-      const int r = idx / (BK / 8), c = (idx % (BK / 8)) * 8;
-      ra[i] = *reinterpret_cast<const uint4*>(A + (long)(m0 + r) * K + k0 + c);
-    }
+  for (int i = 0; i < ACH; ++i) {
+    const int idx = tid + i * 256;
+    const int r = idx / (BK / 8), c = (idx % (BK / 8)) * 8;
+    ra[i] = *reinterpret_cast<const uint4*>(A + (long)(m0 + r) * K + c);
+  }
 #pragma unroll
-    for (int i = 0; i < BCH; ++i) {
-      const int idx = tid + i * 256;
-      const int r = idx / (BK / 4), c = (idx % (BK / 4)) * 4;
-      rb[i] = (n0 + r < N) ? *reinterpret_cast<const float4*>(B + (long)(n0 + r) * K + k0 + c)
-                           : make_float4(0.f, 0.f, 0.f, 0.f);
-    }
-  };
-  auto sstore = [&]() {
+  for (int i = 0; i < BCH; ++i) {
+    const int idx = tid + i * 256;
+    const int r = idx / (BK / 4), c = (idx % (BK / 4)) * 4;
+    rb[i] = (n0 + r < N) ? *reinterpret_cast<const float4*>(B + (long)(n0 + r) * K + c)
+                         : make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+  for (int k0 = 0; k0 < K; k0 += BK) {
+    __syncthreads();
 #pragma unroll
     for (int i = 0; i < ACH; ++i) {
       const int idx = tid + i * 256;
@@ -449,32 +449,38 @@ __global__ __launch_bounds__(256) void gemm_bf16x2_kernel(const uint16_t* __rest
     for (int i = 0; i < BCH; ++i) {
       const int idx = tid + i * 256;
       const int r = idx / (BK / 4), c = (idx % (BK / 4)) * 4;
-      const float v[4] = {rb[i].x, rb[i].y, rb[i].z, rb[i].w};
-      uint16_t h[4], lo[4];
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        h[q] = bf16_bits(v[q]);
-        lo[q] = bf16_bits(v[q] - bf16_float(h[q]));
-      }
+      const uint16_t h0 = bf16_bits(rb[i].x), h1 = bf16_bits(rb[i].y);
+      const uint16_t h2 = bf16_bits(rb[i].z), h3 = bf16_bits(rb[i].w);
+      const uint16_t l0 = bf16_bits(rb[i].x - bf16_float(h0)), l1 = bf16_bits(rb[i].y - bf16_float(h1));
+      const uint16_t l2 = bf16_bits(rb[i].z - bf16_float(h2)), l3 = bf16_bits(rb[i].w - bf16_float(h3));
       uint2 hv, lv;
-      hv.x = (uint32_t)h[0] | ((uint32_t)h[1] << 16);
-      hv.y = (uint32_t)h[2] | ((uint32_t)h[3] << 16);
-      lv.x = (uint32_t)lo[0] | ((uint32_t)lo[1] << 16);
-      lv.y = (uint32_t)lo[2] | ((uint32_t)lo[3] << 16);
+      hv.x = (uint32_t)h0 | ((uint32_t)h1 << 16);
+      hv.y = (uint32_t)h2 | ((uint32_t)h3 << 16);
+      lv.x = (uint32_t)l0 | ((uint32_t)l1 << 16);
+      lv.y = (uint32_t)l2 | ((uint32_t)l3 << 16);
       *reinterpret_cast<uint2*>(&Bh[r * LDK + c]) = hv;
       *reinterpret_cast<uint2*>(&Bl[r * LDK + c]) = lv;
     }
-  };
-  gload(0);
-  for (int k0 = 0; k0 < K; k0 += BK) {
     __syncthreads();
-    sstore();
-    __syncthreads();
-    if (k0 + BK < K) gload(k0 + BK);
+    if (k0 + BK < K) {
+#pragma unroll
+      for (int i = 0; i < ACH; ++i) {
+        const int idx = tid + i * 256;
+        const int r = idx / (BK / 8), c = (idx % (BK / 8)) * 8;
+        ra[i] = *reinterpret_cast<const uint4*>(A + (long)(m0 + r) * K + k0 + BK + c);
+      }
+#pragma unroll
+      for (int i = 0; i < BCH; ++i) {
+        const int idx = tid + i * 256;
+        const int r = idx / (BK / 4), c = (idx % (BK / 4)) * 4;
+        rb[i] = (n0 + r < N) ? *reinterpret_cast<const float4*>(B + (long)(n0 + r) * K + k0 + BK + c)
+                             : make_float4(0.f, 0.f, 0.f, 0.f);
+      }
+    }
 #pragma unroll
     for (int ks = 0; ks < BK; ks += 32) {
-      bf16x8 a[TM], bh[TN], bl[TN];
       const int kk = ks + 8 * (lane >> 4);
+      bf16x8 a[TM], bh[TN], bl[TN];
 #pragma unroll
       for (int i = 0; i < TM; ++i)
         a[i] = *reinterpret_cast<const bf16x8*>(&As[(wm * WM + i * 16 + (lane & 15)) * LDK + kk]);
@@ -510,173 +516,104 @@ __global__ __launch_bounds__(256) void gemm_bf16x2_kernel(const uint16_t* __rest
 
 // ---------------------------------------------------------------------------
 // Line preconditioner: per wave number k an exact upwind block-bidiagonal
-// solve along x of  D_e(k) z_e = r_e - c(k) * z_{neighbour}  with
+// solve along x of  D_e(k) z_e = r_e - c(k) * s_in  with
 //   D_e(k) = (Q/J/s_A)(k K1 + |k| K2) + diag_k(G)_e,
 // swept forward for k > 0 and backward for k < 0 (the x-line factor of
 // DG_code/core/get_LineTransversePreconditioner_full2D.m:174-220, in the k
-// basis where the rho_x eigen-transform is the identity).  The carried
-// scalar obeys an affine recurrence s_e = alpha_e - beta_e s_{e-1}; chunks
-// of elements are composed in parallel (line_aggregate_kernel) and then
-// swept with the composed carry-in (line_apply_kernel).  Block solves use
-// partial pivoting in FP64 whatever the I/O precision.
+// basis where the rho_x eigen-transform is the identity).  s_in is the
+// upstream neighbour's outflow node value; it obeys the affine recurrence
+// s_e = alpha_e - beta_e s_{e-1} with alpha_e = (D^-1 r_e)_out and
+// beta_e = (D^-1 c)_out.  Three phases:
+//   line_aggregate: per (chunk, k) compose the chunk's map s_out = A + B s_in
+//   line_carry:     per k, sweep the chunk maps to get each chunk's s_in
+//   line_apply:     per (chunk, k) sweep the chunk again and write z.
+// Block solves are Gaussian elimination with partial pivoting in FP64 on
+// the augmented system, whatever the I/O precision; n is a template
+// parameter so the block stays in registers.
 // ---------------------------------------------------------------------------
-struct Blk {
-  double r[kMaxN][kMaxN], i[kMaxN][kMaxN];
-  int piv[kMaxN];
+template <int NN, int NR>
+struct Aug {
+  double r[NN][NN + NR], i[NN][NN + NR];
 };
 
-__device__ __forceinline__ void build_block(const OpConst* __restrict__ oc, const double2* __restrict__ s0s1,
-                                            int e, double k, double cosk, Blk& B) {
-  const int n = oc->n;
+template <int NN, int NR>
+__device__ __forceinline__ void build_aug(const OpConst* __restrict__ oc, const double2* __restrict__ s0s1, int e,
+                                          double k, double cosk, Aug<NN, NR>& B) {
   const double ak = fabs(k);
+  const double2* se = s0s1 + (long)e * NN * NN * 2;
 #pragma unroll
-  for (int a = 0; a < kMaxN; ++a)
+  for (int a = 0; a < NN; ++a)
 #pragma unroll
-    for (int b = 0; b < kMaxN; ++b)
-      if (a < n && b < n) {
-        const double2 s0 = s0s1[((long)e * n * n + a * n + b) * 2 + 0];
-        const double2 s1 = s0s1[((long)e * n * n + a * n + b) * 2 + 1];
-        // (i/L)(S0 + 2 cos(k h') S1)
-        const double gr = s0.x + 2.0 * cosk * s1.x, gi = s0.y + 2.0 * cosk * s1.y;
-        B.r[a][b] = oc->tq * (k * oc->K1[a * kMaxN + b] + ak * oc->K2[a * kMaxN + b]) - oc->inv_len * gi;
-        B.i[a][b] = oc->inv_len * gr;
-      }
+    for (int b = 0; b < NN; ++b) {
+      const double2 s0 = se[(a * NN + b) * 2 + 0];
+      const double2 s1 = se[(a * NN + b) * 2 + 1];
+      // diag_k(G) = (i/L)(S0 + 2 cos(k h') S1)
+      const double gr = s0.x + 2.0 * cosk * s1.x, gi = s0.y + 2.0 * cosk * s1.y;
+      B.r[a][b] = oc->tq * (k * oc->K1[a * kMaxN + b] + ak * oc->K2[a * kMaxN + b]) - oc->inv_len * gi;
+      B.i[a][b] = oc->inv_len * gr;
+    }
 }
 
-__device__ __forceinline__ void lu_block(int n, Blk& B) {
+// Gaussian elimination with partial pivoting; the last NR columns hold the
+// right-hand sides and end up holding the solutions.
+template <int NN, int NR>
+__device__ __forceinline__ void gauss_solve(Aug<NN, NR>& B) {
+  constexpr int NC = NN + NR;
 #pragma unroll
-  for (int c = 0; c < kMaxN; ++c) {
-    if (c >= n) break;
-    int p = c;
+  for (int c = 0; c < NN; ++c) {
+    // pivot search
     double best = B.r[c][c] * B.r[c][c] + B.i[c][c] * B.i[c][c];
+    int p = c;
 #pragma unroll
-    for (int r = c + 1; r < kMaxN; ++r)
-      if (r < n) {
-        const double v = B.r[r][c] * B.r[r][c] + B.i[r][c] * B.i[r][c];
-        if (v > best) { best = v; p = r; }
+    for (int r = c + 1; r < NN; ++r) {
+      const double v = B.r[r][c] * B.r[r][c] + B.i[r][c] * B.i[r][c];
+      if (v > best) { best = v; p = r; }
+    }
+#pragma unroll
+    for (int r = c + 1; r < NN; ++r) {
+      const bool sw = (r == p);
+#pragma unroll
+      for (int q = c; q < NC; ++q) {
+        const double ar = B.r[c][q], ai = B.i[c][q], br = B.r[r][q], bi = B.i[r][q];
+        B.r[c][q] = sw ? br : ar; B.i[c][q] = sw ? bi : ai;
+        B.r[r][q] = sw ? ar : br; B.i[r][q] = sw ? ai : bi;
       }
-    B.piv[c] = p;
-    if (p != c) {
-#pragma unroll
-      for (int r = c + 1; r < kMaxN; ++r)
-        if (r == p) {
-#pragma unroll
-          for (int q = 0; q < kMaxN; ++q) {
-            double t = B.r[c][q]; B.r[c][q] = B.r[r][q]; B.r[r][q] = t;
-            t = B.i[c][q]; B.i[c][q] = B.i[r][q]; B.i[r][q] = t;
-          }
-        }
     }
     const double dd = B.r[c][c] * B.r[c][c] + B.i[c][c] * B.i[c][c];
     const double ir = B.r[c][c] / dd, ii = -B.i[c][c] / dd;
 #pragma unroll
-    for (int r = c + 1; r < kMaxN; ++r)
-      if (r < n) {
-        const double fr = B.r[r][c] * ir - B.i[r][c] * ii;
-        const double fi = B.r[r][c] * ii + B.i[r][c] * ir;
-        B.r[r][c] = fr;
-        B.i[r][c] = fi;
+    for (int q = c + 1; q < NC; ++q) {  // scale pivot row
+      const double xr = B.r[c][q] * ir - B.i[c][q] * ii;
+      const double xi = B.r[c][q] * ii + B.i[c][q] * ir;
+      B.r[c][q] = xr;
+      B.i[c][q] = xi;
+    }
 #pragma unroll
-        for (int q = c + 1; q < kMaxN; ++q)
-          if (q < n) {
-            B.r[r][q] -= fr * B.r[c][q] - fi * B.i[c][q];
-            B.i[r][q] -= fr * B.i[c][q] + fi * B.r[c][q];
-          }
+    for (int r = 0; r < NN; ++r) {
+      if (r == c) continue;
+      const double fr = B.r[r][c], fi = B.i[r][c];
+#pragma unroll
+      for (int q = c + 1; q < NC; ++q) {
+        B.r[r][q] -= fr * B.r[c][q] - fi * B.i[c][q];
+        B.i[r][q] -= fr * B.i[c][q] + fi * B.r[c][q];
       }
+    }
   }
 }
 
-__device__ __forceinline__ void lu_solve_block(int n, const Blk& B, double* xr, double* xi) {
-#pragma unroll
-  for (int c = 0; c < kMaxN; ++c)
-    if (c < n) {
-      const int p = B.piv[c];
-      if (p != c) {
-#pragma unroll
-        for (int r = c + 1; r < kMaxN; ++r)
-          if (r == p) {
-            double t = xr[c]; xr[c] = xr[r]; xr[r] = t;
-            t = xi[c]; xi[c] = xi[r]; xi[r] = t;
-          }
-      }
-    }
-#pragma unroll
-  for (int r = 1; r < kMaxN; ++r)
-    if (r < n) {
-#pragma unroll
-      for (int q = 0; q < kMaxN; ++q)
-        if (q < r) {
-          xr[r] -= B.r[r][q] * xr[q] - B.i[r][q] * xi[q];
-          xi[r] -= B.r[r][q] * xi[q] + B.i[r][q] * xr[q];
-        }
-    }
-#pragma unroll
-  for (int r = kMaxN - 1; r >= 0; --r)
-    if (r < n) {
-#pragma unroll
-      for (int q = 0; q < kMaxN; ++q)
-        if (q > r && q < n) {
-          xr[r] -= B.r[r][q] * xr[q] - B.i[r][q] * xi[q];
-          xi[r] -= B.r[r][q] * xi[q] + B.i[r][q] * xr[q];
-        }
-      const double dd = B.r[r][r] * B.r[r][r] + B.i[r][r] * B.i[r][r];
-      const double tr = (xr[r] * B.r[r][r] + xi[r] * B.i[r][r]) / dd;
-      const double ti = (xi[r] * B.r[r][r] - xr[r] * B.i[r][r]) / dd;
-      xr[r] = tr;
-      xi[r] = ti;
-    }
-}
-
-// Element e of the sweep for wave number k (direction by sign of k):
-// z = D^-1 (r - c * s_in); returns the outgoing carried value.
-template <typename T>
-__device__ __forceinline__ double2 line_step(const OpConst* __restrict__ oc, const double2* __restrict__ s0s1,
-                                             const typename cvec<T>::type* __restrict__ r,
-                                             typename cvec<T>::type* __restrict__ z, int e, int j, double k,
-                                             double cosk, double2 s_in, bool write) {
-  using T2 = typename cvec<T>::type;
-  const int n = oc->n, nk = oc->nk;
-  Blk B;
-  build_block(oc, s0s1, e, k, cosk, B);
-  lu_block(n, B);
-  double xr[kMaxN], xi[kMaxN];
+// coupling coefficient tq*c_a of the upstream value for wave number k
+__device__ __forceinline__ double line_coupling(const OpConst* __restrict__ oc, int a, double k) {
   const double ak = fabs(k);
-#pragma unroll
-  for (int a = 0; a < kMaxN; ++a)
-    if (a < n) {
-      const T2 v = r[((long)e * n + a) * nk + j];
-      // coupling coefficient times carried value
-      const double c = (k > 0) ? (-0.5 * (k + ak) * oc->invM0[a]) : (0.5 * (k - ak) * oc->invMn[a]);
-      xr[a] = (double)v.x - oc->tq * c * s_in.x;
-      xi[a] = (double)v.y - oc->tq * c * s_in.y;
-    }
-  lu_solve_block(n, B, xr, xi);
-  if (write) {
-#pragma unroll
-    for (int a = 0; a < kMaxN; ++a)
-      if (a < n) {
-        T2 o;
-        o.x = (T)xr[a];
-        o.y = (T)xi[a];
-        z[((long)e * n + a) * nk + j] = o;
-      }
-  }
-  double2 out;
-  if (k > 0) { out.x = xr[0]; out.y = xi[0]; }
-  else { out.x = xr[0]; out.y = xi[0]; }
-#pragma unroll
-  for (int a = 0; a < kMaxN; ++a)
-    if (k > 0 && a == n - 1) { out.x = xr[a]; out.y = xi[a]; }
-  return out;
+  return oc->tq * ((k > 0) ? (-0.5 * (k + ak) * oc->invM0[a]) : (0.5 * (k - ak) * oc->invMn[a]));
 }
 
-// Per (chunk, k): compose the chunk's affine map s_out = A + B s_in.
-template <typename T>
+template <int NN, typename T>
 __global__ __launch_bounds__(256) void line_aggregate_kernel(const OpConst* __restrict__ oc,
                                                              const double* __restrict__ kvec,
                                                              const double2* __restrict__ s0s1,
                                                              const typename cvec<T>::type* __restrict__ r,
                                                              double2* __restrict__ agg, const GmresState* st) {
+  using T2 = typename cvec<T>::type;
   if (skip(st)) return;
   const int nk = oc->nk, ne = oc->n_chi, S = oc->chunk, nc = oc->n_chunks;
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -685,56 +622,52 @@ __global__ __launch_bounds__(256) void line_aggregate_kernel(const OpConst* __re
   const double k = kvec[j];
   const double cosk = cos(k * oc->hprime);
   const int e_lo = c * S, e_hi = min(ne, e_lo + S);
-  // response to s_in = 0 gives A; response to s_in = 1 minus A gives B.
-  double2 s0 = mk(0, 0), s1 = mk(1, 0);
-  if (k > 0) {
-    for (int e = e_lo; e < e_hi; ++e) {
-      s0 = line_step<T>(oc, s0s1, r, nullptr, e, j, k, cosk, s0, false);
-    }
-  } else {
-    for (int e = e_hi - 1; e >= e_lo; --e) {
-      s0 = line_step<T>(oc, s0s1, r, nullptr, e, j, k, cosk, s0, false);
-    }
-  }
-  // B: homogeneous response (r = 0) to a unit carry-in; computed with a
-  // zero right-hand side pointer-free variant: reuse line_step on r by
-  // linearity: B = step(r, 1) - step(r, 0) accumulates rounding, so do the
-  // homogeneous sweep explicitly.
-  {
-    const int n = oc->n;
-    const double ak = fabs(k);
-    double2 h = s1;
-    const int e_first = (k > 0) ? e_lo : e_hi - 1;
-    const int step = (k > 0) ? 1 : -1;
-    for (int e = e_first; (k > 0) ? (e < e_hi) : (e >= e_lo); e += step) {
-      Blk Bk;
-      build_block(oc, s0s1, e, k, cosk, Bk);
-      lu_block(n, Bk);
-      double xr[kMaxN], xi[kMaxN];
+  const int out = (k > 0) ? NN - 1 : 0;
+  double Ar = 0, Ai = 0, Br = 1, Bi = 0;
+  for (int step = 0; step < e_hi - e_lo; ++step) {
+    const int e = (k > 0) ? e_lo + step : e_hi - 1 - step;
+    Aug<NN, 2> B;
+    build_aug<NN, 2>(oc, s0s1, e, k, cosk, B);
 #pragma unroll
-      for (int a = 0; a < kMaxN; ++a)
-        if (a < n) {
-          const double cc = (k > 0) ? (-0.5 * (k + ak) * oc->invM0[a]) : (0.5 * (k - ak) * oc->invMn[a]);
-          xr[a] = -oc->tq * cc * h.x;
-          xi[a] = -oc->tq * cc * h.y;
-        }
-      lu_solve_block(n, Bk, xr, xi);
-      if (k > 0) {
-        h = mk(xr[0], xi[0]);
-#pragma unroll
-        for (int a = 0; a < kMaxN; ++a)
-          if (a == n - 1) h = mk(xr[a], xi[a]);
-      } else {
-        h = mk(xr[0], xi[0]);
-      }
+    for (int a = 0; a < NN; ++a) {
+      const T2 v = r[((long)e * NN + a) * nk + j];
+      B.r[a][NN] = v.x; B.i[a][NN] = v.y;
+      B.r[a][NN + 1] = line_coupling(oc, a, k); B.i[a][NN + 1] = 0.0;
     }
-    s1 = h;
+    gauss_solve<NN, 2>(B);
+    double al_r = 0, al_i = 0, be_r = 0, be_i = 0;
+#pragma unroll
+    for (int a = 0; a < NN; ++a)
+      if (a == out) { al_r = B.r[a][NN]; al_i = B.i[a][NN]; be_r = B.r[a][NN + 1]; be_i = B.i[a][NN + 1]; }
+    // s_out = alpha - beta * s_in, composed with the running map s_in = A + B s0
+    const double nAr = al_r - (be_r * Ar - be_i * Ai), nAi = al_i - (be_r * Ai + be_i * Ar);
+    const double nBr = -(be_r * Br - be_i * Bi), nBi = -(be_r * Bi + be_i * Br);
+    Ar = nAr; Ai = nAi; Br = nBr; Bi = nBi;
   }
-  agg[((long)c * nk + j) * 2 + 0] = s0;  // A
-  agg[((long)c * nk + j) * 2 + 1] = s1;  // B
+  agg[((long)c * nk + j) * 2 + 0] = mk(Ar, Ai);
+  agg[((long)c * nk + j) * 2 + 1] = mk(Br, Bi);
 }
 
-template <typename T>
+// Per k: carry-in of every chunk (overwrites the chunk's A with its s_in).
+__global__ __launch_bounds__(256) void line_carry_kernel(const OpConst* __restrict__ oc, const double* __restrict__ kvec,
+                                                         double2* __restrict__ agg, const GmresState* st) {
+  if (skip(st)) return;
+  const int nk = oc->nk, nc = oc->n_chunks;
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nk) return;
+  const bool fwd = kvec[j] > 0;
+  double sr = 0, si = 0;
+  for (int step = 0; step < nc; ++step) {
+    const int c = fwd ? step : nc - 1 - step;
+    const double2 A = agg[((long)c * nk + j) * 2 + 0], B = agg[((long)c * nk + j) * 2 + 1];
+    agg[((long)c * nk + j) * 2 + 0] = mk(sr, si);
+    const double nr = A.x + B.x * sr - B.y * si, ni = A.y + B.x * si + B.y * sr;
+    sr = nr;
+    si = ni;
+  }
+}
+
+template <int NN, typename T>
 __global__ __launch_bounds__(256) void line_apply_kernel(const OpConst* __restrict__ oc,
                                                          const double* __restrict__ kvec,
                                                          const double2* __restrict__ s0s1,
@@ -742,6 +675,7 @@ __global__ __launch_bounds__(256) void line_apply_kernel(const OpConst* __restri
                                                          const typename cvec<T>::type* __restrict__ r,
                                                          typename cvec<T>::type* __restrict__ z,
                                                          const GmresState* st) {
+  using T2 = typename cvec<T>::type;
   if (skip(st)) return;
   const int nk = oc->nk, ne = oc->n_chi, S = oc->chunk, nc = oc->n_chunks;
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -749,31 +683,38 @@ __global__ __launch_bounds__(256) void line_apply_kernel(const OpConst* __restri
   const int c = (int)(t / nk), j = (int)(t % nk);
   const double k = kvec[j];
   const double cosk = cos(k * oc->hprime);
-  // carry-in: compose the chunks upstream of c (inflow side has s = 0)
-  double2 s = mk(0, 0);
-  if (k > 0) {
-    for (int cc = 0; cc < c; ++cc) {
-      const double2 A = agg[((long)cc * nk + j) * 2 + 0], B = agg[((long)cc * nk + j) * 2 + 1];
-      s = mk(A.x + B.x * s.x - B.y * s.y, A.y + B.x * s.y + B.y * s.x);
-    }
-  } else {
-    for (int cc = nc - 1; cc > c; --cc) {
-      const double2 A = agg[((long)cc * nk + j) * 2 + 0], B = agg[((long)cc * nk + j) * 2 + 1];
-      s = mk(A.x + B.x * s.x - B.y * s.y, A.y + B.x * s.y + B.y * s.x);
-    }
-  }
   const int e_lo = c * S, e_hi = min(ne, e_lo + S);
-  if (k > 0) {
-    for (int e = e_lo; e < e_hi; ++e) s = line_step<T>(oc, s0s1, r, z, e, j, k, cosk, s, true);
-  } else {
-    for (int e = e_hi - 1; e >= e_lo; --e) s = line_step<T>(oc, s0s1, r, z, e, j, k, cosk, s, true);
+  const int out = (k > 0) ? NN - 1 : 0;
+  const double2 s_in = agg[((long)c * nk + j) * 2 + 0];
+  double sr = s_in.x, si = s_in.y;
+  for (int step = 0; step < e_hi - e_lo; ++step) {
+    const int e = (k > 0) ? e_lo + step : e_hi - 1 - step;
+    Aug<NN, 1> B;
+    build_aug<NN, 1>(oc, s0s1, e, k, cosk, B);
+#pragma unroll
+    for (int a = 0; a < NN; ++a) {
+      const T2 v = r[((long)e * NN + a) * nk + j];
+      const double cf = line_coupling(oc, a, k);
+      B.r[a][NN] = (double)v.x - cf * sr;
+      B.i[a][NN] = (double)v.y - cf * si;
+    }
+    gauss_solve<NN, 1>(B);
+#pragma unroll
+    for (int a = 0; a < NN; ++a) {
+      T2 o;
+      o.x = (T)B.r[a][NN];
+      o.y = (T)B.i[a][NN];
+      z[((long)e * NN + a) * nk + j] = o;
+      if (a == out) { sr = B.r[a][NN]; si = B.i[a][NN]; }
+    }
   }
 }
 
 // ---------------------------------------------------------------------------
 // GMRES device state and reductions.  Reductions write per-block partials
-// and the last block to finish (atomic ticket) sums them in block order, so
-// results are deterministic.  Every Arnoldi kernel returns at once when the
+// that a single-block finalize kernel sums in block order, so results are
+// deterministic and no device-scope fence (an L2 write-back per block on a
+// multi-XCD part) is needed inside a kernel.  Every Arnoldi kernel returns at once when the
 // cycle has converged (st->done), so a whole cycle can be enqueued (or
 // captured in a graph) without host round trips.
 // ---------------------------------------------------------------------------
@@ -795,14 +736,15 @@ struct GmresBufs {
 
 __device__ __forceinline__ double2 cmul(double2 a, double2 b) { return mk(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
 
-// Block reduction of nv (<= kDotGroup+1) complex values per thread into part_out.
+// Block reduction of the kDotGroup+1 complex values of every thread into
+// part_out[0..kDotGroup] (thread 0 of each wave, then thread < 9).
 template <int NT>
-__device__ __forceinline__ void block_reduce_store(const double2* vals, int nv, double2* part_out) {
+__device__ __forceinline__ void block_reduce_store(const double2 (&vals)[kDotGroup + 1], double2* part_out) {
   __shared__ double2 red[kDotGroup + 1][NT / 64];
 #pragma unroll
   for (int i = 0; i < kDotGroup + 1; ++i) {
-    if (i >= nv) break;
     double2 v = vals[i];
+#pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
       v.x += __shfl_down(v.x, off, 64);
       v.y += __shfl_down(v.y, off, 64);
@@ -810,132 +752,187 @@ __device__ __forceinline__ void block_reduce_store(const double2* vals, int nv, 
     if ((threadIdx.x & 63) == 0) red[i][threadIdx.x >> 6] = v;
   }
   __syncthreads();
-  if (threadIdx.x < nv) {
+  if (threadIdx.x < kDotGroup + 1) {
     double2 s = red[threadIdx.x][0];
+#pragma unroll
     for (int w = 1; w < NT / 64; ++w) { s.x += red[threadIdx.x][w].x; s.y += red[threadIdx.x][w].y; }
     part_out[threadIdx.x] = s;
   }
 }
 
-// True in every thread of the last block to finish; its partials reads are
-// then ordered after all other blocks' writes.
-__device__ __forceinline__ bool last_block(unsigned int* ticket) {
-  __shared__ int is_last;
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const unsigned int nb = gridDim.x * gridDim.y;
-    const unsigned int prev = atomicAdd(ticket, 1u);
-    is_last = (prev == nb - 1);
-  }
-  __syncthreads();
-  if (is_last) __threadfence();
-  return is_last != 0;
-}
-
-// out[i] = sum over blocks b < nblocks of part[(base + b)*stride + i], i < nv.
-__device__ __forceinline__ void finalize_sums(const double2* part, long base, int stride, int nblocks, int nv,
-                                              double2* out) {
-  __shared__ double2 red[256];
-  for (int i = 0; i < nv; ++i) {
-    double2 s = mk(0, 0);
-    for (int b = threadIdx.x; b < nblocks; b += blockDim.x) {
-      const double2 v = part[(base + b) * stride + i];
-      s.x += v.x;
-      s.y += v.y;
-    }
-    red[threadIdx.x] = s;
-    __syncthreads();
-    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
-      if (threadIdx.x < st) { red[threadIdx.x].x += red[threadIdx.x + st].x; red[threadIdx.x].y += red[threadIdx.x + st].y; }
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) out[i] = red[0];
-    __syncthreads();
-  }
-}
-
-// Givens update of column j; sets done when |g_{j+1}| <= target.
-__device__ __forceinline__ void givens_step(GmresBufs gb, int j) {
-  const int m = gb.m;
-  GmresState* st = gb.st;
-  double2* Hc = gb.H + (long)j * (m + 1);
-  for (int i = 0; i < j; ++i) {
-    const double2 c = gb.cs[i], s = gb.sn[i];
-    const double2 a = Hc[i], b = Hc[i + 1];
-    // G = [c s; -conj(s) conj(c)]
-    const double2 ca = cmul(c, a), sb = cmul(s, b);
-    const double2 sa = cmul(mk(s.x, -s.y), a), cb = cmul(mk(c.x, -c.y), b);
-    Hc[i] = mk(ca.x + sb.x, ca.y + sb.y);
-    Hc[i + 1] = mk(cb.x - sa.x, cb.y - sa.y);
-  }
-  const double2 a = Hc[j], b = Hc[j + 1];
-  const double r = sqrt(a.x * a.x + a.y * a.y + b.x * b.x + b.y * b.y);
-  double2 c, s;
-  if (r == 0.0) { c = mk(1, 0); s = mk(0, 0); }
-  else { c = mk(a.x / r, -a.y / r); s = mk(b.x / r, -b.y / r); }
-  gb.cs[j] = c;
-  gb.sn[j] = s;
-  Hc[j] = mk(r, 0);
-  Hc[j + 1] = mk(0, 0);
-  const double2 gj = gb.g[j];
-  const double2 t = cmul(mk(s.x, -s.y), gj);
-  gb.g[j + 1] = mk(-t.x, -t.y);
-  gb.g[j] = cmul(c, gj);
-  const double2 gn = gb.g[j + 1];
-  st->resid = sqrt(gn.x * gn.x + gn.y * gn.y);
-  st->jlast = j + 1;
-  if (st->resid <= st->target || r == 0.0) st->done = 1;
-  if (r == 0.0) st->breakdown = 1;
-}
-
-// Multi-dot: out[i] = <V_i, w> for i < nv, plus ||w||^2 in out[nv] when
-// with_norm.  Grid (blocks, ceil(nv/kDotGroup)); block row y handles basis
-// vectors [8y, 8y+8).  mode 0: writes h1 and nrm0; mode 1: writes h2 (only
-// when st->reorth).
-template <typename T, int NT>
-__global__ __launch_bounds__(NT) void arnoldi_dot_kernel(GmresBufs gb, const typename cvec<T>::type* __restrict__ V,
-                                                         const typename cvec<T>::type* __restrict__ w, long N, int nv,
-                                                         int mode) {
-  using T2 = typename cvec<T>::type;
-  if (gb.st->done || (mode == 1 && !gb.st->reorth)) return;
-  const int i0 = blockIdx.y * kDotGroup;
-  const int ng = min(kDotGroup, nv - i0);
-  const bool norm = (mode == 0) && blockIdx.y == 0;
+// Second-stage reduction (single-block finalize kernels): res[i] = sum over blocks
+// b < nblocks of part[(base + b)*stride + i] for all kDotGroup+1 slots at
+// once (strided loads, wave shuffles, one LDS exchange).  res is valid in
+// every thread on return.
+template <int NT>
+__device__ __forceinline__ void finalize_group(const double2* part, long base, int stride, int nblocks,
+                                               double2 (&res)[kDotGroup + 1]) {
+  __shared__ double2 red[kDotGroup + 1][NT / 64];
   double2 acc[kDotGroup + 1];
 #pragma unroll
   for (int i = 0; i < kDotGroup + 1; ++i) acc[i] = mk(0, 0);
-  for (long x = (long)blockIdx.x * NT + threadIdx.x; x < N; x += (long)gridDim.x * NT) {
-    const T2 wv = w[x];
-    const double wr = wv.x, wi = wv.y;
+  for (int b = threadIdx.x; b < nblocks; b += NT) {
+    const double2* pp = part + (base + b) * stride;
 #pragma unroll
-    for (int i = 0; i < kDotGroup; ++i) {
-      if (i < ng) {
-        const T2 v = V[(long)(i0 + i) * N + x];
-        acc[i].x += (double)v.x * wr + (double)v.y * wi;
-        acc[i].y += (double)v.x * wi - (double)v.y * wr;
-      }
+    for (int i = 0; i < kDotGroup + 1; ++i) {
+      const double2 v = pp[i];
+      acc[i].x += v.x;
+      acc[i].y += v.y;
     }
-    acc[kDotGroup].x += wr * wr + wi * wi;
   }
-  if (norm) acc[ng] = acc[kDotGroup];
+#pragma unroll
+  for (int i = 0; i < kDotGroup + 1; ++i) {
+    double2 v = acc[i];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      v.x += __shfl_down(v.x, off, 64);
+      v.y += __shfl_down(v.y, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0) red[i][threadIdx.x >> 6] = v;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < kDotGroup + 1; ++i) {
+    double2 t = red[i][0];
+#pragma unroll
+    for (int w = 1; w < NT / 64; ++w) { t.x += red[i][w].x; t.y += red[i][w].y; }
+    res[i] = t;
+  }
+  __syncthreads();
+}
+
+// Finish Hessenberg column j (entries col[0..j+1]) and apply the Givens
+// rotations; sets done when |g_{j+1}| <= target.  Called by all threads of
+// one block: previous rotations are staged in LDS, thread 0 does the serial
+// recurrence, the column is written back in parallel.
+__device__ __forceinline__ void givens_column(GmresBufs gb, int j, const double2* col) {
+  __shared__ double2 hc[kMaxRestart + 2], cs_s[kMaxRestart], sn_s[kMaxRestart];
+  const int m = gb.m;
+  GmresState* st = gb.st;
+  for (int i = threadIdx.x; i <= j + 1; i += blockDim.x) hc[i] = col[i];
+  for (int i = threadIdx.x; i < j; i += blockDim.x) { cs_s[i] = gb.cs[i]; sn_s[i] = gb.sn[i]; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int i = 0; i < j; ++i) {
+      const double2 c = cs_s[i], s = sn_s[i];
+      const double2 a = hc[i], b = hc[i + 1];
+      // G = [c s; -conj(s) conj(c)]
+      const double2 ca = cmul(c, a), sb = cmul(s, b);
+      const double2 sa = cmul(mk(s.x, -s.y), a), cb = cmul(mk(c.x, -c.y), b);
+      hc[i] = mk(ca.x + sb.x, ca.y + sb.y);
+      hc[i + 1] = mk(cb.x - sa.x, cb.y - sa.y);
+    }
+    const double2 a = hc[j], b = hc[j + 1];
+    const double r = sqrt(a.x * a.x + a.y * a.y + b.x * b.x + b.y * b.y);
+    double2 c, s;
+    if (r == 0.0) { c = mk(1, 0); s = mk(0, 0); }
+    else { c = mk(a.x / r, -a.y / r); s = mk(b.x / r, -b.y / r); }
+    gb.cs[j] = c;
+    gb.sn[j] = s;
+    hc[j] = mk(r, 0);
+    hc[j + 1] = mk(0, 0);
+    const double2 gj = gb.g[j];
+    const double2 t = cmul(mk(s.x, -s.y), gj);
+    gb.g[j + 1] = mk(-t.x, -t.y);
+    gb.g[j] = cmul(c, gj);
+    const double gn = sqrt(t.x * t.x + t.y * t.y);
+    st->resid = gn;
+    st->jlast = j + 1;
+    if (gn <= st->target || r == 0.0) st->done = 1;
+    if (r == 0.0) st->breakdown = 1;
+  }
+  __syncthreads();
+  double2* Hc = gb.H + (long)j * (m + 1);
+  for (int i = threadIdx.x; i <= j + 1; i += blockDim.x) Hc[i] = hc[i];
+}
+
+// 16-byte packets of a complex vector: one double complex or two float
+// complex values.
+template <typename T> struct Pk;
+template <> struct Pk<double> {
+  using type = double2;
+  static constexpr int per = 1;
+  __device__ static __forceinline__ void dot(double& ar, double& ai, type v, type w) {
+    ar += v.x * w.x + v.y * w.y;
+    ai += v.x * w.y - v.y * w.x;
+  }
+  __device__ static __forceinline__ double nrm2(type w) { return w.x * w.x + w.y * w.y; }
+  __device__ static __forceinline__ void axpy(type& w, double2 h, type v) {  // w -= h v
+    w.x -= h.x * v.x - h.y * v.y;
+    w.y -= h.x * v.y + h.y * v.x;
+  }
+  __device__ static __forceinline__ type scale(type v, double s) { return make_double2(v.x * s, v.y * s); }
+};
+template <> struct Pk<float> {
+  using type = float4;
+  static constexpr int per = 2;
+  __device__ static __forceinline__ void dot(double& ar, double& ai, type v, type w) {
+    ar += (double)v.x * w.x + (double)v.y * w.y + (double)v.z * w.z + (double)v.w * w.w;
+    ai += (double)v.x * w.y - (double)v.y * w.x + (double)v.z * w.w - (double)v.w * w.z;
+  }
+  __device__ static __forceinline__ double nrm2(type w) {
+    return (double)w.x * w.x + (double)w.y * w.y + (double)w.z * w.z + (double)w.w * w.w;
+  }
+  __device__ static __forceinline__ void axpy(type& w, double2 h, type v) {
+    const float hr = (float)h.x, hi = (float)h.y;
+    w.x -= hr * v.x - hi * v.y;
+    w.y -= hr * v.y + hi * v.x;
+    w.z -= hr * v.z - hi * v.w;
+    w.w -= hr * v.w + hi * v.z;
+  }
+  __device__ static __forceinline__ type scale(type v, double s) {
+    const float f = (float)s;
+    return make_float4(v.x * f, v.y * f, v.z * f, v.w * f);
+  }
+};
+
+// Multi-dot: h[i] = <V_i, w> for i < nv (plus ||w||^2 when mode 0).
+// Grid (blocks, ceil(nv/kDotGroup)); block row y handles basis vectors
+// [8y, 8y+8).  mode 0 writes h1 and nrm0; mode 1 (only when st->reorth)
+// writes h2.  NP = number of 16-byte packets per vector.
+template <typename T, int NT>
+__global__ __launch_bounds__(NT) void arnoldi_dot_kernel(GmresBufs gb, const typename Pk<T>::type* __restrict__ V,
+                                                         const typename Pk<T>::type* __restrict__ w, long NP, int nv,
+                                                         int mode) {
+  using P = Pk<T>;
+  if (gb.st->done || (mode == 1 && !gb.st->reorth)) return;
+  const int i0 = blockIdx.y * kDotGroup;
+  const int ng = min(kDotGroup, nv - i0);
+  double2 acc[kDotGroup + 1];
+#pragma unroll
+  for (int i = 0; i < kDotGroup + 1; ++i) acc[i] = mk(0, 0);
+  const typename P::type* V0 = V + (long)i0 * NP;
+  for (long x = (long)blockIdx.x * NT + threadIdx.x; x < NP; x += (long)gridDim.x * NT) {
+    const typename P::type wv = w[x];
+#pragma unroll
+    for (int i = 0; i < kDotGroup; ++i)
+      if (i < ng) P::dot(acc[i].x, acc[i].y, V0[(long)i * NP + x], wv);
+    acc[kDotGroup].x += P::nrm2(wv);
+  }
   const int stride = kDotGroup + 1;
   const long blin = (long)blockIdx.y * gridDim.x + blockIdx.x;
-  block_reduce_store<NT>(acc, ng + (norm ? 1 : 0), gb.part + blin * stride);
-  if (last_block(&gb.st->ticket)) {
-    double2* out = (mode == 0) ? gb.h1 : gb.h2;
-    for (int gy = 0; gy < (int)gridDim.y; ++gy) {
-      const int gi0 = gy * kDotGroup;
-      const int gn = min(kDotGroup, nv - gi0);
-      const bool gnorm = (mode == 0) && gy == 0;
-      double2 tmp[kDotGroup + 1];
-      finalize_sums(gb.part, (long)gy * gridDim.x, stride, gridDim.x, gn + (gnorm ? 1 : 0), tmp);
-      if (threadIdx.x == 0) {
-        for (int i = 0; i < gn; ++i) out[gi0 + i] = tmp[i];
-        if (gnorm) gb.st->nrm0 = sqrt(tmp[gn].x);
-      }
+  block_reduce_store<NT>(acc, gb.part + blin * stride);
+}
+
+// Finalize of arnoldi_dot_kernel (one block): sums the per-block partials
+// of every group into h1 (+ nrm0) or h2.
+template <int NT>
+__global__ __launch_bounds__(NT) void arnoldi_dot_finalize_kernel(GmresBufs gb, int nblocks, int nv, int mode) {
+  if (gb.st->done || (mode == 1 && !gb.st->reorth)) return;
+  const int stride = kDotGroup + 1;
+  double2* out = (mode == 0) ? gb.h1 : gb.h2;
+  const int ngy = (nv + kDotGroup - 1) / kDotGroup;
+  for (int gy = 0; gy < ngy; ++gy) {
+    const int gi0 = gy * kDotGroup;
+    const int gn = min(kDotGroup, nv - gi0);
+    double2 res[kDotGroup + 1];
+    finalize_group<NT>(gb.part, (long)gy * nblocks, stride, nblocks, res);
+    if (threadIdx.x == 0) {
+#pragma unroll
+      for (int i = 0; i < kDotGroup; ++i)
+        if (i < gn) out[gi0 + i] = res[i];
+      if (mode == 0 && gy == 0) gb.st->nrm0 = sqrt(res[kDotGroup].x);
     }
-    if (threadIdx.x == 0) gb.st->ticket = 0;
   }
 }
 
@@ -944,74 +941,83 @@ __global__ __launch_bounds__(NT) void arnoldi_dot_kernel(GmresBufs gb, const typ
 // requests a second pass; mode 1 (only when st->reorth) uses h2 and finishes
 // the column with h1 + h2.
 template <typename T, int NT>
-__global__ __launch_bounds__(NT) void arnoldi_update_kernel(GmresBufs gb, const typename cvec<T>::type* __restrict__ V,
-                                                            typename cvec<T>::type* __restrict__ w, long N, int j,
+__global__ __launch_bounds__(NT) void arnoldi_update_kernel(GmresBufs gb, const typename Pk<T>::type* __restrict__ V,
+                                                            typename Pk<T>::type* __restrict__ w, long NP, int j,
                                                             int mode) {
-  using T2 = typename cvec<T>::type;
+  using P = Pk<T>;
   if (gb.st->done || (mode == 1 && !gb.st->reorth)) return;
   const int nv = j + 1;
   __shared__ double2 hs[kMaxRestart + 1];
   const double2* h = (mode == 0) ? gb.h1 : gb.h2;
   for (int i = threadIdx.x; i < nv; i += NT) hs[i] = h[i];
   __syncthreads();
-  double2 acc[1] = {mk(0, 0)};
-  for (long x = (long)blockIdx.x * NT + threadIdx.x; x < N; x += (long)gridDim.x * NT) {
-    const T2 wv = w[x];
-    double wr = wv.x, wi = wv.y;
-    for (int i = 0; i < nv; ++i) {
-      const T2 v = V[(long)i * N + x];
-      const double2 hv = hs[i];
-      wr -= hv.x * (double)v.x - hv.y * (double)v.y;
-      wi -= hv.x * (double)v.y + hv.y * (double)v.x;
+  double2 acc[kDotGroup + 1];
+#pragma unroll
+  for (int i = 0; i < kDotGroup + 1; ++i) acc[i] = mk(0, 0);
+  for (long x = (long)blockIdx.x * NT + threadIdx.x; x < NP; x += (long)gridDim.x * NT) {
+    typename P::type wv = w[x];
+    int i = 0;
+    for (; i + 4 <= nv; i += 4) {
+      const typename P::type v0 = V[(long)(i + 0) * NP + x];
+      const typename P::type v1 = V[(long)(i + 1) * NP + x];
+      const typename P::type v2 = V[(long)(i + 2) * NP + x];
+      const typename P::type v3 = V[(long)(i + 3) * NP + x];
+      P::axpy(wv, hs[i + 0], v0);
+      P::axpy(wv, hs[i + 1], v1);
+      P::axpy(wv, hs[i + 2], v2);
+      P::axpy(wv, hs[i + 3], v3);
     }
-    T2 o; o.x = (T)wr; o.y = (T)wi;
-    w[x] = o;
-    acc[0].x += (double)o.x * (double)o.x + (double)o.y * (double)o.y;
+    for (; i < nv; ++i) P::axpy(wv, hs[i], V[(long)i * NP + x]);
+    w[x] = wv;
+    acc[0].x += P::nrm2(wv);
   }
   const int stride = kDotGroup + 1;
-  block_reduce_store<NT>(acc, 1, gb.part + (long)blockIdx.x * stride);
-  if (last_block(&gb.st->ticket)) {
-    double2 tot[1];
-    finalize_sums(gb.part, 0, stride, gridDim.x, 1, tot);
-    if (threadIdx.x == 0) {
-      GmresState* st = gb.st;
-      st->ticket = 0;
-      const double nrm = sqrt(tot[0].x);
-      const int m = gb.m;
-      double2* Hc = gb.H + (long)j * (m + 1);
-      if (mode == 0) {
-        for (int i = 0; i < nv; ++i) Hc[i] = gb.h1[i];
-        if (nrm < 0.7071067811865476 * st->nrm0) {
-          st->reorth = 1;
-        } else {
-          st->reorth = 0;
-          Hc[j + 1] = mk(nrm, 0);
-          st->hnext = nrm;
-          givens_step(gb, j);
-        }
-      } else {
-        for (int i = 0; i < nv; ++i) Hc[i] = mk(gb.h1[i].x + gb.h2[i].x, gb.h1[i].y + gb.h2[i].y);
-        Hc[j + 1] = mk(nrm, 0);
-        st->hnext = nrm;
-        givens_step(gb, j);
-      }
+  block_reduce_store<NT>(acc, gb.part + (long)blockIdx.x * stride);
+}
+
+// Finalize of arnoldi_update_kernel (one block): ||w||, then either the
+// DGKS request for a second pass (mode 0) or the Hessenberg column with its
+// Givens rotation.
+template <int NT>
+__global__ __launch_bounds__(NT) void arnoldi_update_finalize_kernel(GmresBufs gb, int nblocks, int j, int mode) {
+  if (gb.st->done || (mode == 1 && !gb.st->reorth)) return;
+  const int nv = j + 1;
+  const int stride = kDotGroup + 1;
+  double2 res[kDotGroup + 1];
+  finalize_group<NT>(gb.part, 0, stride, nblocks, res);
+  const double nrm = sqrt(res[0].x);
+  GmresState* st = gb.st;
+  __shared__ double2 col[kMaxRestart + 2];
+  __shared__ int finish;
+  if (threadIdx.x == 0) {
+    finish = 1;
+    if (mode == 0 && nrm < 0.7071067811865476 * st->nrm0) {
+      st->reorth = 1;
+      st->reorth_count += 1;
+      finish = 0;
+    } else if (mode == 0) {
+      st->reorth = 0;
     }
+    st->hnext = nrm;
+  }
+  __syncthreads();
+  if (finish) {
+    for (int i = threadIdx.x; i < nv; i += NT)
+      col[i] = (mode == 0) ? gb.h1[i] : mk(gb.h1[i].x + gb.h2[i].x, gb.h1[i].y + gb.h2[i].y);
+    if (threadIdx.x == 0) col[nv] = mk(nrm, 0);
+    __syncthreads();
+    givens_column(gb, j, col);
   }
 }
 
-// V_{j+1} = w / H[j+1,j]; skipped once the cycle is done (the column of the
-// converging step is not needed).
+// V_{j+1} = w / H[j+1,j]; skipped once the cycle is done.
 template <typename T>
-__global__ void arnoldi_normalize_kernel(GmresBufs gb, const typename cvec<T>::type* __restrict__ w,
-                                         typename cvec<T>::type* __restrict__ vnext, long N, int j) {
-  using T2 = typename cvec<T>::type;
+__global__ void arnoldi_normalize_kernel(GmresBufs gb, const typename Pk<T>::type* __restrict__ w,
+                                         typename Pk<T>::type* __restrict__ vnext, long NP) {
   if (gb.st->done) return;
   const double s = (gb.st->hnext > 0) ? 1.0 / gb.st->hnext : 0.0;
-  for (long x = (long)blockIdx.x * blockDim.x + threadIdx.x; x < N; x += (long)gridDim.x * blockDim.x) {
-    const T2 v = w[x];
-    T2 o; o.x = (T)(v.x * s); o.y = (T)(v.y * s);
-    vnext[x] = o;
-  }
+  for (long x = (long)blockIdx.x * blockDim.x + threadIdx.x; x < NP; x += (long)gridDim.x * blockDim.x)
+    vnext[x] = Pk<T>::scale(w[x], s);
 }
 
 // v0 = r / ||r|| and Givens/rhs initialisation for a new cycle.
@@ -1026,26 +1032,34 @@ __global__ void cycle_init_kernel(GmresBufs gb, double tol_abs, double inner_tol
   st->reorth = 0;
   st->breakdown = 0;
   st->ticket = 0;
+  st->reorth_count = 0;
   st->nrm0 = st->resnorm;
   st->hnext = st->resnorm;
   for (int i = 0; i <= gb.m; ++i) gb.g[i] = mk(0, 0);
   gb.g[0] = mk(st->resnorm, 0);
 }
 
-// y = H(1:jl,1:jl) \ g(1:jl), one thread.
+// y = H(1:jl,1:jl) \ g(1:jl): H staged in LDS by the block, solved by thread 0.
 __global__ void hess_solve_kernel(GmresBufs gb) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  __shared__ double2 Hs[(kMaxRestart + 1) * kMaxRestart];
+  __shared__ double2 ys[kMaxRestart];
   const int m = gb.m, jl = gb.st->jlast;
-  for (int r = jl - 1; r >= 0; --r) {
-    double2 s = gb.g[r];
-    for (int c = r + 1; c < jl; ++c) {
-      const double2 p = cmul(gb.H[(long)c * (m + 1) + r], gb.y[c]);
-      s.x -= p.x; s.y -= p.y;
+  for (int i = threadIdx.x; i < (m + 1) * jl; i += blockDim.x) Hs[i] = gb.H[i];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int r = jl - 1; r >= 0; --r) {
+      double2 s = gb.g[r];
+      for (int c = r + 1; c < jl; ++c) {
+        const double2 p = cmul(Hs[c * (m + 1) + r], ys[c]);
+        s.x -= p.x; s.y -= p.y;
+      }
+      const double2 d = Hs[r * (m + 1) + r];
+      const double dd = d.x * d.x + d.y * d.y;
+      ys[r] = (dd > 0) ? mk((s.x * d.x + s.y * d.y) / dd, (s.y * d.x - s.x * d.y) / dd) : mk(0, 0);
     }
-    const double2 d = gb.H[(long)r * (m + 1) + r];
-    const double dd = d.x * d.x + d.y * d.y;
-    gb.y[r] = (dd > 0) ? mk((s.x * d.x + s.y * d.y) / dd, (s.y * d.x - s.x * d.y) / dd) : mk(0, 0);
   }
+  __syncthreads();
+  for (int i = threadIdx.x; i < jl; i += blockDim.x) gb.y[i] = ys[i];
 }
 
 // s = sum_{i < jlast} y_i V_i
@@ -1086,21 +1100,22 @@ template <int NT>
 __global__ __launch_bounds__(NT) void residual_kernel(const double2* __restrict__ b, const double2* __restrict__ y,
                                                       double2* __restrict__ r, long N, double2* part,
                                                       GmresState* st) {
-  double2 acc[1] = {mk(0, 0)};
+  double2 acc[kDotGroup + 1];
+#pragma unroll
+  for (int i = 0; i < kDotGroup + 1; ++i) acc[i] = mk(0, 0);
   for (long i = (long)blockIdx.x * NT + threadIdx.x; i < N; i += (long)gridDim.x * NT) {
     const double2 v = mk(b[i].x - y[i].x, b[i].y - y[i].y);
     r[i] = v;
     acc[0].x += v.x * v.x + v.y * v.y;
   }
-  block_reduce_store<NT>(acc, 1, part + (long)blockIdx.x * (kDotGroup + 1));
-  if (last_block(&st->ticket)) {
-    double2 tot[1];
-    finalize_sums(part, 0, kDotGroup + 1, gridDim.x, 1, tot);
-    if (threadIdx.x == 0) {
-      st->resnorm = sqrt(tot[0].x);
-      st->ticket = 0;
-    }
-  }
+  block_reduce_store<NT>(acc, part + (long)blockIdx.x * (kDotGroup + 1));
+}
+
+template <int NT>
+__global__ __launch_bounds__(NT) void residual_finalize_kernel(const double2* part, int nblocks, GmresState* st) {
+  double2 res[kDotGroup + 1];
+  finalize_group<NT>(part, 0, kDotGroup + 1, nblocks, res);
+  if (threadIdx.x == 0) st->resnorm = sqrt(res[0].x);
 }
 
 // v0 = r / ||r|| in T precision.

@@ -216,12 +216,22 @@ void op_gemm64(wdg_ctx* c, int which, const double* A, const double* B, double* 
 void op_gemm32(wdg_ctx* c, int which, const uint16_t* A, const float* B, float* C, int M, int K, bool accum,
                const GmresState* st) {
   ProfScope ps(c, which, gemm_flops(c, M, K));
-  constexpr int BM = 64, BN = 64;
-  dim3 grid(M / BM, (c->ncols + BN - 1) / BN);
-  if (accum)
-    hipLaunchKernelGGL((gemm_bf16x2_kernel<BM, BN, true>), grid, dim3(256), 0, c->stream, A, B, C, M, c->ncols, K, st);
-  else
-    hipLaunchKernelGGL((gemm_bf16x2_kernel<BM, BN, false>), grid, dim3(256), 0, c->stream, A, B, C, M, c->ncols, K, st);
+  const int N = c->ncols;
+  if (M % 128 == 0) {
+    constexpr int BM = 128, BN = 128;
+    dim3 grid(M / BM, (N + BN - 1) / BN);
+    if (accum)
+      hipLaunchKernelGGL((gemm_bf16x2_kernel<BM, BN, true>), grid, dim3(256), 0, c->stream, A, B, C, M, N, K, st);
+    else
+      hipLaunchKernelGGL((gemm_bf16x2_kernel<BM, BN, false>), grid, dim3(256), 0, c->stream, A, B, C, M, N, K, st);
+  } else {
+    constexpr int BM = 64, BN = 64;
+    dim3 grid(M / BM, (N + BN - 1) / BN);
+    if (accum)
+      hipLaunchKernelGGL((gemm_bf16x2_kernel<BM, BN, true>), grid, dim3(256), 0, c->stream, A, B, C, M, N, K, st);
+    else
+      hipLaunchKernelGGL((gemm_bf16x2_kernel<BM, BN, false>), grid, dim3(256), 0, c->stream, A, B, C, M, N, K, st);
+  }
 }
 
 // Theta combine: read T, write Z (complex of size s), read the edge table.
@@ -254,18 +264,34 @@ void op_apply(wdg_ctx* c, const typename cvec<T>::type* u, typename cvec<T>::typ
   }
 }
 
-template <typename T>
-void op_precond(wdg_ctx* c, const typename cvec<T>::type* r, typename cvec<T>::type* z, const GmresState* st) {
+template <int NN, typename T>
+void op_precond_n(wdg_ctx* c, const typename cvec<T>::type* r, typename cvec<T>::type* z, const GmresState* st) {
   const long work = (long)c->oc.n_chunks * c->g.nk;
   {
     ProfScope ps(c, P_LINE_AGG, 1.0 * c->N * sizeof(typename cvec<T>::type));
-    hipLaunchKernelGGL((line_aggregate_kernel<T>), dim3(grid1(work, 128)), dim3(128), 0, c->stream, c->d_oc.p,
+    hipLaunchKernelGGL((line_aggregate_kernel<NN, T>), dim3(grid1(work, 256)), dim3(256), 0, c->stream, c->d_oc.p,
                        c->d_k.p, c->d_s0s1.p, r, c->d_agg.p, st);
   }
+  hipLaunchKernelGGL(line_carry_kernel, dim3(grid1(c->g.nk, 256)), dim3(256), 0, c->stream, c->d_oc.p, c->d_k.p,
+                     c->d_agg.p, st);
   {
     ProfScope ps(c, P_LINE_APPLY, 2.0 * c->N * sizeof(typename cvec<T>::type));
-    hipLaunchKernelGGL((line_apply_kernel<T>), dim3(grid1(work, 128)), dim3(128), 0, c->stream, c->d_oc.p,
+    hipLaunchKernelGGL((line_apply_kernel<NN, T>), dim3(grid1(work, 256)), dim3(256), 0, c->stream, c->d_oc.p,
                        c->d_k.p, c->d_s0s1.p, c->d_agg.p, r, z, st);
+  }
+}
+
+template <typename T>
+void op_precond(wdg_ctx* c, const typename cvec<T>::type* r, typename cvec<T>::type* z, const GmresState* st) {
+  switch (c->g.n) {
+    case 2: op_precond_n<2, T>(c, r, z, st); break;
+    case 3: op_precond_n<3, T>(c, r, z, st); break;
+    case 4: op_precond_n<4, T>(c, r, z, st); break;
+    case 5: op_precond_n<5, T>(c, r, z, st); break;
+    case 6: op_precond_n<6, T>(c, r, z, st); break;
+    case 7: op_precond_n<7, T>(c, r, z, st); break;
+    case 8: op_precond_n<8, T>(c, r, z, st); break;
+    default: throw std::invalid_argument("n_k_chi outside 2..8");
   }
 }
 
@@ -306,6 +332,8 @@ void residual64(wdg_ctx* c) {
   op_apply<double>(c, c->d_x.p, c->d_y.p, nullptr);
   hipLaunchKernelGGL((residual_kernel<256>), dim3(c->red_blocks), dim3(256), 0, c->stream, c->d_b.p, c->d_y.p,
                      c->d_r.p, c->N, c->d_part.p, c->d_st.p);
+  hipLaunchKernelGGL((residual_finalize_kernel<256>), dim3(1), dim3(256), 0, c->stream, c->d_part.p, c->red_blocks,
+                     c->d_st.p);
 }
 
 // One restart cycle in precision T (no host synchronisation).
@@ -335,22 +363,29 @@ void gmres_cycle(wdg_ctx* c, int m, double tol_abs, double inner_tol, bool preco
     op_apply<T>(c, opin, w, st);
     const int nv = j + 1;
     const dim3 dg(c->red_blocks, (nv + kDotGroup - 1) / kDotGroup);
+    using PT = typename Pk<T>::type;
+    const long NP = N * (long)sizeof(T2) / 16;
+    PT* Vp = reinterpret_cast<PT*>(V);
+    PT* wp = reinterpret_cast<PT*>(w);
+    const int nb = c->red_blocks;
     {
       ProfScope ps(c, P_DOT, (nv + 1.0) * N * sizeof(T2));
-      hipLaunchKernelGGL((arnoldi_dot_kernel<T, 256>), dg, dim3(256), 0, c->stream, gb, V, w, N, nv, 0);
+      hipLaunchKernelGGL((arnoldi_dot_kernel<T, 256>), dg, dim3(256), 0, c->stream, gb, Vp, wp, NP, nv, 0);
     }
+    hipLaunchKernelGGL((arnoldi_dot_finalize_kernel<256>), dim3(1), dim3(256), 0, c->stream, gb, nb, nv, 0);
     {
       ProfScope ps(c, P_UPDATE, (nv + 2.0) * N * sizeof(T2));
-      hipLaunchKernelGGL((arnoldi_update_kernel<T, 256>), dim3(c->red_blocks), dim3(256), 0, c->stream, gb, V, w, N,
-                         j, 0);
+      hipLaunchKernelGGL((arnoldi_update_kernel<T, 256>), dim3(nb), dim3(256), 0, c->stream, gb, Vp, wp, NP, j, 0);
     }
-    hipLaunchKernelGGL((arnoldi_dot_kernel<T, 256>), dg, dim3(256), 0, c->stream, gb, V, w, N, nv, 1);
-    hipLaunchKernelGGL((arnoldi_update_kernel<T, 256>), dim3(c->red_blocks), dim3(256), 0, c->stream, gb, V, w, N, j,
-                       1);
-    hipLaunchKernelGGL((arnoldi_normalize_kernel<T>), dim3(c->red_blocks), dim3(256), 0, c->stream, gb, w,
-                       V + (size_t)(j + 1) * N, N, j);
+    hipLaunchKernelGGL((arnoldi_update_finalize_kernel<256>), dim3(1), dim3(256), 0, c->stream, gb, nb, j, 0);
+    hipLaunchKernelGGL((arnoldi_dot_kernel<T, 256>), dg, dim3(256), 0, c->stream, gb, Vp, wp, NP, nv, 1);
+    hipLaunchKernelGGL((arnoldi_dot_finalize_kernel<256>), dim3(1), dim3(256), 0, c->stream, gb, nb, nv, 1);
+    hipLaunchKernelGGL((arnoldi_update_kernel<T, 256>), dim3(nb), dim3(256), 0, c->stream, gb, Vp, wp, NP, j, 1);
+    hipLaunchKernelGGL((arnoldi_update_finalize_kernel<256>), dim3(1), dim3(256), 0, c->stream, gb, nb, j, 1);
+    hipLaunchKernelGGL((arnoldi_normalize_kernel<T>), dim3(c->red_blocks), dim3(256), 0, c->stream, gb, wp,
+                       Vp + (size_t)(j + 1) * NP, NP);
   }
-  hipLaunchKernelGGL(hess_solve_kernel, dim3(1), dim3(1), 0, c->stream, gb);
+  hipLaunchKernelGGL(hess_solve_kernel, dim3(1), dim3(256), 0, c->stream, gb);
   hipLaunchKernelGGL((lincomb_kernel<T>), dim3(c->red_blocks), dim3(256), 0, c->stream, gb, V, s, N);
   const T2* corr = s;
   if (precond) {
@@ -387,8 +422,14 @@ void create_impl(wdg_ctx* c, const wdg_desc& d) {
   oc.nk = g.nk;
   oc.n_xi = g.n_xi;
   oc.n_pairs = n * (n + 1) / 2;
-  oc.chunk = std::max(1, std::min(16, g.n_chi));
-  oc.n_chunks = (g.n_chi + oc.chunk - 1) / oc.chunk;
+  // Line-solve chunks: enough (chunk, k) threads to fill the chip.
+  {
+    const long want = 131072;
+    int S = (int)std::max<long>(2, ((long)g.n_chi * g.nk + want - 1) / want);
+    S = std::min(S, 16);
+    oc.chunk = std::min(S, g.n_chi);
+    oc.n_chunks = (g.n_chi + oc.chunk - 1) / oc.chunk;
+  }
   oc.tq = g.q_diff / g.jac / g.scale;
   oc.inv_len = 1.0 / g.l_xi;
   oc.hprime = g.y[1] - g.y[0];
@@ -737,7 +778,7 @@ int wdg_solve(wdg_ctx* ctx, const wdg_solve_opts* opts, const double* x0, double
     HIPCHK(hipStreamSynchronize(ctx->stream));
     info->initial_true_relres = hs.resnorm;
     double relres = hs.resnorm;
-    int its = 0, cycles = 0;
+    int its = 0, cycles = 0, reorth = 0;
     const double tol = opts->tol;
     const double inner = (opts->inner_tol > 0) ? opts->inner_tol : 1e-4;
     const int max_outer = (opts->max_outer > 0) ? opts->max_outer : 1000000;
@@ -754,6 +795,7 @@ int wdg_solve(wdg_ctx* ctx, const wdg_solve_opts* opts, const double* x0, double
       HIPCHK(hipStreamSynchronize(ctx->stream));
       prof_collect(ctx);
       its += hs.jlast;
+      reorth += hs.reorth_count;
       cycles += 1;
       reported = hs.resid;
       relres = hs.resnorm;
@@ -769,6 +811,7 @@ int wdg_solve(wdg_ctx* ctx, const wdg_solve_opts* opts, const double* x0, double
     info->seconds = ms * 1e-3;
     info->iterations = its;
     info->restarts = cycles;
+    info->reorth_steps = reorth;
     info->outer_iterations = fp64 ? 1 : cycles;
     info->true_relres = relres;
     info->reported_relres = reported;
