@@ -77,6 +77,8 @@ typedef struct wdg_solve_opts {
   double  inner_tol;    /* mixed: relative tolerance of each FP32 inner solve      */
   int32_t max_outer;    /* mixed: cap on FP64 refinement steps                     */
   int32_t use_precond;  /* 1: right line preconditioner, 0: none                   */
+  int32_t use_graph;    /* 1: replay each restart cycle from a captured HIP graph   */
+  int32_t orthogonalization; /* 0: CGS with DGKS second pass, 1: single-pass CGS      */
 } wdg_solve_opts;
 
 typedef struct wdg_solve_info {

@@ -34,6 +34,9 @@ struct GmresState {
   double resnorm;    // output of residual_kernel
   double hnext;      // ||w|| after orthogonalisation = H[j+1,j]
   int reorth_count;  // second Gram-Schmidt passes taken (diagnostic)
+  double tol_abs;    // solve tolerance (normalised units)
+  double inner_tol;  // per-cycle relative reduction target (0 = none)
+  int single_pass;   // 1: classical Gram-Schmidt without the second pass
   int done;          // cycle converged
   int jlast;         // number of completed Arnoldi steps in this cycle
   int reorth;        // second CGS pass needed for the current step
@@ -160,7 +163,7 @@ __global__ void precond_sums_kernel(const OpConst* __restrict__ oc, const double
 // (H1 (x) D + H2 (x) |D|, DG_code/core/get_Diff.m:243-249, full upwind).
 // One thread per (e, j); the n nodes of (e, j) are handled together.
 // ---------------------------------------------------------------------------
-template <typename T>
+template <int NN, typename T>
 __global__ __launch_bounds__(256) void transport_kernel(const OpConst* __restrict__ oc,
                                                         const double* __restrict__ kvec,
                                                         const typename cvec<T>::type* __restrict__ u,
@@ -168,50 +171,59 @@ __global__ __launch_bounds__(256) void transport_kernel(const OpConst* __restric
                                                         const GmresState* st) {
   using T2 = typename cvec<T>::type;
   if (skip(st)) return;
-  const int n = oc->n, nk = oc->nk, ne = oc->n_chi;
+  // One thread per (e, pair of wave numbers j, j+1): the n nodes of the
+  // element and the upstream node are read once, 2 complex values per load.
+  constexpr int n = NN;
+  const int nk = oc->nk, ne = oc->n_chi;
+  const int nk2 = nk >> 1;
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (long)ne * nk) return;
-  const int e = (int)(t / nk), j = (int)(t % nk);
-  const double k = kvec[j], ak = fabs(k);
+  if (t >= (long)ne * nk2) return;
+  const int e = (int)(t / nk2), j = 2 * (int)(t % nk2);
   const double tq = oc->tq;
-  double ur[kMaxN], ui[kMaxN];
+  double k[2] = {kvec[j], kvec[j + 1]};
+  double ak[2] = {fabs(k[0]), fabs(k[1])};
+  double ur[NN][2], ui[NN][2];
   const T2* ue = u + (long)e * n * nk + j;
 #pragma unroll
-  for (int b = 0; b < kMaxN; ++b)
-    if (b < n) {
-      T2 v = ue[(long)b * nk];
-      ur[b] = v.x;
-      ui[b] = v.y;
+  for (int b = 0; b < NN; ++b) {
+      const T2 v0 = ue[(long)b * nk], v1 = ue[(long)b * nk + 1];
+      ur[b][0] = v0.x; ui[b][0] = v0.y;
+      ur[b][1] = v1.x; ui[b][1] = v1.y;
     }
-  double cr = 0, ci = 0, dr = 0, di = 0;  // upstream inflow values
-  if (k > 0 && e > 0) {
-    T2 v = u[((long)(e - 1) * n + (n - 1)) * nk + j];
-    cr = v.x; ci = v.y;
-  }
-  if (k < 0 && e < ne - 1) {
-    T2 v = u[((long)(e + 1) * n) * nk + j];
-    dr = v.x; di = v.y;
+  // upstream inflow values (k > 0: last node of e-1; k < 0: first node of e+1)
+  double cr[2] = {0, 0}, ci[2] = {0, 0};
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    if (k[q] > 0 && e > 0) {
+      const T2 v = u[((long)(e - 1) * n + (n - 1)) * nk + j + q];
+      cr[q] = v.x; ci[q] = v.y;
+    } else if (k[q] < 0 && e < ne - 1) {
+      const T2 v = u[((long)(e + 1) * n) * nk + j + q];
+      cr[q] = v.x; ci[q] = v.y;
+    }
   }
   T2* ye = y + (long)e * n * nk + j;
 #pragma unroll
-  for (int a = 0; a < kMaxN; ++a)
-    if (a < n) {
-      double sr = 0, si = 0;
+  for (int a = 0; a < NN; ++a) {
+      T2 o[2];
 #pragma unroll
-      for (int b = 0; b < kMaxN; ++b)
-        if (b < n) {
-          const double c = k * oc->K1[a * kMaxN + b] + ak * oc->K2[a * kMaxN + b];
-          sr += c * ur[b];
-          si += c * ui[b];
+      for (int q = 0; q < 2; ++q) {
+        double sr = 0, si = 0;
+#pragma unroll
+        for (int b = 0; b < NN; ++b) {
+          const double c = k[q] * oc->K1[a * kMaxN + b] + ak[q] * oc->K2[a * kMaxN + b];
+          sr += c * ur[b][q];
+          si += c * ui[b][q];
         }
-      const double cl = -0.5 * (k + ak) * oc->invM0[a];
-      const double cr2 = 0.5 * (k - ak) * oc->invMn[a];
-      sr += cl * cr + cr2 * dr;
-      si += cl * ci + cr2 * di;
-      T2 o;
-      o.x = (T)(tq * sr);
-      o.y = (T)(tq * si);
-      ye[(long)a * nk] = o;
+        // -(k+|k|)/2 invM(:,0) u_{e-1,n-1}  or  (k-|k|)/2 invM(:,n-1) u_{e+1,0}
+        const double cf = (k[q] > 0) ? (-0.5 * (k[q] + ak[q]) * oc->invM0[a]) : (0.5 * (k[q] - ak[q]) * oc->invMn[a]);
+        sr += cf * cr[q];
+        si += cf * ci[q];
+        o[q].x = (T)(tq * sr);
+        o[q].y = (T)(tq * si);
+      }
+      ye[(long)a * nk] = o[0];
+      ye[(long)a * nk + 1] = o[1];
     }
 }
 
@@ -222,7 +234,7 @@ __global__ __launch_bounds__(256) void transport_kernel(const OpConst* __restric
 // projected with invM o (W~'W), :200-219).  Tm is GEMM1's output T[c][l],
 // c = e*n + b.  One thread per (e, l).
 // ---------------------------------------------------------------------------
-template <typename T, typename TS>
+template <int NN, typename T, typename TS>
 __global__ __launch_bounds__(256) void combine_kernel(const OpConst* __restrict__ oc,
                                                       const TS* __restrict__ sbar,
                                                       const double* __restrict__ cap,
@@ -231,14 +243,14 @@ __global__ __launch_bounds__(256) void combine_kernel(const OpConst* __restrict_
                                                       const GmresState* st) {
   using T2 = typename cvec<T>::type;
   if (skip(st)) return;
-  const int n = oc->n, nx = oc->n_xi, ne = oc->n_chi, np = oc->n_pairs, nl = nx + 1;
+  constexpr int n = NN, np = NN * (NN + 1) / 2;
+  const int nx = oc->n_xi, ne = oc->n_chi, nl = nx + 1;
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long)ne * nx) return;
   const int e = (int)(t / nx), l = (int)(t % nx);
-  double tr[3][kMaxN], ti[3][kMaxN];
+  double tr[3][NN], ti[3][NN];
 #pragma unroll
-  for (int b = 0; b < kMaxN; ++b)
-    if (b < n) {
+  for (int b = 0; b < NN; ++b) {
       const T2* row = Tm + ((long)e * n + b) * nx;
       T2 v0 = (l > 0) ? row[l - 1] : T2{0, 0};
       T2 v1 = row[l];
@@ -250,13 +262,13 @@ __global__ __launch_bounds__(256) void combine_kernel(const OpConst* __restrict_
   const double c0 = cap[l], c1 = cap[l + 1], c2 = (l + 2 < nl) ? cap[l + 2] : 0.0;
   const TS* sb = sbar + (long)e * np * nl;
 #pragma unroll
-  for (int a = 0; a < kMaxN; ++a)
-    if (a < n) {
+  for (int a = 0; a < NN; ++a) {
       double zr = 0, zi = 0;
 #pragma unroll
-      for (int b = 0; b < kMaxN; ++b)
-        if (b < n) {
-          const int p = oc->pidx[a * kMaxN + b];
+      for (int b = 0; b < NN; ++b) {
+          // pair index of (min(a,b), max(a,b)) in row-major upper-triangle order
+          const int lo = a < b ? a : b, hi = a < b ? b : a;
+          const int p = lo * NN - lo * (lo - 1) / 2 + (hi - lo);
           const TS* s = sb + (long)p * nl;
           const double2 B0 = bbar(oc, a, b, (double)s[l], c0);      // edge l
           const double2 B1 = bbar(oc, a, b, (double)s[l + 1], c1);  // edge l+1
@@ -398,23 +410,23 @@ __device__ __forceinline__ uint16_t bf16_bits(float f) {
 }
 __device__ __forceinline__ float bf16_float(uint16_t h) { return __uint_as_float(((uint32_t)h) << 16); }
 
-template <int BM, int BN, bool ACCUM>
-__global__ __launch_bounds__(256) void gemm_bf16x2_kernel(const uint16_t* __restrict__ A,
+template <int BM, int BN, bool ACCUM, int WGM = 2, int WGN = 2, int BK = 64>
+__global__ __launch_bounds__(64 * WGM * WGN) void gemm_bf16x2_kernel(const uint16_t* __restrict__ A,
                                                           const float* __restrict__ B,
                                                           float* __restrict__ C, int M, int N,
                                                           int K, const GmresState* st) {
   if (skip(st)) return;
-  constexpr int BK = 64;
+  constexpr int NT = 64 * WGM * WGN;
   constexpr int LDK = BK + 8;                   // bf16 elements per LDS row (16 B pad)
-  constexpr int WM = BM / 2, WN = BN / 2;       // per-wave tile (2 x 2 waves)
+  constexpr int WM = BM / WGM, WN = BN / WGN;   // per-wave tile
   constexpr int TM = WM / 16, TN = WN / 16;     // 16x16 MFMA tiles per wave
-  constexpr int ACH = BM * BK / 8 / 256;        // 16-B chunks of A per thread
-  constexpr int BCH = BN * BK / 4 / 256;        // 16-B chunks of B per thread
+  constexpr int ACH = BM * BK / 8 / NT;         // 16-B chunks of A per thread
+  constexpr int BCH = BN * BK / 4 / NT;         // 16-B chunks of B per thread
   __shared__ __attribute__((aligned(16))) uint16_t As[BM * LDK];
   __shared__ __attribute__((aligned(16))) uint16_t Bh[BN * LDK];
   __shared__ __attribute__((aligned(16))) uint16_t Bl[BN * LDK];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm = wave & 1, wn = wave >> 1;
+  const int wm = wave % WGM, wn = wave / WGM;
   const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
   flt4 acc[TM][TN];
 #pragma unroll
@@ -426,13 +438,13 @@ __global__ __launch_bounds__(256) void gemm_bf16x2_kernel(const uint16_t* __rest
   // prefetch of k-step 0
 #pragma unroll
   for (int i = 0; i < ACH; ++i) {
-    const int idx = tid + i * 256;
+    const int idx = tid + i * NT;
     const int r = idx / (BK / 8), c = (idx % (BK / 8)) * 8;
     ra[i] = *reinterpret_cast<const uint4*>(A + (long)(m0 + r) * K + c);
   }
 #pragma unroll
   for (int i = 0; i < BCH; ++i) {
-    const int idx = tid + i * 256;
+    const int idx = tid + i * NT;
     const int r = idx / (BK / 4), c = (idx % (BK / 4)) * 4;
     rb[i] = (n0 + r < N) ? *reinterpret_cast<const float4*>(B + (long)(n0 + r) * K + c)
                          : make_float4(0.f, 0.f, 0.f, 0.f);
@@ -441,13 +453,13 @@ __global__ __launch_bounds__(256) void gemm_bf16x2_kernel(const uint16_t* __rest
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < ACH; ++i) {
-      const int idx = tid + i * 256;
+      const int idx = tid + i * NT;
       const int r = idx / (BK / 8), c = (idx % (BK / 8)) * 8;
       *reinterpret_cast<uint4*>(&As[r * LDK + c]) = ra[i];
     }
 #pragma unroll
     for (int i = 0; i < BCH; ++i) {
-      const int idx = tid + i * 256;
+      const int idx = tid + i * NT;
       const int r = idx / (BK / 4), c = (idx % (BK / 4)) * 4;
       const uint16_t h0 = bf16_bits(rb[i].x), h1 = bf16_bits(rb[i].y);
       const uint16_t h2 = bf16_bits(rb[i].z), h3 = bf16_bits(rb[i].w);
@@ -465,13 +477,13 @@ __global__ __launch_bounds__(256) void gemm_bf16x2_kernel(const uint16_t* __rest
     if (k0 + BK < K) {
 #pragma unroll
       for (int i = 0; i < ACH; ++i) {
-        const int idx = tid + i * 256;
+        const int idx = tid + i * NT;
         const int r = idx / (BK / 8), c = (idx % (BK / 8)) * 8;
         ra[i] = *reinterpret_cast<const uint4*>(A + (long)(m0 + r) * K + k0 + BK + c);
       }
 #pragma unroll
       for (int i = 0; i < BCH; ++i) {
-        const int idx = tid + i * 256;
+        const int idx = tid + i * NT;
         const int r = idx / (BK / 4), c = (idx % (BK / 4)) * 4;
         rb[i] = (n0 + r < N) ? *reinterpret_cast<const float4*>(B + (long)(n0 + r) * K + k0 + BK + c)
                              : make_float4(0.f, 0.f, 0.f, 0.f);
@@ -489,13 +501,18 @@ __global__ __launch_bounds__(256) void gemm_bf16x2_kernel(const uint16_t* __rest
         bh[j] = *reinterpret_cast<const bf16x8*>(&Bh[(wn * WN + j * 16 + (lane & 15)) * LDK + kk]);
         bl[j] = *reinterpret_cast<const bf16x8*>(&Bl[(wn * WN + j * 16 + (lane & 15)) * LDK + kk]);
       }
+      // all hi products first, then all lo products: consecutive MFMAs never
+      // share an accumulator
 #pragma unroll
       for (int i = 0; i < TM; ++i)
 #pragma unroll
-        for (int j = 0; j < TN; ++j) {
+        for (int j = 0; j < TN; ++j)
           acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[j], a[i], acc[i][j], 0, 0, 0);
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
           acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl[j], a[i], acc[i][j], 0, 0, 0);
-        }
     }
   }
 #pragma unroll
@@ -991,7 +1008,7 @@ __global__ __launch_bounds__(NT) void arnoldi_update_finalize_kernel(GmresBufs g
   __shared__ int finish;
   if (threadIdx.x == 0) {
     finish = 1;
-    if (mode == 0 && nrm < 0.7071067811865476 * st->nrm0) {
+    if (mode == 0 && !st->single_pass && nrm < 0.7071067811865476 * st->nrm0) {
       st->reorth = 1;
       st->reorth_count += 1;
       finish = 0;
@@ -1021,11 +1038,22 @@ __global__ void arnoldi_normalize_kernel(GmresBufs gb, const typename Pk<T>::typ
 }
 
 // v0 = r / ||r|| and Givens/rhs initialisation for a new cycle.
-__global__ void cycle_init_kernel(GmresBufs gb, double tol_abs, double inner_tol) {
+__global__ void solve_init_kernel(GmresState* st, double tol_abs, double inner_tol, int single_pass) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    st->tol_abs = tol_abs;
+    st->inner_tol = inner_tol;
+    st->single_pass = single_pass;
+  }
+}
+
+// Tolerances come from the state (set once per solve) so that a captured
+// cycle graph is valid for any tolerance.
+__global__ void cycle_init_kernel(GmresBufs gb) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   GmresState* st = gb.st;
+  const double tol_abs = st->tol_abs;
   st->beta = st->resnorm;
-  st->target = fmax(tol_abs, inner_tol * st->resnorm);
+  st->target = fmax(tol_abs, st->inner_tol * st->resnorm);
   st->resid = st->resnorm;
   st->done = (st->resnorm <= tol_abs) ? 1 : 0;
   st->jlast = 0;

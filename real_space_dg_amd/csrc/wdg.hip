@@ -20,6 +20,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -111,6 +112,7 @@ struct wdg_ctx {
   double prof_ms[P_COUNT] = {0};
   long prof_n[P_COUNT] = {0};
   double prof_units[P_COUNT] = {0};   // algorithmic bytes (HBM kernels) or flops (GEMMs)
+  std::map<long, hipGraphExec_t> graphs;  // captured cycles, keyed by (restart, precision, precond)
   GmresBufs gb() {
     GmresBufs b;
     b.H = d_H.p; b.cs = d_cs.p; b.sn = d_sn.p; b.g = d_g.p; b.h1 = d_h1.p; b.h2 = d_h2.p;
@@ -194,9 +196,44 @@ namespace {
 template <typename T>
 void op_transport(wdg_ctx* c, const typename cvec<T>::type* u, typename cvec<T>::type* y, const GmresState* st) {
   ProfScope ps(c, P_TRANSPORT, 2.0 * c->N * sizeof(typename cvec<T>::type));
-  const long work = (long)c->g.n_chi * c->g.nk;
-  hipLaunchKernelGGL((transport_kernel<T>), dim3(grid1(work, 256)), dim3(256), 0, c->stream, c->d_oc.p, c->d_k.p,
-                     u, y, st);
+  const long work = (long)c->g.n_chi * (c->g.nk / 2);
+  const dim3 grid(grid1(work, 256));
+#define WDG_TRANSPORT(NN)                                                                                   \
+  hipLaunchKernelGGL((transport_kernel<NN, T>), grid, dim3(256), 0, c->stream, c->d_oc.p, c->d_k.p, u, y, st); \
+  break;
+  switch (c->g.n) {
+    case 2: WDG_TRANSPORT(2)
+    case 3: WDG_TRANSPORT(3)
+    case 4: WDG_TRANSPORT(4)
+    case 5: WDG_TRANSPORT(5)
+    case 6: WDG_TRANSPORT(6)
+    case 7: WDG_TRANSPORT(7)
+    case 8: WDG_TRANSPORT(8)
+    default: throw std::invalid_argument("n_k_chi outside 2..8");
+  }
+#undef WDG_TRANSPORT
+}
+
+template <typename T, typename TS>
+void op_combine(wdg_ctx* c, const TS* sbar, const typename cvec<T>::type* Tm, typename cvec<T>::type* Z,
+                const GmresState* st) {
+  const long work = (long)c->g.n_chi * c->g.n_xi;
+  const dim3 grid(grid1(work, 256));
+#define WDG_COMBINE(NN)                                                                                  \
+  hipLaunchKernelGGL((combine_kernel<NN, T, TS>), grid, dim3(256), 0, c->stream, c->d_oc.p, sbar, c->d_cap.p, \
+                     Tm, Z, st);                                                                         \
+  break;
+  switch (c->g.n) {
+    case 2: WDG_COMBINE(2)
+    case 3: WDG_COMBINE(3)
+    case 4: WDG_COMBINE(4)
+    case 5: WDG_COMBINE(5)
+    case 6: WDG_COMBINE(6)
+    case 7: WDG_COMBINE(7)
+    case 8: WDG_COMBINE(8)
+    default: throw std::invalid_argument("n_k_chi outside 2..8");
+  }
+#undef WDG_COMBINE
 }
 
 // Algorithmic flops of one complex GEMM: 8 real flops per complex MAC.
@@ -217,21 +254,13 @@ void op_gemm32(wdg_ctx* c, int which, const uint16_t* A, const float* B, float* 
                const GmresState* st) {
   ProfScope ps(c, which, gemm_flops(c, M, K));
   const int N = c->ncols;
-  if (M % 128 == 0) {
-    constexpr int BM = 128, BN = 128;
-    dim3 grid(M / BM, (N + BN - 1) / BN);
-    if (accum)
-      hipLaunchKernelGGL((gemm_bf16x2_kernel<BM, BN, true>), grid, dim3(256), 0, c->stream, A, B, C, M, N, K, st);
-    else
-      hipLaunchKernelGGL((gemm_bf16x2_kernel<BM, BN, false>), grid, dim3(256), 0, c->stream, A, B, C, M, N, K, st);
-  } else {
-    constexpr int BM = 64, BN = 64;
-    dim3 grid(M / BM, (N + BN - 1) / BN);
-    if (accum)
-      hipLaunchKernelGGL((gemm_bf16x2_kernel<BM, BN, true>), grid, dim3(256), 0, c->stream, A, B, C, M, N, K, st);
-    else
-      hipLaunchKernelGGL((gemm_bf16x2_kernel<BM, BN, false>), grid, dim3(256), 0, c->stream, A, B, C, M, N, K, st);
-  }
+  // 64 (m') x 128 (columns) tiles, 8 waves as 2 x 4 (M' % 64 == 0, K' % 64 == 0)
+  constexpr int BM = 64, BN = 128;
+  dim3 grid(M / BM, (N + BN - 1) / BN);
+  if (accum)
+    hipLaunchKernelGGL((gemm_bf16x2_kernel<BM, BN, true, 2, 4>), grid, dim3(512), 0, c->stream, A, B, C, M, N, K, st);
+  else
+    hipLaunchKernelGGL((gemm_bf16x2_kernel<BM, BN, false, 2, 4>), grid, dim3(512), 0, c->stream, A, B, C, M, N, K, st);
 }
 
 // Theta combine: read T, write Z (complex of size s), read the edge table.
@@ -249,16 +278,14 @@ void op_apply(wdg_ctx* c, const typename cvec<T>::type* u, typename cvec<T>::typ
     op_gemm64(c, P_GEMM1, c->d_A1.p, (const double*)u, (double*)c->d_T64.p, M1, K1, false, st);
     {
       ProfScope ps(c, P_COMBINE, combine_bytes(c, 16, 8));
-      hipLaunchKernelGGL((combine_kernel<double, double>), dim3(grid1(work, 256)), dim3(256), 0, c->stream,
-                         c->d_oc.p, c->d_sbar64.p, c->d_cap.p, c->d_T64.p, c->d_Z64.p, st);
+      op_combine<double, double>(c, c->d_sbar64.p, c->d_T64.p, c->d_Z64.p, st);
     }
     op_gemm64(c, P_GEMM2, c->d_A2.p, (const double*)c->d_Z64.p, (double*)y, 2 * c->g.nk, 2 * c->g.n_xi, true, st);
   } else {
     op_gemm32(c, P_GEMM1, c->d_A1b.p, (const float*)u, (float*)c->d_T32.p, M1, K1, false, st);
     {
       ProfScope ps(c, P_COMBINE, combine_bytes(c, 8, 4));
-      hipLaunchKernelGGL((combine_kernel<float, float>), dim3(grid1(work, 256)), dim3(256), 0, c->stream,
-                         c->d_oc.p, c->d_sbar32.p, c->d_cap.p, c->d_T32.p, c->d_Z32.p, st);
+      op_combine<float, float>(c, c->d_sbar32.p, c->d_T32.p, c->d_Z32.p, st);
     }
     op_gemm32(c, P_GEMM2, c->d_A2b.p, (const float*)c->d_Z32.p, (float*)y, 2 * c->g.nk, 2 * c->g.n_xi, true, st);
   }
@@ -295,9 +322,14 @@ void op_precond(wdg_ctx* c, const typename cvec<T>::type* r, typename cvec<T>::t
   }
 }
 
+void clear_graphs(wdg_ctx* c);
+
 void ensure_krylov(wdg_ctx* c, int m, bool fp64) {
   if (m > kMaxRestart) throw std::invalid_argument("restart exceeds the supported maximum of 64");
+  // Captured graphs hold raw pointers: drop them before any buffer moves.
+  const void* before[6] = {c->d_V64.p, c->d_V32.p, c->d_part.p, c->d_H.p, c->d_w64.p, c->d_w32.p};
   if (c->m_alloc != m) {
+    clear_graphs(c);
     c->d_H.alloc((size_t)(m + 1) * m);
     c->d_cs.alloc(m);
     c->d_sn.alloc(m);
@@ -325,6 +357,9 @@ void ensure_krylov(wdg_ctx* c, int m, bool fp64) {
     c->d_z32.alloc(N);
     c->d_s32.alloc(N);
   }
+  const void* after[6] = {c->d_V64.p, c->d_V32.p, c->d_part.p, c->d_H.p, c->d_w64.p, c->d_w32.p};
+  for (int i = 0; i < 6; ++i)
+    if (before[i] && before[i] != after[i]) clear_graphs(c);
 }
 
 // r = b - A x in FP64; ||r|| into st->resnorm.
@@ -338,7 +373,7 @@ void residual64(wdg_ctx* c) {
 
 // One restart cycle in precision T (no host synchronisation).
 template <typename T>
-void gmres_cycle(wdg_ctx* c, int m, double tol_abs, double inner_tol, bool precond) {
+void gmres_cycle(wdg_ctx* c, int m, bool precond) {
   using T2 = typename cvec<T>::type;
   T2 *V, *w, *z, *s;
   if constexpr (std::is_same<T, double>::value) {
@@ -349,8 +384,7 @@ void gmres_cycle(wdg_ctx* c, int m, double tol_abs, double inner_tol, bool preco
   const long N = c->N;
   GmresBufs gb = c->gb();
   const GmresState* st = c->d_st.p;
-  (void)inner_tol;
-  hipLaunchKernelGGL(cycle_init_kernel, dim3(1), dim3(1), 0, c->stream, gb, tol_abs, inner_tol);
+  hipLaunchKernelGGL(cycle_init_kernel, dim3(1), dim3(1), 0, c->stream, gb);
   hipLaunchKernelGGL((residual_normalize_kernel<T>), dim3(c->red_blocks), dim3(256), 0, c->stream, c->d_r.p, V, N,
                      st);
   for (int j = 0; j < m; ++j) {
@@ -393,6 +427,40 @@ void gmres_cycle(wdg_ctx* c, int m, double tol_abs, double inner_tol, bool preco
     corr = z;
   }
   hipLaunchKernelGGL((axpy_to64_kernel<T>), dim3(c->red_blocks), dim3(256), 0, c->stream, c->d_x.p, corr, 1.0, N);
+}
+
+// One refinement cycle followed by the FP64 true residual.  With graphs
+// enabled (and no per-kernel event profiling) the whole sequence is captured
+// once per (precision, restart, preconditioner) and replayed.
+void run_cycle(wdg_ctx* c, bool fp64, int m, bool precond, bool use_graph) {
+  auto enqueue = [&]() {
+    if (fp64) gmres_cycle<double>(c, m, precond);
+    else gmres_cycle<float>(c, m, precond);
+    residual64(c);
+  };
+  if (!use_graph || c->prof_mask) {
+    enqueue();
+    return;
+  }
+  const long key = ((long)m << 2) | (fp64 ? 2 : 0) | (precond ? 1 : 0);
+  auto it = c->graphs.find(key);
+  if (it == c->graphs.end()) {
+    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    enqueue();
+    hipGraph_t g = nullptr;
+    HIPCHK(hipStreamEndCapture(c->stream, &g));
+    hipGraphExec_t ge = nullptr;
+    const hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    HIPCHK(e);
+    it = c->graphs.emplace(key, ge).first;
+  }
+  HIPCHK(hipGraphLaunch(it->second, c->stream));
+}
+
+void clear_graphs(wdg_ctx* c) {
+  for (auto& kv : c->graphs) (void)hipGraphExecDestroy(kv.second);
+  c->graphs.clear();
 }
 
 void upload_const(wdg_ctx* c) {
@@ -600,6 +668,7 @@ int wdg_destroy(wdg_ctx* ctx) {
   if (!ctx) return WDG_OK;
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   for (auto e : ctx->ev_pool) (void)hipEventDestroy(e);
+  clear_graphs(ctx);
   hipStream_t s = ctx->stream;
   delete ctx;
   if (s) (void)hipStreamDestroy(s);
@@ -783,13 +852,11 @@ int wdg_solve(wdg_ctx* ctx, const wdg_solve_opts* opts, const double* x0, double
     const double inner = (opts->inner_tol > 0) ? opts->inner_tol : 1e-4;
     const int max_outer = (opts->max_outer > 0) ? opts->max_outer : 1000000;
     double reported = relres;
+    hipLaunchKernelGGL(solve_init_kernel, dim3(1), dim3(1), 0, ctx->stream, ctx->d_st.p, tol, fp64 ? 0.0 : inner,
+                       opts->orthogonalization == 1 ? 1 : 0);
     while (relres > tol && its < opts->max_iter && cycles < max_outer) {
       const int mc = std::min<int>(m, opts->max_iter - its);
-      if (fp64)
-        gmres_cycle<double>(ctx, mc, tol, 0.0, opts->use_precond != 0);
-      else
-        gmres_cycle<float>(ctx, mc, tol, inner, opts->use_precond != 0);
-      residual64(ctx);
+      run_cycle(ctx, fp64, mc, opts->use_precond != 0, opts->use_graph != 0 && mc == m);
       HIPCHK(hipGetLastError());
       HIPCHK(hipMemcpyAsync(&hs, ctx->d_st.p, sizeof(GmresState), hipMemcpyDeviceToHost, ctx->stream));
       HIPCHK(hipStreamSynchronize(ctx->stream));

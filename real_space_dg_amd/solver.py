@@ -40,13 +40,16 @@ class SolveOptions:
     inner_tol: float = 1e-4         # mixed: per-cycle relative reduction target
     max_outer: int = 0              # mixed: cap on refinement cycles (0 = none)
     precondition: bool = True       # right line preconditioner
+    use_graph: bool = True          # replay restart cycles from captured HIP graphs
+    orth: str = "cgs1"              # "cgs1" (single pass, default) or "cgs2" (DGKS second pass)
 
     def to_c(self) -> _lib.wdg_solve_opts:
         if self.precision not in PRECISIONS:
             raise ValueError(f"precision must be one of {sorted(PRECISIONS)}")
         return _lib.wdg_solve_opts(float(self.tol), int(self.restart), int(self.max_iter),
                                    PRECISIONS[self.precision], float(self.inner_tol),
-                                   int(self.max_outer), 1 if self.precondition else 0)
+                                   int(self.max_outer), 1 if self.precondition else 0,
+                                   1 if self.use_graph else 0, {"cgs2": 0, "cgs1": 1}[self.orth])
 
 
 def _as_c128(a, n):

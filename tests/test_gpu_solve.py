@@ -80,3 +80,20 @@ def test_solve_transport_dg_2d_mirror(solved):
     print("mirror n err", en, "RHO err", er)
     assert en < 1e-10 and er < 1e-10
     assert DG["RHO"].shape == ref["RHO"].shape
+
+
+def test_graph_replay_matches_eager(solved):
+    """Captured cycle graphs replayed across restarts (with host syncs and
+    eager work between replays) reproduce the eager launches exactly."""
+    op = solved["op"]
+    runs = []
+    for use_graph in (False, True, True, False, True):
+        for prec in ("fp64", "mixed"):
+            x, info = op.solve(SolveOptions(tol=1e-10, precision=prec, use_graph=use_graph))
+            runs.append((prec, use_graph, x, info["iterations"], info["true_relres"]))
+    for prec in ("fp64", "mixed"):
+        rs = [r for r in runs if r[0] == prec]
+        x0 = rs[0][2]
+        for _, g, x, its, rr in rs[1:]:
+            assert its == rs[0][3]
+            assert np.array_equal(x, x0), (prec, g, np.max(np.abs(x - x0)))
