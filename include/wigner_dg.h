@@ -79,6 +79,7 @@ typedef struct wdg_solve_opts {
   int32_t use_precond;  /* 1: right line preconditioner, 0: none                   */
   int32_t use_graph;    /* 1: replay each restart cycle from a captured HIP graph   */
   int32_t orthogonalization; /* 0: CGS with DGKS second pass, 1: single-pass CGS      */
+  int32_t restart_max;  /* adaptive restart cap (<= restart: fixed restart)           */
 } wdg_solve_opts;
 
 typedef struct wdg_solve_info {

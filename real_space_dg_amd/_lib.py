@@ -55,7 +55,7 @@ class wdg_solve_opts(C.Structure):
     _fields_ = [("tol", C.c_double), ("restart", C.c_int32), ("max_iter", C.c_int32),
                 ("precision", C.c_int32), ("inner_tol", C.c_double), ("max_outer", C.c_int32),
                 ("use_precond", C.c_int32), ("use_graph", C.c_int32),
-                ("orthogonalization", C.c_int32)]
+                ("orthogonalization", C.c_int32), ("restart_max", C.c_int32)]
 
 
 class wdg_solve_info(C.Structure):

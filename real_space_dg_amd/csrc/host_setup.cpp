@@ -15,7 +15,11 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
+#include <map>
+#include <mutex>
 #include <stdexcept>
+#include <tuple>
 
 namespace wdg {
 
@@ -277,8 +281,26 @@ void build_geometry(const wdg_desc& d, Geometry& g) {
       double ph = g.k[j] * g.y[l];
       g.phi[(size_t)l * g.nk + j] = cplx(isl * std::cos(ph), isl * std::sin(ph));
     }
-  g.phi_lu = g.phi;
-  lu_factor(g.phi_lu, g.phi_piv, g.nk);
+  // phi depends only on (n_xi, nk, L_xi): instances of a sweep that share a
+  // geometry share one factorisation.
+  {
+    static std::mutex mu;
+    static std::map<std::tuple<int, int, uint64_t>, std::pair<std::vector<cplx>, std::vector<int>>> cache;
+    uint64_t lbits;
+    std::memcpy(&lbits, &g.l_xi, sizeof(lbits));
+    const auto key = std::make_tuple(g.n_xi, g.nk, lbits);
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = cache.find(key);
+    if (it == cache.end()) {
+      std::vector<cplx> lu = g.phi;
+      std::vector<int> piv;
+      lu_factor(lu, piv, g.nk);
+      if (cache.size() > 64) cache.clear();
+      it = cache.emplace(key, std::make_pair(std::move(lu), std::move(piv))).first;
+    }
+    g.phi_lu = it->second.first;
+    g.phi_piv = it->second.second;
+  }
   // Theta sample points (get_Drift_accel_trial_permutedShell.m:15-55).
   g.edges = linspace(-g.l_xi / 2, g.l_xi / 2, g.n_xi + 1);
   g.cap.resize(g.n_xi + 1);

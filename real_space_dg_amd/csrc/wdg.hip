@@ -22,6 +22,8 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
+#include <tuple>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -113,6 +115,9 @@ struct wdg_ctx {
   long prof_n[P_COUNT] = {0};
   double prof_units[P_COUNT] = {0};   // algorithmic bytes (HBM kernels) or flops (GEMMs)
   std::map<long, hipGraphExec_t> graphs;  // captured cycles, keyed by (restart, precision, precond)
+  GmresState* h_st = nullptr;        // pinned host copy of the device state
+  std::vector<double> stage;         // host staging for warm starts
+  int gemm_cfg32 = -1, gemm_cfg64 = -1;  // autotuned GEMM tile configurations
   GmresBufs gb() {
     GmresBufs b;
     b.H = d_H.p; b.cs = d_cs.p; b.sn = d_sn.p; b.g = d_g.p; b.h1 = d_h1.p; b.h2 = d_h2.p;
@@ -236,31 +241,110 @@ void op_combine(wdg_ctx* c, const TS* sbar, const typename cvec<T>::type* Tm, ty
 #undef WDG_COMBINE
 }
 
+// GEMM tile configurations (chosen per shape by autotune()).
+constexpr int kGemm64Cfgs = 3;   // (BM, BN) = (64,64), (128,64), (64,128)
+constexpr int kGemm32Cfgs = 4;   // (BM, BN, waves) = (64,128,2x4), (128,128,2x2), (128,128,2x4), (64,64,2x2)
+
+void launch_gemm64(wdg_ctx* c, int cfg, const double* A, const double* B, double* C, int M, int K, bool accum,
+                   const GmresState* st) {
+  const int N = c->ncols;
+#define WDG_G64(BM, BN)                                                                                     \
+  {                                                                                                         \
+    dim3 grid(M / BM, (N + BN - 1) / BN);                                                                   \
+    if (accum)                                                                                              \
+      hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, true>), grid, dim3(256), 0, c->stream, A, B, C, M, N, K, st);  \
+    else                                                                                                    \
+      hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, false>), grid, dim3(256), 0, c->stream, A, B, C, M, N, K, st); \
+  }
+  if (cfg == 1 && M % 128 == 0) WDG_G64(128, 64)
+  else if (cfg == 2) WDG_G64(64, 128)
+  else WDG_G64(64, 64)
+#undef WDG_G64
+}
+
+void launch_gemm32(wdg_ctx* c, int cfg, const uint16_t* A, const float* B, float* C, int M, int K, bool accum,
+                   const GmresState* st) {
+  const int N = c->ncols;
+#define WDG_G32(BM, BN, WGM, WGN)                                                                            \
+  {                                                                                                          \
+    dim3 grid(M / BM, (N + BN - 1) / BN);                                                                    \
+    if (accum)                                                                                               \
+      hipLaunchKernelGGL((gemm_bf16x2_kernel<BM, BN, true, WGM, WGN>), grid, dim3(64 * WGM * WGN), 0, c->stream, \
+                         A, B, C, M, N, K, st);                                                              \
+    else                                                                                                     \
+      hipLaunchKernelGGL((gemm_bf16x2_kernel<BM, BN, false, WGM, WGN>), grid, dim3(64 * WGM * WGN), 0, c->stream, \
+                         A, B, C, M, N, K, st);                                                              \
+  }
+  if (cfg == 1 && M % 128 == 0) WDG_G32(128, 128, 2, 2)
+  else if (cfg == 2 && M % 128 == 0) WDG_G32(128, 128, 2, 4)
+  else if (cfg == 3) WDG_G32(64, 64, 2, 2)
+  else WDG_G32(64, 128, 2, 4)
+#undef WDG_G32
+}
+
 // Algorithmic flops of one complex GEMM: 8 real flops per complex MAC.
 double gemm_flops(wdg_ctx* c, int M, int K) { return 8.0 * (M / 2) * (double)(K / 2) * c->ncols; }
 
 void op_gemm64(wdg_ctx* c, int which, const double* A, const double* B, double* C, int M, int K, bool accum,
                const GmresState* st) {
   ProfScope ps(c, which, gemm_flops(c, M, K));
-  constexpr int BM = 64, BN = 64;
-  dim3 grid(M / BM, (c->ncols + BN - 1) / BN);
-  if (accum)
-    hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, true>), grid, dim3(256), 0, c->stream, A, B, C, M, c->ncols, K, st);
-  else
-    hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, false>), grid, dim3(256), 0, c->stream, A, B, C, M, c->ncols, K, st);
+  launch_gemm64(c, c->gemm_cfg64 < 0 ? 0 : c->gemm_cfg64, A, B, C, M, K, accum, st);
 }
 
 void op_gemm32(wdg_ctx* c, int which, const uint16_t* A, const float* B, float* C, int M, int K, bool accum,
                const GmresState* st) {
   ProfScope ps(c, which, gemm_flops(c, M, K));
-  const int N = c->ncols;
-  // 64 (m') x 128 (columns) tiles, 8 waves as 2 x 4 (M' % 64 == 0, K' % 64 == 0)
-  constexpr int BM = 64, BN = 128;
-  dim3 grid(M / BM, (N + BN - 1) / BN);
-  if (accum)
-    hipLaunchKernelGGL((gemm_bf16x2_kernel<BM, BN, true, 2, 4>), grid, dim3(512), 0, c->stream, A, B, C, M, N, K, st);
-  else
-    hipLaunchKernelGGL((gemm_bf16x2_kernel<BM, BN, false, 2, 4>), grid, dim3(512), 0, c->stream, A, B, C, M, N, K, st);
+  launch_gemm32(c, c->gemm_cfg32 < 0 ? 0 : c->gemm_cfg32, A, B, C, M, K, accum, st);
+}
+
+// Tile autotuner: times every configuration on this context's GEMM shape
+// (M' = 2 n_xi, K' = 2 nk, N = n_chi*n_k_chi) and keeps the fastest.
+// Results are cached per shape for the process.
+void autotune(wdg_ctx* c, bool fp64) {
+  int& cfg = fp64 ? c->gemm_cfg64 : c->gemm_cfg32;
+  if (cfg >= 0) return;
+  static std::mutex mu;
+  static std::map<std::tuple<int, int, int, bool>, int> cache;
+  const int M = 2 * c->g.n_xi, K = 2 * c->g.nk, N = c->ncols;
+  const auto key = std::make_tuple(M, N, K, fp64);
+  {
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = cache.find(key);
+    if (it != cache.end()) {
+      cfg = it->second;
+      return;
+    }
+  }
+  hipEvent_t t0, t1;
+  HIPCHK(hipEventCreate(&t0));
+  HIPCHK(hipEventCreate(&t1));
+  const int ncfg = fp64 ? kGemm64Cfgs : kGemm32Cfgs;
+  int best = 0;
+  float best_ms = 1e30f;
+  for (int k = 0; k < ncfg; ++k) {
+    auto launch = [&]() {
+      if (fp64)
+        launch_gemm64(c, k, c->d_A1.p, (const double*)c->d_r.p, (double*)c->d_T64.p, M, K, false, nullptr);
+      else
+        launch_gemm32(c, k, c->d_A1b.p, (const float*)c->d_Z32.p, (float*)c->d_T32.p, M, K, false, nullptr);
+    };
+    launch();
+    HIPCHK(hipEventRecord(t0, c->stream));
+    for (int r = 0; r < 3; ++r) launch();
+    HIPCHK(hipEventRecord(t1, c->stream));
+    HIPCHK(hipEventSynchronize(t1));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, t0, t1));
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = k;
+    }
+  }
+  (void)hipEventDestroy(t0);
+  (void)hipEventDestroy(t1);
+  cfg = best;
+  std::lock_guard<std::mutex> lock(mu);
+  cache[key] = best;
 }
 
 // Theta combine: read T, write Z (complex of size s), read the edge table.
@@ -635,6 +719,121 @@ void check_ready(wdg_ctx* c) {
   if (!c->have_potential) throw std::logic_error("wdg_set_potential has not been called");
 }
 
+// Drives one GMRES solve as a sequence of asynchronous restart cycles
+// (solveStationarySystem gmres branch, DG_code/solve_transport_DG_full2D.m
+// :276-334): normalised RHS, optional warm start (a correction solve with the
+// absolute target tol*||b||), FP64 true residual after every cycle, flag
+// overridden by the true residual.
+struct SolveRun {
+  wdg_ctx* c;
+  wdg_solve_opts o;
+  wdg_solve_info* info;
+  bool fp64 = true, active = false, pending = false;
+  int m = 0, its = 0, cycles = 0, reorth = 0, max_outer = 0;
+  double tol = 0, inner = 0, relres = 1, reported = 1, xs = 1, prev_relres = 1;
+  int m_max = 0;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  SolveRun(wdg_ctx* c_, const wdg_solve_opts& o_, wdg_solve_info* info_) : c(c_), o(o_), info(info_) {
+    if (!(o.tol > 0) || o.restart < 1 || o.max_iter < 1)
+      throw std::invalid_argument("tol must be positive, restart and max_iter at least 1");
+    if (o.precision != WDG_PREC_FP64 && o.precision != WDG_PREC_MIXED)
+      throw std::invalid_argument("unknown precision");
+    std::memset(info, 0, sizeof(*info));
+  }
+  ~SolveRun() {
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+  }
+  void begin(const double* x0) {
+    const long N = c->N;
+    fp64 = o.precision == WDG_PREC_FP64;
+    m = (int)std::min<long>(o.restart, N);
+    m_max = (int)std::min<long>(std::max(o.restart_max, o.restart), std::min<long>(N, kMaxRestart));
+    ensure_krylov(c, m, fp64);
+    autotune(c, fp64);
+    if (!c->h_st) HIPCHK(hipHostMalloc((void**)&c->h_st, sizeof(GmresState), hipHostMallocDefault));
+    xs = (c->rhs_norm > 0) ? c->rhs_norm / c->g.scale : 1.0;  // physical x = xs * x_device
+    c->sol_scale = xs;
+    if (x0) {
+      c->stage.resize(2 * N);
+      for (long i = 0; i < 2 * N; ++i) {
+        if (!std::isfinite(x0[i])) throw std::invalid_argument("x0 must contain only finite values");
+        c->stage[i] = x0[i] / xs;
+      }
+      HIPCHK(hipMemcpy(c->d_x.p, c->stage.data(), 2 * N * 8, hipMemcpyHostToDevice));
+    } else {
+      HIPCHK(hipMemsetAsync(c->d_x.p, 0, N * 16, c->stream));
+    }
+    tol = o.tol;
+    inner = (o.inner_tol > 0) ? o.inner_tol : 1e-4;
+    max_outer = (o.max_outer > 0) ? o.max_outer : 1000000;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventRecord(e0, c->stream));
+    hipLaunchKernelGGL(solve_init_kernel, dim3(1), dim3(1), 0, c->stream, c->d_st.p, tol, fp64 ? 0.0 : inner,
+                       o.orthogonalization == 1 ? 1 : 0);
+    residual64(c);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(c->h_st, c->d_st.p, sizeof(GmresState), hipMemcpyDeviceToHost, c->stream));
+    pending = true;
+    active = true;
+  }
+  // Wait for the last enqueued cycle (or the initial residual) and decide.
+  void sync_poll() {
+    HIPCHK(hipStreamSynchronize(c->stream));
+    pending = false;
+    prof_collect(c);
+    const GmresState& hs = *c->h_st;
+    if (cycles == 0 && its == 0 && info->initial_true_relres == 0) {
+      info->initial_true_relres = hs.resnorm;
+      relres = reported = hs.resnorm;
+    } else {
+      its += hs.jlast;
+      reorth += hs.reorth_count;
+      reported = hs.resid;
+      prev_relres = relres;
+      relres = hs.resnorm;
+      if (hs.jlast == 0) active = false;  // breakdown at the start of a cycle
+      // Adaptive restart: a full cycle that cut the residual by less than 3x
+      // gets a 1.5x longer Krylov space next time (up to restart_max).
+      if (hs.jlast == m && m < m_max && relres > prev_relres / 3.0) {
+        m = std::min(m_max, m + m / 2);
+        ensure_krylov(c, m, fp64);
+      }
+    }
+    if (!(relres > tol) || its >= o.max_iter || cycles >= max_outer || !std::isfinite(relres)) active = false;
+  }
+  void step() {
+    const int mc = std::min<int>(m, o.max_iter - its);
+    run_cycle(c, fp64, mc, o.use_precond != 0, o.use_graph != 0 && mc == m);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(c->h_st, c->d_st.p, sizeof(GmresState), hipMemcpyDeviceToHost, c->stream));
+    cycles += 1;
+    pending = true;
+  }
+  void end(double* x) {
+    if (pending) sync_poll();
+    HIPCHK(hipEventRecord(e1, c->stream));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    const long N = c->N;
+    info->seconds = ms * 1e-3;
+    info->iterations = its;
+    info->restarts = cycles;
+    info->reorth_steps = reorth;
+    info->outer_iterations = fp64 ? 1 : cycles;
+    info->true_relres = relres;
+    info->reported_relres = reported;
+    info->flag = (std::isfinite(relres) && relres <= tol) ? 0 : 1;
+    c->have_solution = true;
+    if (x) {
+      HIPCHK(hipMemcpy(x, c->d_x.p, 2 * N * 8, hipMemcpyDeviceToHost));
+      for (long i = 0; i < 2 * N; ++i) x[i] *= xs;
+    }
+  }
+};
+
 }  // namespace
 
 // ===========================================================================
@@ -669,6 +868,7 @@ int wdg_destroy(wdg_ctx* ctx) {
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   for (auto e : ctx->ev_pool) (void)hipEventDestroy(e);
   clear_graphs(ctx);
+  if (ctx->h_st) (void)hipHostFree(ctx->h_st);
   hipStream_t s = ctx->stream;
   delete ctx;
   if (s) (void)hipStreamDestroy(s);
@@ -806,91 +1006,67 @@ int wdg_diagonal(wdg_ctx* ctx, double* d) {
 int wdg_solve(wdg_ctx* ctx, const wdg_solve_opts* opts, const double* x0, double* x, wdg_solve_info* info) {
   if (!ctx || !opts || !info) return fail(ctx, WDG_ERR_ARG, "null argument");
   if (!ctx->have_potential) return fail(ctx, WDG_ERR_STATE, "wdg_set_potential has not been called");
-  if (!(opts->tol > 0) || opts->restart < 1 || opts->max_iter < 1)
-    return fail(ctx, WDG_ERR_ARG, "tol must be positive, restart and max_iter at least 1");
-  if (opts->precision != WDG_PREC_FP64 && opts->precision != WDG_PREC_MIXED)
-    return fail(ctx, WDG_ERR_ARG, "unknown precision");
   return guarded(ctx, [&]() {
-    std::memset(info, 0, sizeof(*info));
-    const long N = ctx->N;
-    const bool fp64 = opts->precision == WDG_PREC_FP64;
-    const int m = std::min<long>(opts->restart, N);
-    ensure_krylov(ctx, m, fp64);
-    const double sA = ctx->g.scale;
-    // physical x = (||b|| / s_A) * x_device
-    const double xs = (ctx->rhs_norm > 0) ? ctx->rhs_norm / sA : 1.0;
-    ctx->sol_scale = xs;
-    if (x0) {
-      std::vector<double> xd(2 * N);
-      for (long i = 0; i < 2 * N; ++i) {
-        if (!std::isfinite(x0[i])) throw std::invalid_argument("x0 must contain only finite values");
-        xd[i] = x0[i] / xs;
-      }
-      HIPCHK(hipMemcpyAsync(ctx->d_x.p, xd.data(), 2 * N * 8, hipMemcpyHostToDevice, ctx->stream));
-    } else {
-      HIPCHK(hipMemsetAsync(ctx->d_x.p, 0, N * 16, ctx->stream));
+    SolveRun r(ctx, *opts, info);
+    r.begin(x0);
+    r.sync_poll();
+    while (r.active) {
+      r.step();
+      r.sync_poll();
     }
-    if (ctx->rhs_norm == 0 && !x0) {
-      info->flag = 0;
-      info->true_relres = 0;
-      ctx->have_solution = true;
-      if (x) std::memset(x, 0, 2 * N * sizeof(double));
-      return WDG_OK;
-    }
-    hipEvent_t e0, e1;
-    HIPCHK(hipEventCreate(&e0));
-    HIPCHK(hipEventCreate(&e1));
-    HIPCHK(hipEventRecord(e0, ctx->stream));
-    residual64(ctx);
-    GmresState hs{};
-    HIPCHK(hipMemcpyAsync(&hs, ctx->d_st.p, sizeof(GmresState), hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(hipStreamSynchronize(ctx->stream));
-    info->initial_true_relres = hs.resnorm;
-    double relres = hs.resnorm;
-    int its = 0, cycles = 0, reorth = 0;
-    const double tol = opts->tol;
-    const double inner = (opts->inner_tol > 0) ? opts->inner_tol : 1e-4;
-    const int max_outer = (opts->max_outer > 0) ? opts->max_outer : 1000000;
-    double reported = relres;
-    hipLaunchKernelGGL(solve_init_kernel, dim3(1), dim3(1), 0, ctx->stream, ctx->d_st.p, tol, fp64 ? 0.0 : inner,
-                       opts->orthogonalization == 1 ? 1 : 0);
-    while (relres > tol && its < opts->max_iter && cycles < max_outer) {
-      const int mc = std::min<int>(m, opts->max_iter - its);
-      run_cycle(ctx, fp64, mc, opts->use_precond != 0, opts->use_graph != 0 && mc == m);
-      HIPCHK(hipGetLastError());
-      HIPCHK(hipMemcpyAsync(&hs, ctx->d_st.p, sizeof(GmresState), hipMemcpyDeviceToHost, ctx->stream));
-      HIPCHK(hipStreamSynchronize(ctx->stream));
-      prof_collect(ctx);
-      its += hs.jlast;
-      reorth += hs.reorth_count;
-      cycles += 1;
-      reported = hs.resid;
-      relres = hs.resnorm;
-      if (!std::isfinite(relres)) break;
-      if (hs.jlast == 0) break;  // no progress possible (breakdown at start)
-    }
-    HIPCHK(hipEventRecord(e1, ctx->stream));
-    HIPCHK(hipEventSynchronize(e1));
-    float ms = 0;
-    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    info->seconds = ms * 1e-3;
-    info->iterations = its;
-    info->restarts = cycles;
-    info->reorth_steps = reorth;
-    info->outer_iterations = fp64 ? 1 : cycles;
-    info->true_relres = relres;
-    info->reported_relres = reported;
-    info->flag = (std::isfinite(relres) && relres <= tol) ? 0 : 1;
-    ctx->have_solution = true;
-    if (x) {
-      HIPCHK(hipMemcpyAsync(x, ctx->d_x.p, 2 * N * 8, hipMemcpyDeviceToHost, ctx->stream));
-      HIPCHK(hipStreamSynchronize(ctx->stream));
-      for (long i = 0; i < 2 * N; ++i) x[i] *= xs;
-    }
+    r.end(x);
     return WDG_OK;
   });
+}
+
+int wdg_solve_batch(wdg_ctx* const* ctxs, int32_t n, const wdg_solve_opts* opts, wdg_solve_info* infos) {
+  if (!ctxs || n < 0 || !opts || !infos) return fail(nullptr, WDG_ERR_ARG, "null argument");
+  for (int i = 0; i < n; ++i) {
+    if (!ctxs[i]) return fail(nullptr, WDG_ERR_ARG, "null context in batch");
+    if (!ctxs[i]->have_potential) return fail(ctxs[i], WDG_ERR_STATE, "wdg_set_potential has not been called");
+    for (int j = 0; j < i; ++j)
+      if (ctxs[j] == ctxs[i]) return fail(ctxs[i], WDG_ERR_ARG, "a context appears twice in the batch");
+  }
+  std::vector<std::unique_ptr<SolveRun>> runs;
+  runs.reserve(n);
+  int failed = -1;
+  int rc = WDG_OK;
+  // Every instance runs its own restart cycles on its own stream; the host
+  // polls each instance once per cycle and relaunches it, so the GPU keeps
+  // all unconverged instances in flight together.
+  for (int i = 0; i < n && rc == WDG_OK; ++i) {
+    rc = guarded(ctxs[i], [&]() {
+      runs.emplace_back(new SolveRun(ctxs[i], *opts, &infos[i]));
+      runs.back()->begin(nullptr);
+      return WDG_OK;
+    });
+    if (rc != WDG_OK) failed = i;
+  }
+  bool any = true;
+  while (rc == WDG_OK && any) {
+    any = false;
+    for (int i = 0; i < (int)runs.size() && rc == WDG_OK; ++i) {
+      rc = guarded(ctxs[i], [&]() {
+        SolveRun& r = *runs[i];
+        if (r.pending) r.sync_poll();
+        if (r.active) {
+          r.step();
+          any = true;
+        }
+        return WDG_OK;
+      });
+      if (rc != WDG_OK) failed = i;
+    }
+  }
+  for (int i = 0; i < (int)runs.size() && rc == WDG_OK; ++i) {
+    rc = guarded(ctxs[i], [&]() {
+      runs[i]->end(nullptr);
+      return WDG_OK;
+    });
+    if (rc != WDG_OK) failed = i;
+  }
+  if (rc != WDG_OK && failed >= 0) g_create_error = std::string("instance ") + std::to_string(failed) + ": " + ctxs[failed]->err;
+  return rc;
 }
 
 int wdg_observables(wdg_ctx* ctx, const double* rho, const double* m_nodes, double* n_out, double* j_out,

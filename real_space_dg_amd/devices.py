@@ -64,3 +64,23 @@ def rtd_1d(n_elem: int, degree: int, nk: int, bias_v: float = 0.0, barrier_ev: f
     esub = rtd_potential(x, length_nm, barrier_ev, bias_v=bias_v)[None, :, None]
     vsub = np.ones((1, nx, 1, 1))
     return mat, esub, vsub, fermi_ev, fermi_ev - bias_v
+
+
+def rtd_sweep(n_instances: int = 64, seed: int = 0, degree: int = 3,
+              n_elems=(128, 256, 384, 512), nks=(256, 512, 768, 1024),
+              bias_range=(0.0, 0.4), barrier_range=(0.25, 0.35)):
+    """Batched bias sweep of RTD instances with uneven grids, all from ``seed``.
+
+    Per instance: bias ~ U[bias_range] V, elements ~ n_elems, plane waves ~ nks,
+    barrier height ~ U[barrier_range] eV (numpy default_rng(seed), drawn in
+    that order per instance).  Returns a list of (mat, Esub, Vsub, EfL, EfR).
+    """
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n_instances):
+        bias = float(rng.uniform(*bias_range))
+        ne = int(rng.choice(n_elems))
+        nk = int(rng.choice(nks))
+        barrier = float(rng.uniform(*barrier_range))
+        out.append(rtd_1d(ne, degree, nk, bias_v=bias, barrier_ev=barrier))
+    return out

@@ -42,6 +42,7 @@ class SolveOptions:
     precondition: bool = True       # right line preconditioner
     use_graph: bool = True          # replay restart cycles from captured HIP graphs
     orth: str = "cgs1"              # "cgs1" (single pass, default) or "cgs2" (DGKS second pass)
+    restart_max: int = 0            # adaptive restart cap (0: fixed restart)
 
     def to_c(self) -> _lib.wdg_solve_opts:
         if self.precision not in PRECISIONS:
@@ -49,7 +50,8 @@ class SolveOptions:
         return _lib.wdg_solve_opts(float(self.tol), int(self.restart), int(self.max_iter),
                                    PRECISIONS[self.precision], float(self.inner_tol),
                                    int(self.max_outer), 1 if self.precondition else 0,
-                                   1 if self.use_graph else 0, {"cgs2": 0, "cgs1": 1}[self.orth])
+                                   1 if self.use_graph else 0, {"cgs2": 0, "cgs1": 1}[self.orth],
+                                   int(self.restart_max))
 
 
 def _as_c128(a, n):
@@ -189,6 +191,31 @@ class WignerOperator:
         keys = names.value.decode().split(",")
         return {k: {"avg_ms": avg[i], "launches": cnt[i], "units": units[i]}
                 for i, k in enumerate(keys[:ne.value])}
+
+
+def solve_batch(ops, opts: SolveOptions | None = None):
+    """Solve several independent instances together (wdg_solve_batch).
+
+    Each operator keeps its own grid and potential; the solutions stay on the
+    device (use ``op.observables``).  Returns one info dict per operator.
+    """
+    opts = opts or SolveOptions()
+    ops = list(ops)
+    if not ops:
+        return []
+    lib = ops[0]._lib
+    arr = (C.c_void_p * len(ops))(*[op._h for op in ops])
+    infos = (_lib.wdg_solve_info * len(ops))()
+    co = opts.to_c()
+    rc = lib.wdg_solve_batch(arr, len(ops), C.byref(co), infos)
+    check(rc, None)
+    out = []
+    for info in infos:
+        d = {f: getattr(info, f) for f, _ in info._fields_}
+        d["status"] = "solved-gmres" if info.flag == 0 else "iterative-gmres-not-converged"
+        d["precision"] = opts.precision
+        out.append(d)
+    return out
 
 
 PROFILE_BITS = {"transport": 0, "theta_gemm1": 1, "theta_combine": 2, "theta_gemm2": 3,
