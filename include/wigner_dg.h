@@ -39,7 +39,11 @@ enum {
   WDG_ERR_UNSUPPORTED = -6   /* option outside the restated method */
 };
 
-enum { WDG_FERMI_MSA_3D = 0, WDG_FERMI_MSA_2D = 1 };     /* DG_code/core/verteilung.m:77,118 */
+enum { WDG_FERMI_MSA_3D = 0,      /* pure 1D Fermi-Dirac, DG_code/core/verteilung.m:77-114        */
+       WDG_FERMI_MSA_2D = 1,      /* kz-integrated Fermi-Dirac, verteilung.m:118-168              */
+       WDG_FERMI_TSU_ESAKI = 2 }; /* both transverse directions integrated: m kT/(pi hbar^2)
+                                     ln(1+exp((EF-E)/kT)), the form of verteilung.m:64-65 with the
+                                     device's EF, band edge and mass (volume density, m^-3)        */
 enum { WDG_PREC_FP64 = 0, WDG_PREC_MIXED = 1 };
 
 /* Geometry: the fields of mat / mat.dg.params that initParams reads
@@ -80,6 +84,8 @@ typedef struct wdg_solve_opts {
   int32_t use_graph;    /* 1: replay each restart cycle from a captured HIP graph   */
   int32_t orthogonalization; /* 0: CGS with DGKS second pass, 1: single-pass CGS      */
   int32_t restart_max;  /* adaptive restart cap (<= restart: fixed restart)           */
+  int32_t warm_device;  /* 1 and x0 == NULL: start from the context's last solution
+                           (kept on the device across wdg_set_potential calls)     */
 } wdg_solve_opts;
 
 typedef struct wdg_solve_info {
@@ -149,6 +155,37 @@ int wdg_solve(wdg_ctx* ctx, const wdg_solve_opts* opts, const double* x0, double
  * (xi fastest) like RHO_DG's transpose. */
 int wdg_observables(wdg_ctx* ctx, const double* rho, const double* m_nodes, double* n,
                     double* j, double* rho_u);
+
+/* ---- self-consistent electrostatics (RTD Gummel loop) ---- */
+/* 1D Poisson problem on the material grid, Dirichlet contacts phi(0) = 0,
+ * phi(L) = bias (RTD/eval_Poisson2D_rtd.m, RTD/solve_Poisson2D_rtd.m with
+ * Ny = 1).  Electron energy is vc - phi [eV]. */
+typedef struct wdg_poisson_desc {
+  int32_t n_mat;          /* material grid points (mat.Nx), uniform spacing  */
+  int32_t method;         /* 0 'direct', 1 'newton-raphson' (Boltzmann Jacobian) */
+  const double* x_mat;    /* material grid [m]                                */
+  const double* eps_r;    /* relative permittivity per node                   */
+  const double* net_doping; /* Nd - Na per node [m^-3]                        */
+  double temp;            /* [K]                                              */
+} wdg_poisson_desc;
+
+typedef struct wdg_poisson_info {
+  double physical_relres_before;  /* ||F_free||_inf / scale at phi_in        */
+  double unmixed_step;            /* ||delta||_inf [V]                        */
+  double charge_integral;         /* integral of the projected density [m^-2] */
+  double dg_integral;             /* integral of the DG density [m^-2]        */
+} wdg_poisson_info;
+
+/* Geometry of the Poisson problem; builds the conservative DG -> dual-volume
+ * density projection (RTD/project_DG_density_RTD.m) and the operator. */
+int wdg_poisson_setup(wdg_ctx* ctx, const wdg_poisson_desc* desc);
+
+/* One FP64 Poisson update on the device from the density of the context's
+ * last solution: phi_out = phi_in + delta (unmixed candidate; Gummel mixing
+ * is the caller's).  n_out (may be NULL) receives the projected density.
+ * RTD/make_RTD_poisson_step.m:44-54 + RTD/solve_Poisson2D_rtd.m:85-130. */
+int wdg_poisson_step(wdg_ctx* ctx, const double* phi_in, double bias, double* phi_out, double* n_out,
+                     wdg_poisson_info* info);
 
 /* ---- device-resident variants (for callers that keep data in HBM) ---- */
 /* u, y: device pointers, interleaved complex of the precision's type

@@ -258,6 +258,14 @@ def verteilung(mat, p, N_xi, phi, EfL, EfR, V, m):
     if p["choose_fermi"] == "DG_MSA_3D":                                   # :77-114
         fL = 2 * p["deg_factor_temp"] * (1.0 / (1.0 + np.exp(q * (EL - EfL) / (kB * Temp))))
         fR = 2 * p["deg_factor_temp"] * (1.0 / (1.0 + np.exp(q * (ER - EfR) / (kB * Temp))))
+    elif p["choose_fermi"] == "TSU_ESAKI":
+        # transverse-integrated contacts: the form of verteilung.m:64-65 ('LS')
+        # m kT/(pi hbar^2) ln(1+exp(q(EF-E)/kT)) with the device's EF, band
+        # edge and mass (the reference branch hard-codes EF and m)
+        xl = q * (EfL - EL) / (kB * Temp)
+        xr = q * (EfR - ER) / (kB * Temp)
+        fL = m[0] * kB * Temp / (np.pi * hb ** 2) * np.logaddexp(0.0, xl)
+        fR = m[Nx - 1] * kB * Temp / (np.pi * hb ** 2) * np.logaddexp(0.0, xr)
     elif p["choose_fermi"] == "DG_MSA_2D":                                 # :118-152
         me_z = float(mat["me_z_ch"])
         kz_max = np.sqrt(2 * 2 * M0_VT * me_z * q * (np.max(V + 0.3))) / hb

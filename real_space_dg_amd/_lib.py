@@ -17,6 +17,7 @@ LIB_PATH = Path(__file__).resolve().parent / "libwdg.so"
 WDG_OK = 0
 WDG_FERMI_MSA_3D = 0
 WDG_FERMI_MSA_2D = 1
+WDG_FERMI_TSU_ESAKI = 2
 WDG_PREC_FP64 = 0
 WDG_PREC_MIXED = 1
 
@@ -55,7 +56,7 @@ class wdg_solve_opts(C.Structure):
     _fields_ = [("tol", C.c_double), ("restart", C.c_int32), ("max_iter", C.c_int32),
                 ("precision", C.c_int32), ("inner_tol", C.c_double), ("max_outer", C.c_int32),
                 ("use_precond", C.c_int32), ("use_graph", C.c_int32),
-                ("orthogonalization", C.c_int32), ("restart_max", C.c_int32)]
+                ("orthogonalization", C.c_int32), ("restart_max", C.c_int32), ("warm_device", C.c_int32)]
 
 
 class wdg_solve_info(C.Structure):
@@ -63,6 +64,16 @@ class wdg_solve_info(C.Structure):
                 ("restarts", C.c_int32), ("true_relres", C.c_double), ("reported_relres", C.c_double),
                 ("initial_true_relres", C.c_double), ("seconds", C.c_double),
                 ("reorth_steps", C.c_int32)]
+
+
+class wdg_poisson_desc(C.Structure):
+    _fields_ = [("n_mat", C.c_int32), ("method", C.c_int32), ("x_mat", C.POINTER(C.c_double)),
+                ("eps_r", C.POINTER(C.c_double)), ("net_doping", C.POINTER(C.c_double)), ("temp", C.c_double)]
+
+
+class wdg_poisson_info(C.Structure):
+    _fields_ = [("physical_relres_before", C.c_double), ("unmixed_step", C.c_double),
+                ("charge_integral", C.c_double), ("dg_integral", C.c_double)]
 
 
 # name -> (restype, argtypes)
@@ -80,6 +91,8 @@ SIGNATURES = {
     "wdg_diagonal": (C.c_int, [_P, _D]),
     "wdg_solve": (C.c_int, [_P, C.POINTER(wdg_solve_opts), _D, _D, C.POINTER(wdg_solve_info)]),
     "wdg_observables": (C.c_int, [_P, _D, _D, _D, _D, _D]),
+    "wdg_poisson_setup": (C.c_int, [_P, C.POINTER(wdg_poisson_desc)]),
+    "wdg_poisson_step": (C.c_int, [_P, _D, C.c_double, _D, _D, C.POINTER(wdg_poisson_info)]),
     "wdg_apply_device": (C.c_int, [_P, _P, _P, C.c_int32]),
     "wdg_precond_device": (C.c_int, [_P, _P, _P, C.c_int32]),
     "wdg_host_setup": (C.c_int, [C.POINTER(wdg_desc), C.POINTER(wdg_potential), _D, _D]),

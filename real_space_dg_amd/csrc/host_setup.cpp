@@ -338,6 +338,14 @@ void inflow(const Geometry& g, const wdg_potential& pot, std::vector<cplx>& fnl,
     if (pot.fermi_model == WDG_FERMI_MSA_3D) {                          // :98-99
       fL[j] = 2.0 * pot.deg_factor / (1.0 + std::exp(q * (EL - pot.ef_left) / (kB * T)));
       fR[j] = 2.0 * pot.deg_factor / (1.0 + std::exp(q * (ER - pot.ef_right) / (kB * T)));
+    } else if (pot.fermi_model == WDG_FERMI_TSU_ESAKI) {
+      // m kT/(pi hbar^2) ln(1 + exp(q(EF - E)/kT)): verteilung.m:64-65 form
+      // with the device's Fermi level, band edge and mass.
+      const double aL = pot.m_left * kB * T / (M_PI * hb * hb);
+      const double aR = pot.m_right * kB * T / (M_PI * hb * hb);
+      const double xl = q * (pot.ef_left - EL) / (kB * T), xr = q * (pot.ef_right - ER) / (kB * T);
+      fL[j] = aL * ((xl > 30) ? xl + std::log1p(std::exp(-xl)) : std::log1p(std::exp(xl)));
+      fR[j] = aR * ((xr > 30) ? xr + std::log1p(std::exp(-xr)) : std::log1p(std::exp(xr)));
     } else {                                                            // :139-152
       double vmax = pot.em[0];
       for (int i = 0; i < pot.n_mat; ++i) vmax = std::max(vmax, pot.em[i]);

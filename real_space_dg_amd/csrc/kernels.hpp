@@ -1189,6 +1189,124 @@ __global__ void observables_kernel(const double2* __restrict__ rho, const double
   }
 }
 
+// ---------------------------------------------------------------------------
+// Electrostatics (RTD Gummel loop).  One block: DG nodal density from the
+// two centre phi rows, conservative projection onto the Poisson dual cells
+// (CSR rows of the exact polynomial integrals), then one FP64 Poisson update
+// M phi = q (n - Nd + Na) with Dirichlet contacts, 'direct' or Newton with
+// the local Boltzmann Jacobian J = A_ff - diag(q^2 n/(kB T))
+// (RTD/solve_Poisson2D_rtd.m:85-130, RTD/project_DG_density_RTD.m:1-37).
+// The tridiagonal free-node system is solved by the Thomas algorithm.
+// ---------------------------------------------------------------------------
+struct PoissonDev {
+  int nmat, nodes, method;
+  double temp, q, kb, scale_n;   // scale_n: device solution -> density units
+  const double2* obs;            // 2 complex per DG node (centre rho_u rows)
+  const int* t_ptr;              // CSR of the projection (nmat + 1)
+  const int* t_col;
+  const double* t_val;
+  const double* widths;
+  const double* lo;              // M(k, k-1)
+  const double* up;              // M(k, k+1)
+  const double* dg;              // M(k, k)
+  const double* doping;          // Nd - Na
+  const double* phi_in;
+  double bias;
+  double* n_dg;                  // scratch, nodes
+  double* n_cv;                  // projected density, nmat
+  double* work;                  // 3 * nmat scratch (F, c', d')
+  double* phi_out;
+  double* info;                  // [relres_before, unmixed_step, charge_integral, dg_integral]
+};
+
+__global__ void poisson_newton_kernel(PoissonDev P) {
+  const int t = threadIdx.x, nt = blockDim.x;
+  for (int i = t; i < P.nodes; i += nt) {
+    const double2 a = P.obs[2 * i], b = P.obs[2 * i + 1];
+    P.n_dg[i] = P.scale_n * 0.5 * (a.x + b.x);
+  }
+  __syncthreads();
+  for (int r = t; r < P.nmat; r += nt) {
+    double s = 0.0;
+    for (int p = P.t_ptr[r]; p < P.t_ptr[r + 1]; ++p) s += P.t_val[p] * P.n_dg[P.t_col[p]];
+    P.n_cv[r] = s / P.widths[r];
+  }
+  __syncthreads();
+  const int nm = P.nmat;
+  double* F = P.work;
+  double* cp = P.work + nm;
+  double* dp = P.work + 2 * nm;
+  __shared__ double red[4][256];
+  double fmx = 0, qn = 0, qd = 0, av = 0, csum = 0;
+  for (int k = t; k < nm; k += nt) {
+    double v = P.phi_in[k];
+    if (k == 0) v = 0.0;
+    if (k == nm - 1) v = P.bias;
+    csum += P.n_cv[k] * P.widths[k];
+    if (k == 0 || k == nm - 1) { F[k] = 0.0; continue; }
+    const double vl = (k - 1 == 0) ? 0.0 : P.phi_in[k - 1];
+    const double vr = (k + 1 == nm - 1) ? P.bias : P.phi_in[k + 1];
+    const double Av = P.lo[k] * vl + P.dg[k] * v + P.up[k] * vr;
+    const double f = Av - P.q * (P.n_cv[k] - P.doping[k]);
+    F[k] = f;
+    fmx = fmax(fmx, fabs(f));
+    qn = fmax(qn, fabs(P.q * P.n_cv[k]));
+    qd = fmax(qd, fabs(P.q * P.doping[k]));
+    av = fmax(av, fabs(Av));
+  }
+  red[0][t] = fmx; red[1][t] = qn; red[2][t] = qd; red[3][t] = av;
+  __shared__ double red_sum[256];
+  red_sum[t] = csum;
+  __syncthreads();
+  if (t == 0) {
+    double m0 = 0, m1 = 0, m2 = 0, m3 = 0, cs = 0;
+    for (int i = 0; i < nt; ++i) {
+      m0 = fmax(m0, red[0][i]); m1 = fmax(m1, red[1][i]); m2 = fmax(m2, red[2][i]); m3 = fmax(m3, red[3][i]);
+      cs += red_sum[i];
+    }
+    double dgs = 0.0;
+    for (int r = 0; r < nm; ++r)
+      for (int p = P.t_ptr[r]; p < P.t_ptr[r + 1]; ++p) dgs += P.t_val[p] * P.n_dg[P.t_col[p]];
+    const double scale = fmax(fmax(m1, m2), fmax(m3, P.q));
+    P.info[0] = m0 / scale;
+    P.info[2] = cs;
+    P.info[3] = dgs;
+    // Thomas algorithm on the free nodes 1..nm-2: J delta = -F
+    const double beta = (P.method == 1) ? P.q * P.q / (P.kb * P.temp) : 0.0;
+    double dmax = 0.0;
+    for (int k = 1; k <= nm - 2; ++k) {
+      const double a = (k > 1) ? P.lo[k] : 0.0;
+      const double b = P.dg[k] - beta * P.n_cv[k];
+      const double c = (k < nm - 2) ? P.up[k] : 0.0;
+      const double rhs = -F[k];
+      if (k == 1) {
+        cp[k] = c / b;
+        dp[k] = rhs / b;
+      } else {
+        const double den = b - a * cp[k - 1];
+        cp[k] = c / den;
+        dp[k] = (rhs - a * dp[k - 1]) / den;
+      }
+    }
+    double x = 0.0;
+    P.phi_out[nm - 1] = P.bias;
+    P.phi_out[0] = 0.0;
+    for (int k = nm - 2; k >= 1; --k) {
+      x = dp[k] - ((k < nm - 2) ? cp[k] * x : 0.0);
+      P.phi_out[k] = P.phi_in[k] + x;
+      dmax = fmax(dmax, fabs(x));
+    }
+    P.info[1] = dmax;
+  }
+}
+
+__global__ void scale_complex_kernel(double2* __restrict__ x, double s, long N) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) {
+    x[i].x *= s;
+    x[i].y *= s;
+  }
+}
+
 // Precision conversion helpers.
 __global__ void d2f_kernel(const double2* __restrict__ a, float2* __restrict__ b, double s, long N) {
   for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x)

@@ -118,6 +118,12 @@ struct wdg_ctx {
   GmresState* h_st = nullptr;        // pinned host copy of the device state
   std::vector<double> stage;         // host staging for warm starts
   int gemm_cfg32 = -1, gemm_cfg64 = -1;  // autotuned GEMM tile configurations
+  // electrostatics
+  bool have_poisson = false;
+  int p_nmat = 0, p_method = 1;
+  double p_temp = 300;
+  DevBuf<int> d_tptr, d_tcol;
+  DevBuf<double> d_tval, d_pw, d_plo, d_pup, d_pdg, d_pdop, d_pin, d_pout, d_pndg, d_pncv, d_pwork, d_pinfo;
   GmresBufs gb() {
     GmresBufs b;
     b.H = d_H.p; b.cs = d_cs.p; b.sn = d_sn.p; b.g = d_g.p; b.h1 = d_h1.p; b.h2 = d_h2.p;
@@ -683,12 +689,14 @@ void set_potential_impl(wdg_ctx* c, const wdg_potential& pot) {
     if (!std::isfinite(pot.em[i])) throw std::invalid_argument("em must be finite");
   if (!(pot.m_left > 0) || !(pot.m_right > 0) || !(pot.temp > 0))
     throw std::invalid_argument("masses and temperature must be positive");
-  if (pot.fermi_model != WDG_FERMI_MSA_3D && pot.fermi_model != WDG_FERMI_MSA_2D)
+  if (pot.fermi_model != WDG_FERMI_MSA_3D && pot.fermi_model != WDG_FERMI_MSA_2D &&
+      pot.fermi_model != WDG_FERMI_TSU_ESAKI)
     throw std::invalid_argument("unknown fermi_model");
   if (pot.fermi_model == WDG_FERMI_MSA_2D && !(pot.me_z > 0)) throw std::invalid_argument("me_z must be positive");
   // inflow and rhs (host, FP64)
   std::vector<cplx> fnl, fnr;
   inflow(g, pot, fnl, fnr);
+  const double old_norm = c->rhs_norm;
   build_rhs(g, fnl, fnr, c->rhs);
   double s = 0;
   for (double v : c->rhs) s += v * v;
@@ -711,8 +719,15 @@ void set_potential_impl(wdg_ctx* c, const wdg_potential& pot) {
                      c->d_sbar64.p, c->d_cap.p, c->d_s0s1.p);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(c->stream));
+  // A previous solution stays usable as a warm start: keep its physical value
+  // (x_phys = ||b|| / s_A * x_device) under the new RHS normalisation.
+  if (c->have_solution && old_norm > 0 && c->rhs_norm > 0 && old_norm != c->rhs_norm) {
+    hipLaunchKernelGGL(scale_complex_kernel, dim3(c->red_blocks), dim3(256), 0, c->stream, c->d_x.p,
+                       old_norm / c->rhs_norm, c->N);
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
+  if (c->rhs_norm > 0) c->sol_scale = c->rhs_norm / g.scale;
   c->have_potential = true;
-  c->have_solution = false;
 }
 
 void check_ready(wdg_ctx* c) {
@@ -761,7 +776,7 @@ struct SolveRun {
         c->stage[i] = x0[i] / xs;
       }
       HIPCHK(hipMemcpy(c->d_x.p, c->stage.data(), 2 * N * 8, hipMemcpyHostToDevice));
-    } else {
+    } else if (!(o.warm_device && c->have_solution)) {
       HIPCHK(hipMemsetAsync(c->d_x.p, 0, N * 16, c->stream));
     }
     tol = o.tol;
@@ -833,6 +848,116 @@ struct SolveRun {
     }
   }
 };
+
+// Gauss-Legendre rule on [-1, 1] by Newton iteration on P_n.
+void gauss_legendre(int n, std::vector<double>& x, std::vector<double>& w) {
+  x.resize(n);
+  w.resize(n);
+  for (int i = 0; i < n; ++i) {
+    double z = std::cos(M_PI * (i + 0.75) / (n + 0.5));
+    for (int it = 0; it < 100; ++it) {
+      double p0 = 1.0, p1 = z;
+      for (int k = 1; k < n; ++k) {
+        const double p2 = ((2.0 * k + 1.0) * z * p1 - k * p0) / (k + 1.0);
+        p0 = p1;
+        p1 = p2;
+      }
+      const double dp = n * (z * p1 - p0) / (z * z - 1.0);
+      const double dz = p1 / dp;
+      z -= dz;
+      if (std::abs(dz) < 1e-16) {
+        x[i] = z;
+        w[i] = 2.0 / ((1.0 - z * z) * dp * dp);
+        break;
+      }
+      x[i] = z;
+      w[i] = 2.0 / ((1.0 - z * z) * dp * dp);
+    }
+  }
+}
+
+void poisson_setup_impl(wdg_ctx* c, const wdg_poisson_desc& d) {
+  const Geometry& g = c->g;
+  const int nm = d.n_mat, n = g.n;
+  if (nm != g.n_chi * (n - 1) + 1) throw std::invalid_argument("n_mat must equal n_chi*(n_k_chi-1)+1");
+  if (!d.x_mat || !d.eps_r || !d.net_doping) throw std::invalid_argument("x_mat, eps_r and net_doping are required");
+  if (d.method != 0 && d.method != 1) throw std::invalid_argument("method must be 0 (direct) or 1 (newton)");
+  if (!(d.temp > 0)) throw std::invalid_argument("temperature must be positive");
+  const double dx = (d.x_mat[nm - 1] - d.x_mat[0]) / (nm - 1);
+  for (int i = 1; i < nm; ++i)
+    if (std::abs(d.x_mat[i] - d.x_mat[i - 1] - dx) > 1e-8 * dx)
+      throw std::invalid_argument("the Poisson grid must be uniform (RTD/eval_Poisson2D_rtd.m:202-209)");
+  for (int i = 0; i < nm; ++i)
+    if (!(d.eps_r[i] > 0)) throw std::invalid_argument("eps_r must be positive");
+  // projection rows: exact integrals of each element's Lagrange basis over
+  // the dual cells (8-point Gauss-Legendre, exact to degree 15)
+  std::vector<double> edges(nm + 1), widths(nm);
+  edges[0] = d.x_mat[0];
+  edges[nm] = d.x_mat[nm - 1];
+  for (int i = 1; i < nm; ++i) edges[i] = 0.5 * (d.x_mat[i - 1] + d.x_mat[i]);
+  for (int i = 0; i < nm; ++i) widths[i] = edges[i + 1] - edges[i];
+  std::vector<double> gx, gw;
+  gauss_legendre(8, gx, gw);
+  std::vector<int> ptr(nm + 1, 0), col;
+  std::vector<double> val;
+  const double h = g.l_chi / g.n_chi;
+  for (int r = 0; r < nm; ++r) {
+    for (int e = 0; e < g.n_chi; ++e) {
+      const double left = e * h, right = (e + 1) * h;
+      const double a = std::max(edges[r], left), b = std::min(edges[r + 1], right);
+      if (!(b > a)) continue;
+      for (int j = 0; j < n; ++j) {
+        double s = 0.0;
+        for (int q = 0; q < 8; ++q) {
+          const double xx = 0.5 * (a + b) + 0.5 * (b - a) * gx[q];
+          const double rr = (xx - 0.5 * (left + right)) / (0.5 * h);
+          double l = 1.0;
+          for (int m = 0; m < n; ++m)
+            if (m != j) l *= (rr - g.r[m]) / (g.r[j] - g.r[m]);
+          s += gw[q] * l;
+        }
+        col.push_back(e * n + j);
+        val.push_back(0.5 * (b - a) * s);
+      }
+    }
+    ptr[r + 1] = (int)col.size();
+  }
+  // operator rows (harmonic-mean permittivity, identity contact rows)
+  const double cx = 8.8541878128e-12 / (dx * dx);
+  std::vector<double> lo(nm, 0.0), up(nm, 0.0), dg(nm, 1.0);
+  for (int k = 1; k < nm - 1; ++k) {
+    const double e0 = d.eps_r[k];
+    lo[k] = cx * (2 * e0 * d.eps_r[k - 1] / (e0 + d.eps_r[k - 1]));
+    up[k] = cx * (2 * e0 * d.eps_r[k + 1] / (e0 + d.eps_r[k + 1]));
+    dg[k] = -(lo[k] + up[k]);
+  }
+  auto up_i = [&](DevBuf<int>& bb, const std::vector<int>& v) {
+    bb.alloc(v.size());
+    HIPCHK(hipMemcpy(bb.p, v.data(), v.size() * sizeof(int), hipMemcpyHostToDevice));
+  };
+  auto up_d = [&](DevBuf<double>& bb, const double* v, size_t nn) {
+    bb.alloc(nn);
+    HIPCHK(hipMemcpy(bb.p, v, nn * sizeof(double), hipMemcpyHostToDevice));
+  };
+  up_i(c->d_tptr, ptr);
+  up_i(c->d_tcol, col);
+  up_d(c->d_tval, val.data(), val.size());
+  up_d(c->d_pw, widths.data(), nm);
+  up_d(c->d_plo, lo.data(), nm);
+  up_d(c->d_pup, up.data(), nm);
+  up_d(c->d_pdg, dg.data(), nm);
+  up_d(c->d_pdop, d.net_doping, nm);
+  c->d_pin.alloc(nm);
+  c->d_pout.alloc(nm);
+  c->d_pndg.alloc(c->ncols);
+  c->d_pncv.alloc(nm);
+  c->d_pwork.alloc(3 * (size_t)nm);
+  c->d_pinfo.alloc(4);
+  c->p_nmat = nm;
+  c->p_method = d.method;
+  c->p_temp = d.temp;
+  c->have_poisson = true;
+}
 
 }  // namespace
 
@@ -1110,6 +1235,67 @@ int wdg_observables(wdg_ctx* ctx, const double* rho, const double* m_nodes, doub
     }
     if (rho_u)
       for (long i = 0; i < (long)nodes * g.n_xi * 2; ++i) rho_u[i] *= scale;
+    return WDG_OK;
+  });
+}
+
+int wdg_poisson_setup(wdg_ctx* ctx, const wdg_poisson_desc* desc) {
+  if (!ctx || !desc) return fail(ctx, WDG_ERR_ARG, "null argument");
+  return guarded(ctx, [&]() {
+    poisson_setup_impl(ctx, *desc);
+    return WDG_OK;
+  });
+}
+
+int wdg_poisson_step(wdg_ctx* ctx, const double* phi_in, double bias, double* phi_out, double* n_out,
+                     wdg_poisson_info* info) {
+  if (!ctx || !phi_in || !phi_out) return fail(ctx, WDG_ERR_ARG, "null argument");
+  if (!ctx->have_poisson) return fail(ctx, WDG_ERR_STATE, "wdg_poisson_setup has not been called");
+  if (!ctx->have_solution) return fail(ctx, WDG_ERR_STATE, "no solution available: call wdg_solve first");
+  return guarded(ctx, [&]() {
+    const int nm = ctx->p_nmat;
+    for (int i = 0; i < nm; ++i)
+      if (!std::isfinite(phi_in[i])) throw std::invalid_argument("phi_in must be finite");
+    HIPCHK(hipMemcpyAsync(ctx->d_pin.p, phi_in, nm * 8, hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(observables_kernel, dim3(ctx->ncols), dim3(256), 0, ctx->stream, ctx->d_x.p, ctx->d_phic.p,
+                       ctx->g.nk, ctx->d_obs.p);
+    PoissonDev P;
+    P.nmat = nm;
+    P.nodes = ctx->ncols;
+    P.method = ctx->p_method;
+    P.temp = ctx->p_temp;
+    P.q = 1.602176634e-19;   // RTD/solve_Poisson2D_rtd.m:24
+    P.kb = 1.380649e-23;     // RTD/solve_Poisson2D_rtd.m:25
+    P.scale_n = ctx->sol_scale;
+    P.obs = ctx->d_obs.p;
+    P.t_ptr = ctx->d_tptr.p;
+    P.t_col = ctx->d_tcol.p;
+    P.t_val = ctx->d_tval.p;
+    P.widths = ctx->d_pw.p;
+    P.lo = ctx->d_plo.p;
+    P.up = ctx->d_pup.p;
+    P.dg = ctx->d_pdg.p;
+    P.doping = ctx->d_pdop.p;
+    P.phi_in = ctx->d_pin.p;
+    P.bias = bias;
+    P.n_dg = ctx->d_pndg.p;
+    P.n_cv = ctx->d_pncv.p;
+    P.work = ctx->d_pwork.p;
+    P.phi_out = ctx->d_pout.p;
+    P.info = ctx->d_pinfo.p;
+    hipLaunchKernelGGL(poisson_newton_kernel, dim3(1), dim3(256), 0, ctx->stream, P);
+    HIPCHK(hipGetLastError());
+    double inf[4];
+    HIPCHK(hipMemcpyAsync(phi_out, ctx->d_pout.p, nm * 8, hipMemcpyDeviceToHost, ctx->stream));
+    if (n_out) HIPCHK(hipMemcpyAsync(n_out, ctx->d_pncv.p, nm * 8, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(inf, ctx->d_pinfo.p, 4 * 8, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (info) {
+      info->physical_relres_before = inf[0];
+      info->unmixed_step = inf[1];
+      info->charge_integral = inf[2];
+      info->dg_integral = inf[3];
+    }
     return WDG_OK;
   });
 }

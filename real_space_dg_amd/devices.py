@@ -84,3 +84,32 @@ def rtd_sweep(n_instances: int = 64, seed: int = 0, degree: int = 3,
         barrier = float(rng.uniform(*barrier_range))
         out.append(rtd_1d(ne, degree, nk, bias_v=bias, barrier_ev=barrier))
     return out
+
+
+def rtd_self_consistent(n_elem: int = 256, degree: int = 3, nk: int = 512, n_bias: int = 16,
+                        bias_max: float = 0.3, contact_nm: float = 20.0, nd_contact: float = 1e24,
+                        nd_intrinsic: float = 1e21, eps_r: float = 12.9, barrier_ev: float = 0.3,
+                        fermi_offset_ev: float = 0.05, length_nm: float = 60.0):
+    """Self-consistent RTD (Gummel loop) description.
+
+    Returns ``mat`` with the band-edge offset ``Vc`` [eV] (double barrier),
+    donor density ``Nd`` [m^-3] (n+ in the two contact regions of length
+    ``contact_nm``, ``nd_intrinsic`` elsewhere), ``eps`` (relative
+    permittivity), contact lengths ``Lc1``/``Lc2`` [nm], the bias list
+    ``Vdd`` and the Fermi-level offset of the reference default
+    EfL = Vc(1) + 0.05 eV, EfR = Vc(end) + 0.05 eV - bias
+    (RTD/solve2D_rtd_self_consistent.m:109-110).  The transport contacts
+    use the transverse-integrated (Tsu-Esaki) inflow so that the density is
+    a volume density comparable with Nd.
+    """
+    mat, _, _, _, _ = rtd_1d(n_elem, degree, nk, bias_v=0.0, barrier_ev=barrier_ev, length_nm=length_nm,
+                             fermi_model="TSU_ESAKI")
+    x = mat["x"]
+    mat["Vc"] = rtd_potential(x, length_nm, barrier_ev, bias_v=0.0)
+    mat["Nd"] = np.where((x <= contact_nm) | (x >= length_nm - contact_nm), nd_contact, nd_intrinsic)
+    mat["eps"] = np.full(x.size, eps_r)
+    mat["Lc1"] = contact_nm
+    mat["Lc2"] = contact_nm
+    mat["Vdd"] = np.linspace(0.0, bias_max, n_bias)
+    mat["fermi_offset"] = fermi_offset_ev
+    return mat

@@ -26,7 +26,8 @@ from . import _lib
 from ._lib import check, dptr
 
 M0 = 9.109e-31          # DG_code/solve_transport_DG_2D.m:7
-FERMI_MODELS = {"DG_MSA_3D": _lib.WDG_FERMI_MSA_3D, "DG_MSA_2D": _lib.WDG_FERMI_MSA_2D}
+FERMI_MODELS = {"DG_MSA_3D": _lib.WDG_FERMI_MSA_3D, "DG_MSA_2D": _lib.WDG_FERMI_MSA_2D,
+                "TSU_ESAKI": _lib.WDG_FERMI_TSU_ESAKI}
 PRECISIONS = {"fp64": _lib.WDG_PREC_FP64, "mixed": _lib.WDG_PREC_MIXED}
 
 
@@ -43,6 +44,7 @@ class SolveOptions:
     use_graph: bool = True          # replay restart cycles from captured HIP graphs
     orth: str = "cgs1"              # "cgs1" (single pass, default) or "cgs2" (DGKS second pass)
     restart_max: int = 0            # adaptive restart cap (0: fixed restart)
+    warm_device: bool = False       # start from the operator's last (device) solution
 
     def to_c(self) -> _lib.wdg_solve_opts:
         if self.precision not in PRECISIONS:
@@ -51,7 +53,7 @@ class SolveOptions:
                                    PRECISIONS[self.precision], float(self.inner_tol),
                                    int(self.max_outer), 1 if self.precondition else 0,
                                    1 if self.use_graph else 0, {"cgs2": 0, "cgs1": 1}[self.orth],
-                                   int(self.restart_max))
+                                   int(self.restart_max), 1 if self.warm_device else 0)
 
 
 def _as_c128(a, n):
@@ -176,6 +178,29 @@ class WignerOperator:
         ru = np.empty((nodes, self.n_xi), np.complex128) if want_rho_u else None
         check(self._lib.wdg_observables(self._h, dptr(r), dptr(m), dptr(n), dptr(j), dptr(ru)), self._h)
         return n, j, (None if ru is None else ru.T)
+
+    # ---- electrostatics (Gummel loop) ----
+    def poisson_setup(self, x_mat_m, eps_r, net_doping, temp: float = 300.0, method: str = "newton-raphson"):
+        x = np.ascontiguousarray(np.asarray(x_mat_m, float))
+        e = np.ascontiguousarray(np.asarray(eps_r, float))
+        d = np.ascontiguousarray(np.asarray(net_doping, float))
+        if not (x.shape == e.shape == d.shape):
+            raise ValueError("x_mat, eps_r and net_doping must have the same length")
+        meth = {"direct": 0, "newton-raphson": 1, "newton-rhapson": 1}[method]
+        self._poisson_keep = (x, e, d)
+        desc = _lib.wdg_poisson_desc(x.size, meth, dptr(x), dptr(e), dptr(d), float(temp))
+        check(self._lib.wdg_poisson_setup(self._h, C.byref(desc)), self._h)
+        self._n_mat = x.size
+
+    def poisson_step(self, phi, bias: float):
+        """Unmixed FP64 Poisson update from the last solution's density."""
+        phi = np.ascontiguousarray(np.asarray(phi, float))
+        out = np.empty_like(phi)
+        n_cv = np.empty_like(phi)
+        info = _lib.wdg_poisson_info()
+        check(self._lib.wdg_poisson_step(self._h, dptr(phi), float(bias), dptr(out), dptr(n_cv), C.byref(info)),
+              self._h)
+        return out, n_cv, {f: getattr(info, f) for f, _ in info._fields_}
 
     # ---- profiling ----
     def set_profiling(self, mask: int):
