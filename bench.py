@@ -47,10 +47,14 @@ def parse():
     ap.add_argument("--fp64-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-iters", type=int, default=6)
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the batched-sweep and self-consistent workloads reported beside the metric")
+    ap.add_argument("--calibrate-copy", type=float, default=0.0, metavar="BYTES",
+                    help="first run one plain device copy of BYTES bytes (HBM-counter calibration)")
     return ap.parse_args()
 
 
-def kernel_roofline(prof: dict, name: str, peak_kind: str) -> dict:
+def kernel_roofline(prof: dict, name: str, peak_kind: str, precision: str = "mixed") -> dict:
     st = prof[name]
     t_ms = st["avg_ms"]
     per_launch = st["units"] / max(st["launches"], 1)
@@ -62,18 +66,20 @@ def kernel_roofline(prof: dict, name: str, peak_kind: str) -> dict:
         ach = per_launch / (t_ms * 1e-3) / 1e9
         out = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS}
     out.update({"kernel": name, "avg_launch_ms": t_ms, "launches": st["launches"],
-                "algorithmic_per_launch": per_launch, "traffic": traffic_for(name)})
+                "algorithmic_per_launch": per_launch, "traffic": traffic_for(name, precision)})
     return out
 
 
-def traffic_for(name: str):
-    """HBM bytes per launch measured with rocprofv3 --pmc (profiles/traffic.json), if recorded."""
+def traffic_for(name: str, precision: str = "mixed"):
+    """HBM bytes per launch of the bench workload's kernel, measured with two
+    rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) and recorded in
+    profiles/traffic.json; None if not recorded."""
     f = ROOT / "profiles" / "traffic.json"
     if not f.exists():
         return None
     try:
         d = json.loads(f.read_text())
-        v = d.get("kernels", {}).get(name)
+        v = d.get("kernels_fp64" if precision == "fp64" else "kernels", {}).get(name)
         return None if v is None else v.get("bytes_per_launch")
     except (OSError, ValueError):
         return None
@@ -111,12 +117,89 @@ def cpu_baseline(mat, E, efl, efr, its_per_solve: int, sample_iters: int) -> dic
             "seconds_per_iteration": t_iter, "setup_seconds": t_setup}
 
 
+def copy_calibration(nbytes: float):
+    """One plain device-to-device copy of a known size (counter calibration)."""
+    import torch
+    n = int(nbytes) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.mul(a, 1.0, out=b)
+    torch.cuda.synchronize()
+    del a, b
+
+
+def batched_sweep(n_instances: int = 64, seed: int = 0) -> dict:
+    """64-instance bias sweep (uneven grids from ``seed``) solved by one
+    batched call per precision; solves/s over the whole batch."""
+    from real_space_dg_amd.devices import rtd_sweep
+    from real_space_dg_amd.solver import SolveOptions, WignerOperator, solve_batch
+    cases = rtd_sweep(n_instances, seed=seed)
+    ops = []
+    for mat, E, _, efl, efr in cases:
+        op = WignerOperator.from_mat(mat)
+        m = mat["me_x"][0, :, 0] * M0
+        op.set_potential(mat["x"] * 1e-9, E[0, :, 0], efl, efr, m[0], m[-1], temp=mat["Temp"])
+        ops.append(op)
+    out = {"workload": f"{n_instances} RTD instances, seed {seed}: bias U[0,0.4] V, elements in "
+                       "{128,256,384,512} (p=3), k in {256,512,768,1024}, barrier U[0.25,0.35] eV; "
+                       "one wdg_solve_batch call, tol 1e-10",
+           "unknowns_total": int(sum(op.N for op in ops))}
+    for prec in ("mixed", "fp64"):
+        opts = SolveOptions(tol=1e-10, precision=prec, restart=30, restart_max=64)
+        solve_batch(ops[:2], opts)   # graph/tile warmup
+        t0 = time.perf_counter()
+        infos = solve_batch(ops, opts)
+        dt = time.perf_counter() - t0
+        its = [i["iterations"] for i in infos]
+        out[prec] = {"solves_per_s": n_instances / dt, "seconds": dt, "converged": int(sum(i["flag"] == 0 for i in infos)),
+                     "iterations_mean": float(np.mean(its)), "iterations_max": int(max(its)),
+                     "true_relres_max": float(max(i["true_relres"] for i in infos))}
+        print(f"[bench] batched sweep {prec}: {dt:.2f} s", file=sys.stderr, flush=True)
+    out["mixed_over_fp64"] = out["mixed"]["solves_per_s"] / out["fp64"]["solves_per_s"]
+    for op in ops:
+        op.close()
+    return out
+
+
+def self_consistent(n_bias: int = 16) -> dict:
+    """Gummel loop of 16 bias points (256 elements p=3, 512 k), FP64 Poisson
+    Newton updates on the device, Anderson mixing, stop at 1e-6 V."""
+    from real_space_dg_amd.devices import rtd_self_consistent
+    from real_space_dg_amd.gummel import GummelOptions, solve_rtd_self_consistent
+    from real_space_dg_amd.solver import SolveOptions
+    mat = rtd_self_consistent(n_elem=256, degree=3, nk=512, n_bias=n_bias, bias_max=0.3)
+    out = {"workload": f"self-consistent RTD, {n_bias} biases in [0,0.3] V, 256 elements p=3 x 512 k, "
+                       "n+ 1e24 m^-3 in 20 nm contacts, 1e21 m^-3 elsewhere, eps_r 12.9, Gummel tol 1e-6 V"}
+    res = {}
+    for prec in ("mixed", "fp64"):
+        opt = GummelOptions(max_iterations=60, potential_tolerance=1e-6, acceleration="anderson", mixing=0.5)
+        opt.transport = SolveOptions(tol=1e-10, precision=prec, restart=30, restart_max=64)
+        r = solve_rtd_self_consistent(mat, opt)
+        res[prec] = r
+        out[prec] = {"seconds": r["seconds"], "bias_points_per_s": n_bias / r["seconds"],
+                     "converged": int(np.sum(r["converged"])), "gummel_iterations_max": int(np.max(r["iterations"])),
+                     "gmres_iterations_total": int(np.sum(r["gmres_iterations"]))}
+        print(f"[bench] self-consistent {prec}: {r['seconds']:.2f} s", file=sys.stderr, flush=True)
+    a, b = res["mixed"], res["fp64"]
+    out["mixed_vs_fp64"] = {
+        "phi_max_abs_diff_V": float(max(np.max(np.abs(x - y)) for x, y in zip(a["phi"], b["phi"]))),
+        "density_max_rel_diff": float(max(np.max(np.abs(x - y)) / np.max(np.abs(y))
+                                          for x, y in zip(a["density"], b["density"]))),
+        "current_max_rel_diff": float(np.max(np.abs(a["current"] - b["current"])) / np.max(np.abs(b["current"]))),
+    }
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (--gpus {args.gpus} requested); "
               "multi-GPU sharding is design-only.", file=sys.stderr)
         sys.exit(2)
+    if args.calibrate_copy > 0:
+        copy_calibration(args.calibrate_copy)
     from real_space_dg_amd.devices import rtd_1d
     from real_space_dg_amd.solver import PROFILE_BITS, SolveOptions, WignerOperator
 
@@ -198,15 +281,18 @@ def main():
         "roofline": roof,
         "wigner_apply_roofline": kernel_roofline(prof_mixed, "transport", "fp32"),
         "theta_gemm_roofline_mixed": kernel_roofline(prof_mixed, "theta_gemm1", "bf16"),
-        "theta_gemm_roofline_fp64": kernel_roofline(prof_fp64, "theta_gemm1", "fp64"),
+        "theta_gemm_roofline_fp64": kernel_roofline(prof_fp64, "theta_gemm1", "fp64", "fp64"),
         "kernels_mixed": kernels,
     }
+    op.close()
+    if not args.no_extras:
+        line["batched_sweep"] = batched_sweep()
+        line["self_consistent"] = self_consistent()
     if not args.no_cpu_baseline:
         print("[bench] CPU baseline sample", file=sys.stderr, flush=True)
         line["cpu_baseline"] = cpu_baseline(mat, E, efl, efr, info_d["iterations"], args.cpu_sample_iters)
     else:
         line["cpu_baseline"] = None
-    op.close()
     print(json.dumps(line))
 
 

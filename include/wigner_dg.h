@@ -201,6 +201,19 @@ int wdg_precond_device(wdg_ctx* ctx, const void* d_r, void* d_z, int32_t precisi
  * Either output may be NULL. */
 int wdg_host_setup(const wdg_desc* desc, const wdg_potential* pot, double* rhs, double* kmats);
 
+/* Formulation of the Theta[V] apply.  WDG_THETA_FFT evaluates phi U and
+ * i phi^H Z as chirp-z transforms (FFTs of length 2^p 3^q >= nk+n_xi-1) in
+ * one fused kernel per element; WDG_THETA_GEMM runs them as dense batched
+ * MFMA GEMMs (split-BF16 in the mixed inner solver, FP64 otherwise).  Both
+ * compute the same operator as get_Drift_accel_trial_permutedShell.m:95-151
+ * (G_q = i phi^H C_q phi); AUTO (the default) picks FFT when the grid allows
+ * it.  No counterpart in the reference: an implementation choice.
+ * Returns WDG_ERR_ARG if FFT is requested for a grid it does not apply to. */
+enum { WDG_THETA_AUTO = 0, WDG_THETA_FFT = 1, WDG_THETA_GEMM = 2 };
+int wdg_set_theta_method(wdg_ctx* ctx, int32_t method);
+/* The formulation in use (WDG_THETA_FFT or WDG_THETA_GEMM), -1 for NULL. */
+int wdg_theta_method(const wdg_ctx* ctx);
+
 /* Enable per-kernel timing with HIP events on the context stream for the
  * kernels selected by mask (bit i = entry i of wdg_profile's list); 0 turns
  * it off.  Resets the accumulated statistics. */

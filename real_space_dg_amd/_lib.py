@@ -18,6 +18,9 @@ WDG_OK = 0
 WDG_FERMI_MSA_3D = 0
 WDG_FERMI_MSA_2D = 1
 WDG_FERMI_TSU_ESAKI = 2
+WDG_THETA_AUTO = 0
+WDG_THETA_FFT = 1
+WDG_THETA_GEMM = 2
 WDG_PREC_FP64 = 0
 WDG_PREC_MIXED = 1
 
@@ -96,6 +99,8 @@ SIGNATURES = {
     "wdg_apply_device": (C.c_int, [_P, _P, _P, C.c_int32]),
     "wdg_precond_device": (C.c_int, [_P, _P, _P, C.c_int32]),
     "wdg_host_setup": (C.c_int, [C.POINTER(wdg_desc), C.POINTER(wdg_potential), _D, _D]),
+    "wdg_set_theta_method": (C.c_int, [_P, C.c_int32]),
+    "wdg_theta_method": (C.c_int, [_P]),
     "wdg_set_profiling": (C.c_int, [_P, C.c_uint32]),
     "wdg_profile": (C.c_int, [_P, C.c_char_p, C.c_int32, _D, C.POINTER(C.c_int64), _D, C.c_int32,
                               C.POINTER(C.c_int32)]),

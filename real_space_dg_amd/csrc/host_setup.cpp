@@ -323,6 +323,120 @@ void build_geometry(const wdg_desc& d, Geometry& g) {
   g.scale = g.q_diff / g.jac * kmax * k1max;
 }
 
+namespace {
+
+bool smooth23(int p) {
+  while (p % 2 == 0) p /= 2;
+  while (p % 3 == 0) p /= 3;
+  return p == 1;
+}
+
+std::vector<int> fft_radices(int P) {
+  std::vector<int> r;
+  if (P == 1) return r;
+  while (P % 8 == 0) { r.push_back(8); P /= 8; }
+  while (P % 4 == 0) { r.push_back(4); P /= 4; }
+  while (P % 2 == 0) { r.push_back(2); P /= 2; }
+  while (P % 3 == 0) { r.push_back(3); P /= 3; }
+  if (P != 1) throw std::runtime_error("FFT length must be of the form 2^p 3^q");
+  return r;
+}
+
+// exp(i pi s t^2), the phase reduced modulo 2 pi before the trigonometry.
+cplx chirp(double s, double t) {
+  const double ph = std::fmod(s * t * t, 2.0);
+  return cplx(std::cos(M_PI * ph), std::sin(M_PI * ph));
+}
+
+}  // namespace
+
+void fft_host(std::vector<cplx>& x, bool inverse) {
+  const int P = (int)x.size();
+  const std::vector<int> rad = fft_radices(P);
+  std::vector<cplx> y(P);
+  const double sgn = inverse ? 1.0 : -1.0;
+  int Ns = 1;
+  for (int R : rad) {
+    const int nb = P / R;
+    std::vector<cplx> v(R), o(R);
+    for (int b = 0; b < nb; ++b) {
+      const int jm = b % Ns;
+      for (int r = 0; r < R; ++r) {
+        const double ang = sgn * 2.0 * M_PI * (double)jm * r / ((double)Ns * R);
+        v[r] = x[b + r * nb] * cplx(std::cos(ang), std::sin(ang));
+      }
+      for (int s = 0; s < R; ++s) {
+        cplx acc = 0;
+        for (int r = 0; r < R; ++r) {
+          const double ang = sgn * 2.0 * M_PI * (double)((r * s) % R) / R;
+          acc += v[r] * cplx(std::cos(ang), std::sin(ang));
+        }
+        o[s] = acc;
+      }
+      const int base = (b / Ns) * Ns * R + jm;
+      for (int r = 0; r < R; ++r) y[base + r * Ns] = o[r];
+    }
+    x.swap(y);
+    Ns *= R;
+  }
+}
+
+void build_chirp(const Geometry& g, ChirpPlan& cp) {
+  cp = ChirpPlan();
+  const int nk = g.nk, nx = g.n_xi;
+  const double L = g.l_xi;
+  const double hp = (g.l_xi - g.delta_xi) / (nx - 1);
+  const double beta = hp / L;
+  const double ck = 0.5 * (nk - 1), cx = 0.5 * (nx - 1);
+  for (int j = 0; j < nk; ++j)
+    if (std::abs(g.k[j] + 2.0 * M_PI / L * (j - ck)) > 1e-12 * 2.0 * M_PI / L * nk) return;
+  for (int l = 0; l < nx; ++l)
+    if (std::abs(g.y[l] - hp * (l - cx)) > 1e-12 * L) return;
+  // P = 64 m (m = 2^a 3^b): the device transform starts and ends with a
+  // radix-8 pass around the radices of m.
+  int P = nk + nx - 1;
+  while (P % 64 != 0 || !smooth23(P / 64)) ++P;
+  cp.P = P;
+  cp.rad.push_back(8);
+  for (int r : fft_radices(P / 64)) cp.rad.push_back(r);
+  cp.rad.push_back(8);
+  if (P / 64 == 1) cp.rad = {8, 8};
+  {
+    int Ns = 1;
+    for (int R : cp.rad) {
+      cp.tw_off.push_back((int)cp.tw.size());
+      for (int jm = 0; jm < Ns; ++jm) {
+        const double ang = -2.0 * M_PI * (double)jm / ((double)Ns * R);
+        cp.tw.emplace_back(std::cos(ang), std::sin(ang));
+      }
+      Ns *= R;
+    }
+  }
+  const double norm = 1.0 / (std::sqrt(L) * P);
+  cp.a_pre.resize(nk);
+  cp.a_post.resize(nk);
+  for (int j = 0; j < nk; ++j) {
+    const double u = j - ck;
+    cp.a_pre[j] = chirp(-beta, u);
+    cp.a_post[j] = cplx(0.0, 1.0) * chirp(beta, u) * norm;
+  }
+  cp.w_post.resize(nx);
+  cp.b_pre.resize(nx);
+  for (int l = 0; l < nx; ++l) {
+    const double v = l - cx;
+    cp.w_post[l] = chirp(-beta, v) * norm;
+    cp.b_pre[l] = chirp(beta, v);
+  }
+  const double dl = 0.5 * (nk - nx);
+  cp.g1.assign(P, cplx(0.0, 0.0));
+  for (int m = -(nk - 1); m <= nx - 1; ++m) cp.g1[(m + P) % P] = chirp(beta, m + dl);
+  cp.g2.assign(P, cplx(0.0, 0.0));
+  for (int m = -(nx - 1); m <= nk - 1; ++m) cp.g2[(m + P) % P] = chirp(-beta, m - dl);
+  fft_host(cp.g1, false);
+  fft_host(cp.g2, false);
+  cp.ok = true;
+}
+
 void inflow(const Geometry& g, const wdg_potential& pot, std::vector<cplx>& fnl,
             std::vector<cplx>& fnr) {
   const double hb = kHVT / 2.0 / M_PI;

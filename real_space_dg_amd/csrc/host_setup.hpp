@@ -49,8 +49,36 @@ struct Geometry {
   std::vector<int> phi_piv;
 };
 
+// Chirp-z factorisation of the plane-wave basis:
+//   phi[l][j] = exp(-i pi beta u_j^2) exp(-i pi beta v_l^2) exp(i pi beta (v_l-u_j)^2) / sqrt(L)
+// with u_j = j-(nk-1)/2, v_l = l-(n_xi-1)/2, beta = h'/L, so that phi U and
+// i phi^H Z are circular convolutions of length P (= 2^p 3^q >= nk+n_xi-1)
+// between chirp-multiplied vectors.  All tables are FP64.
+struct ChirpPlan {
+  bool ok = false;            // the grids have the structure above
+  int P = 0;
+  std::vector<int> rad;       // radix sequence of the length-P FFT (8, 4, 2, 3)
+  // Stockham base twiddles per pass: pass p (stride Ns, radix R) holds
+  // exp(-2 pi i jm/(Ns R)) for jm in [0, Ns) at tw[tw_off[p] + jm].
+  std::vector<int> tw_off;
+  std::vector<cplx> tw;
+  std::vector<cplx> a_pre;    // nk: exp(-i pi beta u^2)
+  std::vector<cplx> w_post;   // n_xi: exp(-i pi beta v^2)/(sqrt(L) P)
+  std::vector<cplx> g1;       // P: FFT of exp(i pi beta m^2), m in [-(nk-1), n_xi-1]
+  std::vector<cplx> b_pre;    // n_xi: exp(+i pi beta v^2)
+  std::vector<cplx> a_post;   // nk: i exp(+i pi beta u^2)/(sqrt(L) P)
+  std::vector<cplx> g2;       // P: FFT of exp(-i pi beta m^2), m in [-(n_xi-1), nk-1]
+};
+
 // Throws std::runtime_error on invalid input.
 void build_geometry(const wdg_desc& d, Geometry& g);
+
+// Fills cp (cp.ok = false when the grids are not of the chirp form).
+void build_chirp(const Geometry& g, ChirpPlan& cp);
+
+// In-place FFT of length x.size() (a 2^p 3^q length): forward
+// X_k = sum_m x_m exp(-2 pi i k m / P), or the unnormalised inverse.
+void fft_host(std::vector<cplx>& x, bool inverse);
 
 // Inflow distributions fnl, fnr (k basis), DG_code/core/verteilung.m:77-170.
 void inflow(const Geometry& g, const wdg_potential& pot, std::vector<cplx>& fnl,

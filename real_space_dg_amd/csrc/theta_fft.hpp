@@ -1,0 +1,483 @@
+// Theta[V] through chirp-z transforms (gfx950).
+//
+// The plane-wave basis of the FV-in-xi branch samples
+//   phi[l][j] = exp(i k_j y_l) / sqrt(L),  k_j = -(2 pi / L) u_j,  y_l = h' v_l,
+//   u_j = j - (nk-1)/2,  v_l = l - (n_xi-1)/2
+// (DG_code/core/Basisfunktionen_Q_m.m:22-29, DG_code/initParams.m:178-183), so
+// with beta = h'/L and u v = (u^2 + v^2 - (v-u)^2)/2
+//   phi U   = w o ( g  (*) (a o U) )     (Bluestein / chirp-z)
+//   i phi^H Z = a' o ( g^ (*) (b o Z) )
+// where (*) is a linear convolution evaluated as a circular one of length
+// P >= nk + n_xi - 1 through FFTs of size P = 2^p 3^q, and the chirps a, w,
+// b, a' and the spectra G1 = FFT(g), G2 = FFT(g^) are tables built in FP64
+// on the host.  The two dense k x xi contractions of the GEMM formulation
+// (16 n_xi nk flops per DG node) become four FFTs of size P per node.
+//
+// theta_fft_kernel fuses, per element e (one workgroup):
+//   T_b = phi U_{e,b}            for the n nodes b    (n forward chirp-z)
+//   Z_a = sum_b Cbar_ab (*) T_b  tridiagonal in l     (the Theta combine)
+//   Y_{e,a} += i phi^H Z_a       for the n nodes a    (n backward chirp-z)
+// with T held in LDS, so HBM sees one read of u, one read-modify-write of y
+// and the edge table.  FFTs are Stockham autosort passes whose first and
+// last passes keep the data in registers (see fft_regs), so the chirp and
+// spectrum multiplies and the HBM traffic need no extra LDS round trips.
+#pragma once
+
+#include "kernels.hpp"
+
+namespace wdg {
+
+constexpr int kFftThreads = 256;
+constexpr int kFftMaxPasses = 16;
+constexpr int kMaxLdsBytes = 160 * 1024;   // LDS per workgroup on gfx950
+
+struct FftPlan {
+  int P;
+  int npass;
+  int ntw;                    // entries of the twiddle table
+  int rad[kFftMaxPasses];
+  int toff[kFftMaxPasses];    // per-pass offset into the twiddle table
+};
+
+// Pointers into LDS carry the LDS address space explicitly: a generic
+// pointer passed through the transform helpers and swapped between passes
+// compiles to flat memory instructions instead of ds_read/ds_write.
+// HIP's float2/double2 are classes whose operators take generic references,
+// so LDS holds clang ext-vector pairs and lds_ld/lds_st convert.
+#define WDG_LDS __attribute__((address_space(3)))
+template <typename T2>
+struct LdsVec;
+template <>
+struct LdsVec<float2> {
+  typedef float type __attribute__((ext_vector_type(2)));
+};
+template <>
+struct LdsVec<double2> {
+  typedef double type __attribute__((ext_vector_type(2)));
+};
+template <typename T2>
+using LdsP = WDG_LDS typename LdsVec<T2>::type*;
+
+template <typename T2>
+__device__ __forceinline__ T2 lds_ld(LdsP<T2> p, int i) {
+  const typename LdsVec<T2>::type w = p[i];
+  T2 r;
+  r.x = w.x;
+  r.y = w.y;
+  return r;
+}
+template <typename T2>
+__device__ __forceinline__ void lds_st(LdsP<T2> p, int i, T2 v) {
+  typename LdsVec<T2>::type w;
+  w.x = v.x;
+  w.y = v.y;
+  p[i] = w;
+}
+
+// LDS work buffers are padded by one element every 16 so that the strided
+// Stockham stores of the early passes spread over the banks.
+__device__ __forceinline__ int lds_pad(int i) { return i + (i >> 4); }
+__host__ __device__ constexpr int padded_len(int P) { return P + (P >> 4) + 1; }
+
+template <typename T2>
+struct ThetaTabs {
+  const T2* tw;      // per-pass base twiddles exp(-2 pi i jm / (Ns R))
+  const T2* a_pre;   // nk: exp(-i pi beta u^2)
+  const T2* w_post;  // n_xi: exp(-i pi beta v^2) / sqrt(L) / P
+  const T2* g1;      // P: FFT of the forward chirp filter
+  const T2* b_pre;   // n_xi: exp(+i pi beta v^2)
+  const T2* a_post;  // nk: i exp(+i pi beta u^2) / sqrt(L) / P
+  const T2* g2;      // P: FFT of the backward chirp filter
+};
+
+template <typename T2>
+__device__ __forceinline__ T2 c2mul(T2 a, T2 b) {
+  T2 r;
+  r.x = a.x * b.x - a.y * b.y;
+  r.y = a.x * b.y + a.y * b.x;
+  return r;
+}
+template <typename T2>
+__device__ __forceinline__ T2 c2add(T2 a, T2 b) {
+  T2 r;
+  r.x = a.x + b.x;
+  r.y = a.y + b.y;
+  return r;
+}
+template <typename T2>
+__device__ __forceinline__ T2 c2sub(T2 a, T2 b) {
+  T2 r;
+  r.x = a.x - b.x;
+  r.y = a.y - b.y;
+  return r;
+}
+// multiply by -i (forward transform) or +i (inverse transform)
+template <bool INV, typename T2>
+__device__ __forceinline__ T2 c2rot(T2 a) {
+  T2 r;
+  if (INV) {
+    r.x = -a.y;
+    r.y = a.x;
+  } else {
+    r.x = a.y;
+    r.y = -a.x;
+  }
+  return r;
+}
+
+template <int R, bool INV, typename T2>
+struct SmallDft;
+
+template <bool INV, typename T2>
+struct SmallDft<2, INV, T2> {
+  __device__ __forceinline__ static void run(T2 (&v)[2]) {
+    const T2 a = v[0], b = v[1];
+    v[0] = c2add(a, b);
+    v[1] = c2sub(a, b);
+  }
+};
+
+template <bool INV, typename T2>
+struct SmallDft<3, INV, T2> {
+  __device__ __forceinline__ static void run(T2 (&v)[3]) {
+    using S = decltype(v[0].x);
+    const T2 t = c2add(v[1], v[2]);
+    T2 s = c2sub(v[1], v[2]);
+    const S h = (S)0.86602540378443864676;   // sqrt(3)/2
+    s.x *= h;
+    s.y *= h;
+    T2 m;
+    m.x = v[0].x - (S)0.5 * t.x;
+    m.y = v[0].y - (S)0.5 * t.y;
+    v[0] = c2add(v[0], t);
+    const T2 rs = c2rot<INV>(s);
+    v[1] = c2add(m, rs);
+    v[2] = c2sub(m, rs);
+  }
+};
+
+template <bool INV, typename T2>
+__device__ __forceinline__ void dft4(T2& v0, T2& v1, T2& v2, T2& v3) {
+  const T2 t0 = c2add(v0, v2), t1 = c2sub(v0, v2);
+  const T2 t2 = c2add(v1, v3), t3 = c2rot<INV>(c2sub(v1, v3));
+  v0 = c2add(t0, t2);
+  v1 = c2add(t1, t3);
+  v2 = c2sub(t0, t2);
+  v3 = c2sub(t1, t3);
+}
+
+template <bool INV, typename T2>
+struct SmallDft<4, INV, T2> {
+  __device__ __forceinline__ static void run(T2 (&v)[4]) { dft4<INV>(v[0], v[1], v[2], v[3]); }
+};
+
+template <bool INV, typename T2>
+struct SmallDft<8, INV, T2> {
+  __device__ __forceinline__ static void run(T2 (&v)[8]) {
+    using S = decltype(v[0].x);
+    T2 e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
+    T2 o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
+    dft4<INV>(e0, e1, e2, e3);
+    dft4<INV>(o0, o1, o2, o3);
+    const S r = (S)0.70710678118654752440;
+    // o1 *= W8, o2 *= W8^2, o3 *= W8^3 with W8 = exp(-+ i pi/4)
+    T2 w1, w3;
+    if (INV) {
+      w1.x = (o1.x - o1.y) * r; w1.y = (o1.x + o1.y) * r;
+      w3.x = (-o3.x - o3.y) * r; w3.y = (o3.x - o3.y) * r;
+    } else {
+      w1.x = (o1.x + o1.y) * r; w1.y = (o1.y - o1.x) * r;
+      w3.x = (o3.y - o3.x) * r; w3.y = (-o3.x - o3.y) * r;
+    }
+    const T2 w2 = c2rot<INV>(o2);
+    v[0] = c2add(e0, o0); v[4] = c2sub(e0, o0);
+    v[1] = c2add(e1, w1); v[5] = c2sub(e1, w1);
+    v[2] = c2add(e2, w2); v[6] = c2sub(e2, w2);
+    v[3] = c2add(e3, w3); v[7] = c2sub(e3, w3);
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Register-ended Stockham FFT of length P = 64 M, run by a group of TT = P/8
+// threads.  The first pass (Ns = 1) and the last pass (Ns = P/8) are radix 8,
+// so thread t owns the elements t + TT r (r = 0..7) both at the input of the
+// first pass and at the output of the last pass: a transform takes v[8] in
+// registers and returns v[8] in registers, and pointwise factors between
+// transforms never touch LDS.  Middle passes (radices 8/4/3/2) run between
+// two padded LDS buffers X, Y owned by the group.  Every pass but the last
+// ends with a workgroup barrier (all groups of a workgroup step together).
+// ---------------------------------------------------------------------------
+template <int R, bool INV, typename T2>
+__device__ __forceinline__ void fft_mid_pass(LdsP<T2> X, LdsP<T2> Y, LdsP<T2> tw, int P, int Ns, int t,
+                                             int TT) {
+  const int nb = P / R;
+  // Ns is a power of two unless a radix-3 pass precedes this one
+  const bool pow2 = (Ns & (Ns - 1)) == 0;
+  const int lg = __ffs(Ns) - 1;
+  for (int b = t; b < nb; b += TT) {
+    const int q = pow2 ? (b >> lg) : (b / Ns);
+    const int jm = b - q * Ns;
+    T2 v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[r] = lds_ld<T2>(X, lds_pad(b + r * nb));
+    T2 w = lds_ld<T2>(tw, jm);
+    if (INV) w.y = -w.y;
+    T2 wr = w;
+    v[1] = c2mul(v[1], w);
+#pragma unroll
+    for (int r = 2; r < R; ++r) {
+      wr = c2mul(wr, w);
+      v[r] = c2mul(v[r], wr);
+    }
+    SmallDft<R, INV, T2>::run(v);
+    const int base = q * Ns * R + jm;
+#pragma unroll
+    for (int r = 0; r < R; ++r) lds_st<T2>(Y, lds_pad(base + r * Ns), v[r]);
+  }
+}
+
+// The group's two work buffers are buf[0], buf[1]; *cur names the one the
+// next transform's first pass writes (never the one the previous
+// transform's last pass read, so no extra barrier is needed between
+// transforms).
+template <bool INV, typename T2>
+__device__ __forceinline__ void fft_regs(T2 (&v)[8], LdsP<T2> buf0, LdsP<T2> buf1, int& cur, LdsP<T2> tw,
+                                         const FftPlan& pl, int t, int TT) {
+  const int P = pl.P;
+  // first pass: radix 8, Ns = 1, butterfly t, outputs 8 t + r
+  SmallDft<8, INV, T2>::run(v);
+  LdsP<T2> X = cur ? buf1 : buf0;
+  LdsP<T2> Y = cur ? buf0 : buf1;
+#pragma unroll
+  for (int r = 0; r < 8; ++r) lds_st<T2>(X, lds_pad(8 * t + r), v[r]);
+  __syncthreads();
+  int Ns = 8;
+  for (int p = 1; p < pl.npass - 1; ++p) {
+    const int R = pl.rad[p];
+    LdsP<T2> twp = tw + pl.toff[p];
+    if (R == 8) fft_mid_pass<8, INV, T2>(X, Y, twp, P, Ns, t, TT);
+    else if (R == 4) fft_mid_pass<4, INV, T2>(X, Y, twp, P, Ns, t, TT);
+    else if (R == 3) fft_mid_pass<3, INV, T2>(X, Y, twp, P, Ns, t, TT);
+    else fft_mid_pass<2, INV, T2>(X, Y, twp, P, Ns, t, TT);
+    __syncthreads();
+    Ns *= R;
+    LdsP<T2> tmp = X;
+    X = Y;
+    Y = tmp;
+  }
+  // last pass: radix 8, Ns = P/8 = TT, butterfly t (jm = t), elements t + TT r
+#pragma unroll
+  for (int r = 0; r < 8; ++r) v[r] = lds_ld<T2>(X, lds_pad(t + TT * r));
+  T2 w = lds_ld<T2>(tw, pl.toff[pl.npass - 1] + t);
+  if (INV) w.y = -w.y;
+  T2 wr = w;
+  v[1] = c2mul(v[1], w);
+#pragma unroll
+  for (int r = 2; r < 8; ++r) {
+    wr = c2mul(wr, w);
+    v[r] = c2mul(v[r], wr);
+  }
+  SmallDft<8, INV, T2>::run(v);
+  cur = (X == buf0) ? 1 : 0;
+}
+
+// y += Theta u for the elements of the grid (one workgroup per element).
+// blockDim = G * TT (G groups of TT = P/8 threads, each group transforms one
+// DG node's vector at a time).  Dynamic LDS: n*n_xi (T, then Z), 2 G padded
+// work buffers and the twiddle table, in complex values of type T2.
+template <int NN, typename T, typename TS>
+__global__ __launch_bounds__(kFftThreads) void theta_fft_kernel(const OpConst* __restrict__ oc, FftPlan pl,
+                                                                ThetaTabs<typename cvec<T>::type> tb,
+                                                                const TS* __restrict__ sbar,
+                                                                const double* __restrict__ cap,
+                                                                const typename cvec<T>::type* __restrict__ u,
+                                                                typename cvec<T>::type* __restrict__ y,
+                                                                const GmresState* st) {
+  using T2 = typename cvec<T>::type;
+  if (skip(st)) return;
+  extern __shared__ __align__(16) unsigned char theta_smem[];
+  constexpr int np = NN * (NN + 1) / 2;
+  const int nk = oc->nk, nx = oc->n_xi, nl = nx + 1, P = pl.P;
+  const int TT = P >> 3, G = blockDim.x / TT;
+  const int g = threadIdx.x / TT, t = threadIdx.x % TT;
+  const int e = blockIdx.x;
+  const int PL = padded_len(P);
+  LdsP<T2> Tst = (LdsP<T2>)theta_smem;
+  LdsP<T2> buf0 = Tst + NN * nx + 2 * g * PL;
+  LdsP<T2> buf1 = buf0 + PL;
+  LdsP<T2> twl = Tst + NN * nx + 2 * G * PL;
+  for (int i = threadIdx.x; i < pl.ntw; i += blockDim.x) lds_st<T2>(twl, i, tb.tw[i]);
+  int cur = 0;
+  T2 zero;
+  zero.x = 0;
+  zero.y = 0;
+  // this thread's table entries (elements t + TT r)
+  T2 a_pre[8], g1[8], w_post[8];
+#pragma unroll
+  for (int r = 0; r < 8; ++r) {
+    const int i = t + TT * r;
+    a_pre[r] = (i < nk) ? tb.a_pre[i] : zero;
+    g1[r] = tb.g1[i];
+    w_post[r] = (i < nx) ? tb.w_post[i] : zero;
+  }
+  __syncthreads();
+
+  // forward: T_b = w o IFFT(G1 o FFT(a o U_b)) for b = g, g + G, ...
+  for (int b0 = 0; b0 < NN; b0 += G) {
+    const int b = b0 + g;
+    const bool act = b < NN;
+    const T2* __restrict__ ub = u + ((long)e * NN + (act ? b : 0)) * nk;
+    T2 v[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      const int i = t + TT * r;
+      v[r] = (act && i < nk) ? c2mul(a_pre[r], ub[i]) : zero;
+    }
+    fft_regs<false, T2>(v, buf0, buf1, cur, twl, pl, t, TT);
+#pragma unroll
+    for (int r = 0; r < 8; ++r) v[r] = c2mul(v[r], g1[r]);
+    fft_regs<true, T2>(v, buf0, buf1, cur, twl, pl, t, TT);
+    if (act) {
+#pragma unroll
+      for (int r = 0; r < 8; ++r) {
+        const int i = t + TT * r;
+        if (i < nx) lds_st<T2>(Tst, b * nx + i, c2mul(w_post[r], v[r]));
+      }
+    }
+  }
+  __syncthreads();
+
+  // combine: Z_a(l) = sum_b [dg_ab(l) T_b(l) + Bbar_ab(l+1) T_b(l+1) + Bbar_ab(l) T_b(l-1)],
+  // dg = Bbar(l) + Bbar(l+1), Bbar_ab(l) = c_ab (S_ab(l) - i CAP(l) Mq_ab).  With the
+  // edge sums A_b = T_b(l) + T_b(l-1), C_b = T_b(l) + T_b(l+1):
+  //   Z_a = sum_b c_ab (S_ab(l) A_b + S_ab(l+1) C_b) - i sum_b c_ab Mq_ab (CAP(l) A_b + CAP(l+1) C_b).
+  // All a at once per l (in the kernel's precision T), then Z overwrites T.
+  constexpr int kMaxL = 4;   // l values per thread (the host guarantees n_xi <= 4 blockDim)
+  T2 zreg[kMaxL][NN];
+  const TS* sb = sbar + (long)e * np * nl;
+#pragma unroll
+  for (int k = 0; k < kMaxL; ++k) {
+    const int l = threadIdx.x + k * blockDim.x;
+    if (l >= nx) continue;
+    T sv0[np], sv1[np];
+#pragma unroll
+    for (int p = 0; p < np; ++p) {
+      sv0[p] = (T)sb[(long)p * nl + l];
+      sv1[p] = (T)sb[(long)p * nl + l + 1];
+    }
+    const T c0 = (T)cap[l], c1 = (T)cap[l + 1];
+    T2 A[NN], Cs[NN];
+#pragma unroll
+    for (int bb = 0; bb < NN; ++bb) {
+      const T2 t1 = lds_ld<T2>(Tst, bb * nx + l);
+      const T2 t0 = (l > 0) ? lds_ld<T2>(Tst, bb * nx + l - 1) : zero;
+      const T2 t2 = (l < nx - 1) ? lds_ld<T2>(Tst, bb * nx + l + 1) : zero;
+      A[bb] = c2add(t1, t0);
+      Cs[bb] = c2add(t1, t2);
+    }
+#pragma unroll
+    for (int a = 0; a < NN; ++a) {
+      T pr = 0, pi = 0, qr = 0, qi = 0;   // potential part; CAP part before the factor -i
+#pragma unroll
+      for (int bb = 0; bb < NN; ++bb) {
+        const int lo = a < bb ? a : bb, hi = a < bb ? bb : a;
+        const int p = lo * NN - lo * (lo - 1) / 2 + (hi - lo);
+        const T c = (T)oc->cab[a * kMaxN + bb];
+        const T cm = c * (T)oc->Mq[a * kMaxN + bb];
+        const T f0 = c * sv0[p], f1 = c * sv1[p];
+        pr += f0 * A[bb].x + f1 * Cs[bb].x;
+        pi += f0 * A[bb].y + f1 * Cs[bb].y;
+        const T g0 = cm * c0, g1c = cm * c1;
+        qr += g0 * A[bb].x + g1c * Cs[bb].x;
+        qi += g0 * A[bb].y + g1c * Cs[bb].y;
+      }
+      // Z = P - i Q
+      zreg[k][a].x = pr + qi;
+      zreg[k][a].y = pi - qr;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < kMaxL; ++k) {
+    const int l = threadIdx.x + k * blockDim.x;
+    if (l >= nx) continue;
+#pragma unroll
+    for (int a = 0; a < NN; ++a) lds_st<T2>(Tst, a * nx + l, zreg[k][a]);
+  }
+  // this thread's backward table entries
+  T2 b_pre[8], g2[8], a_post[8];
+#pragma unroll
+  for (int r = 0; r < 8; ++r) {
+    const int i = t + TT * r;
+    b_pre[r] = (i < nx) ? tb.b_pre[i] : zero;
+    g2[r] = tb.g2[i];
+    a_post[r] = (i < nk) ? tb.a_post[i] : zero;
+  }
+  __syncthreads();
+
+  // backward: Y_a += a' o IFFT(G2 o FFT(b o Z_a)) for a = g, g + G, ...
+  for (int a0 = 0; a0 < NN; a0 += G) {
+    const int a = a0 + g;
+    const bool act = a < NN;
+    T2* __restrict__ yb = y + ((long)e * NN + (act ? a : 0)) * nk;
+    T2 v[8], yo[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      const int i = t + TT * r;
+      v[r] = (act && i < nx) ? c2mul(b_pre[r], lds_ld<T2>(Tst, a * nx + i)) : zero;
+      yo[r] = (act && i < nk) ? yb[i] : zero;
+    }
+    fft_regs<false, T2>(v, buf0, buf1, cur, twl, pl, t, TT);
+#pragma unroll
+    for (int r = 0; r < 8; ++r) v[r] = c2mul(v[r], g2[r]);
+    fft_regs<true, T2>(v, buf0, buf1, cur, twl, pl, t, TT);
+    if (act) {
+#pragma unroll
+      for (int r = 0; r < 8; ++r) {
+        const int j = t + TT * r;
+        if (j < nk) yb[j] = c2add(yo[r], c2mul(a_post[r], v[r]));
+      }
+    }
+  }
+}
+
+// out[c][l] = (phi u_c)(l) in FP64 (the rho_u rows of the observables).
+// blockDim = G * TT; group g of workgroup w transforms node w G + g.
+// Dynamic LDS: 2 G padded work buffers and the twiddle table.
+__global__ __launch_bounds__(kFftThreads) void phi_fft_kernel(FftPlan pl, ThetaTabs<double2> tb,
+                                                              const double2* __restrict__ u, int nodes, int nk,
+                                                              int nx, double2* __restrict__ out) {
+  extern __shared__ __align__(16) unsigned char phi_smem[];
+  const int P = pl.P, TT = P >> 3, G = blockDim.x / TT;
+  const int g = threadIdx.x / TT, t = threadIdx.x % TT;
+  const int PL = padded_len(P);
+  LdsP<double2> base = (LdsP<double2>)phi_smem;
+  LdsP<double2> buf0 = base + 2 * g * PL;
+  LdsP<double2> buf1 = buf0 + PL;
+  LdsP<double2> twl = base + 2 * G * PL;
+  for (int i = threadIdx.x; i < pl.ntw; i += blockDim.x) lds_st<double2>(twl, i, tb.tw[i]);
+  __syncthreads();
+  const long c = (long)blockIdx.x * G + g;
+  const bool act = c < nodes;
+  const double2 zero = make_double2(0.0, 0.0);
+  double2 v[8];
+#pragma unroll
+  for (int r = 0; r < 8; ++r) {
+    const int i = t + TT * r;
+    v[r] = (act && i < nk) ? c2mul(tb.a_pre[i], u[c * nk + i]) : zero;
+  }
+  int cur = 0;
+  fft_regs<false, double2>(v, buf0, buf1, cur, twl, pl, t, TT);
+#pragma unroll
+  for (int r = 0; r < 8; ++r) v[r] = c2mul(v[r], tb.g1[t + TT * r]);
+  fft_regs<true, double2>(v, buf0, buf1, cur, twl, pl, t, TT);
+  if (act) {
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      const int l = t + TT * r;
+      if (l < nx) out[c * nx + l] = c2mul(tb.w_post[l], v[r]);
+    }
+  }
+}
+
+}  // namespace wdg

@@ -30,6 +30,7 @@
 
 #include "host_setup.hpp"
 #include "kernels.hpp"
+#include "theta_fft.hpp"
 
 using namespace wdg;
 
@@ -66,9 +67,11 @@ struct DevBuf {
   ~DevBuf() { release(); }
 };
 
-enum ProfId { P_TRANSPORT = 0, P_GEMM1, P_COMBINE, P_GEMM2, P_LINE_AGG, P_LINE_APPLY, P_DOT, P_UPDATE, P_COUNT };
-const char* kProfNames[P_COUNT] = {"transport", "theta_gemm1", "theta_combine", "theta_gemm2",
-                                   "line_aggregate", "line_apply", "arnoldi_dot", "arnoldi_update"};
+enum ProfId {
+  P_TRANSPORT = 0, P_GEMM1, P_COMBINE, P_GEMM2, P_LINE_AGG, P_LINE_APPLY, P_DOT, P_UPDATE, P_THETA_FFT, P_COUNT
+};
+const char* kProfNames[P_COUNT] = {"transport",      "theta_gemm1", "theta_combine", "theta_gemm2", "line_aggregate",
+                                   "line_apply",     "arnoldi_dot", "arnoldi_update", "theta_fft"};
 
 }  // namespace
 
@@ -82,10 +85,22 @@ struct wdg_ctx {
   DevBuf<double> d_sbar64;
   DevBuf<float> d_sbar32;
   DevBuf<double2> d_s0s1, d_agg;
-  DevBuf<double> d_A1, d_A2;          // realified phi / i*phi^H, FP64
+  DevBuf<double> d_A1, d_A2;          // realified phi / i*phi^H, FP64 (GEMM formulation)
   DevBuf<uint16_t> d_A1b, d_A2b;      // same in BF16
-  DevBuf<double2> d_T64, d_Z64;       // Theta intermediates
+  DevBuf<double2> d_T64, d_Z64;       // Theta intermediates (GEMM formulation)
   DevBuf<float2> d_T32, d_Z32;
+  // Theta by chirp-z FFTs
+  ChirpPlan chirp;
+  bool fft_ok = false;                // grids of chirp form and the LDS plan fits
+  bool use_fft = false;
+  bool gemm_ready = false;
+  FftPlan fplan{};
+  DevBuf<double2> d_tab64;            // tw, a_pre, w_post, g1, b_pre, a_post, g2
+  DevBuf<float2> d_tab32;
+  ThetaTabs<double2> tt64{};
+  ThetaTabs<float2> tt32{};
+  size_t lds64 = 0, lds32 = 0, lds_phi = 0;
+  int fft_block = 0, fft_groups = 0;  // threads per workgroup, transforms per workgroup
   DevBuf<double2> d_b, d_x, d_r, d_y; // FP64 solve vectors (normalised units)
   DevBuf<double2> d_u64, d_v64;       // scratch for host-buffer entry points
   DevBuf<float2> d_u32, d_v32;
@@ -358,10 +373,55 @@ double combine_bytes(wdg_ctx* c, int s, int ts) {
   return 2.0 * c->ncols * c->g.n_xi * s + (double)c->g.n_chi * c->oc.n_pairs * (c->g.n_xi + 1) * ts;
 }
 
+// y += Theta u by the fused chirp-z kernel.
+template <typename T>
+void op_theta_fft(wdg_ctx* c, const typename cvec<T>::type* u, typename cvec<T>::type* y, const GmresState* st) {
+  constexpr bool f64 = std::is_same<T, double>::value;
+  const size_t s = sizeof(typename cvec<T>::type), ts = f64 ? 8 : 4;
+  // algorithmic HBM bytes: read u, read and write y, read the edge table
+  ProfScope ps(c, P_THETA_FFT, 3.0 * c->N * s + (double)c->g.n_chi * c->oc.n_pairs * (c->g.n_xi + 1) * ts);
+  const dim3 grid(c->g.n_chi), block(c->fft_block);
+  const size_t lds = f64 ? c->lds64 : c->lds32;
+#define WDG_THETA(NN)                                                                                         \
+  if constexpr (f64)                                                                                          \
+    hipLaunchKernelGGL((theta_fft_kernel<NN, double, double>), grid, block, lds, c->stream, c->d_oc.p, c->fplan, \
+                       c->tt64, c->d_sbar64.p, c->d_cap.p, u, y, st);                                          \
+  else                                                                                                        \
+    hipLaunchKernelGGL((theta_fft_kernel<NN, float, float>), grid, block, lds, c->stream, c->d_oc.p, c->fplan,  \
+                       c->tt32, c->d_sbar32.p, c->d_cap.p, u, y, st);                                          \
+  break;
+  switch (c->g.n) {
+    case 2: WDG_THETA(2)
+    case 3: WDG_THETA(3)
+    case 4: WDG_THETA(4)
+    case 5: WDG_THETA(5)
+    case 6: WDG_THETA(6)
+    case 7: WDG_THETA(7)
+    case 8: WDG_THETA(8)
+    default: throw std::invalid_argument("n_k_chi outside 2..8");
+  }
+#undef WDG_THETA
+}
+
+// The attribute is process-wide per kernel, so it is set to the hardware
+// maximum rather than to this context's plan (contexts of other grids share
+// the kernel).
+template <int NN>
+void theta_fft_attrs() {
+  HIPCHK(hipFuncSetAttribute((const void*)theta_fft_kernel<NN, double, double>,
+                             hipFuncAttributeMaxDynamicSharedMemorySize, kMaxLdsBytes));
+  HIPCHK(hipFuncSetAttribute((const void*)theta_fft_kernel<NN, float, float>,
+                             hipFuncAttributeMaxDynamicSharedMemorySize, kMaxLdsBytes));
+}
+
 // y = A_hat u in precision T.
 template <typename T>
 void op_apply(wdg_ctx* c, const typename cvec<T>::type* u, typename cvec<T>::type* y, const GmresState* st) {
   op_transport<T>(c, u, y, st);
+  if (c->use_fft) {
+    op_theta_fft<T>(c, u, y, st);
+    return;
+  }
   const int M1 = 2 * c->g.n_xi, K1 = 2 * c->g.nk;
   const long work = (long)c->g.n_chi * c->g.n_xi;
   if constexpr (std::is_same<T, double>::value) {
@@ -565,6 +625,120 @@ uint16_t host_bf16(double v) {
   return (uint16_t)(u >> 16);
 }
 
+// Realified phi for GEMM1 (T = phi U) and i*phi^H for GEMM2 (Y = i phi^H Z),
+// and the T/Z intermediates: only needed by the GEMM formulation of Theta.
+void ensure_gemm(wdg_ctx* c) {
+  if (c->gemm_ready) return;
+  const Geometry& g = c->g;
+  auto up = [&](DevBuf<double>& b, const std::vector<double>& v) {
+    b.alloc(v.size());
+    HIPCHK(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  };
+  const int nx = g.n_xi, nk = g.nk;
+  std::vector<double> A1((size_t)2 * nx * 2 * nk), A2((size_t)2 * nk * 2 * nx);
+  for (int l = 0; l < nx; ++l)
+    for (int j = 0; j < nk; ++j) {
+      const double pr = g.phi[(size_t)l * nk + j].real(), pi = g.phi[(size_t)l * nk + j].imag();
+      const size_t K1 = 2 * nk, K2 = 2 * nx;
+      A1[(size_t)(2 * l) * K1 + 2 * j] = pr;
+      A1[(size_t)(2 * l) * K1 + 2 * j + 1] = -pi;
+      A1[(size_t)(2 * l + 1) * K1 + 2 * j] = pi;
+      A1[(size_t)(2 * l + 1) * K1 + 2 * j + 1] = pr;
+      // psi(j,l) = i conj(phi(l,j)) = pi + i pr
+      A2[(size_t)(2 * j) * K2 + 2 * l] = pi;
+      A2[(size_t)(2 * j) * K2 + 2 * l + 1] = -pr;
+      A2[(size_t)(2 * j + 1) * K2 + 2 * l] = pr;
+      A2[(size_t)(2 * j + 1) * K2 + 2 * l + 1] = pi;
+    }
+  up(c->d_A1, A1);
+  up(c->d_A2, A2);
+  std::vector<uint16_t> A1b(A1.size()), A2b(A2.size());
+  for (size_t i = 0; i < A1.size(); ++i) A1b[i] = host_bf16(A1[i]);
+  for (size_t i = 0; i < A2.size(); ++i) A2b[i] = host_bf16(A2[i]);
+  c->d_A1b.alloc(A1b.size());
+  c->d_A2b.alloc(A2b.size());
+  HIPCHK(hipMemcpyAsync(c->d_A1b.p, A1b.data(), A1b.size() * 2, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->d_A2b.p, A2b.data(), A2b.size() * 2, hipMemcpyHostToDevice, c->stream));
+  const size_t tz = (size_t)c->ncols * nx;
+  c->d_T64.alloc(tz);
+  c->d_Z64.alloc(tz);
+  c->d_T32.alloc(tz);
+  c->d_Z32.alloc(tz);
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->gemm_ready = true;
+}
+
+// Chirp-z tables and the LDS plan of the fused Theta kernel.  Theta runs on
+// FFTs whenever the grids allow it and the plan fits in LDS, else on GEMMs.
+void setup_theta(wdg_ctx* c) {
+  const Geometry& g = c->g;
+  build_chirp(g, c->chirp);
+  const ChirpPlan& cp = c->chirp;
+  c->fft_ok = false;
+  if (cp.ok && (int)cp.rad.size() <= kFftMaxPasses) {
+    const int tt = cp.P / 8;
+    c->fft_groups = std::max(1, kFftThreads / tt);
+    c->fft_block = tt * c->fft_groups;
+    const size_t words =
+        (size_t)g.n * g.n_xi + 2 * (size_t)c->fft_groups * padded_len(cp.P) + cp.tw.size();
+    c->lds64 = words * sizeof(double2);
+    c->lds32 = words * sizeof(float2);
+    c->lds_phi = (2 * (size_t)c->fft_groups * padded_len(cp.P) + cp.tw.size()) * sizeof(double2);
+    c->fft_ok = tt <= kFftThreads && (cp.P % 64) == 0 && c->lds64 <= (size_t)kMaxLdsBytes;
+  }
+  if (c->fft_ok) {
+    FftPlan& pl = c->fplan;
+    std::memset(&pl, 0, sizeof(pl));
+    pl.P = cp.P;
+    pl.npass = (int)cp.rad.size();
+    pl.ntw = (int)cp.tw.size();
+    for (int i = 0; i < pl.npass; ++i) {
+      pl.rad[i] = cp.rad[i];
+      pl.toff[i] = cp.tw_off[i];
+    }
+    const std::vector<cplx>* parts[7] = {&cp.tw, &cp.a_pre, &cp.w_post, &cp.g1, &cp.b_pre, &cp.a_post, &cp.g2};
+    size_t tot = 0, off[7];
+    for (int i = 0; i < 7; ++i) {
+      off[i] = tot;
+      tot += parts[i]->size();
+    }
+    std::vector<double2> h64(tot);
+    std::vector<float2> h32(tot);
+    for (int i = 0; i < 7; ++i)
+      for (size_t k = 0; k < parts[i]->size(); ++k) {
+        const cplx v = (*parts[i])[k];
+        h64[off[i] + k] = make_double2(v.real(), v.imag());
+        h32[off[i] + k] = make_float2((float)v.real(), (float)v.imag());
+      }
+    c->d_tab64.alloc(tot);
+    c->d_tab32.alloc(tot);
+    HIPCHK(hipMemcpyAsync(c->d_tab64.p, h64.data(), tot * sizeof(double2), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->d_tab32.p, h32.data(), tot * sizeof(float2), hipMemcpyHostToDevice, c->stream));
+    const double2* b64 = c->d_tab64.p;
+    const float2* b32 = c->d_tab32.p;
+    c->tt64 = ThetaTabs<double2>{b64 + off[0], b64 + off[1], b64 + off[2], b64 + off[3],
+                                 b64 + off[4], b64 + off[5], b64 + off[6]};
+    c->tt32 = ThetaTabs<float2>{b32 + off[0], b32 + off[1], b32 + off[2], b32 + off[3],
+                                b32 + off[4], b32 + off[5], b32 + off[6]};
+    switch (g.n) {
+      case 2: theta_fft_attrs<2>(); break;
+      case 3: theta_fft_attrs<3>(); break;
+      case 4: theta_fft_attrs<4>(); break;
+      case 5: theta_fft_attrs<5>(); break;
+      case 6: theta_fft_attrs<6>(); break;
+      case 7: theta_fft_attrs<7>(); break;
+      case 8: theta_fft_attrs<8>(); break;
+      default: break;
+    }
+    HIPCHK(hipFuncSetAttribute((const void*)phi_fft_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               kMaxLdsBytes));
+    c->use_fft = true;
+  } else {
+    c->use_fft = false;
+    ensure_gemm(c);
+  }
+}
+
 void create_impl(wdg_ctx* c, const wdg_desc& d) {
   build_geometry(d, c->g);
   const Geometry& g = c->g;
@@ -623,32 +797,8 @@ void create_impl(wdg_ctx* c, const wdg_desc& d) {
   up(c->d_xquad, g.xquad);
   up(c->d_edges, g.edges);
   up(c->d_cap, g.cap);
-  // Realified phi for GEMM1 (T = phi U) and i*phi^H for GEMM2 (Y = i phi^H Z).
+  setup_theta(c);
   const int nx = g.n_xi, nk = g.nk;
-  std::vector<double> A1((size_t)2 * nx * 2 * nk), A2((size_t)2 * nk * 2 * nx);
-  for (int l = 0; l < nx; ++l)
-    for (int j = 0; j < nk; ++j) {
-      const double pr = g.phi[(size_t)l * nk + j].real(), pi = g.phi[(size_t)l * nk + j].imag();
-      const size_t K1 = 2 * nk, K2 = 2 * nx;
-      A1[(size_t)(2 * l) * K1 + 2 * j] = pr;
-      A1[(size_t)(2 * l) * K1 + 2 * j + 1] = -pi;
-      A1[(size_t)(2 * l + 1) * K1 + 2 * j] = pi;
-      A1[(size_t)(2 * l + 1) * K1 + 2 * j + 1] = pr;
-      // psi(j,l) = i conj(phi(l,j)) = pi + i pr
-      A2[(size_t)(2 * j) * K2 + 2 * l] = pi;
-      A2[(size_t)(2 * j) * K2 + 2 * l + 1] = -pr;
-      A2[(size_t)(2 * j + 1) * K2 + 2 * l] = pr;
-      A2[(size_t)(2 * j + 1) * K2 + 2 * l + 1] = pi;
-    }
-  up(c->d_A1, A1);
-  up(c->d_A2, A2);
-  std::vector<uint16_t> A1b(A1.size()), A2b(A2.size());
-  for (size_t i = 0; i < A1.size(); ++i) A1b[i] = host_bf16(A1[i]);
-  for (size_t i = 0; i < A2.size(); ++i) A2b[i] = host_bf16(A2[i]);
-  c->d_A1b.alloc(A1b.size());
-  c->d_A2b.alloc(A2b.size());
-  HIPCHK(hipMemcpyAsync(c->d_A1b.p, A1b.data(), A1b.size() * 2, hipMemcpyHostToDevice, c->stream));
-  HIPCHK(hipMemcpyAsync(c->d_A2b.p, A2b.data(), A2b.size() * 2, hipMemcpyHostToDevice, c->stream));
   // Centre phi rows for the observables.
   const int lc = nx / 2;
   std::vector<double2> ph(2 * nk);
@@ -658,11 +808,6 @@ void create_impl(wdg_ctx* c, const wdg_desc& d) {
   }
   c->d_phic.alloc(2 * nk);
   HIPCHK(hipMemcpyAsync(c->d_phic.p, ph.data(), ph.size() * sizeof(double2), hipMemcpyHostToDevice, c->stream));
-  const size_t tz = (size_t)c->ncols * nx;
-  c->d_T64.alloc(tz);
-  c->d_Z64.alloc(tz);
-  c->d_T32.alloc(tz);
-  c->d_Z32.alloc(tz);
   c->d_s0s1.alloc((size_t)g.n_chi * n * n * 2);
   c->d_agg.alloc((size_t)oc.n_chunks * nk * 2);
   c->d_sbar64.alloc((size_t)g.n_chi * oc.n_pairs * (nx + 1));
@@ -765,7 +910,7 @@ struct SolveRun {
     m = (int)std::min<long>(o.restart, N);
     m_max = (int)std::min<long>(std::max(o.restart_max, o.restart), std::min<long>(N, kMaxRestart));
     ensure_krylov(c, m, fp64);
-    autotune(c, fp64);
+    if (!c->use_fft) autotune(c, fp64);
     if (!c->h_st) HIPCHK(hipHostMalloc((void**)&c->h_st, sizeof(GmresState), hipHostMallocDefault));
     xs = (c->rhs_norm > 0) ? c->rhs_norm / c->g.scale : 1.0;  // physical x = xs * x_device
     c->sol_scale = xs;
@@ -1219,9 +1364,17 @@ int wdg_observables(wdg_ctx* ctx, const double* rho, const double* m_nodes, doub
     HIPCHK(hipMemcpyAsync(obs.data(), ctx->d_obs.p, obs.size() * sizeof(double2), hipMemcpyDeviceToHost,
                           ctx->stream));
     if (rho_u) {
-      op_gemm64(ctx, P_GEMM1, ctx->d_A1.p, (const double*)src, (double*)ctx->d_T64.p, 2 * g.n_xi, 2 * g.nk, false,
-                nullptr);
-      HIPCHK(hipMemcpyAsync(rho_u, ctx->d_T64.p, (size_t)nodes * g.n_xi * 16, hipMemcpyDeviceToHost, ctx->stream));
+      if (ctx->use_fft) {
+        ctx->d_v64.alloc((size_t)nodes * g.n_xi);
+        const int G = ctx->fft_groups;
+        hipLaunchKernelGGL(phi_fft_kernel, dim3((nodes + G - 1) / G), dim3(ctx->fft_block), ctx->lds_phi,
+                           ctx->stream, ctx->fplan, ctx->tt64, src, nodes, g.nk, g.n_xi, ctx->d_v64.p);
+        HIPCHK(hipMemcpyAsync(rho_u, ctx->d_v64.p, (size_t)nodes * g.n_xi * 16, hipMemcpyDeviceToHost, ctx->stream));
+      } else {
+        op_gemm64(ctx, P_GEMM1, ctx->d_A1.p, (const double*)src, (double*)ctx->d_T64.p, 2 * g.n_xi, 2 * g.nk, false,
+                  nullptr);
+        HIPCHK(hipMemcpyAsync(rho_u, ctx->d_T64.p, (size_t)nodes * g.n_xi * 16, hipMemcpyDeviceToHost, ctx->stream));
+      }
     }
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(ctx->stream));
@@ -1321,6 +1474,26 @@ int wdg_host_setup(const wdg_desc* desc, const wdg_potential* pot, double* rhs, 
     }
     return WDG_OK;
   });
+}
+
+int wdg_set_theta_method(wdg_ctx* ctx, int32_t method) {
+  if (!ctx) return fail(nullptr, WDG_ERR_ARG, "null argument");
+  return guarded(ctx, [&]() {
+    if (method != WDG_THETA_AUTO && method != WDG_THETA_FFT && method != WDG_THETA_GEMM)
+      throw std::invalid_argument("unknown Theta method");
+    if (method == WDG_THETA_FFT && !ctx->fft_ok)
+      throw std::invalid_argument("the chirp-z FFT formulation does not apply to this grid (or exceeds LDS)");
+    const bool fft = (method == WDG_THETA_FFT) || (method == WDG_THETA_AUTO && ctx->fft_ok);
+    if (!fft) ensure_gemm(ctx);
+    if (fft != ctx->use_fft) clear_graphs(ctx);   // captured cycles hold the other kernels
+    ctx->use_fft = fft;
+    return WDG_OK;
+  });
+}
+
+int wdg_theta_method(const wdg_ctx* ctx) {
+  if (!ctx) return -1;
+  return ctx->use_fft ? WDG_THETA_FFT : WDG_THETA_GEMM;
 }
 
 int wdg_set_profiling(wdg_ctx* ctx, uint32_t mask) {

@@ -202,6 +202,19 @@ class WignerOperator:
               self._h)
         return out, n_cv, {f: getattr(info, f) for f, _ in info._fields_}
 
+    # ---- Theta formulation ----
+    THETA_METHODS = {"auto": _lib.WDG_THETA_AUTO, "fft": _lib.WDG_THETA_FFT, "gemm": _lib.WDG_THETA_GEMM}
+
+    def set_theta_method(self, method: str):
+        """'fft' (chirp-z transforms), 'gemm' (dense MFMA GEMMs) or 'auto'."""
+        if method not in self.THETA_METHODS:
+            raise ValueError(f"method must be one of {sorted(self.THETA_METHODS)}")
+        check(self._lib.wdg_set_theta_method(self._h, self.THETA_METHODS[method]), self._h)
+
+    @property
+    def theta_method(self) -> str:
+        return {_lib.WDG_THETA_FFT: "fft", _lib.WDG_THETA_GEMM: "gemm"}[self._lib.wdg_theta_method(self._h)]
+
     # ---- profiling ----
     def set_profiling(self, mask: int):
         check(self._lib.wdg_set_profiling(self._h, C.c_uint32(mask)), self._h)
@@ -244,7 +257,7 @@ def solve_batch(ops, opts: SolveOptions | None = None):
 
 
 PROFILE_BITS = {"transport": 0, "theta_gemm1": 1, "theta_combine": 2, "theta_gemm2": 3,
-                "line_aggregate": 4, "line_apply": 5, "arnoldi_dot": 6, "arnoldi_update": 7}
+                "line_aggregate": 4, "line_apply": 5, "arnoldi_dot": 6, "arnoldi_update": 7, "theta_fft": 8}
 
 
 # ---------------------------------------------------------------------------

@@ -25,36 +25,68 @@ def test_rhs(case):
     assert np.linalg.norm(r - case["rhs"]) <= 1e-12 * np.linalg.norm(case["rhs"])
 
 
-def test_apply_fp64(case):
+METHODS = ["fft", "gemm"]
+
+
+def test_auto_picks_fft(case):
+    case["op"].set_theta_method("auto")
+    assert case["op"].theta_method == "fft"
+
+
+@pytest.mark.parametrize("method", METHODS)
+def test_apply_fp64(case, method):
+    case["op"].set_theta_method(method)
     u = rand_c(case["A"].shape[0], 1)
     y_ref = (case["A"] + case["G"]) @ u
     y = case["op"].apply(u, "fp64")
     err = np.linalg.norm(y - y_ref) / np.linalg.norm(y_ref)
-    print("apply fp64 rel err", err)
+    print(method, "apply fp64 rel err", err)
     assert err < 1e-13
 
 
-def test_apply_transport_only_part(case):
-    # constant-in-k smooth vector hits mostly the transport part
+@pytest.mark.parametrize("method", METHODS)
+def test_apply_theta_part(case, method):
+    case["op"].set_theta_method(method)
     u = rand_c(case["A"].shape[0], 2)
     y_ref = case["G"] @ u
     y = case["op"].apply(u, "fp64") - case["A"] @ u
     err = np.linalg.norm(y - y_ref) / np.linalg.norm(y_ref)
-    print("theta fp64 rel err", err)
+    print(method, "theta fp64 rel err", err)
     assert err < 1e-12
 
 
-def test_apply_mixed(case):
+@pytest.mark.parametrize("method", METHODS)
+def test_apply_mixed(case, method):
+    case["op"].set_theta_method(method)
     u = rand_c(case["A"].shape[0], 3)
     y_ref = (case["A"] + case["G"]) @ u
     y = case["op"].apply(u, "mixed")
     err = np.linalg.norm(y - y_ref) / np.linalg.norm(y_ref)
     th = np.linalg.norm((y - case["A"] @ u) - case["G"] @ u) / np.linalg.norm(case["G"] @ u)
-    print("apply mixed rel err", err, "theta part", th)
-    # BF16 rounding of phi (unit roundoff 2^-9) in two GEMMs bounds the Theta
-    # part; the transport part is FP32.
-    assert err < 3e-3
-    assert th < 5e-3
+    print(method, "apply mixed rel err", err, "theta part", th)
+    if method == "gemm":
+        # BF16 rounding of phi (unit roundoff 2^-9) in two GEMMs bounds the
+        # Theta part; the transport part is FP32.
+        assert err < 3e-3
+        assert th < 5e-3
+    else:
+        # FP32 chirp-z transforms: a few FP32 roundings times log2(P)
+        assert err < 1e-5
+        assert th < 1e-5
+
+
+@pytest.mark.parametrize("method", METHODS)
+def test_observables_rho_u(case, method):
+    case["op"].set_theta_method(method)
+    p = case["p"]
+    u = rand_c(case["A"].shape[0], 5)
+    m_nodes = np.full(p["N_chi"] * p["N_K_chi"], 0.067 * 9.109e-31)
+    from oracle import wigner1d as W
+    n_ref, j_ref, ru_ref = W.observables(p, u, m_nodes)
+    n, j, ru = case["op"].observables(m_nodes, rho=u, want_rho_u=True)
+    assert np.max(np.abs(ru - ru_ref)) <= 1e-12 * np.max(np.abs(ru_ref))
+    assert np.max(np.abs(n - n_ref)) <= 1e-12 * np.max(np.abs(n_ref))
+    assert np.max(np.abs(j - j_ref)) <= 1e-10 * np.max(np.abs(j_ref))
 
 
 def test_diagonal(case):

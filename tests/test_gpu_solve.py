@@ -43,7 +43,16 @@ def test_fp64_matches_direct(solved):
     assert ej < 1e-8
 
 
-def test_mixed_reaches_tolerance(solved):
+@pytest.mark.parametrize("method", ["fft", "gemm"])
+def test_mixed_reaches_tolerance(solved, method):
+    solved["op"].set_theta_method(method)
+    try:
+        _mixed_reaches_tolerance(solved)
+    finally:
+        solved["op"].set_theta_method("auto")
+
+
+def _mixed_reaches_tolerance(solved):
     x, info = solved["op"].solve(SolveOptions(tol=1e-10, restart=30, precision="mixed"))
     print(info)
     assert info["flag"] == 0 and info["true_relres"] <= 1e-10
