@@ -177,21 +177,34 @@ void lu_factor(std::vector<cplx>& a, std::vector<int>& piv, int n) {
   }
 }
 
-std::vector<cplx> lu_solve(const std::vector<cplx>& lu, const std::vector<int>& piv, int n,
-                           std::vector<cplx> b) {
+// Two right-hand sides at once (one pass over the factors).
+void lu_solve2(const std::vector<cplx>& lu, const std::vector<int>& piv, int n, std::vector<cplx>& b1,
+               std::vector<cplx>& b2) {
   for (int c = 0; c < n; ++c)
-    if (piv[c] != c) std::swap(b[c], b[piv[c]]);
+    if (piv[c] != c) {
+      std::swap(b1[c], b1[piv[c]]);
+      std::swap(b2[c], b2[piv[c]]);
+    }
   for (int r = 1; r < n; ++r) {
-    cplx s = b[r];
-    for (int k = 0; k < r; ++k) s -= lu[(size_t)r * n + k] * b[k];
-    b[r] = s;
+    cplx s1 = b1[r], s2 = b2[r];
+    const cplx* row = lu.data() + (size_t)r * n;
+    for (int k = 0; k < r; ++k) {
+      s1 -= row[k] * b1[k];
+      s2 -= row[k] * b2[k];
+    }
+    b1[r] = s1;
+    b2[r] = s2;
   }
   for (int r = n - 1; r >= 0; --r) {
-    cplx s = b[r];
-    for (int k = r + 1; k < n; ++k) s -= lu[(size_t)r * n + k] * b[k];
-    b[r] = s / lu[(size_t)r * n + r];
+    cplx s1 = b1[r], s2 = b2[r];
+    const cplx* row = lu.data() + (size_t)r * n;
+    for (int k = r + 1; k < n; ++k) {
+      s1 -= row[k] * b1[k];
+      s2 -= row[k] * b2[k];
+    }
+    b1[r] = s1 / row[r];
+    b2[r] = s2 / row[r];
   }
-  return b;
 }
 
 double cap_profile(double q, double l_xi, double delta_xi) {
@@ -274,32 +287,32 @@ void build_geometry(const wdg_desc& d, Geometry& g) {
   // Wave numbers and basis (Basisfunktionen_Q_m.m:22-32).
   g.k.resize(g.nk);
   for (int j = 0; j < g.nk; ++j) g.k[j] = -2.0 * M_PI / g.l_xi * ((j + 1) - 0.5 * (g.nk + 1));
-  g.phi.resize((size_t)g.n_xi * g.nk);
-  const double isl = 1.0 / std::sqrt(g.l_xi);
-  for (int l = 0; l < g.n_xi; ++l)
-    for (int j = 0; j < g.nk; ++j) {
-      double ph = g.k[j] * g.y[l];
-      g.phi[(size_t)l * g.nk + j] = cplx(isl * std::cos(ph), isl * std::sin(ph));
-    }
-  // phi depends only on (n_xi, nk, L_xi): instances of a sweep that share a
-  // geometry share one factorisation.
   {
     static std::mutex mu;
-    static std::map<std::tuple<int, int, uint64_t>, std::pair<std::vector<cplx>, std::vector<int>>> cache;
+    static std::map<std::tuple<int, int, uint64_t>, std::shared_ptr<const BasisData>> cache;
     uint64_t lbits;
     std::memcpy(&lbits, &g.l_xi, sizeof(lbits));
     const auto key = std::make_tuple(g.n_xi, g.nk, lbits);
     std::lock_guard<std::mutex> lock(mu);
     auto it = cache.find(key);
     if (it == cache.end()) {
-      std::vector<cplx> lu = g.phi;
-      std::vector<int> piv;
-      lu_factor(lu, piv, g.nk);
+      auto bd = std::make_shared<BasisData>();
+      const double isl = 1.0 / std::sqrt(g.l_xi);
+      bd->phi.resize((size_t)g.n_xi * g.nk);
+      bd->cosm.resize((size_t)g.n_xi * g.nk);
+      for (int l = 0; l < g.n_xi; ++l)
+        for (int j = 0; j < g.nk; ++j) {
+          const double ph = g.k[j] * g.y[l];
+          const double c = std::cos(ph), s = std::sin(ph);
+          bd->phi[(size_t)l * g.nk + j] = cplx(isl * c, isl * s);
+          bd->cosm[(size_t)l * g.nk + j] = c;
+        }
+      bd->lu = bd->phi;
+      lu_factor(bd->lu, bd->piv, g.nk);
       if (cache.size() > 64) cache.clear();
-      it = cache.emplace(key, std::make_pair(std::move(lu), std::move(piv))).first;
+      it = cache.emplace(key, std::move(bd)).first;
     }
-    g.phi_lu = it->second.first;
-    g.phi_piv = it->second.second;
+    g.basis = it->second;
   }
   // Theta sample points (get_Drift_accel_trial_permutedShell.m:15-55).
   g.edges = linspace(-g.l_xi / 2, g.l_xi / 2, g.n_xi + 1);
@@ -479,18 +492,41 @@ void inflow(const Geometry& g, const wdg_potential& pot, std::vector<cplx>& fnl,
   }
   // Cosine transform to xi, then phi \ f (verteilung.m:109-114).
   std::vector<cplx> bl(g.n_xi), br(g.n_xi);
+  const double* cm = g.basis->cosm.data();
   for (int l = 0; l < g.n_xi; ++l) {
     double sl = 0.0, sr = 0.0;
+    const double* row = cm + (size_t)l * nk;
     for (int j = 0; j < nk; ++j) {
-      const double c = std::cos(g.y[l] * g.k[j]);
-      sl += c * fL[j];
-      sr += c * fR[j];
+      sl += row[j] * fL[j];
+      sr += row[j] * fR[j];
     }
     bl[l] = sl * dk / 2.0 / M_PI;
     br[l] = sr * dk / 2.0 / M_PI;
   }
-  fnl = lu_solve(g.phi_lu, g.phi_piv, nk, bl);
-  fnr = lu_solve(g.phi_lu, g.phi_piv, nk, br);
+  lu_solve2(g.basis->lu, g.basis->piv, nk, bl, br);
+  fnl = std::move(bl);
+  fnr = std::move(br);
+}
+
+void build_rhs_blocks(const Geometry& g, const std::vector<cplx>& fnl, const std::vector<cplx>& fnr,
+                      std::vector<double>& first, std::vector<double>& last) {
+  const int n = g.n, nk = g.nk;
+  first.assign((size_t)2 * n * nk, 0.0);
+  last.assign((size_t)2 * n * nk, 0.0);
+  for (int a = 0; a < n; ++a)
+    for (int j = 0; j < nk; ++j) {
+      const double kk = g.k[j];
+      const size_t i = (size_t)a * nk + j;
+      if (kk > 0) {  // Q1 = invM(:,1)/detJx on element 1 (get_Diff.m:253-254,257)
+        const cplx v = g.q_diff * g.invM[a * n + 0] / g.jac * std::abs(kk) * fnl[j];
+        first[2 * i] = v.real();
+        first[2 * i + 1] = v.imag();
+      } else if (kk < 0) {  // Q2 = invM(:,n)/detJx on the last element (:255-256,258)
+        const cplx v = g.q_diff * g.invM[a * n + (n - 1)] / g.jac * std::abs(kk) * fnr[j];
+        last[2 * i] = v.real();
+        last[2 * i + 1] = v.imag();
+      }
+    }
 }
 
 void build_rhs(const Geometry& g, const std::vector<cplx>& fnl, const std::vector<cplx>& fnr,

@@ -4,6 +4,7 @@
 #pragma once
 
 #include <complex>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -43,11 +44,20 @@ struct Geometry {
   std::vector<double> cap;                      // CAP at the edges
   std::vector<double> xquad;                    // n_chi*kNq chi quadrature points
   std::vector<double> chi;                      // n_chi*n node coordinates (equispaced, as reference)
-  std::vector<cplx> phi;                        // n_xi x nk, row-major phi[l*nk + j]
-  // LU factors of phi (row-major, partial pivoting) for the phi\f solves.
-  std::vector<cplx> phi_lu;
-  std::vector<int> phi_piv;
+  // phi and what derives from it depend only on (n_xi, nk, L_xi): instances
+  // of a sweep that share a geometry share one copy.
+  std::shared_ptr<const struct BasisData> basis;
+  const cplx* phi() const;                      // n_xi x nk, row-major phi[l*nk + j]
 };
+
+struct BasisData {
+  std::vector<cplx> phi;     // n_xi x nk, row-major
+  std::vector<cplx> lu;      // LU factors of phi (row-major, partial pivoting) for phi\f
+  std::vector<int> piv;
+  std::vector<double> cosm;  // n_xi x nk: cos(y_l k_j), the cosine transform of verteilung.m:109-110
+};
+
+inline const cplx* Geometry::phi() const { return basis->phi.data(); }
 
 // Chirp-z factorisation of the plane-wave basis:
 //   phi[l][j] = exp(-i pi beta u_j^2) exp(-i pi beta v_l^2) exp(i pi beta (v_l-u_j)^2) / sqrt(L)
@@ -88,5 +98,10 @@ void inflow(const Geometry& g, const wdg_potential& pot, std::vector<cplx>& fnl,
 // (get_Diff.m:253-259).
 void build_rhs(const Geometry& g, const std::vector<cplx>& fnl, const std::vector<cplx>& fnr,
                std::vector<double>& rhs);
+
+// The only nonzero blocks of that right-hand side: element 0 (inflow of the
+// k > 0 half) and element n_chi-1 (k < 0), each n*nk interleaved complex.
+void build_rhs_blocks(const Geometry& g, const std::vector<cplx>& fnl, const std::vector<cplx>& fnr,
+                      std::vector<double>& first, std::vector<double>& last);
 
 }  // namespace wdg

@@ -666,19 +666,58 @@ __global__ __launch_bounds__(256) void line_aggregate_kernel(const OpConst* __re
 }
 
 // Per k: carry-in of every chunk (overwrites the chunk's A with its s_in).
+// One wavefront per k: lane l composes the chunk maps at sweep positions
+// [l*cpl, (l+1)*cpl), a Kogge-Stone scan over the lanes composes the
+// prefixes (s_out = A + B s_in; later map after earlier), and each lane
+// re-walks its chunks from the carry-in of its first one (s = 0 enters the
+// first chunk of the sweep).
+__device__ __forceinline__ void affine_after(double& Ar, double& Ai, double& Br, double& Bi, double pAr, double pAi,
+                                             double pBr, double pBi) {
+  // (A, B) after (pA, pB): s -> A + B (pA + pB s)
+  const double nAr = Ar + (Br * pAr - Bi * pAi), nAi = Ai + (Br * pAi + Bi * pAr);
+  const double nBr = Br * pBr - Bi * pBi, nBi = Br * pBi + Bi * pBr;
+  Ar = nAr; Ai = nAi; Br = nBr; Bi = nBi;
+}
+
 __global__ __launch_bounds__(256) void line_carry_kernel(const OpConst* __restrict__ oc, const double* __restrict__ kvec,
                                                          double2* __restrict__ agg, const GmresState* st) {
   if (skip(st)) return;
   const int nk = oc->nk, nc = oc->n_chunks;
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= nk) return;
+  const int j = (int)(((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  const int lane = threadIdx.x & 63;
+  if (j >= nk) return;   // uniform per wavefront
   const bool fwd = kvec[j] > 0;
-  double sr = 0, si = 0;
-  for (int step = 0; step < nc; ++step) {
-    const int c = fwd ? step : nc - 1 - step;
-    const double2 A = agg[((long)c * nk + j) * 2 + 0], B = agg[((long)c * nk + j) * 2 + 1];
-    agg[((long)c * nk + j) * 2 + 0] = mk(sr, si);
-    const double nr = A.x + B.x * sr - B.y * si, ni = A.y + B.x * si + B.y * sr;
+  const int cpl = (nc + 63) >> 6;
+  double Ar = 0, Ai = 0, Br = 1, Bi = 0;
+  for (int q = 0; q < cpl; ++q) {
+    const int pos = lane * cpl + q;
+    if (pos >= nc) break;
+    const int c = fwd ? pos : nc - 1 - pos;
+    const double2 a = agg[((long)c * nk + j) * 2 + 0], b = agg[((long)c * nk + j) * 2 + 1];
+    double cAr = a.x, cAi = a.y, cBr = b.x, cBi = b.y;
+    affine_after(cAr, cAi, cBr, cBi, Ar, Ai, Br, Bi);
+    Ar = cAr; Ai = cAi; Br = cBr; Bi = cBi;
+  }
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const double pAr = __shfl_up(Ar, off, 64), pAi = __shfl_up(Ai, off, 64);
+    const double pBr = __shfl_up(Br, off, 64), pBi = __shfl_up(Bi, off, 64);
+    if (lane >= off) affine_after(Ar, Ai, Br, Bi, pAr, pAi, pBr, pBi);
+  }
+  // carry into this lane's first chunk: the inclusive prefix of lane-1 at s = 0
+  double sr = __shfl_up(Ar, 1, 64), si = __shfl_up(Ai, 1, 64);
+  if (lane == 0) {
+    sr = 0.0;
+    si = 0.0;
+  }
+  for (int q = 0; q < cpl; ++q) {
+    const int pos = lane * cpl + q;
+    if (pos >= nc) break;
+    const int c = fwd ? pos : nc - 1 - pos;
+    const long o = ((long)c * nk + j) * 2;
+    const double2 a = agg[o], b = agg[o + 1];
+    agg[o] = mk(sr, si);
+    const double nr = a.x + b.x * sr - b.y * si, ni = a.y + b.x * si + b.y * sr;
     sr = nr;
     si = ni;
   }

@@ -68,10 +68,12 @@ struct DevBuf {
 };
 
 enum ProfId {
-  P_TRANSPORT = 0, P_GEMM1, P_COMBINE, P_GEMM2, P_LINE_AGG, P_LINE_APPLY, P_DOT, P_UPDATE, P_THETA_FFT, P_COUNT
+  P_TRANSPORT = 0, P_GEMM1, P_COMBINE, P_GEMM2, P_LINE_AGG, P_LINE_APPLY, P_DOT, P_UPDATE, P_THETA_FFT, P_LINE_CARRY,
+  P_COUNT
 };
-const char* kProfNames[P_COUNT] = {"transport",      "theta_gemm1", "theta_combine", "theta_gemm2", "line_aggregate",
-                                   "line_apply",     "arnoldi_dot", "arnoldi_update", "theta_fft"};
+const char* kProfNames[P_COUNT] = {"transport",   "theta_gemm1",    "theta_combine", "theta_gemm2",
+                                   "line_aggregate", "line_apply",  "arnoldi_dot",   "arnoldi_update",
+                                   "theta_fft",   "line_carry"};
 
 }  // namespace
 
@@ -111,7 +113,7 @@ struct wdg_ctx {
   DevBuf<float2> d_V32, d_w32, d_z32, d_s32;
   DevBuf<double2> d_H, d_cs, d_sn, d_g, d_h1, d_h2, d_yv, d_part;
   DevBuf<GmresState> d_st;
-  std::vector<double> rhs;           // physical rhs, interleaved
+  std::vector<cplx> fnl, fnr;        // inflow data of the current potential (k basis)
   double rhs_norm = 0;
   bool have_potential = false;
   bool have_solution = false;
@@ -449,8 +451,12 @@ void op_precond_n(wdg_ctx* c, const typename cvec<T>::type* r, typename cvec<T>:
     hipLaunchKernelGGL((line_aggregate_kernel<NN, T>), dim3(grid1(work, 256)), dim3(256), 0, c->stream, c->d_oc.p,
                        c->d_k.p, c->d_s0s1.p, r, c->d_agg.p, st);
   }
-  hipLaunchKernelGGL(line_carry_kernel, dim3(grid1(c->g.nk, 256)), dim3(256), 0, c->stream, c->d_oc.p, c->d_k.p,
-                     c->d_agg.p, st);
+  {
+    // chunk maps read twice, carries written once
+    ProfScope ps(c, P_LINE_CARRY, 3.0 * c->oc.n_chunks * c->g.nk * 2 * sizeof(double2));
+    hipLaunchKernelGGL(line_carry_kernel, dim3(grid1(64L * c->g.nk, 256)), dim3(256), 0, c->stream, c->d_oc.p,
+                       c->d_k.p, c->d_agg.p, st);
+  }
   {
     ProfScope ps(c, P_LINE_APPLY, 2.0 * c->N * sizeof(typename cvec<T>::type));
     hipLaunchKernelGGL((line_apply_kernel<NN, T>), dim3(grid1(work, 256)), dim3(256), 0, c->stream, c->d_oc.p,
@@ -638,7 +644,7 @@ void ensure_gemm(wdg_ctx* c) {
   std::vector<double> A1((size_t)2 * nx * 2 * nk), A2((size_t)2 * nk * 2 * nx);
   for (int l = 0; l < nx; ++l)
     for (int j = 0; j < nk; ++j) {
-      const double pr = g.phi[(size_t)l * nk + j].real(), pi = g.phi[(size_t)l * nk + j].imag();
+      const double pr = g.phi()[(size_t)l * nk + j].real(), pi = g.phi()[(size_t)l * nk + j].imag();
       const size_t K1 = 2 * nk, K2 = 2 * nx;
       A1[(size_t)(2 * l) * K1 + 2 * j] = pr;
       A1[(size_t)(2 * l) * K1 + 2 * j + 1] = -pi;
@@ -803,8 +809,8 @@ void create_impl(wdg_ctx* c, const wdg_desc& d) {
   const int lc = nx / 2;
   std::vector<double2> ph(2 * nk);
   for (int j = 0; j < nk; ++j) {
-    ph[j] = make_double2(g.phi[(size_t)(lc - 1) * nk + j].real(), g.phi[(size_t)(lc - 1) * nk + j].imag());
-    ph[nk + j] = make_double2(g.phi[(size_t)lc * nk + j].real(), g.phi[(size_t)lc * nk + j].imag());
+    ph[j] = make_double2(g.phi()[(size_t)(lc - 1) * nk + j].real(), g.phi()[(size_t)(lc - 1) * nk + j].imag());
+    ph[nk + j] = make_double2(g.phi()[(size_t)lc * nk + j].real(), g.phi()[(size_t)lc * nk + j].imag());
   }
   c->d_phic.alloc(2 * nk);
   HIPCHK(hipMemcpyAsync(c->d_phic.p, ph.data(), ph.size() * sizeof(double2), hipMemcpyHostToDevice, c->stream));
@@ -842,14 +848,27 @@ void set_potential_impl(wdg_ctx* c, const wdg_potential& pot) {
   std::vector<cplx> fnl, fnr;
   inflow(g, pot, fnl, fnr);
   const double old_norm = c->rhs_norm;
-  build_rhs(g, fnl, fnr, c->rhs);
+  // Only the boundary elements carry inflow: upload those two blocks of the
+  // normalised RHS and clear the rest on the device.
+  std::vector<double> first, last;
+  build_rhs_blocks(g, fnl, fnr, first, last);
   double s = 0;
-  for (double v : c->rhs) s += v * v;
+  for (double v : first) s += v * v;
+  for (double v : last) s += v * v;
   c->rhs_norm = std::sqrt(s);
-  std::vector<double> bn(c->rhs.size());
   const double inv = (c->rhs_norm > 0) ? 1.0 / c->rhs_norm : 0.0;
-  for (size_t i = 0; i < bn.size(); ++i) bn[i] = c->rhs[i] * inv;
-  HIPCHK(hipMemcpyAsync(c->d_b.p, bn.data(), bn.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  for (double& v : first) v *= inv;
+  for (double& v : last) v *= inv;
+  c->stage.resize(2 * first.size());
+  std::memcpy(c->stage.data(), first.data(), first.size() * sizeof(double));
+  std::memcpy(c->stage.data() + first.size(), last.data(), last.size() * sizeof(double));
+  const size_t blk = (size_t)g.n * g.nk;   // complex values per element
+  HIPCHK(hipMemsetAsync(c->d_b.p, 0, (size_t)c->N * sizeof(double2), c->stream));
+  HIPCHK(hipMemcpyAsync(c->d_b.p, c->stage.data(), blk * sizeof(double2), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->d_b.p + (size_t)(g.n_chi - 1) * blk, c->stage.data() + first.size(),
+                        blk * sizeof(double2), hipMemcpyHostToDevice, c->stream));
+  c->fnl = std::move(fnl);
+  c->fnr = std::move(fnr);
   // potential tables on the device
   std::vector<double> xg(pot.x_mat, pot.x_mat + pot.n_mat), ug(pot.em, pot.em + pot.n_mat);
   c->d_xg.alloc(xg.size());
@@ -1158,7 +1177,9 @@ int wdg_set_potential(wdg_ctx* ctx, const wdg_potential* pot) {
 int wdg_rhs(wdg_ctx* ctx, double* rhs) {
   if (!ctx || !rhs) return fail(ctx, WDG_ERR_ARG, "null argument");
   if (!ctx->have_potential) return fail(ctx, WDG_ERR_STATE, "wdg_set_potential has not been called");
-  std::memcpy(rhs, ctx->rhs.data(), ctx->rhs.size() * sizeof(double));
+  std::vector<double> full;
+  build_rhs(ctx->g, ctx->fnl, ctx->fnr, full);
+  std::memcpy(rhs, full.data(), full.size() * sizeof(double));
   return WDG_OK;
 }
 

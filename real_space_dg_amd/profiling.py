@@ -11,8 +11,8 @@
 
 Kernel names are the demangled template names of the library; ``role``
 maps them to the names the library's own HIP-event profiler uses
-(transport, theta_gemm1, theta_combine, theta_gemm2, line_aggregate,
-line_apply, arnoldi_dot, arnoldi_update) plus the precision of the
+(transport, theta_fft, theta_gemm1, theta_combine, theta_gemm2,
+line_aggregate, line_carry, line_apply, arnoldi_dot, arnoldi_update) plus the precision of the
 instantiation ('mixed' for float/bf16, 'fp64' for double).
 """
 from __future__ import annotations
@@ -29,6 +29,8 @@ from collections import defaultdict
 _ROLES = [
     (r"transport_kernel<", "transport"),
     (r"combine_kernel<", "theta_combine"),
+    (r"theta_fft_kernel<", "theta_fft"),
+    (r"phi_fft_kernel", "phi_fft"),
     (r"gemm_(?:bf16x2|f64)_kernel<[^>]*?\bfalse\b", "theta_gemm1"),
     (r"gemm_(?:bf16x2|f64)_kernel<[^>]*?\btrue\b", "theta_gemm2"),
     (r"line_aggregate_kernel<", "line_aggregate"),

@@ -257,7 +257,8 @@ def solve_batch(ops, opts: SolveOptions | None = None):
 
 
 PROFILE_BITS = {"transport": 0, "theta_gemm1": 1, "theta_combine": 2, "theta_gemm2": 3,
-                "line_aggregate": 4, "line_apply": 5, "arnoldi_dot": 6, "arnoldi_update": 7, "theta_fft": 8}
+                "line_aggregate": 4, "line_apply": 5, "arnoldi_dot": 6, "arnoldi_update": 7, "theta_fft": 8,
+                "line_carry": 9}
 
 
 # ---------------------------------------------------------------------------
