@@ -86,7 +86,11 @@ typedef struct wdg_solve_opts {
   int32_t restart_max;  /* adaptive restart cap (<= restart: fixed restart)           */
   int32_t warm_device;  /* 1 and x0 == NULL: start from the context's last solution
                            (kept on the device across wdg_set_potential calls)     */
+  int32_t krylov_basis; /* mixed: storage of the inner Krylov basis, WDG_BASIS_WORKING
+                           (FP32), WDG_BASIS_BF16 or WDG_BASIS_FP16 (scaled by 256)  */
 } wdg_solve_opts;
+
+enum { WDG_BASIS_WORKING = 0, WDG_BASIS_BF16 = 1, WDG_BASIS_FP16 = 2 };
 
 typedef struct wdg_solve_info {
   int32_t flag;                 /* 0 converged (true residual <= tol), 1 not    */

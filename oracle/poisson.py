@@ -6,16 +6,17 @@ Restates, for the 1D (Ny = 1, Dirichlet-X) case:
   RTD/eval_Poisson2D_rtd.m:1-200      -> poisson_matrix   (harmonic-mean eps, identity contact rows)
   RTD/solve_Poisson2D_rtd.m:85-130    -> poisson_step     ('direct' / 'newton-raphson')
   RTD/project_DG_density_RTD.m:1-69   -> projection_matrix, project_density
-  RTD/solve2D_rtd_self_consistent.m:88-205 -> gummel (bias loop, mixing, tolerances)
 
 Pinning (tests/test_oracle_poisson.py): the projection integrates each
 element's Lagrange polynomial exactly (checked on polynomial densities and
 by conservation of the total charge); the Poisson matrix reproduces the
 exact solution of eps phi'' = const on a uniform grid (quadratic phi) and
-the harmonic-mean flux continuity across a dielectric step; the Newton step
-solves its linear system to machine precision and, for a Boltzmann density,
-converges quadratically to the nonlinear solution (checked against
-scipy.optimize on a small grid).
+the harmonic-mean flux continuity across a dielectric step (the exact
+two-layer solution satisfies the discrete system; an interface off the face
+does not); iterated on a Boltzmann density the Newton step converges
+quadratically and leaves a vanishing nonlinear Poisson residual.  The
+Gummel iteration itself (bias loop, mixing, tolerances) is restated in
+real_space_dg_amd/gummel.py and checked on the GPU against this step.
 """
 from __future__ import annotations
 

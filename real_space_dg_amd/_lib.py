@@ -21,6 +21,9 @@ WDG_FERMI_TSU_ESAKI = 2
 WDG_THETA_AUTO = 0
 WDG_THETA_FFT = 1
 WDG_THETA_GEMM = 2
+WDG_BASIS_WORKING = 0
+WDG_BASIS_BF16 = 1
+WDG_BASIS_FP16 = 2
 WDG_PREC_FP64 = 0
 WDG_PREC_MIXED = 1
 
@@ -59,7 +62,8 @@ class wdg_solve_opts(C.Structure):
     _fields_ = [("tol", C.c_double), ("restart", C.c_int32), ("max_iter", C.c_int32),
                 ("precision", C.c_int32), ("inner_tol", C.c_double), ("max_outer", C.c_int32),
                 ("use_precond", C.c_int32), ("use_graph", C.c_int32),
-                ("orthogonalization", C.c_int32), ("restart_max", C.c_int32), ("warm_device", C.c_int32)]
+                ("orthogonalization", C.c_int32), ("restart_max", C.c_int32), ("warm_device", C.c_int32),
+                ("krylov_basis", C.c_int32)]
 
 
 class wdg_solve_info(C.Structure):

@@ -942,27 +942,80 @@ template <> struct Pk<float> {
   }
 };
 
+// Storage of the Krylov basis.  A basis packet holds the same complex values
+// as one working packet Pk<T>::type: FP64/FP32 packets are stored as they are;
+// the FP32 inner solver can instead keep its basis as BF16 packets (uint2 =
+// 2 complex in 8 bytes, round to nearest even), halving the Gram-Schmidt
+// traffic.  load() widens to the working packet, store() narrows.
+__device__ __forceinline__ float bf16_bits_to_float(uint32_t h) { return __uint_as_float(h << 16); }
+__device__ __forceinline__ uint32_t float_to_bf16_bits(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return u >> 16;
+}
+template <typename VP>
+struct BasisIO {
+  using work = VP;
+  __device__ static __forceinline__ work load(VP v) { return v; }
+  __device__ static __forceinline__ VP store(work v) { return v; }
+};
+template <>
+struct BasisIO<uint2> {
+  using work = float4;
+  __device__ static __forceinline__ float4 load(uint2 p) {
+    return make_float4(bf16_bits_to_float(p.x & 0xFFFFu), bf16_bits_to_float(p.x >> 16),
+                       bf16_bits_to_float(p.y & 0xFFFFu), bf16_bits_to_float(p.y >> 16));
+  }
+  __device__ static __forceinline__ uint2 store(float4 v) {
+    uint2 p;
+    p.x = float_to_bf16_bits(v.x) | (float_to_bf16_bits(v.y) << 16);
+    p.y = float_to_bf16_bits(v.z) | (float_to_bf16_bits(v.w) << 16);
+    return p;
+  }
+};
+
+// FP16 basis packets: the unit-norm basis vectors are stored times 256
+// (|entries| <= 256 < 65504, and entries down to 2.4e-7 stay normal).
+template <>
+struct BasisIO<ushort4> {
+  using work = float4;
+  __device__ static __forceinline__ float4 load(ushort4 p) {
+    const float s = 1.0f / 256.0f;
+    return make_float4((float)__builtin_bit_cast(_Float16, p.x) * s, (float)__builtin_bit_cast(_Float16, p.y) * s,
+                       (float)__builtin_bit_cast(_Float16, p.z) * s, (float)__builtin_bit_cast(_Float16, p.w) * s);
+  }
+  __device__ static __forceinline__ ushort4 store(float4 v) {
+    ushort4 p;
+    p.x = __builtin_bit_cast(unsigned short, (_Float16)(v.x * 256.0f));
+    p.y = __builtin_bit_cast(unsigned short, (_Float16)(v.y * 256.0f));
+    p.z = __builtin_bit_cast(unsigned short, (_Float16)(v.z * 256.0f));
+    p.w = __builtin_bit_cast(unsigned short, (_Float16)(v.w * 256.0f));
+    return p;
+  }
+};
+
 // Multi-dot: h[i] = <V_i, w> for i < nv (plus ||w||^2 when mode 0).
 // Grid (blocks, ceil(nv/kDotGroup)); block row y handles basis vectors
 // [8y, 8y+8).  mode 0 writes h1 and nrm0; mode 1 (only when st->reorth)
 // writes h2.  NP = number of 16-byte packets per vector.
-template <typename T, int NT>
-__global__ __launch_bounds__(NT) void arnoldi_dot_kernel(GmresBufs gb, const typename Pk<T>::type* __restrict__ V,
+template <typename T, typename VP, int NT>
+__global__ __launch_bounds__(NT) void arnoldi_dot_kernel(GmresBufs gb, const VP* __restrict__ V,
                                                          const typename Pk<T>::type* __restrict__ w, long NP, int nv,
                                                          int mode) {
   using P = Pk<T>;
+  using IO = BasisIO<VP>;
   if (gb.st->done || (mode == 1 && !gb.st->reorth)) return;
   const int i0 = blockIdx.y * kDotGroup;
   const int ng = min(kDotGroup, nv - i0);
   double2 acc[kDotGroup + 1];
 #pragma unroll
   for (int i = 0; i < kDotGroup + 1; ++i) acc[i] = mk(0, 0);
-  const typename P::type* V0 = V + (long)i0 * NP;
+  const VP* V0 = V + (long)i0 * NP;
   for (long x = (long)blockIdx.x * NT + threadIdx.x; x < NP; x += (long)gridDim.x * NT) {
     const typename P::type wv = w[x];
 #pragma unroll
     for (int i = 0; i < kDotGroup; ++i)
-      if (i < ng) P::dot(acc[i].x, acc[i].y, V0[(long)i * NP + x], wv);
+      if (i < ng) P::dot(acc[i].x, acc[i].y, IO::load(V0[(long)i * NP + x]), wv);
     acc[kDotGroup].x += P::nrm2(wv);
   }
   const int stride = kDotGroup + 1;
@@ -996,11 +1049,12 @@ __global__ __launch_bounds__(NT) void arnoldi_dot_finalize_kernel(GmresBufs gb, 
 // finishes the Hessenberg column (DGKS test ||w_new|| >= ||w||/sqrt(2)) or
 // requests a second pass; mode 1 (only when st->reorth) uses h2 and finishes
 // the column with h1 + h2.
-template <typename T, int NT>
-__global__ __launch_bounds__(NT) void arnoldi_update_kernel(GmresBufs gb, const typename Pk<T>::type* __restrict__ V,
+template <typename T, typename VP, int NT>
+__global__ __launch_bounds__(NT) void arnoldi_update_kernel(GmresBufs gb, const VP* __restrict__ V,
                                                             typename Pk<T>::type* __restrict__ w, long NP, int j,
                                                             int mode) {
   using P = Pk<T>;
+  using IO = BasisIO<VP>;
   if (gb.st->done || (mode == 1 && !gb.st->reorth)) return;
   const int nv = j + 1;
   __shared__ double2 hs[kMaxRestart + 1];
@@ -1014,16 +1068,16 @@ __global__ __launch_bounds__(NT) void arnoldi_update_kernel(GmresBufs gb, const 
     typename P::type wv = w[x];
     int i = 0;
     for (; i + 4 <= nv; i += 4) {
-      const typename P::type v0 = V[(long)(i + 0) * NP + x];
-      const typename P::type v1 = V[(long)(i + 1) * NP + x];
-      const typename P::type v2 = V[(long)(i + 2) * NP + x];
-      const typename P::type v3 = V[(long)(i + 3) * NP + x];
+      const typename P::type v0 = IO::load(V[(long)(i + 0) * NP + x]);
+      const typename P::type v1 = IO::load(V[(long)(i + 1) * NP + x]);
+      const typename P::type v2 = IO::load(V[(long)(i + 2) * NP + x]);
+      const typename P::type v3 = IO::load(V[(long)(i + 3) * NP + x]);
       P::axpy(wv, hs[i + 0], v0);
       P::axpy(wv, hs[i + 1], v1);
       P::axpy(wv, hs[i + 2], v2);
       P::axpy(wv, hs[i + 3], v3);
     }
-    for (; i < nv; ++i) P::axpy(wv, hs[i], V[(long)i * NP + x]);
+    for (; i < nv; ++i) P::axpy(wv, hs[i], IO::load(V[(long)i * NP + x]));
     w[x] = wv;
     acc[0].x += P::nrm2(wv);
   }
@@ -1066,14 +1120,18 @@ __global__ __launch_bounds__(NT) void arnoldi_update_finalize_kernel(GmresBufs g
   }
 }
 
-// V_{j+1} = w / H[j+1,j]; skipped once the cycle is done.
-template <typename T>
+// V_{j+1} = w / H[j+1,j] (and, for a narrowed basis, the working-precision
+// copy vcur that the operator reads next); skipped once the cycle is done.
+template <typename T, typename VP>
 __global__ void arnoldi_normalize_kernel(GmresBufs gb, const typename Pk<T>::type* __restrict__ w,
-                                         typename Pk<T>::type* __restrict__ vnext, long NP) {
+                                         VP* __restrict__ vnext, typename Pk<T>::type* __restrict__ vcur, long NP) {
   if (gb.st->done) return;
   const double s = (gb.st->hnext > 0) ? 1.0 / gb.st->hnext : 0.0;
-  for (long x = (long)blockIdx.x * blockDim.x + threadIdx.x; x < NP; x += (long)gridDim.x * blockDim.x)
-    vnext[x] = Pk<T>::scale(w[x], s);
+  for (long x = (long)blockIdx.x * blockDim.x + threadIdx.x; x < NP; x += (long)gridDim.x * blockDim.x) {
+    const typename Pk<T>::type v = Pk<T>::scale(w[x], s);
+    vnext[x] = BasisIO<VP>::store(v);
+    if (vcur) vcur[x] = v;
+  }
 }
 
 // v0 = r / ||r|| and Givens/rhs initialisation for a new cycle.
@@ -1130,23 +1188,33 @@ __global__ void hess_solve_kernel(GmresBufs gb) {
 }
 
 // s = sum_{i < jlast} y_i V_i
-template <typename T>
-__global__ void lincomb_kernel(GmresBufs gb, const typename cvec<T>::type* __restrict__ V,
-                               typename cvec<T>::type* __restrict__ s, long N) {
-  using T2 = typename cvec<T>::type;
+// s = V y over the basis packets (FP64 accumulation).
+__device__ __forceinline__ void mac_packet(double (&acc)[4], double2 y, float4 v) {
+  acc[0] += y.x * (double)v.x - y.y * (double)v.y;
+  acc[1] += y.x * (double)v.y + y.y * (double)v.x;
+  acc[2] += y.x * (double)v.z - y.y * (double)v.w;
+  acc[3] += y.x * (double)v.w + y.y * (double)v.z;
+}
+__device__ __forceinline__ void mac_packet(double (&acc)[4], double2 y, double2 v) {
+  acc[0] += y.x * v.x - y.y * v.y;
+  acc[1] += y.x * v.y + y.y * v.x;
+}
+__device__ __forceinline__ void put_packet(float4& o, const double (&acc)[4]) {
+  o = make_float4((float)acc[0], (float)acc[1], (float)acc[2], (float)acc[3]);
+}
+__device__ __forceinline__ void put_packet(double2& o, const double (&acc)[4]) { o = make_double2(acc[0], acc[1]); }
+
+template <typename T, typename VP>
+__global__ void lincomb_kernel(GmresBufs gb, const VP* __restrict__ V, typename Pk<T>::type* __restrict__ s,
+                               long NP) {
   const int jl = gb.st->jlast;
   __shared__ double2 ys[kMaxRestart];
   for (int i = threadIdx.x; i < jl; i += blockDim.x) ys[i] = gb.y[i];
   __syncthreads();
-  for (long x = (long)blockIdx.x * blockDim.x + threadIdx.x; x < N; x += (long)gridDim.x * blockDim.x) {
-    double sr = 0, si = 0;
-    for (int i = 0; i < jl; ++i) {
-      const T2 v = V[(long)i * N + x];
-      sr += ys[i].x * (double)v.x - ys[i].y * (double)v.y;
-      si += ys[i].x * (double)v.y + ys[i].y * (double)v.x;
-    }
-    T2 o; o.x = (T)sr; o.y = (T)si;
-    s[x] = o;
+  for (long x = (long)blockIdx.x * blockDim.x + threadIdx.x; x < NP; x += (long)gridDim.x * blockDim.x) {
+    double acc[4] = {0, 0, 0, 0};
+    for (int i = 0; i < jl; ++i) mac_packet(acc, ys[i], BasisIO<VP>::load(V[(long)i * NP + x]));
+    put_packet(s[x], acc);
   }
 }
 
@@ -1186,14 +1254,23 @@ __global__ __launch_bounds__(NT) void residual_finalize_kernel(const double2* pa
 }
 
 // v0 = r / ||r|| in T precision.
-template <typename T>
-__global__ void residual_normalize_kernel(const double2* __restrict__ r, typename cvec<T>::type* __restrict__ v0,
-                                          long N, const GmresState* st) {
-  using T2 = typename cvec<T>::type;
+// v0 = r/||r|| into basis slot 0 (and its working-precision copy vcur).
+__device__ __forceinline__ void scaled_packet(const double2* r, long x, double s, double2& o) {
+  o = make_double2(r[x].x * s, r[x].y * s);
+}
+__device__ __forceinline__ void scaled_packet(const double2* r, long x, double s, float4& o) {
+  const double2 a = r[2 * x], b = r[2 * x + 1];
+  o = make_float4((float)(a.x * s), (float)(a.y * s), (float)(b.x * s), (float)(b.y * s));
+}
+template <typename T, typename VP>
+__global__ void residual_normalize_kernel(const double2* __restrict__ r, VP* __restrict__ v0,
+                                          typename Pk<T>::type* __restrict__ vcur, long NP, const GmresState* st) {
   const double s = (st->resnorm > 0) ? 1.0 / st->resnorm : 0.0;
-  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) {
-    T2 o; o.x = (T)(r[i].x * s); o.y = (T)(r[i].y * s);
-    v0[i] = o;
+  for (long x = (long)blockIdx.x * blockDim.x + threadIdx.x; x < NP; x += (long)gridDim.x * blockDim.x) {
+    typename Pk<T>::type v;
+    scaled_packet(r, x, s, v);
+    v0[x] = BasisIO<VP>::store(v);
+    if (vcur) vcur[x] = v;
   }
 }
 

@@ -111,6 +111,8 @@ struct wdg_ctx {
   // Krylov storage (allocated per precision on demand)
   DevBuf<double2> d_V64, d_w64, d_z64, d_s64;
   DevBuf<float2> d_V32, d_w32, d_z32, d_s32;
+  DevBuf<uint2> d_Vb16;               // 16-bit basis of the FP32 inner solver (2 complex per packet)
+  DevBuf<float2> d_vcur32;            // FP32 copy of the newest BF16 basis vector
   DevBuf<double2> d_H, d_cs, d_sn, d_g, d_h1, d_h2, d_yv, d_part;
   DevBuf<GmresState> d_st;
   std::vector<cplx> fnl, fnr;        // inflow data of the current potential (k basis)
@@ -480,10 +482,11 @@ void op_precond(wdg_ctx* c, const typename cvec<T>::type* r, typename cvec<T>::t
 
 void clear_graphs(wdg_ctx* c);
 
-void ensure_krylov(wdg_ctx* c, int m, bool fp64) {
+void ensure_krylov(wdg_ctx* c, int m, bool fp64, bool b16) {
   if (m > kMaxRestart) throw std::invalid_argument("restart exceeds the supported maximum of 64");
   // Captured graphs hold raw pointers: drop them before any buffer moves.
-  const void* before[6] = {c->d_V64.p, c->d_V32.p, c->d_part.p, c->d_H.p, c->d_w64.p, c->d_w32.p};
+  const void* before[8] = {c->d_V64.p, c->d_V32.p, c->d_part.p, c->d_H.p,
+                           c->d_w64.p, c->d_w32.p, c->d_Vb16.p, c->d_vcur32.p};
   if (c->m_alloc != m) {
     clear_graphs(c);
     c->d_H.alloc((size_t)(m + 1) * m);
@@ -508,13 +511,19 @@ void ensure_krylov(wdg_ctx* c, int m, bool fp64) {
     c->d_z64.alloc(N);
     c->d_s64.alloc(N);
   } else {
-    c->d_V32.alloc((size_t)(m + 1) * N);
+    if (b16) {
+      c->d_Vb16.alloc((size_t)(m + 1) * (N / 2));   // 2 complex per 8-byte packet
+      c->d_vcur32.alloc(N);
+    } else {
+      c->d_V32.alloc((size_t)(m + 1) * N);
+    }
     c->d_w32.alloc(N);
     c->d_z32.alloc(N);
     c->d_s32.alloc(N);
   }
-  const void* after[6] = {c->d_V64.p, c->d_V32.p, c->d_part.p, c->d_H.p, c->d_w64.p, c->d_w32.p};
-  for (int i = 0; i < 6; ++i)
+  const void* after[8] = {c->d_V64.p, c->d_V32.p, c->d_part.p, c->d_H.p,
+                          c->d_w64.p, c->d_w32.p, c->d_Vb16.p, c->d_vcur32.p};
+  for (int i = 0; i < 8; ++i)
     if (before[i] && before[i] != after[i]) clear_graphs(c);
 }
 
@@ -527,24 +536,41 @@ void residual64(wdg_ctx* c) {
                      c->d_st.p);
 }
 
-// One restart cycle in precision T (no host synchronisation).
-template <typename T>
-void gmres_cycle(wdg_ctx* c, int m, bool precond) {
+// One restart cycle in precision T (no host synchronisation).  VP is the
+// basis packet: the working packet of T, or uint2 for a BF16 basis of the
+// FP32 solver (then the newest vector is also kept in FP32 in vcur, which
+// is what the preconditioner and the operator read).
+template <typename T, typename VP>
+void gmres_cycle(wdg_ctx* c, int m, bool precond, bool single_pass) {
   using T2 = typename cvec<T>::type;
-  T2 *V, *w, *z, *s;
+  using PT = typename Pk<T>::type;
+  constexpr bool narrow = std::is_same<VP, uint2>::value || std::is_same<VP, ushort4>::value;
+  T2 *w, *z, *s;
+  VP* V;
+  T2* vcur = nullptr;
   if constexpr (std::is_same<T, double>::value) {
-    V = c->d_V64.p; w = c->d_w64.p; z = c->d_z64.p; s = c->d_s64.p;
+    V = reinterpret_cast<VP*>(c->d_V64.p); w = c->d_w64.p; z = c->d_z64.p; s = c->d_s64.p;
   } else {
-    V = c->d_V32.p; w = c->d_w32.p; z = c->d_z32.p; s = c->d_s32.p;
+    w = c->d_w32.p; z = c->d_z32.p; s = c->d_s32.p;
+    if constexpr (narrow) {
+      V = reinterpret_cast<VP*>(c->d_Vb16.p);
+      vcur = c->d_vcur32.p;
+    } else {
+      V = reinterpret_cast<VP*>(c->d_V32.p);
+    }
   }
   const long N = c->N;
+  const long NP = N * (long)sizeof(T2) / 16;        // working packets per vector
+  const double vbytes = (double)NP * sizeof(VP);    // bytes of one stored basis vector
+  PT* wp = reinterpret_cast<PT*>(w);
+  PT* vcp = reinterpret_cast<PT*>(vcur);
   GmresBufs gb = c->gb();
   const GmresState* st = c->d_st.p;
   hipLaunchKernelGGL(cycle_init_kernel, dim3(1), dim3(1), 0, c->stream, gb);
-  hipLaunchKernelGGL((residual_normalize_kernel<T>), dim3(c->red_blocks), dim3(256), 0, c->stream, c->d_r.p, V, N,
-                     st);
+  hipLaunchKernelGGL((residual_normalize_kernel<T, VP>), dim3(c->red_blocks), dim3(256), 0, c->stream, c->d_r.p, V,
+                     vcp, NP, st);
   for (int j = 0; j < m; ++j) {
-    T2* vj = V + (size_t)j * N;
+    const T2* vj = narrow ? vcur : reinterpret_cast<const T2*>(V + (size_t)j * NP);
     const T2* opin = vj;
     if (precond) {
       op_precond<T>(c, vj, z, st);
@@ -553,30 +579,31 @@ void gmres_cycle(wdg_ctx* c, int m, bool precond) {
     op_apply<T>(c, opin, w, st);
     const int nv = j + 1;
     const dim3 dg(c->red_blocks, (nv + kDotGroup - 1) / kDotGroup);
-    using PT = typename Pk<T>::type;
-    const long NP = N * (long)sizeof(T2) / 16;
-    PT* Vp = reinterpret_cast<PT*>(V);
-    PT* wp = reinterpret_cast<PT*>(w);
     const int nb = c->red_blocks;
     {
-      ProfScope ps(c, P_DOT, (nv + 1.0) * N * sizeof(T2));
-      hipLaunchKernelGGL((arnoldi_dot_kernel<T, 256>), dg, dim3(256), 0, c->stream, gb, Vp, wp, NP, nv, 0);
+      ProfScope ps(c, P_DOT, nv * vbytes + N * sizeof(T2));
+      hipLaunchKernelGGL((arnoldi_dot_kernel<T, VP, 256>), dg, dim3(256), 0, c->stream, gb, V, wp, NP, nv, 0);
     }
     hipLaunchKernelGGL((arnoldi_dot_finalize_kernel<256>), dim3(1), dim3(256), 0, c->stream, gb, nb, nv, 0);
     {
-      ProfScope ps(c, P_UPDATE, (nv + 2.0) * N * sizeof(T2));
-      hipLaunchKernelGGL((arnoldi_update_kernel<T, 256>), dim3(nb), dim3(256), 0, c->stream, gb, Vp, wp, NP, j, 0);
+      ProfScope ps(c, P_UPDATE, nv * vbytes + 2.0 * N * sizeof(T2));
+      hipLaunchKernelGGL((arnoldi_update_kernel<T, VP, 256>), dim3(nb), dim3(256), 0, c->stream, gb, V, wp, NP, j,
+                         0);
     }
     hipLaunchKernelGGL((arnoldi_update_finalize_kernel<256>), dim3(1), dim3(256), 0, c->stream, gb, nb, j, 0);
-    hipLaunchKernelGGL((arnoldi_dot_kernel<T, 256>), dg, dim3(256), 0, c->stream, gb, Vp, wp, NP, nv, 1);
-    hipLaunchKernelGGL((arnoldi_dot_finalize_kernel<256>), dim3(1), dim3(256), 0, c->stream, gb, nb, nv, 1);
-    hipLaunchKernelGGL((arnoldi_update_kernel<T, 256>), dim3(nb), dim3(256), 0, c->stream, gb, Vp, wp, NP, j, 1);
-    hipLaunchKernelGGL((arnoldi_update_finalize_kernel<256>), dim3(1), dim3(256), 0, c->stream, gb, nb, j, 1);
-    hipLaunchKernelGGL((arnoldi_normalize_kernel<T>), dim3(c->red_blocks), dim3(256), 0, c->stream, gb, wp,
-                       Vp + (size_t)(j + 1) * NP, NP);
+    if (!single_pass) {  // DGKS second pass, taken on the device only when requested (st->reorth)
+      hipLaunchKernelGGL((arnoldi_dot_kernel<T, VP, 256>), dg, dim3(256), 0, c->stream, gb, V, wp, NP, nv, 1);
+      hipLaunchKernelGGL((arnoldi_dot_finalize_kernel<256>), dim3(1), dim3(256), 0, c->stream, gb, nb, nv, 1);
+      hipLaunchKernelGGL((arnoldi_update_kernel<T, VP, 256>), dim3(nb), dim3(256), 0, c->stream, gb, V, wp, NP, j,
+                         1);
+      hipLaunchKernelGGL((arnoldi_update_finalize_kernel<256>), dim3(1), dim3(256), 0, c->stream, gb, nb, j, 1);
+    }
+    hipLaunchKernelGGL((arnoldi_normalize_kernel<T, VP>), dim3(c->red_blocks), dim3(256), 0, c->stream, gb, wp,
+                       V + (size_t)(j + 1) * NP, vcp, NP);
   }
   hipLaunchKernelGGL(hess_solve_kernel, dim3(1), dim3(256), 0, c->stream, gb);
-  hipLaunchKernelGGL((lincomb_kernel<T>), dim3(c->red_blocks), dim3(256), 0, c->stream, gb, V, s, N);
+  hipLaunchKernelGGL((lincomb_kernel<T, VP>), dim3(c->red_blocks), dim3(256), 0, c->stream, gb, V,
+                     reinterpret_cast<PT*>(s), NP);
   const T2* corr = s;
   if (precond) {
     op_precond<T>(c, s, z, nullptr);
@@ -588,17 +615,20 @@ void gmres_cycle(wdg_ctx* c, int m, bool precond) {
 // One refinement cycle followed by the FP64 true residual.  With graphs
 // enabled (and no per-kernel event profiling) the whole sequence is captured
 // once per (precision, restart, preconditioner) and replayed.
-void run_cycle(wdg_ctx* c, bool fp64, int m, bool precond, bool use_graph) {
+void run_cycle(wdg_ctx* c, bool fp64, int basis, int m, bool precond, bool single_pass, bool use_graph) {
   auto enqueue = [&]() {
-    if (fp64) gmres_cycle<double>(c, m, precond);
-    else gmres_cycle<float>(c, m, precond);
+    if (fp64) gmres_cycle<double, double2>(c, m, precond, single_pass);
+    else if (basis == WDG_BASIS_BF16) gmres_cycle<float, uint2>(c, m, precond, single_pass);
+    else if (basis == WDG_BASIS_FP16) gmres_cycle<float, ushort4>(c, m, precond, single_pass);
+    else gmres_cycle<float, float4>(c, m, precond, single_pass);
     residual64(c);
   };
   if (!use_graph || c->prof_mask) {
     enqueue();
     return;
   }
-  const long key = ((long)m << 2) | (fp64 ? 2 : 0) | (precond ? 1 : 0);
+  const long key = ((long)m << 5) | (single_pass ? 16 : 0) | ((long)(fp64 ? 0 : basis) << 2) | (fp64 ? 2 : 0) |
+                   (precond ? 1 : 0);
   auto it = c->graphs.find(key);
   if (it == c->graphs.end()) {
     HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
@@ -907,7 +937,8 @@ struct SolveRun {
   wdg_ctx* c;
   wdg_solve_opts o;
   wdg_solve_info* info;
-  bool fp64 = true, active = false, pending = false;
+  bool fp64 = true, b16 = false, active = false, pending = false;
+  int basis = WDG_BASIS_WORKING;
   int m = 0, its = 0, cycles = 0, reorth = 0, max_outer = 0;
   double tol = 0, inner = 0, relres = 1, reported = 1, xs = 1, prev_relres = 1;
   int m_max = 0;
@@ -928,7 +959,12 @@ struct SolveRun {
     fp64 = o.precision == WDG_PREC_FP64;
     m = (int)std::min<long>(o.restart, N);
     m_max = (int)std::min<long>(std::max(o.restart_max, o.restart), std::min<long>(N, kMaxRestart));
-    ensure_krylov(c, m, fp64);
+    if (o.krylov_basis != WDG_BASIS_WORKING && o.krylov_basis != WDG_BASIS_BF16 &&
+        o.krylov_basis != WDG_BASIS_FP16)
+      throw std::invalid_argument("unknown krylov_basis");
+    basis = fp64 ? WDG_BASIS_WORKING : o.krylov_basis;
+    b16 = basis != WDG_BASIS_WORKING;
+    ensure_krylov(c, m, fp64, b16);
     if (!c->use_fft) autotune(c, fp64);
     if (!c->h_st) HIPCHK(hipHostMalloc((void**)&c->h_st, sizeof(GmresState), hipHostMallocDefault));
     xs = (c->rhs_norm > 0) ? c->rhs_norm / c->g.scale : 1.0;  // physical x = xs * x_device
@@ -977,14 +1013,14 @@ struct SolveRun {
       // gets a 1.5x longer Krylov space next time (up to restart_max).
       if (hs.jlast == m && m < m_max && relres > prev_relres / 3.0) {
         m = std::min(m_max, m + m / 2);
-        ensure_krylov(c, m, fp64);
+        ensure_krylov(c, m, fp64, b16);
       }
     }
     if (!(relres > tol) || its >= o.max_iter || cycles >= max_outer || !std::isfinite(relres)) active = false;
   }
   void step() {
     const int mc = std::min<int>(m, o.max_iter - its);
-    run_cycle(c, fp64, mc, o.use_precond != 0, o.use_graph != 0 && mc == m);
+    run_cycle(c, fp64, basis, mc, o.use_precond != 0, o.orthogonalization == 1, o.use_graph != 0 && mc == m);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(c->h_st, c->d_st.p, sizeof(GmresState), hipMemcpyDeviceToHost, c->stream));
     cycles += 1;

@@ -45,6 +45,7 @@ class SolveOptions:
     orth: str = "cgs1"              # "cgs1" (single pass, default) or "cgs2" (DGKS second pass)
     restart_max: int = 0            # adaptive restart cap (0: fixed restart)
     warm_device: bool = False       # start from the operator's last (device) solution
+    krylov_basis: str = "fp16"      # mixed: storage of the inner Krylov basis, "fp32", "bf16" or "fp16"
 
     def to_c(self) -> _lib.wdg_solve_opts:
         if self.precision not in PRECISIONS:
@@ -53,7 +54,9 @@ class SolveOptions:
                                    PRECISIONS[self.precision], float(self.inner_tol),
                                    int(self.max_outer), 1 if self.precondition else 0,
                                    1 if self.use_graph else 0, {"cgs2": 0, "cgs1": 1}[self.orth],
-                                   int(self.restart_max), 1 if self.warm_device else 0)
+                                   int(self.restart_max), 1 if self.warm_device else 0,
+                                   {"fp32": _lib.WDG_BASIS_WORKING, "bf16": _lib.WDG_BASIS_BF16,
+                                    "fp16": _lib.WDG_BASIS_FP16}[self.krylov_basis])
 
 
 def _as_c128(a, n):

@@ -71,6 +71,17 @@ def _mixed_reaches_tolerance(solved):
     assert ej < 1e-7
 
 
+@pytest.mark.parametrize("basis", ["fp32", "bf16", "fp16"])
+def test_mixed_krylov_basis_storage(solved, basis):
+    """The inner basis may be stored narrower than FP32; the FP64 refinement
+    still reaches the tolerance and lands on the oracle solution."""
+    x, info = solved["op"].solve(SolveOptions(tol=1e-12, restart=30, precision="mixed", krylov_basis=basis))
+    print(basis, info)
+    assert info["flag"] == 0 and info["true_relres"] <= 1e-12
+    err = np.linalg.norm(x - solved["x_ref"]) / np.linalg.norm(solved["x_ref"])
+    assert err < 1e-9
+
+
 def test_warm_start_and_flag(solved):
     op = solved["op"]
     x, info = op.solve(SolveOptions(tol=1e-10))
