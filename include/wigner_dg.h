@@ -205,6 +205,25 @@ int wdg_precond_device(wdg_ctx* ctx, const void* d_r, void* d_z, int32_t precisi
  * Either output may be NULL. */
 int wdg_host_setup(const wdg_desc* desc, const wdg_potential* pot, double* rhs, double* kmats);
 
+/* Explicit RK4 time stepping of d rho/dt = b_scale*rhs - a_scale*A rho
+ * (DG_code/core/transient.m:12-16; DG_code/solve_transport_DG_2D_transient_SC_new.m:145-158,
+ * where A is prescaled by 2e-10 and rhs is not).  Starts from rho0 (host,
+ * 2N doubles) or, when rho0 is NULL, from the context's last solution;
+ * the final state replaces that solution (rho_out may be NULL).  Every
+ * save_every steps the observables of the state (as wdg_observables, with
+ * m_nodes) are written to column s of n_t and j_t (n_chi*n_k_chi x n_saves,
+ * column-major); n_saves = nsteps / save_every must not exceed max_saves.
+ * The potential and inflow of the last wdg_set_potential stay fixed. */
+typedef struct wdg_transient_opts {
+  double  dt;          /* time step [s] (mat.dt)                                 */
+  int32_t nsteps;      /* number of steps (mat.Nt)                               */
+  int32_t save_every;  /* observables every save_every steps (>= 1)              */
+  double  a_scale;     /* factor on A (the reference uses 2e-10)                 */
+  double  b_scale;     /* factor on rhs (the reference uses 1)                   */
+} wdg_transient_opts;
+int wdg_transient_rk4(wdg_ctx* ctx, const wdg_transient_opts* opts, const double* rho0, double* rho_out,
+                      const double* m_nodes, double* n_t, double* j_t, int32_t max_saves);
+
 /* Formulation of the Theta[V] apply.  WDG_THETA_FFT evaluates phi U and
  * i phi^H Z as chirp-z transforms (FFTs of length 2^p 3^q >= nk+n_xi-1) in
  * one fused kernel per element; WDG_THETA_GEMM runs them as dense batched

@@ -1,0 +1,45 @@
+"""FP64 CPU restatement of the reference's explicit RK4 transient (1D path).
+
+TEST INFRASTRUCTURE ONLY (see oracle/__init__.py).
+
+  DG_code/core/transient.m:12-16 and
+  DG_code/solve_transport_DG_2D_transient_SC_new.m:145-158   -> rk4
+
+    k1 = rhs - A rho;  k2 = rhs - A(rho + dt/2 k1);  k3 = rhs - A(rho + dt/2 k2)
+    k4 = rhs - A(rho + dt k3);  rho += dt/6 (k1 + 2 k2 + 2 k3 + k4)
+
+with A prescaled by a_scale (2e-10 in the reference, :148) and rhs by
+b_scale (1 there).  Observables after each saved step as
+solve_transport_DG_2D_transient_SC_new.m:172-178 (n_xi even branch).
+
+Pinning (tests/test_oracle_transient.py): against the exact solution of the
+linear ODE, rho(t) = rho_inf + expm(-a A t)(rho0 - rho_inf), with the global
+error falling as dt^4; the stationary state is a fixed point.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import wigner1d as W
+
+
+def rk4(A, rhs, rho0, dt, nsteps, a_scale=2e-10, b_scale=1.0, save_every=1, p=None, m_nodes=None):
+    """Returns (rho, n_t, j_t); n_t, j_t are (nodes, nsteps // save_every)
+    when p and m_nodes are given, else None."""
+    As = A * a_scale
+    b = rhs * b_scale
+    rho = np.asarray(rho0, complex).copy()
+    n_t, j_t = [], []
+    for step in range(1, nsteps + 1):
+        k1 = b - As @ rho
+        k2 = b - As @ (rho + dt / 2 * k1)
+        k3 = b - As @ (rho + dt / 2 * k2)
+        k4 = b - As @ (rho + dt * k3)
+        rho = rho + dt / 6 * (k1 + 2 * k2 + 2 * k3 + k4)
+        if p is not None and step % save_every == 0:
+            n, j, _ = W.observables(p, rho, m_nodes)
+            n_t.append(n)
+            j_t.append(j)
+    if p is None:
+        return rho, None, None
+    return rho, np.array(n_t).T, np.array(j_t).T

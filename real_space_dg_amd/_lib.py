@@ -73,6 +73,11 @@ class wdg_solve_info(C.Structure):
                 ("reorth_steps", C.c_int32)]
 
 
+class wdg_transient_opts(C.Structure):
+    _fields_ = [("dt", C.c_double), ("nsteps", C.c_int32), ("save_every", C.c_int32),
+                ("a_scale", C.c_double), ("b_scale", C.c_double)]
+
+
 class wdg_poisson_desc(C.Structure):
     _fields_ = [("n_mat", C.c_int32), ("method", C.c_int32), ("x_mat", C.POINTER(C.c_double)),
                 ("eps_r", C.POINTER(C.c_double)), ("net_doping", C.POINTER(C.c_double)), ("temp", C.c_double)]
@@ -103,6 +108,7 @@ SIGNATURES = {
     "wdg_apply_device": (C.c_int, [_P, _P, _P, C.c_int32]),
     "wdg_precond_device": (C.c_int, [_P, _P, _P, C.c_int32]),
     "wdg_host_setup": (C.c_int, [C.POINTER(wdg_desc), C.POINTER(wdg_potential), _D, _D]),
+    "wdg_transient_rk4": (C.c_int, [_P, C.POINTER(wdg_transient_opts), _D, _D, _D, _D, _D, C.c_int32]),
     "wdg_set_theta_method": (C.c_int, [_P, C.c_int32]),
     "wdg_theta_method": (C.c_int, [_P]),
     "wdg_set_profiling": (C.c_int, [_P, C.c_uint32]),

@@ -1423,6 +1423,45 @@ __global__ void scale_complex_kernel(double2* __restrict__ x, double s, long N) 
   }
 }
 
+// ---------------------------------------------------------------------------
+// Explicit RK4 of d rho/dt = c_b rhs - c_A A rho (DG_code/core/transient.m:12-16,
+// DG_code/solve_transport_DG_2D_transient_SC_new.m:145-158) in device units:
+// with rho = sigma x, sigma = ||b||/s_A, the ODE is dx/dt' = c_b b^ - c_A A^ x
+// for t' = s_A t.  Stage s gets y = A^ tmp_s and forms k_s = c_b b^ - c_A y;
+//   s = 1: acc = k1,    tmp = x + h/2 k1
+//   s = 2: acc += 2k2,  tmp = x + h/2 k2
+//   s = 3: acc += 2k3,  tmp = x + h k3
+//   s = 4: x += h/6 (acc + k4)
+// ---------------------------------------------------------------------------
+__global__ void rk4_stage_kernel(int stage, const double2* __restrict__ b, const double2* __restrict__ y,
+                                 double2* __restrict__ x, double2* __restrict__ tmp, double2* __restrict__ acc,
+                                 double cb, double ca, double h, long N) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) {
+    const double2 bi = b[i], yi = y[i];
+    const double kr = cb * bi.x - ca * yi.x, ki = cb * bi.y - ca * yi.y;
+    if (stage == 4) {
+      const double2 a = acc[i];
+      double2 xi = x[i];
+      xi.x += h / 6.0 * (a.x + kr);
+      xi.y += h / 6.0 * (a.y + ki);
+      x[i] = xi;
+      continue;
+    }
+    double2 a;
+    if (stage == 1) {
+      a = mk(kr, ki);
+    } else {
+      a = acc[i];
+      a.x += 2.0 * kr;
+      a.y += 2.0 * ki;
+    }
+    acc[i] = a;
+    const double c = (stage == 3) ? h : 0.5 * h;
+    const double2 xi = x[i];
+    tmp[i] = mk(xi.x + c * kr, xi.y + c * ki);
+  }
+}
+
 // Precision conversion helpers.
 __global__ void d2f_kernel(const double2* __restrict__ a, float2* __restrict__ b, double s, long N) {
   for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x)

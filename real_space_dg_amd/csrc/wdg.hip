@@ -169,6 +169,8 @@ int guarded(wdg_ctx* c, F&& f) {
     return fail(c, WDG_ERR_MEMORY, "host allocation failed");
   } catch (const std::invalid_argument& e) {
     return fail(c, WDG_ERR_ARG, e.what());
+  } catch (const std::logic_error& e) {   // a call out of order (no potential, no solution)
+    return fail(c, WDG_ERR_STATE, e.what());
   } catch (const std::runtime_error& e) {
     return fail(c, WDG_ERR_ARG, e.what());
   }
@@ -1529,6 +1531,76 @@ int wdg_host_setup(const wdg_desc* desc, const wdg_potential* pot, double* rhs, 
       build_rhs(g, fnl, fnr, r);
       std::memcpy(rhs, r.data(), r.size() * sizeof(double));
     }
+    return WDG_OK;
+  });
+}
+
+int wdg_transient_rk4(wdg_ctx* ctx, const wdg_transient_opts* o, const double* rho0, double* rho_out,
+                      const double* m_nodes, double* n_t, double* j_t, int32_t max_saves) {
+  if (!ctx || !o || !m_nodes) return fail(ctx, WDG_ERR_ARG, "null argument");
+  if (!ctx->have_potential) return fail(ctx, WDG_ERR_STATE, "wdg_set_potential has not been called");
+  return guarded(ctx, [&]() {
+    if (!(o->dt > 0) || o->nsteps < 0 || o->save_every < 1 || !std::isfinite(o->a_scale) ||
+        !std::isfinite(o->b_scale))
+      throw std::invalid_argument("dt must be positive, nsteps >= 0, save_every >= 1, finite scales");
+    const int nsave = o->nsteps / o->save_every;
+    if (nsave > 0 && (!n_t || !j_t)) throw std::invalid_argument("n_t and j_t are required when saving");
+    if (nsave > max_saves) throw std::invalid_argument("nsteps/save_every exceeds max_saves");
+    const Geometry& g = ctx->g;
+    const long N = ctx->N;
+    const int nodes = ctx->ncols;
+    // device units: rho = sigma x with sigma = ||b|| / s_A
+    const double sigma = (ctx->rhs_norm > 0) ? ctx->rhs_norm / g.scale : 1.0;
+    if (rho0) {
+      ctx->stage.resize(2 * N);
+      for (long i = 0; i < 2 * N; ++i) {
+        if (!std::isfinite(rho0[i])) throw std::invalid_argument("rho0 must contain only finite values");
+        ctx->stage[i] = rho0[i] / sigma;
+      }
+      HIPCHK(hipMemcpy(ctx->d_x.p, ctx->stage.data(), 2 * N * 8, hipMemcpyHostToDevice));
+    } else if (!ctx->have_solution) {
+      throw std::logic_error("no initial state: pass rho0 or solve first");
+    } else if (ctx->sol_scale != sigma) {
+      hipLaunchKernelGGL(scale_complex_kernel, dim3(ctx->red_blocks), dim3(256), 0, ctx->stream, ctx->d_x.p,
+                         ctx->sol_scale / sigma, N);
+    }
+    ctx->sol_scale = sigma;
+    ctx->d_u64.alloc(N);   // tmp
+    ctx->d_v64.alloc(N);   // acc
+    DevBuf<double2> obs_t;
+    if (nsave > 0) obs_t.alloc((size_t)nsave * 2 * nodes);
+    const double h = o->dt * g.scale;
+    const double cb = o->b_scale, ca = o->a_scale;
+    const dim3 gr(ctx->red_blocks), bl(256);
+    int saved = 0;
+    for (int step = 1; step <= o->nsteps; ++step) {
+      for (int stage = 1; stage <= 4; ++stage) {
+        op_apply<double>(ctx, stage == 1 ? ctx->d_x.p : ctx->d_u64.p, ctx->d_y.p, nullptr);
+        hipLaunchKernelGGL(rk4_stage_kernel, gr, bl, 0, ctx->stream, stage, ctx->d_b.p, ctx->d_y.p, ctx->d_x.p,
+                           ctx->d_u64.p, ctx->d_v64.p, cb, ca, h, N);
+      }
+      if (step % o->save_every == 0 && saved < nsave) {
+        hipLaunchKernelGGL(observables_kernel, dim3(nodes), dim3(256), 0, ctx->stream, ctx->d_x.p, ctx->d_phic.p,
+                           g.nk, obs_t.p + (size_t)saved * 2 * nodes);
+        ++saved;
+      }
+    }
+    HIPCHK(hipGetLastError());
+    std::vector<double2> obs((size_t)nsave * 2 * nodes);
+    if (nsave > 0)
+      HIPCHK(hipMemcpyAsync(obs.data(), obs_t.p, obs.size() * sizeof(double2), hipMemcpyDeviceToHost, ctx->stream));
+    if (rho_out) HIPCHK(hipMemcpyAsync(rho_out, ctx->d_x.p, N * 16, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    ctx->have_solution = true;
+    if (rho_out)
+      for (long i = 0; i < 2 * N; ++i) rho_out[i] *= sigma;
+    const double hb = kHVT / 2.0 / M_PI;
+    for (int sidx = 0; sidx < nsave; ++sidx)
+      for (int i = 0; i < nodes; ++i) {
+        const double2 a = obs[((size_t)sidx * 2 * nodes) + 2 * i], b = obs[((size_t)sidx * 2 * nodes) + 2 * i + 1];
+        n_t[(size_t)sidx * nodes + i] = sigma * 0.5 * (a.x + b.x);
+        j_t[(size_t)sidx * nodes + i] = sigma * kQIP * hb / m_nodes[i] * (b.y - a.y) / g.delta_xi;
+      }
     return WDG_OK;
   });
 }

@@ -182,6 +182,26 @@ class WignerOperator:
         check(self._lib.wdg_observables(self._h, dptr(r), dptr(m), dptr(n), dptr(j), dptr(ru)), self._h)
         return n, j, (None if ru is None else ru.T)
 
+    # ---- transient ----
+    def transient_rk4(self, dt: float, nsteps: int, m_nodes, save_every: int = 1, a_scale: float = 2e-10,
+                      b_scale: float = 1.0, rho0=None, return_rho: bool = True):
+        """Explicit RK4 of d rho/dt = b_scale*rhs - a_scale*A rho from rho0 (or
+        the last solution).  DG_code/core/transient.m:12-16 and
+        DG_code/solve_transport_DG_2D_transient_SC_new.m:145-158 (a_scale =
+        2e-10 there).  Returns (rho, n_t, j_t) with n_t, j_t of shape
+        (nodes, nsteps // save_every)."""
+        nodes = self.n_chi * self.n_k_chi
+        m = np.ascontiguousarray(np.broadcast_to(np.asarray(m_nodes, float), (nodes,)))
+        nsave = int(nsteps) // int(save_every)
+        n_t = np.zeros((max(nsave, 1), nodes))
+        j_t = np.zeros((max(nsave, 1), nodes))
+        r0 = None if rho0 is None else _as_c128(rho0, self.N)
+        rho = np.empty(self.N, np.complex128) if return_rho else None
+        o = _lib.wdg_transient_opts(float(dt), int(nsteps), int(save_every), float(a_scale), float(b_scale))
+        check(self._lib.wdg_transient_rk4(self._h, C.byref(o), dptr(r0), dptr(rho), dptr(m), dptr(n_t),
+                                          dptr(j_t), max(nsave, 1)), self._h)
+        return rho, n_t[:nsave].T, j_t[:nsave].T
+
     # ---- electrostatics (Gummel loop) ----
     def poisson_setup(self, x_mat_m, eps_r, net_doping, temp: float = 300.0, method: str = "newton-raphson"):
         x = np.ascontiguousarray(np.asarray(x_mat_m, float))

@@ -1,0 +1,62 @@
+"""Device RK4 transient (wdg_transient_rk4) against the oracle's RK4."""
+import numpy as np
+import pytest
+import scipy.sparse.linalg as spla
+
+from oracle import transient as TR
+from real_space_dg_amd.solver import SolveOptions
+from tests._cases import build_case, gpu_operator, oracle_system, rand_c
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def case():
+    c = build_case(8, 2, 64, 0.1)
+    A, G, rhs = oracle_system(c)
+    c["K"] = (A + G).tocsc()
+    c["rhs"] = rhs
+    c["lam"] = np.max(np.abs(np.linalg.eigvals(c["K"].toarray())))
+    c["op"] = gpu_operator(c)
+    c["m_nodes"] = np.full(c["p"]["N_chi"] * c["p"]["N_K_chi"], c["m"][0])
+    yield c
+    c["op"].close()
+
+
+def test_rk4_from_perturbed_state_matches_oracle(case):
+    a = 2e-10
+    x_stat = spla.spsolve(case["K"], case["rhs"])
+    rho0 = x_stat + 0.05 * np.linalg.norm(x_stat) / np.sqrt(x_stat.size) * rand_c(x_stat.size, 7)
+    dt = 0.5 / (a * case["lam"])
+    ref, n_ref, j_ref = TR.rk4(case["K"], case["rhs"], rho0, dt, 24, a_scale=a, b_scale=a, save_every=4,
+                               p=case["p"], m_nodes=case["m_nodes"])
+    rho, n_t, j_t = case["op"].transient_rk4(dt, 24, case["m_nodes"], save_every=4, a_scale=a, b_scale=a,
+                                             rho0=rho0)
+    assert n_t.shape == n_ref.shape == (case["m_nodes"].size, 6)
+    assert np.linalg.norm(rho - ref) <= 1e-11 * np.linalg.norm(ref)
+    assert np.max(np.abs(n_t - n_ref)) <= 1e-11 * np.max(np.abs(n_ref))
+    assert np.max(np.abs(j_t - j_ref)) <= 1e-9 * np.max(np.abs(j_ref))
+    # the perturbation decays towards the stationary state
+    assert np.linalg.norm(rho - x_stat) < np.linalg.norm(rho0 - x_stat)
+
+
+def test_rk4_from_device_solution_with_reference_scales(case):
+    op = case["op"]
+    x, info = op.solve(SolveOptions(tol=1e-13, precision="fp64"))
+    assert info["flag"] == 0
+    a, b = 2e-10, 1.0   # solve_transport_DG_2D_transient_SC_new.m:148 scales A only
+    dt = 0.25 / (a * case["lam"])
+    ref, n_ref, j_ref = TR.rk4(case["K"], case["rhs"], x, dt, 10, a_scale=a, b_scale=b, save_every=5,
+                               p=case["p"], m_nodes=case["m_nodes"])
+    rho, n_t, j_t = op.transient_rk4(dt, 10, case["m_nodes"], save_every=5, a_scale=a, b_scale=b)
+    assert np.linalg.norm(rho - ref) <= 1e-10 * np.linalg.norm(ref)
+    assert np.max(np.abs(n_t - n_ref)) <= 1e-10 * np.max(np.abs(n_ref))
+    # the state stays on the device: observables of the last solution are those of rho
+    n_last, _, _ = op.observables(case["m_nodes"])
+    assert np.allclose(n_last, n_t[:, -1], rtol=1e-12, atol=0)
+
+
+def test_rk4_rejects_bad_input(case):
+    from real_space_dg_amd._lib import WdgError
+    with pytest.raises(WdgError):
+        case["op"].transient_rk4(-1.0, 3, case["m_nodes"])

@@ -1,0 +1,47 @@
+"""Pins of the oracle's RK4 transient against the exact linear-ODE solution."""
+import numpy as np
+import scipy.linalg as sla
+import scipy.sparse.linalg as spla
+
+from oracle import transient as TR
+from tests._cases import build_case, oracle_system
+
+
+def _small():
+    c = build_case(4, 2, 32, 0.1)
+    A, G, rhs = oracle_system(c)
+    K = (A + G).tocsc()
+    return c, K, rhs
+
+
+def test_rk4_matches_matrix_exponential_with_fourth_order_error():
+    c, K, rhs = _small()
+    a = 2e-10
+    Kd = K.toarray() * a
+    lam = np.max(np.abs(np.linalg.eigvals(Kd)))
+    rho_inf = np.linalg.solve(Kd, rhs)   # fixed point of rhs - a K rho
+    rng = np.random.default_rng(0)
+    rho0 = rho_inf + 0.1 * np.linalg.norm(rho_inf) / np.sqrt(rho_inf.size) * (
+        rng.standard_normal(rho_inf.size) + 1j * rng.standard_normal(rho_inf.size))
+    T = 1.0 / lam * 8
+    exact = rho_inf + sla.expm(-Kd * T) @ (rho0 - rho_inf)
+    errs = []
+    for nsteps in (40, 80):
+        dt = T / nsteps
+        rho, _, _ = TR.rk4(K, rhs, rho0, dt, nsteps, a_scale=a)
+        errs.append(np.linalg.norm(rho - exact) / np.linalg.norm(exact))
+    assert errs[1] < 1e-6
+    ratio = errs[0] / errs[1]
+    assert 12 < ratio < 20, (errs, ratio)
+
+
+def test_stationary_state_is_a_fixed_point():
+    c, K, rhs = _small()
+    a, bsc = 2e-10, 3.0
+    rho_inf = spla.spsolve(K, rhs) * bsc / a
+    lam = np.max(np.abs(np.linalg.eigvals(K.toarray() * a)))
+    rho, n_t, j_t = TR.rk4(K, rhs, rho_inf, 0.5 / lam, 5, a_scale=a, b_scale=bsc, p=c["p"],
+                           m_nodes=np.full(c["p"]["N_chi"] * c["p"]["N_K_chi"], c["m"][0]))
+    assert np.linalg.norm(rho - rho_inf) <= 1e-9 * np.linalg.norm(rho_inf)
+    assert n_t.shape == (c["p"]["N_chi"] * c["p"]["N_K_chi"], 5)
+    assert np.allclose(n_t[:, 0], n_t[:, -1], rtol=1e-9)
