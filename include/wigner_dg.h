@@ -249,6 +249,124 @@ int wdg_set_profiling(wdg_ctx* ctx, uint32_t mask);
 int wdg_profile(wdg_ctx* ctx, char* names, int32_t names_len, double* avg_ms, int64_t* launches,
                 double* units, int32_t max_entries, int32_t* n_entries);
 
+/* ==========================================================================
+ * Full-2D rho-basis operator F(rho_x, rho_y, X, Y)
+ * (DG_code/core/get_SysM_full2D.m:1-91, DG_code/solve_transport_DG_full2D.m)
+ *
+ * Vector order as initParams_full2D.m:4-8,105-107: rho_x fastest, then
+ * rho_y, the X DG DOF and the Y DG DOF (N = n_rho_x*n_rho_y*n_X_dof*n_Y_dof
+ * interleaved complex doubles).  A full-2D context is a wdg_ctx: wdg_apply,
+ * wdg_diagonal, wdg_precond, wdg_solve, wdg_solve_batch, wdg_rhs, wdg_size,
+ * the profiling calls and wdg_destroy work on it; the 1D-only calls
+ * (wdg_set_potential, wdg_observables, wdg_poisson_*, wdg_transient_rk4,
+ * wdg_set_theta_method) return WDG_ERR_UNSUPPORTED.
+ * ========================================================================== */
+
+/* Grid (initParams_full2D.m:58-111): relative FV axes and rectangular DG axes. */
+typedef struct wdg2_desc {
+  int32_t n_rho_x, n_rho_y;   /* FV cells of rho_x, rho_y (N_rho_x, N_rho_y), >= 2           */
+  int32_t nl_x, nl_y;         /* DG nodes per element (N_K_X, N_K_Y), 2..4                   */
+  int32_t ne_x, ne_y;         /* DG elements (N_el_X, N_el_Y)                                */
+  double  delta_x, delta_y;   /* FV cell widths; central difference D = (-1, 0, +1)/(2 delta)
+                                 (initParams_full2D.m:199-214)                                */
+} wdg2_desc;
+
+/* One operator part coef * kron(C, R) (get_Diff_full2D.m:463-474 addOperatorPart),
+ * C acting along centre axis `axis` and R along the relative axis of the same
+ * name.  Centre factors:
+ *   BANDED - block tridiagonal along the axis: per element e an nl x nl
+ *            row-major block (c_diag, ne*nl*nl), the column that couples to
+ *            the last node of e-1 (c_lower, ne*nl) and to the first node of
+ *            e+1 (c_upper, ne*nl): H1 / interior H2, get_Diff_full2D.m:206-266.
+ *   FACE   - lift (c_diag, nl values) of the trace node at the low (face 0,
+ *            first element, node 0) or high end (face 1, last element, last
+ *            node) into that element; rows of transverse DOF t are kept when
+ *            mask[t] != 0 (mask may be NULL): leftBoundary/rightBoundary and
+ *            the contact masks, get_Diff_full2D.m:393-452.
+ * Relative factors: CENTRAL -i D (Ax1D/Ay1D, initParams_full2D.m:99-102),
+ * IDENTITY (Rusanov beta I, get_Diff_full2D.m:454-462), DENSE n_rho x n_rho
+ * complex row-major r_dense (|A|, A+ R, A- R: get_Diff_full2D.m:295-347,718-730).
+ * BANDED parts pair with any relative factor, FACE parts with DENSE. */
+enum { WDG2_AXIS_X = 0, WDG2_AXIS_Y = 1 };
+enum { WDG2_CENTER_BANDED = 0, WDG2_CENTER_FACE = 1 };
+enum { WDG2_REL_CENTRAL = 0, WDG2_REL_IDENTITY = 1, WDG2_REL_DENSE = 2 };
+typedef struct wdg2_part {
+  int32_t axis, center, rel, face;
+  double  coef;
+  const double*  c_diag;
+  const double*  c_lower;
+  const double*  c_upper;
+  const int32_t* mask;
+  const double*  r_dense;
+} wdg2_part;
+
+/* Creates a full-2D context from the transport parts (get_Diff_full2D.m:80-146).
+ * The potential must follow (wdg2_set_potential). */
+int wdg2_create(const wdg2_desc* desc, const wdg2_part* parts, int32_t n_parts, wdg_ctx** out);
+
+/* Potential and CAP (get_Drift_full2D.m:1-209, get_CAP_full2D.m:1-39).
+ * FV_CONSISTENT: (1/16) P^T diag(DeltaV) P on the relative vertices, projected
+ *   over each centre element's tensor quadrature (get_Drift_full2D.m:331-364):
+ *   center_x/_y are the quadrature coordinates (ne*nq per axis, element-major),
+ *   rel_x/_y the relative vertex coordinates (n_rho+1), interp_* (nq x nl) the
+ *   local basis at the quadrature points and proj_* (nq x nl) the projection
+ *   w_q (interp M^-1)^T factors of centerProjection.
+ * COLLOCATED: diagonal DeltaV at the DG nodes (center_x/_y: nodes, n_dof per
+ *   axis) and cell centres (rel_x/_y: n_rho).
+ * DeltaV = V(X + rho_x/2, Y + rho_y/2) - V(X - rho_x/2, Y - rho_y/2) with V
+ * bilinear on the ascending material grid after clamping each coordinate
+ * (evaluateExtendedPotential, get_Drift_full2D.m:650-666).  The operator adds
+ * i*pot_coef*DeltaV and cap_coef*(cap_x(rho_x) + cap_y(rho_y)). */
+enum { WDG2_POT_FV_CONSISTENT = 0, WDG2_POT_COLLOCATED = 1 };
+typedef struct wdg2_potential {
+  int32_t mode;
+  int32_t nq_x, nq_y;          /* quadrature points per element axis (FV), 1..8       */
+  int32_t nx_grid, ny_grid;    /* material grid, >= 2 each                            */
+  const double* x_grid;        /* [m], ascending                                     */
+  const double* y_grid;
+  const double* v;             /* nx_grid*ny_grid [V], x fastest                      */
+  const double* center_x;
+  const double* center_y;
+  const double* rel_x;
+  const double* rel_y;
+  const double* cap_x;         /* n_rho_x CAP profile (get_CAP.m)                    */
+  const double* cap_y;
+  const double* interp_x;      /* FV only */
+  const double* interp_y;
+  const double* proj_x;
+  const double* proj_y;
+  double pot_coef;             /* drift_scale*Q_drift (get_Drift_full2D.m:161)       */
+  double cap_coef;             /* drift_scale*cap_scale*Q_drift (:162)               */
+} wdg2_potential;
+int wdg2_set_potential(wdg_ctx* ctx, const wdg2_potential* pot);
+
+/* The inflow right-hand side assembled by the caller (sysInfo.assembleRhs(),
+ * get_Diff_full2D.m:609-616), N interleaved complex doubles; uploaded once
+ * (solve_transport_DG_full2D.m:372). */
+int wdg2_set_rhs(wdg_ctx* ctx, const double* rhs);
+
+/* sysInfo.getRowAbsSum() (get_SysM_full2D.m:165-168): transport row sums of
+ * the Kronecker parts plus the drift triangle-inequality bound
+ * (get_Drift_full2D.m:490-523); N doubles. */
+int wdg2_rowabs(wdg_ctx* ctx, double* s);
+
+/* Diagonal preconditioner of the Krylov solve (buildPreconditioner,
+ * solve_transport_DG_full2D.m:788-925,1187-1236): JACOBI inverts diag(A)
+ * (entries below 1e-12*max(max|d|,1) get identity action) and falls back to
+ * ROWABS when such entries exist; ROWABS divides by the row-absolute sums.
+ * *used receives the method in effect.  wdg_solve then applies it as a right
+ * preconditioner; NONE disables it. */
+enum { WDG2_PRECOND_NONE = 0, WDG2_PRECOND_JACOBI = 1, WDG2_PRECOND_ROWABS = 2 };
+int wdg2_set_precond(wdg_ctx* ctx, int32_t kind, int32_t* used);
+
+/* Zero-separation observables (get_Observables_full2D.m:24-60): per centre
+ * DOF i, out[k*ncen + i] = sum_j w[8k+j] * rho[i*nrel + idx[8k+j]],
+ * j < counts[k] <= 8, for k = 0 (density weights), 1 (d/drho_x row),
+ * 2 (d/drho_y row); 3*ncen complex outputs.  rho NULL: the context's last
+ * solution. */
+int wdg2_observables(wdg_ctx* ctx, const double* rho, const int32_t* idx, const double* w, const int32_t* counts,
+                     double* out);
+
 #ifdef __cplusplus
 }
 #endif

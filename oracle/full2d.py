@@ -346,6 +346,22 @@ def assemble_rhs(rhs_parts, p):
     return r
 
 
+def kron_diagonal(parts, p):
+    """assembleKronDiagonal (get_Diff_full2D.m:495-514): sum c kron(diag C, diag R)."""
+    d = np.zeros(p["nTotal"], complex)
+    for _, c, Cm, Rm in parts:
+        d += c * np.kron(Cm.diagonal(), Rm.diagonal())
+    return d
+
+
+def kron_row_abs_sum(parts, p):
+    """assembleKronRowAbsSum (get_Diff_full2D.m:516-535): sum |c| kron(rowabs C, rowabs R)."""
+    s = np.zeros(p["nTotal"])
+    for _, c, Cm, Rm in parts:
+        s += abs(c) * np.kron(np.asarray(abs(Cm).sum(axis=1)).ravel(), np.asarray(abs(Rm).sum(axis=1)).ravel())
+    return s
+
+
 def apply_parts(parts, u, p):
     """applyKronOperator (get_Diff_full2D.m:618-639): Y = sum c R U C^T."""
     nr, nc = p["relative"]["nDof"], p["dg"]["nCenterDof"]
@@ -600,13 +616,41 @@ class DriftFull2D:
 
     def diagonal(self):
         if self.mode != "collocated":
-            raise NotImplementedError("diagonal of the FV-consistent drift is not restated")
+            return self._fv_diagonal()
         Xc = np.repeat(self.centerX, self.nrel)
         Yc = np.repeat(self.centerY, self.nrel)
         rx = np.tile(self.relX, self.ncen)
         ry = np.tile(self.relY, self.ncen)
         dv = self.V(Xc + 0.5 * rx, Yc + 0.5 * ry) - self.V(Xc - 0.5 * rx, Yc - 0.5 * ry)
         return self.pot_coef * dv + self.cap_coef * np.tile(self.cap, self.ncen)
+
+    def _fv_weights(self):
+        """centerDiagonalWeights / centerRowAbsWeights (get_Drift_full2D.m:240-252)."""
+        Ip = self.interpolation
+        Pi = self.centerProjection
+        return Pi * Ip, np.abs(Pi) * np.abs(Ip).sum(axis=1)[:, None]
+
+    def _fv_diagonal(self):
+        """fvConsistentDiagonal (get_Drift_full2D.m:466-488)."""
+        dw, _ = self._fv_weights()
+        D = self.cap_coef * np.repeat(self.cap[:, None], self.ncen, axis=1).astype(complex)
+        for ids, Xq, Yq in self.elements:
+            rel = (self.P.T @ self.delta_v(Xq, Yq)) / 16.0         # (nrel, nq)
+            D[:, ids] += self.pot_coef * (rel @ dw)
+        return D.T.reshape(-1)
+
+    def row_abs(self):
+        """getDriftRowAbsSum (get_Drift_full2D.m:490-523): |diag| (collocated) or
+        the triangle-inequality bound with vertex degrees (FV-consistent)."""
+        if self.mode == "collocated":
+            return np.abs(self.diagonal())
+        _, rw = self._fv_weights()
+        deg = np.asarray(self.P.sum(axis=1)).ravel()
+        S = abs(self.cap_coef) * np.repeat(np.abs(self.cap)[:, None], self.ncen, axis=1)
+        for ids, Xq, Yq in self.elements:
+            rel = (self.P.T @ (np.abs(self.delta_v(Xq, Yq)) * deg[:, None])) / 16.0
+            S[:, ids] += abs(self.pot_coef) * (rel @ rw)
+        return S.T.reshape(-1)
 
     def assemble(self):
         n = self.nrel * self.ncen

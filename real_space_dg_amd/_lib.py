@@ -26,6 +26,18 @@ WDG_BASIS_BF16 = 1
 WDG_BASIS_FP16 = 2
 WDG_PREC_FP64 = 0
 WDG_PREC_MIXED = 1
+WDG2_AXIS_X = 0
+WDG2_AXIS_Y = 1
+WDG2_CENTER_BANDED = 0
+WDG2_CENTER_FACE = 1
+WDG2_REL_CENTRAL = 0
+WDG2_REL_IDENTITY = 1
+WDG2_REL_DENSE = 2
+WDG2_POT_FV_CONSISTENT = 0
+WDG2_POT_COLLOCATED = 1
+WDG2_PRECOND_NONE = 0
+WDG2_PRECOND_JACOBI = 1
+WDG2_PRECOND_ROWABS = 2
 
 # Status code -> reference error identifier (DG:Full2D:* in the MATLAB code).
 ERROR_IDS = {
@@ -88,6 +100,27 @@ class wdg_poisson_info(C.Structure):
                 ("charge_integral", C.c_double), ("dg_integral", C.c_double)]
 
 
+class wdg2_desc(C.Structure):
+    _fields_ = [("n_rho_x", C.c_int32), ("n_rho_y", C.c_int32), ("nl_x", C.c_int32), ("nl_y", C.c_int32),
+                ("ne_x", C.c_int32), ("ne_y", C.c_int32), ("delta_x", C.c_double), ("delta_y", C.c_double)]
+
+
+class wdg2_part(C.Structure):
+    _fields_ = [("axis", C.c_int32), ("center", C.c_int32), ("rel", C.c_int32), ("face", C.c_int32),
+                ("coef", C.c_double), ("c_diag", C.POINTER(C.c_double)), ("c_lower", C.POINTER(C.c_double)),
+                ("c_upper", C.POINTER(C.c_double)), ("mask", C.POINTER(C.c_int32)),
+                ("r_dense", C.POINTER(C.c_double))]
+
+
+class wdg2_potential(C.Structure):
+    _fields_ = [("mode", C.c_int32), ("nq_x", C.c_int32), ("nq_y", C.c_int32), ("nx_grid", C.c_int32),
+                ("ny_grid", C.c_int32)] + \
+               [(n, C.POINTER(C.c_double)) for n in ("x_grid", "y_grid", "v", "center_x", "center_y", "rel_x",
+                                                     "rel_y", "cap_x", "cap_y", "interp_x", "interp_y", "proj_x",
+                                                     "proj_y")] + \
+               [("pot_coef", C.c_double), ("cap_coef", C.c_double)]
+
+
 # name -> (restype, argtypes)
 _P = C.c_void_p
 _D = C.POINTER(C.c_double)
@@ -111,6 +144,12 @@ SIGNATURES = {
     "wdg_transient_rk4": (C.c_int, [_P, C.POINTER(wdg_transient_opts), _D, _D, _D, _D, _D, C.c_int32]),
     "wdg_set_theta_method": (C.c_int, [_P, C.c_int32]),
     "wdg_theta_method": (C.c_int, [_P]),
+    "wdg2_create": (C.c_int, [C.POINTER(wdg2_desc), C.POINTER(wdg2_part), C.c_int32, C.POINTER(_P)]),
+    "wdg2_set_potential": (C.c_int, [_P, C.POINTER(wdg2_potential)]),
+    "wdg2_set_rhs": (C.c_int, [_P, _D]),
+    "wdg2_rowabs": (C.c_int, [_P, _D]),
+    "wdg2_set_precond": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_int32)]),
+    "wdg2_observables": (C.c_int, [_P, _D, C.POINTER(C.c_int32), _D, C.POINTER(C.c_int32), _D]),
     "wdg_set_profiling": (C.c_int, [_P, C.c_uint32]),
     "wdg_profile": (C.c_int, [_P, C.c_char_p, C.c_int32, _D, C.POINTER(C.c_int64), _D, C.c_int32,
                               C.POINTER(C.c_int32)]),

@@ -10,7 +10,8 @@ PKG = Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 LIB = PKG / "libwdg.so"
 SOURCES = [CSRC / "wdg.hip", CSRC / "host_setup.cpp"]
-HEADERS = [CSRC / "kernels.hpp", CSRC / "theta_fft.hpp", CSRC / "host_setup.hpp", PKG.parent / "include" / "wigner_dg.h"]
+HEADERS = [CSRC / "kernels.hpp", CSRC / "theta_fft.hpp", CSRC / "full2d.hpp", CSRC / "full2d_host.hpp",
+           CSRC / "host_setup.hpp", PKG.parent / "include" / "wigner_dg.h"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

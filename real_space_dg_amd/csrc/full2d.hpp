@@ -1,0 +1,656 @@
+// Full-2D Wigner operator F(rho_x, rho_y, X, Y): device kernels.
+//
+// Vector order (DG_code/initParams_full2D.m:4-8,105-107): rho_x fastest, then
+// rho_y, the X DG DOF and the Y DG DOF, i.e. DOF (i, r) at i*nrel + r with
+// centre DOF i = ix + nxd*iy and relative cell r = rx + nrx*ry.  An operator
+// part c*kron(C, R) (get_Diff_full2D.m:80-146) acts as Y = c R U C^T on
+// U = reshape(u, nrel, ncen).
+//
+// Kernels:
+//  f2_apply_kernel   one workgroup per (centre element, band of rho_y rows):
+//                    central (-i D) and identity (Rusanov) transport parts of
+//                    both axes, the FV-consistent potential
+//                    (1/16) P^T diag(DeltaV) P projected over the element's
+//                    tensor quadrature (get_Drift_full2D.m:331-364) or the
+//                    collocated diagonal, and the CAP.  Writes y.
+//                    With MODE_DIAG / MODE_ROWABS the same tiling produces the
+//                    drift diagonal and row-absolute bound instead
+//                    (get_Drift_full2D.m:466-523).
+//  f2_dense_kernel   a part whose relative factor is a dense matrix along one
+//                    axis (characteristic |A|, A+ R, A- R): banded centre
+//                    factor (interior upwind) or a face lift.  Adds into y.
+//  f2_kron_diag_kernel, f2_observables_kernel, f2_scale_kernel.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "kernels.hpp"
+
+namespace wdg {
+
+constexpr int kF2MaxNl = 4;    // DG nodes per element axis (p <= 3)
+constexpr int kF2MaxNq = 8;    // potential quadrature points per element axis
+constexpr int kF2Block = 256;
+
+enum F2Mode { F2_APPLY = 0, F2_DIAG = 1, F2_ROWABS = 2 };
+
+struct F2Const {
+  int nrx, nry, nlx, nly, nex, ney, nxd, nyd, nrel;
+  int rows, tiles;              // rho_y rows per apply tile, tiles per element
+  int pot_mode;                 // 0 fv-consistent, 1 collocated
+  int nqx, nqy, nxg, nyg;       // quadrature points, material grid size
+  int npx, npy;                 // relative sample points per axis (vertices or cells)
+  double inv2dx, inv2dy;        // 1/(2 delta) of the central difference
+  double pot_coef, cap_coef;    // drift coefficients / operator scale (pot multiplies i*DeltaV)
+  // banded centre parts (coefficient and 1/s_A folded in); null when absent
+  const double *bxc, *bxi, *byc, *byi;
+  // potential samples: per (centre sample, relative sample, sign) the material
+  // cell and its linear weight along one axis
+  const int *tix, *tiy;
+  const double *ttx, *tty;
+  const double* v;              // material grid potential, x fastest
+  const double *capx, *capy;
+  double ipx[kF2MaxNq * kF2MaxNl], ipy[kF2MaxNq * kF2MaxNl];   // interpolation (q, b)
+  double prx[kF2MaxNq * kF2MaxNl], pry[kF2MaxNq * kF2MaxNl];   // projection (q, a)
+};
+
+template <typename T>
+struct F2Vec;
+template <>
+struct F2Vec<double> { using T2 = double2; };
+template <>
+struct F2Vec<float> { using T2 = float2; };
+
+template <typename T2>
+__device__ __forceinline__ T2 f2_zero() { T2 z; z.x = 0; z.y = 0; return z; }
+
+template <typename T, typename T2>
+__device__ __forceinline__ void f2_fma(T2& acc, T s, const T2& v) {
+  acc.x += s * v.x;
+  acc.y += s * v.y;
+}
+
+// Bilinear interpolation of the material-grid potential between cells
+// (ix, iy) and (ix+1, iy+1) with weights tx, ty (griddedInterpolant 'linear').
+__device__ __forceinline__ double f2_bilin(const double* v, int nxg, int ix, double tx, int iy, double ty) {
+  const double* p0 = v + (size_t)iy * nxg + ix;
+  const double* p1 = p0 + nxg;
+  const double a = p0[0] + tx * (p0[1] - p0[0]);
+  const double b = p1[0] + tx * (p1[1] - p1[0]);
+  return a + ty * (b - a);
+}
+
+// DeltaV = V(x+, y+) - V(x-, y-) for table rows kx (x samples) and ky.
+__device__ __forceinline__ double f2_delta_v(const F2Const& c, int kx, int ky) {
+  const double vp = f2_bilin(c.v, c.nxg, c.tix[2 * kx], c.ttx[2 * kx], c.tiy[2 * ky], c.tty[2 * ky]);
+  const double vm = f2_bilin(c.v, c.nxg, c.tix[2 * kx + 1], c.ttx[2 * kx + 1], c.tiy[2 * ky + 1], c.tty[2 * ky + 1]);
+  return vp - vm;
+}
+
+// Neighbour value with zero padding outside [0, n).
+template <typename T2>
+__device__ __forceinline__ T2 f2_ld(const T2* p, long off, bool ok) {
+  return ok ? p[off] : f2_zero<T2>();
+}
+
+// LDS layout of the apply kernel: su[NL][rows+2][nrx] (own element, one halo
+// row each side) followed by sz[NL][rows+1][nrx+1] (vertex values).
+inline size_t f2_lds_bytes(int nl, int rows, int nrx, size_t t2, bool fv) {
+  size_t b = (size_t)nl * (rows + 2) * nrx * t2;
+  if (fv) b += (size_t)nl * (rows + 1) * (nrx + 1) * t2;
+  return b;
+}
+
+template <int NLX, int NLY, typename T, int MODE>
+__global__ void __launch_bounds__(kF2Block) f2_apply_kernel(const F2Const* __restrict__ cp,
+                                                            const typename F2Vec<T>::T2* __restrict__ u,
+                                                            typename F2Vec<T>::T2* __restrict__ y,
+                                                            const GmresState* st) {
+  using T2 = typename F2Vec<T>::T2;
+  constexpr int NL = NLX * NLY;
+  if (skip(st)) return;
+  const F2Const& c = *cp;
+  const int nrx = c.nrx, nry = c.nry, nrel = c.nrel;
+  const int el = blockIdx.x / c.tiles, tile = blockIdx.x - el * c.tiles;
+  const int ex = el % c.nex, ey = el / c.nex;
+  const int ry0 = tile * c.rows;
+  const int nrows = min(c.rows, nry - ry0);
+  const int R2 = c.rows + 2;
+  const bool fv = c.pot_mode == 0;
+  extern __shared__ __align__(16) unsigned char f2_smem[];
+  T2* su = reinterpret_cast<T2*>(f2_smem);
+  T2* sz = su + (size_t)NL * R2 * nrx;
+  auto cdof = [&](int ax, int ay) { return (long)(ex * NLX + ax) + (long)c.nxd * (ey * NLY + ay); };
+
+  if (MODE == F2_APPLY) {
+    for (int idx = threadIdx.x; idx < NL * R2 * nrx; idx += blockDim.x) {
+      const int a = idx / (R2 * nrx);
+      const int rem = idx - a * R2 * nrx;
+      const int rr = rem / nrx, rx = rem - rr * nrx;
+      const int ry = ry0 - 1 + rr;
+      const bool ok = ry >= 0 && ry < nry && rr <= nrows + 1;
+      su[idx] = ok ? u[cdof(a % NLX, a / NLX) * nrel + (long)ry * nrx + rx] : f2_zero<T2>();
+    }
+    __syncthreads();
+  }
+
+  // ---- vertex phase of the FV-consistent potential ----
+  if (fv) {
+    const int nvx = nrx + 1;
+    const int nvert = (nrows + 1) * nvx;
+    for (int v = threadIdx.x; v < nvert; v += blockDim.x) {
+      const int vyl = v / nvx, vx = v - vyl * nvx;
+      const int vy = ry0 + vyl;
+      T2 W[NL], Z[NL];
+      T deg = 0;
+      if (MODE == F2_APPLY) {
+        // P: vertex (vx, vy) sums the cells (vx-1..vx) x (vy-1..vy)
+        const bool xl = vx >= 1, xr = vx < nrx;
+#pragma unroll
+        for (int b = 0; b < NL; ++b) {
+          const T2* s = su + (size_t)b * R2 * nrx;
+          T2 w = f2_zero<T2>();
+#pragma unroll
+          for (int dr = 0; dr < 2; ++dr) {
+            const T2* row = s + (size_t)(vyl + dr) * nrx;   // su rows hold zeros outside the domain
+            if (xl) { w.x += row[vx - 1].x; w.y += row[vx - 1].y; }
+            if (xr) { w.x += row[vx].x; w.y += row[vx].y; }
+          }
+          W[b] = w;
+        }
+      } else {
+        const int cxn = (vx >= 1) + (vx < nrx), cyn = (vy >= 1) + (vy < nry);
+        deg = (T)(cxn * cyn);
+      }
+#pragma unroll
+      for (int a = 0; a < NL; ++a) Z[a] = f2_zero<T2>();
+      for (int qx = 0; qx < c.nqx; ++qx) {
+        const int kx = (ex * c.nqx + qx) * c.npx + vx;
+        T2 t1[NLY];
+        if (MODE == F2_APPLY) {
+#pragma unroll
+          for (int by = 0; by < NLY; ++by) {
+            T2 s = f2_zero<T2>();
+#pragma unroll
+            for (int bx = 0; bx < NLX; ++bx) f2_fma(s, (T)c.ipx[qx * kF2MaxNl + bx], W[bx + NLX * by]);
+            t1[by] = s;
+          }
+        }
+        T2 acc[NLY];
+#pragma unroll
+        for (int ay = 0; ay < NLY; ++ay) acc[ay] = f2_zero<T2>();
+        for (int qy = 0; qy < c.nqy; ++qy) {
+          const int ky = (ey * c.nqy + qy) * c.npy + vy;
+          const T dv = (T)f2_delta_v(c, kx, ky);
+          if (MODE == F2_APPLY) {
+            T2 wq = f2_zero<T2>();
+#pragma unroll
+            for (int by = 0; by < NLY; ++by) f2_fma(wq, (T)c.ipy[qy * kF2MaxNl + by], t1[by]);
+            wq.x *= dv;
+            wq.y *= dv;
+#pragma unroll
+            for (int ay = 0; ay < NLY; ++ay) f2_fma(acc[ay], (T)c.pry[qy * kF2MaxNl + ay], wq);
+          } else if (MODE == F2_DIAG) {
+#pragma unroll
+            for (int ay = 0; ay < NLY; ++ay)
+              acc[ay].x += dv * (T)(c.pry[qy * kF2MaxNl + ay] * c.ipy[qy * kF2MaxNl + ay]);
+          } else {
+            T sy = 0;
+#pragma unroll
+            for (int by = 0; by < NLY; ++by) sy += fabs((T)c.ipy[qy * kF2MaxNl + by]);
+#pragma unroll
+            for (int ay = 0; ay < NLY; ++ay) acc[ay].x += fabs(dv) * fabs((T)c.pry[qy * kF2MaxNl + ay]) * sy;
+          }
+        }
+        T wx[NLX];
+        if (MODE == F2_APPLY) {
+#pragma unroll
+          for (int ax = 0; ax < NLX; ++ax) wx[ax] = (T)c.prx[qx * kF2MaxNl + ax];
+        } else if (MODE == F2_DIAG) {
+#pragma unroll
+          for (int ax = 0; ax < NLX; ++ax) wx[ax] = (T)(c.prx[qx * kF2MaxNl + ax] * c.ipx[qx * kF2MaxNl + ax]);
+        } else {
+          T sx = 0;
+#pragma unroll
+          for (int bx = 0; bx < NLX; ++bx) sx += fabs((T)c.ipx[qx * kF2MaxNl + bx]);
+#pragma unroll
+          for (int ax = 0; ax < NLX; ++ax) wx[ax] = fabs((T)c.prx[qx * kF2MaxNl + ax]) * sx;
+        }
+#pragma unroll
+        for (int ay = 0; ay < NLY; ++ay)
+#pragma unroll
+          for (int ax = 0; ax < NLX; ++ax) f2_fma(Z[ax + NLX * ay], wx[ax], acc[ay]);
+      }
+      if (MODE == F2_ROWABS) {
+#pragma unroll
+        for (int a = 0; a < NL; ++a) Z[a].x *= deg;
+      }
+#pragma unroll
+      for (int a = 0; a < NL; ++a) sz[((size_t)a * (c.rows + 1) + vyl) * nvx + vx] = Z[a];
+    }
+    __syncthreads();
+  }
+
+  // ---- cell phase ----
+  const int ncell = nrows * nrx;
+  const T pc = (T)c.pot_coef, cc = (T)c.cap_coef;
+  for (int t = threadIdx.x; t < ncell; t += blockDim.x) {
+    const int ryl = t / nrx, rx = t - ryl * nrx;
+    const int ry = ry0 + ryl;
+    const long r = (long)ry * nrx + rx;
+    const T cap = (T)(c.capx[rx] + c.capy[ry]);
+    T2 out[NL];
+    // drift
+#pragma unroll
+    for (int a = 0; a < NL; ++a) {
+      const int ax = a % NLX, ay = a / NLX;
+      T2 s = f2_zero<T2>();
+      if (fv) {
+        const int nvx = nrx + 1;
+        const T2* zb = sz + (size_t)a * (c.rows + 1) * nvx;
+        const T2 z00 = zb[(size_t)ryl * nvx + rx], z01 = zb[(size_t)ryl * nvx + rx + 1];
+        const T2 z10 = zb[(size_t)(ryl + 1) * nvx + rx], z11 = zb[(size_t)(ryl + 1) * nvx + rx + 1];
+        s.x = z00.x + z01.x + z10.x + z11.x;
+        s.y = z00.y + z01.y + z10.y + z11.y;
+        const T f = pc * (T)0.0625;
+        if (MODE == F2_APPLY) {
+          // i * pot * (1/16) P^T Z  +  cap * U
+          const T2 uu = su[((size_t)a * R2 + ryl + 1) * nrx + rx];
+          out[a].x = -f * s.y + cc * cap * uu.x;
+          out[a].y = f * s.x + cc * cap * uu.y;
+        } else if (MODE == F2_DIAG) {
+          out[a].x = cc * cap;
+          out[a].y = f * s.x;
+        } else {
+          out[a].x = fabs(f) * s.x + fabs(cc) * fabs(cap);
+          out[a].y = 0;
+        }
+      } else {
+        // collocated: i * pot * DeltaV(node, cell) + cap, diagonal
+        const int kx = (ex * NLX + ax) * c.npx + rx;
+        const int ky = (ey * NLY + ay) * c.npy + ry;
+        const T dv = (T)f2_delta_v(c, kx, ky);
+        const T dre = cc * cap, dim = pc * dv;
+        if (MODE == F2_APPLY) {
+          const T2 uu = su[((size_t)a * R2 + ryl + 1) * nrx + rx];
+          out[a].x = dre * uu.x - dim * uu.y;
+          out[a].y = dre * uu.y + dim * uu.x;
+        } else if (MODE == F2_DIAG) {
+          out[a].x = dre;
+          out[a].y = dim;
+        } else {
+          out[a].x = sqrt(dre * dre + dim * dim);
+          out[a].y = 0;
+        }
+      }
+    }
+    if (MODE == F2_APPLY) {
+      // ---- transport: X parts along rho_x, Y parts along rho_y ----
+      const bool rxl = rx > 0, rxr = rx < nrx - 1, ryd = ry > 0, ryu = ry < nry - 1;
+      const int sX = NLX * NLX + 2 * NLX, sY = NLY * NLY + 2 * NLY;
+#pragma unroll
+      for (int ay = 0; ay < NLY; ++ay) {
+        // neighbour X face nodes on this Y row: last node of ex-1, first of ex+1
+        T2 lm = f2_zero<T2>(), l0 = lm, lp = lm, rm = lm, r0 = lm, rp = lm;
+        const long row_off = r;
+        if (ex > 0) {
+          const T2* p = u + ((long)(ex * NLX - 1) + (long)c.nxd * (ey * NLY + ay)) * nrel + row_off;
+          lm = f2_ld(p, -1, rxl); l0 = p[0]; lp = f2_ld(p, 1, rxr);
+        }
+        if (ex < c.nex - 1) {
+          const T2* p = u + ((long)((ex + 1) * NLX) + (long)c.nxd * (ey * NLY + ay)) * nrel + row_off;
+          rm = f2_ld(p, -1, rxl); r0 = p[0]; rp = f2_ld(p, 1, rxr);
+        }
+#pragma unroll
+        for (int ax = 0; ax < NLX; ++ax) {
+          T2 dif = f2_zero<T2>(), idn = f2_zero<T2>();
+          if (c.bxc) {
+            const double* B = c.bxc + (size_t)ex * sX;
+#pragma unroll
+            for (int bx = 0; bx < NLX; ++bx) {
+              const T2* s = su + ((size_t)(bx + NLX * ay) * R2 + ryl + 1) * nrx;
+              const T w = (T)B[ax * NLX + bx];
+              const T2 up = rxr ? s[rx + 1] : f2_zero<T2>();
+              const T2 dn = rxl ? s[rx - 1] : f2_zero<T2>();
+              dif.x += w * (up.x - dn.x);
+              dif.y += w * (up.y - dn.y);
+            }
+            const T wl = (T)B[NLX * NLX + ax], wr = (T)B[NLX * NLX + NLX + ax];
+            dif.x += wl * (lp.x - lm.x) + wr * (rp.x - rm.x);
+            dif.y += wl * (lp.y - lm.y) + wr * (rp.y - rm.y);
+          }
+          if (c.bxi) {
+            const double* B = c.bxi + (size_t)ex * sX;
+#pragma unroll
+            for (int bx = 0; bx < NLX; ++bx)
+              f2_fma(idn, (T)B[ax * NLX + bx], su[((size_t)(bx + NLX * ay) * R2 + ryl + 1) * nrx + rx]);
+            f2_fma(idn, (T)B[NLX * NLX + ax], l0);
+            f2_fma(idn, (T)B[NLX * NLX + NLX + ax], r0);
+          }
+          // -i/(2 delta) * dif
+          const T h = (T)c.inv2dx;
+          T2& o = out[ax + NLX * ay];
+          o.x += h * dif.y + idn.x;
+          o.y += -h * dif.x + idn.y;
+        }
+      }
+#pragma unroll
+      for (int ax = 0; ax < NLX; ++ax) {
+        T2 dm = f2_zero<T2>(), d0 = dm, dp = dm, tm = dm, t0 = dm, tp = dm;
+        if (ey > 0) {
+          const T2* p = u + ((long)(ex * NLX + ax) + (long)c.nxd * (ey * NLY - 1)) * nrel + r;
+          dm = f2_ld(p, -(long)nrx, ryd); d0 = p[0]; dp = f2_ld(p, (long)nrx, ryu);
+        }
+        if (ey < c.ney - 1) {
+          const T2* p = u + ((long)(ex * NLX + ax) + (long)c.nxd * ((ey + 1) * NLY)) * nrel + r;
+          tm = f2_ld(p, -(long)nrx, ryd); t0 = p[0]; tp = f2_ld(p, (long)nrx, ryu);
+        }
+#pragma unroll
+        for (int ay = 0; ay < NLY; ++ay) {
+          T2 dif = f2_zero<T2>(), idn = f2_zero<T2>();
+          if (c.byc) {
+            const double* B = c.byc + (size_t)ey * sY;
+#pragma unroll
+            for (int by = 0; by < NLY; ++by) {
+              const T2* s = su + (size_t)(ax + NLX * by) * R2 * nrx;
+              const T w = (T)B[ay * NLY + by];
+              const T2 up = s[(size_t)(ryl + 2) * nrx + rx];   // zero rows outside the domain
+              const T2 dn = s[(size_t)ryl * nrx + rx];
+              dif.x += w * (up.x - dn.x);
+              dif.y += w * (up.y - dn.y);
+            }
+            const T wl = (T)B[NLY * NLY + ay], wr = (T)B[NLY * NLY + NLY + ay];
+            dif.x += wl * (dp.x - dm.x) + wr * (tp.x - tm.x);
+            dif.y += wl * (dp.y - dm.y) + wr * (tp.y - tm.y);
+          }
+          if (c.byi) {
+            const double* B = c.byi + (size_t)ey * sY;
+#pragma unroll
+            for (int by = 0; by < NLY; ++by)
+              f2_fma(idn, (T)B[ay * NLY + by], su[((size_t)(ax + NLX * by) * R2 + ryl + 1) * nrx + rx]);
+            f2_fma(idn, (T)B[NLY * NLY + ay], d0);
+            f2_fma(idn, (T)B[NLY * NLY + NLY + ay], t0);
+          }
+          const T h = (T)c.inv2dy;
+          T2& o = out[ax + NLX * ay];
+          o.x += h * dif.y + idn.x;
+          o.y += -h * dif.x + idn.y;
+        }
+      }
+    }
+#pragma unroll
+    for (int a = 0; a < NL; ++a) {
+      const long i = cdof(a % NLX, a / NLX);
+      if (MODE == F2_APPLY) {
+        y[i * nrel + r] = out[a];
+      } else {
+        // diagonal / row sums accumulate onto the transport part
+        T2 o = y[i * nrel + r];
+        o.x += out[a].x;
+        o.y += out[a].y;
+        y[i * nrel + r] = o;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Dense relative factor along one axis.
+// ---------------------------------------------------------------------------
+struct F2Dense {
+  int axis;          // 0: C along X, R along rho_x; 1: along Y / rho_y
+  int face;          // -1: banded centre factor; 0 / 1: face lift at the low / high end
+  int nl, ne;        // nodes per element and elements of the centre axis
+  double coef;       // part coefficient / operator scale
+  const double* blocks;   // banded: per element diag nl*nl, lower nl, upper nl
+  const double* lift;     // face: nl
+  const int* mask;        // face: per transverse centre DOF (null = all)
+  const double2* R;       // n x n row-major, n = n_rho along the axis
+};
+
+inline size_t f2_dense_lds(int n, int nrel, size_t t2) { return ((size_t)n * (n + 1) + nrel) * t2; }
+
+// One workgroup per output line: a centre DOF (banded) or a transverse DOF of
+// the face (lift).  W = C-combination of u in LDS, then R along the axis,
+// 4 outputs per thread sharing one R (or W) load.
+template <typename T>
+__global__ void __launch_bounds__(kF2Block) f2_dense_kernel(const F2Const* __restrict__ cp, F2Dense d,
+                                                            const typename F2Vec<T>::T2* __restrict__ u,
+                                                            typename F2Vec<T>::T2* __restrict__ y,
+                                                            const GmresState* st) {
+  using T2 = typename F2Vec<T>::T2;
+  if (skip(st)) return;
+  const F2Const& c = *cp;
+  const int nrx = c.nrx, nry = c.nry, nrel = c.nrel;
+  const int n = d.axis == 0 ? nrx : nry;
+  const int line = blockIdx.x;
+  extern __shared__ __align__(16) unsigned char f2_smem[];
+  T2* sR = reinterpret_cast<T2*>(f2_smem);          // n x (n+1)
+  T2* sW = sR + (size_t)n * (n + 1);                // nrel
+  // centre DOF helpers: along-axis index k and transverse index t
+  const int nxd = c.nxd;
+  auto dof = [&](int k, int t) -> long { return d.axis == 0 ? (long)k + (long)nxd * t : (long)t + (long)nxd * k; };
+  const int nalong = d.nl * d.ne;
+  int k_out = 0, t = 0;
+  if (d.face < 0) {
+    const long i = line;
+    k_out = d.axis == 0 ? (int)(i % nxd) : (int)(i / nxd);
+    t = d.axis == 0 ? (int)(i / nxd) : (int)(i % nxd);
+  } else {
+    t = line;
+    if (d.mask && !d.mask[t]) return;
+  }
+  for (int idx = threadIdx.x; idx < n * n; idx += blockDim.x) {
+    const int m = idx / n, k = idx - m * n;
+    const double2 v = d.R[idx];
+    T2 w;
+    w.x = (T)(d.coef * v.x);
+    w.y = (T)(d.coef * v.y);
+    sR[(size_t)m * (n + 1) + k] = w;
+  }
+  if (d.face < 0) {
+    const int e = k_out / d.nl, a = k_out - e * d.nl;
+    const int sb = d.nl * d.nl + 2 * d.nl;
+    const double* B = d.blocks + (size_t)e * sb;
+    for (int r = threadIdx.x; r < nrel; r += blockDim.x) {
+      T2 w = f2_zero<T2>();
+      for (int b = 0; b < d.nl; ++b) f2_fma(w, (T)B[a * d.nl + b], u[dof(e * d.nl + b, t) * nrel + r]);
+      if (e > 0) f2_fma(w, (T)B[d.nl * d.nl + a], u[dof(e * d.nl - 1, t) * nrel + r]);
+      if (e < d.ne - 1) f2_fma(w, (T)B[d.nl * d.nl + d.nl + a], u[dof((e + 1) * d.nl, t) * nrel + r]);
+      sW[r] = w;
+    }
+  } else {
+    const int ktr = d.face == 0 ? 0 : nalong - 1;
+    for (int r = threadIdx.x; r < nrel; r += blockDim.x) sW[r] = u[dof(ktr, t) * nrel + r];
+  }
+  __syncthreads();
+  // out[m][cidx]: axis 0: m = rx, cidx = ry; axis 1: m = ry, cidx = rx
+  const int ncol = d.axis == 0 ? nry : nrx;
+  const int nblk = d.axis == 0 ? (ncol + 3) / 4 : (n + 3) / 4;
+  const int items = d.axis == 0 ? n * nblk : ncol * nblk;
+  for (int it = threadIdx.x; it < items; it += blockDim.x) {
+    T2 acc[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[j] = f2_zero<T2>();
+    if (d.axis == 0) {
+      const int m = it % n, cb = it / n;
+      for (int k = 0; k < n; ++k) {
+        const T2 rv = sR[(size_t)m * (n + 1) + k];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int cc = cb * 4 + j;
+          if (cc < ncol) {
+            const T2 w = sW[(size_t)cc * nrx + k];
+            acc[j].x += rv.x * w.x - rv.y * w.y;
+            acc[j].y += rv.x * w.y + rv.y * w.x;
+          }
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int cc = cb * 4 + j;
+        if (cc >= ncol) continue;
+        const long r = (long)cc * nrx + m;
+        if (d.face < 0) {
+          const long o = dof(k_out, t) * nrel + r;
+          T2 v = y[o];
+          v.x += acc[j].x;
+          v.y += acc[j].y;
+          y[o] = v;
+        } else {
+          for (int a = 0; a < d.nl; ++a) {
+            const int k = d.face == 0 ? a : nalong - d.nl + a;
+            const T l = (T)d.lift[a];
+            const long o = dof(k, t) * nrel + r;
+            T2 v = y[o];
+            v.x += l * acc[j].x;
+            v.y += l * acc[j].y;
+            y[o] = v;
+          }
+        }
+      }
+    } else {
+      const int cc = it % ncol, mb = it / ncol;
+      for (int k = 0; k < n; ++k) {
+        const T2 w = sW[(size_t)k * nrx + cc];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int m = mb * 4 + j;
+          if (m < n) {
+            const T2 rv = sR[(size_t)m * (n + 1) + k];
+            acc[j].x += rv.x * w.x - rv.y * w.y;
+            acc[j].y += rv.x * w.y + rv.y * w.x;
+          }
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int m = mb * 4 + j;
+        if (m >= n) continue;
+        const long r = (long)m * nrx + cc;
+        if (d.face < 0) {
+          const long o = dof(k_out, t) * nrel + r;
+          T2 v = y[o];
+          v.x += acc[j].x;
+          v.y += acc[j].y;
+          y[o] = v;
+        } else {
+          for (int a = 0; a < d.nl; ++a) {
+            const int k = d.face == 0 ? a : nalong - d.nl + a;
+            const T l = (T)d.lift[a];
+            const long o = dof(k, t) * nrel + r;
+            T2 v = y[o];
+            v.x += l * acc[j].x;
+            v.y += l * acc[j].y;
+            y[o] = v;
+          }
+        }
+      }
+    }
+  }
+}
+
+// Transport diagonal / row-abs bound of the Kronecker parts:
+// sum_p coef_p * dC_p[i] * dR_p[r] (complex) into y (re, im), and the row
+// bound sum_p |coef_p| aC_p[i] aR_p[r] into the real part when rowabs.
+__global__ void f2_kron_diag_kernel(int nparts, int ncen, int nrel, const double* __restrict__ dc,
+                                    const double2* __restrict__ dr, double2* __restrict__ y) {
+  const long n = (long)ncen * nrel;
+  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < n; idx += (long)gridDim.x * blockDim.x) {
+    const long i = idx / nrel, r = idx - i * nrel;
+    double re = 0, im = 0;
+    for (int p = 0; p < nparts; ++p) {
+      const double cv = dc[(size_t)p * ncen + i];
+      const double2 rv = dr[(size_t)p * nrel + r];
+      re += cv * rv.x;
+      im += cv * rv.y;
+    }
+    y[idx] = make_double2(re, im);
+  }
+}
+
+// Observables at zero separation: per centre DOF, three sparse sums over the
+// relative block (n: value at 0; d/drho_x; d/drho_y), get_Observables_full2D.m:24-60.
+struct F2ObsW {
+  int idx[24];
+  double w[24];
+  int cnt[3];
+};
+
+__global__ void f2_observables_kernel(const double2* __restrict__ rho, int ncen, int nrel, F2ObsW ow, double scale,
+                                      double2* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ncen) return;
+  const double2* p = rho + (size_t)i * nrel;
+  for (int k = 0; k < 3; ++k) {
+    double re = 0, im = 0;
+    for (int j = 0; j < ow.cnt[k]; ++j) {
+      const double2 v = p[ow.idx[8 * k + j]];
+      re += ow.w[8 * k + j] * v.x;
+      im += ow.w[8 * k + j] * v.y;
+    }
+    out[(size_t)k * ncen + i] = make_double2(scale * re, scale * im);
+  }
+}
+
+// z = p o r (complex elementwise): Jacobi / row-scaling preconditioner.
+template <typename T>
+__global__ void f2_scale_kernel(const typename F2Vec<T>::T2* __restrict__ p, const typename F2Vec<T>::T2* __restrict__ r,
+                                typename F2Vec<T>::T2* __restrict__ z, long n, const GmresState* st) {
+  if (skip(st)) return;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const auto a = p[i], b = r[i];
+    typename F2Vec<T>::T2 o;
+    o.x = a.x * b.x - a.y * b.y;
+    o.y = a.x * b.y + a.y * b.x;
+    z[i] = o;
+  }
+}
+
+// max |d| and the number of entries with |d| < floor (preconditioner floors).
+__global__ void f2_absmax_kernel(const double2* __restrict__ d, long n, double* __restrict__ part) {
+  __shared__ double sm[256];
+  double m = 0;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    m = fmax(m, sqrt(d[i].x * d[i].x + d[i].y * d[i].y));
+  sm[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) sm[threadIdx.x] = fmax(sm[threadIdx.x], sm[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sm[0];
+}
+
+// Builds the preconditioner vector from the (scaled) diagonal or row sums:
+// jacobi (kind 1): p = 1/d, identity action (p = s_A) where |d| < floor;
+// rowabs (kind 2): p = 1/s, identity action where s < floor.  Counts the
+// floored entries.
+__global__ void f2_precond_build_kernel(const double2* __restrict__ d, long n, int kind, double floor_dev,
+                                        double ident, double2* __restrict__ p64, float2* __restrict__ p32,
+                                        unsigned long long* __restrict__ nsmall) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const double2 v = d[i];
+    double2 o;
+    if (kind == 1) {
+      const double m2 = v.x * v.x + v.y * v.y;
+      if (sqrt(m2) < floor_dev) {
+        o = make_double2(ident, 0.0);
+        atomicAdd(nsmall, 1ull);
+      } else {
+        o = make_double2(v.x / m2, -v.y / m2);
+      }
+    } else {
+      if (v.x < floor_dev) {
+        o = make_double2(ident, 0.0);
+        atomicAdd(nsmall, 1ull);
+      } else {
+        o = make_double2(1.0 / v.x, 0.0);
+      }
+    }
+    p64[i] = o;
+    p32[i] = make_float2((float)o.x, (float)o.y);
+  }
+}
+
+}  // namespace wdg

@@ -31,6 +31,7 @@
 #include "host_setup.hpp"
 #include "kernels.hpp"
 #include "theta_fft.hpp"
+#include "full2d.hpp"
 
 using namespace wdg;
 
@@ -69,15 +70,19 @@ struct DevBuf {
 
 enum ProfId {
   P_TRANSPORT = 0, P_GEMM1, P_COMBINE, P_GEMM2, P_LINE_AGG, P_LINE_APPLY, P_DOT, P_UPDATE, P_THETA_FFT, P_LINE_CARRY,
+  P_F2_APPLY, P_F2_DENSE,
   P_COUNT
 };
 const char* kProfNames[P_COUNT] = {"transport",   "theta_gemm1",    "theta_combine", "theta_gemm2",
                                    "line_aggregate", "line_apply",  "arnoldi_dot",   "arnoldi_update",
-                                   "theta_fft",   "line_carry"};
+                                   "theta_fft",   "line_carry",     "f2_apply",      "f2_dense"};
 
 }  // namespace
 
+struct F2Data;   // full-2D operator data (full2d_host.hpp)
+
 struct wdg_ctx {
+  std::shared_ptr<F2Data> f2;         // set for full-2D contexts
   Geometry g;
   std::string err;
   hipStream_t stream = nullptr;
@@ -222,6 +227,8 @@ void prof_collect(wdg_ctx* c) {
 }
 
 }  // namespace
+
+#include "full2d_host.hpp"
 
 namespace {
 
@@ -423,6 +430,10 @@ void theta_fft_attrs() {
 // y = A_hat u in precision T.
 template <typename T>
 void op_apply(wdg_ctx* c, const typename cvec<T>::type* u, typename cvec<T>::type* y, const GmresState* st) {
+  if (c->f2) {
+    f2_apply<T>(c, u, y, st);
+    return;
+  }
   op_transport<T>(c, u, y, st);
   if (c->use_fft) {
     op_theta_fft<T>(c, u, y, st);
@@ -470,6 +481,10 @@ void op_precond_n(wdg_ctx* c, const typename cvec<T>::type* r, typename cvec<T>:
 
 template <typename T>
 void op_precond(wdg_ctx* c, const typename cvec<T>::type* r, typename cvec<T>::type* z, const GmresState* st) {
+  if (c->f2) {
+    f2_precond<T>(c, r, z, st);
+    return;
+  }
   switch (c->g.n) {
     case 2: op_precond_n<2, T>(c, r, z, st); break;
     case 3: op_precond_n<3, T>(c, r, z, st); break;
@@ -939,7 +954,7 @@ struct SolveRun {
   wdg_ctx* c;
   wdg_solve_opts o;
   wdg_solve_info* info;
-  bool fp64 = true, b16 = false, active = false, pending = false;
+  bool fp64 = true, b16 = false, active = false, pending = false, precond = true;
   int basis = WDG_BASIS_WORKING;
   int m = 0, its = 0, cycles = 0, reorth = 0, max_outer = 0;
   double tol = 0, inner = 0, relres = 1, reported = 1, xs = 1, prev_relres = 1;
@@ -967,7 +982,9 @@ struct SolveRun {
     basis = fp64 ? WDG_BASIS_WORKING : o.krylov_basis;
     b16 = basis != WDG_BASIS_WORKING;
     ensure_krylov(c, m, fp64, b16);
-    if (!c->use_fft) autotune(c, fp64);
+    if (c->f2) f2_prepare_solve(c);
+    else if (!c->use_fft) autotune(c, fp64);
+    precond = o.use_precond != 0 && !(c->f2 && c->f2->precond == WDG2_PRECOND_NONE);
     if (!c->h_st) HIPCHK(hipHostMalloc((void**)&c->h_st, sizeof(GmresState), hipHostMallocDefault));
     xs = (c->rhs_norm > 0) ? c->rhs_norm / c->g.scale : 1.0;  // physical x = xs * x_device
     c->sol_scale = xs;
@@ -1022,7 +1039,7 @@ struct SolveRun {
   }
   void step() {
     const int mc = std::min<int>(m, o.max_iter - its);
-    run_cycle(c, fp64, basis, mc, o.use_precond != 0, o.orthogonalization == 1, o.use_graph != 0 && mc == m);
+    run_cycle(c, fp64, basis, mc, precond, o.orthogonalization == 1, o.use_graph != 0 && mc == m);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(c->h_st, c->d_st.p, sizeof(GmresState), hipMemcpyDeviceToHost, c->stream));
     cycles += 1;
@@ -1206,6 +1223,7 @@ int64_t wdg_size(const wdg_ctx* ctx) { return ctx ? ctx->N : 0; }
 
 int wdg_set_potential(wdg_ctx* ctx, const wdg_potential* pot) {
   if (!ctx || !pot) return fail(ctx, WDG_ERR_ARG, "null argument");
+  if (ctx->f2) return fail(ctx, WDG_ERR_UNSUPPORTED, "full-2D context: use wdg2_set_potential");
   return guarded(ctx, [&]() {
     set_potential_impl(ctx, *pot);
     return WDG_OK;
@@ -1215,6 +1233,13 @@ int wdg_set_potential(wdg_ctx* ctx, const wdg_potential* pot) {
 int wdg_rhs(wdg_ctx* ctx, double* rhs) {
   if (!ctx || !rhs) return fail(ctx, WDG_ERR_ARG, "null argument");
   if (!ctx->have_potential) return fail(ctx, WDG_ERR_STATE, "wdg_set_potential has not been called");
+  if (ctx->f2) {
+    return guarded(ctx, [&]() {
+      HIPCHK(hipMemcpy(rhs, ctx->d_b.p, (size_t)ctx->N * 16, hipMemcpyDeviceToHost));
+      for (long i = 0; i < 2 * ctx->N; ++i) rhs[i] *= ctx->rhs_norm;
+      return WDG_OK;
+    });
+  }
   std::vector<double> full;
   build_rhs(ctx->g, ctx->fnl, ctx->fnr, full);
   std::memcpy(rhs, full.data(), full.size() * sizeof(double));
@@ -1274,6 +1299,11 @@ int wdg_precond(wdg_ctx* ctx, const double* r, double* z) {
   if (!ctx->have_potential) return fail(ctx, WDG_ERR_STATE, "wdg_set_potential has not been called");
   return guarded(ctx, [&]() {
     const long N = ctx->N;
+    if (ctx->f2) f2_prepare_solve(ctx);
+    if (ctx->f2 && ctx->f2->precond == WDG2_PRECOND_NONE) {   // identity in physical units
+      std::memcpy(z, r, (size_t)N * 16);
+      return WDG_OK;
+    }
     ctx->d_u64.alloc(N);
     ctx->d_v64.alloc(N);
     HIPCHK(hipMemcpyAsync(ctx->d_u64.p, r, N * 16, hipMemcpyHostToDevice, ctx->stream));
@@ -1292,6 +1322,7 @@ int wdg_precond_device(wdg_ctx* ctx, const void* d_r, void* d_z, int32_t precisi
   if (!ctx || !d_r || !d_z) return fail(ctx, WDG_ERR_ARG, "null argument");
   if (!ctx->have_potential) return fail(ctx, WDG_ERR_STATE, "wdg_set_potential has not been called");
   return guarded(ctx, [&]() {
+    if (ctx->f2) f2_prepare_solve(ctx);
     if (precision == WDG_PREC_FP64)
       op_precond<double>(ctx, (const double2*)d_r, (double2*)d_z, nullptr);
     else
@@ -1307,6 +1338,13 @@ int wdg_diagonal(wdg_ctx* ctx, double* d) {
   if (!ctx || !d) return fail(ctx, WDG_ERR_ARG, "null argument");
   if (!ctx->have_potential) return fail(ctx, WDG_ERR_STATE, "wdg_set_potential has not been called");
   return guarded(ctx, [&]() {
+    if (ctx->f2) {
+      f2_diag_into(ctx, F2_DIAG, ctx->d_y.p);
+      HIPCHK(hipMemcpyAsync(d, ctx->d_y.p, (size_t)ctx->N * 16, hipMemcpyDeviceToHost, ctx->stream));
+      HIPCHK(hipStreamSynchronize(ctx->stream));
+      for (long i = 0; i < 2 * ctx->N; ++i) d[i] *= ctx->f2->sA;
+      return WDG_OK;
+    }
     const Geometry& g = ctx->g;
     const int n = g.n, nk = g.nk;
     std::vector<double2> s0s1((size_t)g.n_chi * n * n * 2);
@@ -1401,6 +1439,7 @@ int wdg_solve_batch(wdg_ctx* const* ctxs, int32_t n, const wdg_solve_opts* opts,
 int wdg_observables(wdg_ctx* ctx, const double* rho, const double* m_nodes, double* n_out, double* j_out,
                     double* rho_u) {
   if (!ctx || !m_nodes || !n_out || !j_out) return fail(ctx, WDG_ERR_ARG, "null argument");
+  if (ctx->f2) return fail(ctx, WDG_ERR_UNSUPPORTED, "full-2D context: use wdg2_observables");
   return guarded(ctx, [&]() {
     const Geometry& g = ctx->g;
     const long N = ctx->N;
@@ -1453,6 +1492,7 @@ int wdg_observables(wdg_ctx* ctx, const double* rho, const double* m_nodes, doub
 
 int wdg_poisson_setup(wdg_ctx* ctx, const wdg_poisson_desc* desc) {
   if (!ctx || !desc) return fail(ctx, WDG_ERR_ARG, "null argument");
+  if (ctx->f2) return fail(ctx, WDG_ERR_UNSUPPORTED, "not available for a full-2D context");
   return guarded(ctx, [&]() {
     poisson_setup_impl(ctx, *desc);
     return WDG_OK;
@@ -1462,6 +1502,7 @@ int wdg_poisson_setup(wdg_ctx* ctx, const wdg_poisson_desc* desc) {
 int wdg_poisson_step(wdg_ctx* ctx, const double* phi_in, double bias, double* phi_out, double* n_out,
                      wdg_poisson_info* info) {
   if (!ctx || !phi_in || !phi_out) return fail(ctx, WDG_ERR_ARG, "null argument");
+  if (ctx->f2) return fail(ctx, WDG_ERR_UNSUPPORTED, "not available for a full-2D context");
   if (!ctx->have_poisson) return fail(ctx, WDG_ERR_STATE, "wdg_poisson_setup has not been called");
   if (!ctx->have_solution) return fail(ctx, WDG_ERR_STATE, "no solution available: call wdg_solve first");
   return guarded(ctx, [&]() {
@@ -1538,6 +1579,7 @@ int wdg_host_setup(const wdg_desc* desc, const wdg_potential* pot, double* rhs, 
 int wdg_transient_rk4(wdg_ctx* ctx, const wdg_transient_opts* o, const double* rho0, double* rho_out,
                       const double* m_nodes, double* n_t, double* j_t, int32_t max_saves) {
   if (!ctx || !o || !m_nodes) return fail(ctx, WDG_ERR_ARG, "null argument");
+  if (ctx->f2) return fail(ctx, WDG_ERR_UNSUPPORTED, "not available for a full-2D context");
   if (!ctx->have_potential) return fail(ctx, WDG_ERR_STATE, "wdg_set_potential has not been called");
   return guarded(ctx, [&]() {
     if (!(o->dt > 0) || o->nsteps < 0 || o->save_every < 1 || !std::isfinite(o->a_scale) ||
@@ -1607,6 +1649,7 @@ int wdg_transient_rk4(wdg_ctx* ctx, const wdg_transient_opts* o, const double* r
 
 int wdg_set_theta_method(wdg_ctx* ctx, int32_t method) {
   if (!ctx) return fail(nullptr, WDG_ERR_ARG, "null argument");
+  if (ctx->f2) return fail(ctx, WDG_ERR_UNSUPPORTED, "not available for a full-2D context");
   return guarded(ctx, [&]() {
     if (method != WDG_THETA_AUTO && method != WDG_THETA_FFT && method != WDG_THETA_GEMM)
       throw std::invalid_argument("unknown Theta method");
@@ -1654,6 +1697,127 @@ int wdg_profile(wdg_ctx* ctx, char* names, int32_t names_len, double* avg_ms, in
     names[names_len - 1] = 0;
   }
   return WDG_OK;
+}
+
+// ---- full-2D operator (include/wigner_dg.h) ----
+
+int wdg2_create(const wdg2_desc* desc, const wdg2_part* parts, int32_t n_parts, wdg_ctx** out) {
+  if (!desc || !out) return fail(nullptr, WDG_ERR_ARG, "null argument");
+  *out = nullptr;
+  auto c = std::make_unique<wdg_ctx>();
+  int rc = guarded(nullptr, [&]() {
+    f2_create_impl(c.get(), *desc, parts, n_parts);
+    return WDG_OK;
+  });
+  if (rc != WDG_OK) {
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    c->f2.reset();
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    c->stream = nullptr;
+    return rc;
+  }
+  *out = c.release();
+  return WDG_OK;
+}
+
+int wdg2_set_potential(wdg_ctx* ctx, const wdg2_potential* pot) {
+  if (!ctx || !pot) return fail(ctx, WDG_ERR_ARG, "null argument");
+  if (!ctx->f2) return fail(ctx, WDG_ERR_UNSUPPORTED, "not a full-2D context");
+  return guarded(ctx, [&]() {
+    f2_set_potential_impl(ctx, *pot);
+    return WDG_OK;
+  });
+}
+
+int wdg2_set_rhs(wdg_ctx* ctx, const double* rhs) {
+  if (!ctx || !rhs) return fail(ctx, WDG_ERR_ARG, "null argument");
+  if (!ctx->f2) return fail(ctx, WDG_ERR_UNSUPPORTED, "not a full-2D context");
+  return guarded(ctx, [&]() {
+    const long N = ctx->N;
+    double s = 0;
+    for (long i = 0; i < 2 * N; ++i) {
+      if (!std::isfinite(rhs[i])) throw std::invalid_argument("rhs must be finite");
+      s += rhs[i] * rhs[i];
+    }
+    const double old_norm = ctx->rhs_norm;
+    ctx->rhs_norm = std::sqrt(s);
+    const double inv = ctx->rhs_norm > 0 ? 1.0 / ctx->rhs_norm : 0.0;
+    ctx->stage.resize(2 * N);
+    for (long i = 0; i < 2 * N; ++i) ctx->stage[i] = rhs[i] * inv;
+    HIPCHK(hipMemcpyAsync(ctx->d_b.p, ctx->stage.data(), (size_t)N * 16, hipMemcpyHostToDevice, ctx->stream));
+    // keep a previous device solution's physical value under the new normalisation
+    if (ctx->have_solution && old_norm > 0 && ctx->rhs_norm > 0 && old_norm != ctx->rhs_norm)
+      hipLaunchKernelGGL(scale_complex_kernel, dim3(ctx->red_blocks), dim3(256), 0, ctx->stream, ctx->d_x.p,
+                         old_norm / ctx->rhs_norm, ctx->N);
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (ctx->rhs_norm > 0) ctx->sol_scale = ctx->rhs_norm / ctx->f2->sA;
+    ctx->f2->have_rhs = true;
+    return WDG_OK;
+  });
+}
+
+int wdg2_rowabs(wdg_ctx* ctx, double* out) {
+  if (!ctx || !out) return fail(ctx, WDG_ERR_ARG, "null argument");
+  if (!ctx->f2) return fail(ctx, WDG_ERR_UNSUPPORTED, "not a full-2D context");
+  if (!ctx->have_potential) return fail(ctx, WDG_ERR_STATE, "wdg2_set_potential has not been called");
+  return guarded(ctx, [&]() {
+    f2_diag_into(ctx, F2_ROWABS, ctx->d_y.p);
+    std::vector<double2> h(ctx->N);
+    HIPCHK(hipMemcpyAsync(h.data(), ctx->d_y.p, (size_t)ctx->N * 16, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    for (long i = 0; i < ctx->N; ++i) out[i] = h[i].x * ctx->f2->sA;
+    return WDG_OK;
+  });
+}
+
+int wdg2_set_precond(wdg_ctx* ctx, int32_t kind, int32_t* used) {
+  if (!ctx) return fail(ctx, WDG_ERR_ARG, "null argument");
+  if (!ctx->f2) return fail(ctx, WDG_ERR_UNSUPPORTED, "not a full-2D context");
+  if (!ctx->have_potential) return fail(ctx, WDG_ERR_STATE, "wdg2_set_potential has not been called");
+  return guarded(ctx, [&]() {
+    int u = 0;
+    f2_build_precond(ctx, kind, 1e-12, true, &u);
+    if (used) *used = u;
+    return WDG_OK;
+  });
+}
+
+int wdg2_observables(wdg_ctx* ctx, const double* rho, const int32_t* idx, const double* w, const int32_t* counts,
+                     double* out) {
+  if (!ctx || !idx || !w || !counts || !out) return fail(ctx, WDG_ERR_ARG, "null argument");
+  if (!ctx->f2) return fail(ctx, WDG_ERR_UNSUPPORTED, "not a full-2D context");
+  return guarded(ctx, [&]() {
+    F2Data& f = *ctx->f2;
+    F2ObsW ow{};
+    for (int k = 0; k < 3; ++k) {
+      if (counts[k] < 0 || counts[k] > 8) throw std::invalid_argument("counts must be in 0..8");
+      ow.cnt[k] = counts[k];
+      for (int j = 0; j < counts[k]; ++j) {
+        if (idx[8 * k + j] < 0 || idx[8 * k + j] >= f.nrel) throw std::invalid_argument("weight index out of range");
+        ow.idx[8 * k + j] = idx[8 * k + j];
+        ow.w[8 * k + j] = w[8 * k + j];
+      }
+    }
+    const double2* src;
+    double scale;
+    if (rho) {
+      ctx->d_u64.alloc(ctx->N);
+      HIPCHK(hipMemcpyAsync(ctx->d_u64.p, rho, (size_t)ctx->N * 16, hipMemcpyHostToDevice, ctx->stream));
+      src = ctx->d_u64.p;
+      scale = 1.0;
+    } else {
+      if (!ctx->have_solution) throw std::logic_error("no solution available: call wdg_solve first");
+      src = ctx->d_x.p;
+      scale = ctx->sol_scale;
+    }
+    ctx->d_obs.alloc((size_t)3 * f.ncen);
+    hipLaunchKernelGGL(f2_observables_kernel, dim3((f.ncen + 255) / 256), dim3(256), 0, ctx->stream, src, f.ncen,
+                       f.nrel, ow, scale, ctx->d_obs.p);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out, ctx->d_obs.p, (size_t)3 * f.ncen * 16, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return WDG_OK;
+  });
 }
 
 }  // extern "C"

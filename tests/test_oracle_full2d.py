@@ -255,3 +255,38 @@ def test_constant_state_has_no_interior_transport_residual(flux):
     assert len(interior) > 0
     assert np.max(np.abs(Y[interior])) < 1e-12 * np.max(np.abs(Y))
     assert np.max(np.abs(Y)) > 0   # the boundary elements do see the inflow penalty
+
+
+@pytest.mark.parametrize("flux", ["rusanov", "upwind"])
+def test_kron_diagonal_and_row_sums_match_assembly(flux):
+    """assembleKronDiagonal equals diag(A); assembleKronRowAbsSum bounds the
+    row sums of |A| (full2DVariableMassTest.m:82-91 checks the bound)."""
+    m = small_mat(); m["dg"]["params"]["full2D_X_flux"] = flux
+    m["dg"]["params"]["full2D_Y_flux"] = flux
+    p, b, parts, rparts, info, A, rhs = diff_setup(m)
+    assert np.allclose(F.kron_diagonal(parts, p), A.diagonal(), rtol=0, atol=1e-12 * abs(A).max())
+    rs = np.asarray(abs(A).sum(axis=1)).ravel()
+    bound = F.kron_row_abs_sum(parts, p)
+    assert np.all(bound >= rs - 1e-12 * rs.max())
+    # a single part's bound is exact
+    one = [parts[0]]
+    A1 = F.assemble_parts(one, p)
+    assert np.allclose(F.kron_row_abs_sum(one, p), np.asarray(abs(A1).sum(axis=1)).ravel(), rtol=1e-13, atol=0)
+
+
+@pytest.mark.parametrize("mode", ["fv-consistent", "collocated"])
+def test_drift_diagonal_and_row_bound(mode):
+    """fvConsistentDiagonal equals diag of the assembled drift; the row bound
+    is >= the row sums of |G| and exact for the collocated diagonal."""
+    m = small_mat(); m["dg"]["params"]["full2D_potentialDiscretization"] = mode
+    p = F.init_params_full2d(m, m["V"], 0.2, 0.1)
+    dr = F.DriftFull2D(m, p, m["V"])
+    G = dr.assemble()
+    assert np.allclose(dr.diagonal(), G.diagonal(), rtol=0, atol=1e-12 * abs(G).max())
+    rs = np.asarray(abs(G).sum(axis=1)).ravel()
+    bound = dr.row_abs()
+    assert np.all(bound >= rs - 1e-12 * rs.max())
+    if mode == "collocated":
+        assert np.allclose(bound, rs, rtol=1e-13, atol=0)
+    else:
+        assert np.max(bound - rs) > 0   # the triangle inequality is strict somewhere
