@@ -356,8 +356,31 @@ int wdg2_rowabs(wdg_ctx* ctx, double* s);
  * ROWABS when such entries exist; ROWABS divides by the row-absolute sums.
  * *used receives the method in effect.  wdg_solve then applies it as a right
  * preconditioner; NONE disables it. */
-enum { WDG2_PRECOND_NONE = 0, WDG2_PRECOND_JACOBI = 1, WDG2_PRECOND_ROWABS = 2 };
+enum { WDG2_PRECOND_NONE = 0, WDG2_PRECOND_JACOBI = 1, WDG2_PRECOND_ROWABS = 2, WDG2_PRECOND_LINE = 3 };
 int wdg2_set_precond(wdg_ctx* ctx, int32_t kind, int32_t* used);
+
+/* Line-transverse preconditioner (get_LineTransversePreconditioner_full2D.m:1-239):
+ * M = (sigma I + Xref)(sigma I + Tref)/sigma in the eigenbasis U of -i D_rhoX,
+ * applied as z = U T^-1 X^-1 U^H r / sigma (applyInverse, :199-239):
+ *   modes       n_rho_x x n_rho_x complex, row-major, column k = eigenvector k;
+ *   forward     per mode, nonzero = X sweep from the first element (speed >= 0);
+ *   x_inverse   (I + a K1 + |a| K2)^-1 and x_coupling the neighbour block of
+ *               xFactors (:174-197): n_rho_x * ne_x * nl_x^2 doubles, mode-major,
+ *               then element, each nl_x x nl_x row-major;
+ *   t_inverse   (B_k/sigma + I)^-1 per mode (the dense LU pages of :82-99 as
+ *               explicit inverses): n_rho_x * nT^2 complex, row-major,
+ *               nT = n_rho_y * n_Y_dof, transverse index t = rho_y + n_rho_y*Ydof.
+ * The caller computes the factors on the host (the reference's CPU setup);
+ * this call uploads them and selects WDG2_PRECOND_LINE for wdg_solve. */
+typedef struct wdg2_line_factors {
+  double sigma;
+  const double*  modes;
+  const int32_t* forward;
+  const double*  x_inverse;
+  const double*  x_coupling;
+  const double*  t_inverse;
+} wdg2_line_factors;
+int wdg2_set_line_precond(wdg_ctx* ctx, const wdg2_line_factors* factors);
 
 /* Zero-separation observables (get_Observables_full2D.m:24-60): per centre
  * DOF i, out[k*ncen + i] = sum_j w[8k+j] * rho[i*nrel + idx[8k+j]],

@@ -777,3 +777,129 @@ def solve_direct_full2d(mat, Vxy, EfL, EfR):
     rhs = assemble_rhs(rparts, p)
     rho = spla.spsolve(A.tocsc(), rhs)
     return p, A, rhs, rho
+
+
+# ---------------------------------------------------------------------------
+# Line-transverse preconditioner (get_LineTransversePreconditioner_full2D.m)
+# ---------------------------------------------------------------------------
+
+def _transverse_reference(mat, p, Vxy, ix_ref):
+    """transverseReference (:128-172): a 2-cell rho_x, 2-element X strip with
+    the potential of material column ix_ref, no X transport, no CAP, zero
+    reservoirs; returns the (rho_y, Y) block K of the transport and V of the
+    potential at rho_x cell 1 and X DOF 1 (V doubled for 'fv-consistent')."""
+    import copy
+    ref = copy.deepcopy(mat)
+    Vf = field_to_xy(Vxy, mat, 0)
+    ref["V"] = np.repeat(Vf[ix_ref][None, :], mat["Nx"], axis=0)
+    q = dict(_params(mat))
+    if isinstance(q.get("full2D"), dict):
+        q["full2D"] = dict(q["full2D"])
+    nyd = p["dg"]["Y"]["nDof"]
+    q.update({"N_rho_x": 2, "N_K_X": 2, "N_el_X": 2, "full2D_Q_diff_x": 0, "full2D_cap_scale": 0,
+              "full2D_reservoirModel": "zero", "full2D_sourceRho": np.zeros((2 * p["relative"]["NrhoY"], nyd)),
+              "full2D_drainRho": np.zeros((2 * p["relative"]["NrhoY"], nyd))})
+    ref.setdefault("dg", {})["params"] = q
+    pr = init_params_full2d(ref, ref["V"], None, None)
+    b = boundary_full2d(ref, pr, None, None, ref["V"])
+    parts, _, _ = get_diff_full2d(ref, pr, b)
+    Kfull = assemble_parts(parts, pr)
+    drift = DriftFull2D(ref, pr, ref["V"])
+    Vfull = drift.assemble()
+    nrx, nry = pr["relative"]["NrhoX"], pr["relative"]["NrhoY"]
+    nxd = pr["dg"]["X"]["nDof"]
+    ids = np.array([0 + nrx * (ry + nry * (0 + nxd * iyd)) for iyd in range(nyd) for ry in range(nry)])
+    K = Kfull[ids][:, ids].toarray()
+    V = Vfull[ids][:, ids].toarray()
+    if drift.mode == "fv-consistent":
+        V = 2 * V
+    return K, V
+
+
+class LineTransverse:
+    """get_LineTransversePreconditioner_full2D.m:1-239 with dense LU pages:
+    M = (sigma I + Xref)(sigma I + Tref)/sigma in the eigenbasis U of -i D_rhoX;
+    apply(r) = M^-1 r (applyInverse, :199-239)."""
+
+    def __init__(self, mat, p, Vxy, info, drift):
+        params = _params(mat)
+        nrx, nry = p["relative"]["NrhoX"], p["relative"]["NrhoY"]
+        X, Y = p["dg"]["X"], p["dg"]["Y"]
+        self.sz = (nrx, nry, X["nDof"], Y["nDof"])
+        self.nT = nry * Y["nDof"]
+        self.nl, self.ne = X["nLocal"], X["nElements"]
+        max_bytes = _param(params, "full2D_lineMaxFactorBytes", 4 * 1024 ** 3)
+        self.factor_bytes = (32 * self.nT ** 2 * nrx + 32 * self.nl ** 2 * nrx * self.ne + 16 * nrx ** 2
+                             + 8 * self.nT * nrx)
+        if self.factor_bytes > max_bytes:
+            raise MemoryError("DG:Full2D:LineMemoryLimit")
+        ix_ref = int(_param(params, "full2D_lineReferenceXIndex", 1)) - 1
+        lam, U = np.linalg.eig(np.asarray(p["relative"]["Ax1D"].todense()))
+        lam = lam.real
+        lam[np.abs(lam) < 100 * np.spacing(np.max(np.abs(lam)))] = 0
+        self.U, self.lam = U, lam
+        K, V = _transverse_reference(mat, p, Vxy, ix_ref)
+        self.K, self.V = K, V
+        capY = np.tile(W.get_cap(p["relative"]["rhoY"]["cells"], p["relative"]["rhoY"]["length"],
+                                 p["relative"]["rhoY"]["delta"]), Y["nDof"]) * drift.cap_coef
+        wx = W.get_cap(p["relative"]["rhoX"]["cells"], p["relative"]["rhoX"]["length"], p["relative"]["rhoX"]["delta"])
+        capX = np.real(np.sum(np.conj(U) * (wx[:, None] * U), axis=0)) * drift.cap_coef
+        if drift.mode == "fv-consistent":
+            Pm = sp.diags([np.ones(nrx), np.ones(nrx)], [0, -1], shape=(nrx + 1, nrx)).toarray()
+            weight = np.real(np.sum(np.conj(U) * ((Pm.T @ Pm / 4) @ U), axis=0))
+        else:
+            weight = np.ones(nrx)
+        self.capY, self.capX, self.weight = capY, capX, weight
+        Qx = info["scales"][0]
+        self.Qx = Qx
+        xrate = abs(Qx) * np.max(np.abs(lam)) / np.min(X["jacobian"])
+        trate = (np.abs(K).sum(axis=1).max() + np.abs(V).sum(axis=1).max() + np.max(np.abs(capY))
+                 + np.max(np.abs(capX)))
+        rel_shift = _param(params, "full2D_lineRelativeShift", 0.003)
+        self.sigma = float(_param(params, "full2D_lineShift", rel_shift * max(xrate, trate, np.finfo(float).tiny)))
+        self._x_factors(X, Qx * lam)
+        self.B = []
+        for k in range(nrx):
+            Bk = K + weight[k] * V + np.diag(capY + capX[k])
+            self.B.append(Bk / self.sigma + np.eye(self.nT))
+
+    def _x_factors(self, X, speeds):
+        """xFactors (:174-197)."""
+        n, ne = self.nl, self.ne
+        M = X["M"]
+        e0 = np.zeros((n, n)); e0[0, 0] = 1
+        en = np.zeros((n, n)); en[-1, -1] = 1
+        from_left = np.zeros((n, n)); from_left[0, -1] = 1
+        from_right = np.zeros((n, n)); from_right[-1, 0] = 1
+        K1 = np.linalg.solve(M, X["S"] + 0.5 * e0 - 0.5 * en)
+        K2 = np.linalg.solve(M, 0.5 * e0 + 0.5 * en)
+        self.forward = speeds >= 0
+        self.xinv = np.zeros((ne, speeds.size, n, n))
+        self.xcoup = np.zeros((ne, speeds.size, n, n))
+        for e in range(ne):
+            for k, s in enumerate(speeds):
+                a = s / (self.sigma * X["jacobian"][e])
+                self.xinv[e, k] = np.linalg.inv(np.eye(n) + a * K1 + abs(a) * K2)
+                self.xcoup[e, k] = (-a * np.linalg.solve(M, from_left) if self.forward[k]
+                                    else a * np.linalg.solve(M, from_right))
+        self.speeds = speeds
+
+    def apply(self, r):
+        """applyInverse (:199-239): modes of rho_x, the per-mode upwind X
+        sweep, the per-mode transverse (rho_y, Y) solve, back to rho_x, / sigma."""
+        nrx, nry, nxd, nyd = self.sz
+        n, ne, nT = self.nl, self.ne, self.nT
+        R4 = np.asarray(r).reshape(nyd, nxd, nry, nrx)                      # [Y, X, rho_y, rho_x]
+        w = np.einsum("jk,yxrj->kxyr", self.U.conj(), R4).reshape(nrx, nxd, nT)   # t = rho_y + nry*Y
+        v = np.zeros_like(w)
+        for k in range(nrx):
+            prev = np.zeros((n, nT), complex)
+            order = range(ne) if self.forward[k] else range(ne - 1, -1, -1)
+            for e in order:
+                blk = w[k, e * n:(e + 1) * n] - self.xcoup[e, k] @ prev
+                prev = self.xinv[e, k] @ blk
+                v[k, e * n:(e + 1) * n] = prev
+        out = np.stack([np.linalg.solve(self.B[k], v[k].T).T for k in range(nrx)])   # [k, X, t]
+        out = out.reshape(nrx, nxd, nyd, nry)
+        z = np.einsum("jk,kxyr->yxrj", self.U, out) / self.sigma
+        return z.reshape(-1)

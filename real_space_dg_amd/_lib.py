@@ -38,6 +38,7 @@ WDG2_POT_COLLOCATED = 1
 WDG2_PRECOND_NONE = 0
 WDG2_PRECOND_JACOBI = 1
 WDG2_PRECOND_ROWABS = 2
+WDG2_PRECOND_LINE = 3
 
 # Status code -> reference error identifier (DG:Full2D:* in the MATLAB code).
 ERROR_IDS = {
@@ -121,6 +122,12 @@ class wdg2_potential(C.Structure):
                [("pot_coef", C.c_double), ("cap_coef", C.c_double)]
 
 
+class wdg2_line_factors(C.Structure):
+    _fields_ = [("sigma", C.c_double), ("modes", C.POINTER(C.c_double)), ("forward", C.POINTER(C.c_int32)),
+                ("x_inverse", C.POINTER(C.c_double)), ("x_coupling", C.POINTER(C.c_double)),
+                ("t_inverse", C.POINTER(C.c_double))]
+
+
 # name -> (restype, argtypes)
 _P = C.c_void_p
 _D = C.POINTER(C.c_double)
@@ -149,6 +156,7 @@ SIGNATURES = {
     "wdg2_set_rhs": (C.c_int, [_P, _D]),
     "wdg2_rowabs": (C.c_int, [_P, _D]),
     "wdg2_set_precond": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_int32)]),
+    "wdg2_set_line_precond": (C.c_int, [_P, C.POINTER(wdg2_line_factors)]),
     "wdg2_observables": (C.c_int, [_P, _D, C.POINTER(C.c_int32), _D, C.POINTER(C.c_int32), _D]),
     "wdg_set_profiling": (C.c_int, [_P, C.c_uint32]),
     "wdg_profile": (C.c_int, [_P, C.c_char_p, C.c_int32, _D, C.POINTER(C.c_int64), _D, C.c_int32,

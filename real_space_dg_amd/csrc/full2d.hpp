@@ -399,7 +399,8 @@ __global__ void __launch_bounds__(kF2Block) f2_apply_kernel(const F2Const* __res
 // ---------------------------------------------------------------------------
 struct F2Dense {
   int axis;          // 0: C along X, R along rho_x; 1: along Y / rho_y
-  int face;          // -1: banded centre factor; 0 / 1: face lift at the low / high end
+  int face;          // -1: banded centre factor; 0 / 1: face lift at the low / high end;
+                     // -2: plain y = R u along the axis (overwrites y)
   int nl, ne;        // nodes per element and elements of the centre axis
   double coef;       // part coefficient / operator scale
   const double* blocks;   // banded: per element diag nl*nl, lower nl, upper nl
@@ -432,6 +433,7 @@ __global__ void __launch_bounds__(kF2Block) f2_dense_kernel(const F2Const* __res
   auto dof = [&](int k, int t) -> long { return d.axis == 0 ? (long)k + (long)nxd * t : (long)t + (long)nxd * k; };
   const int nalong = d.nl * d.ne;
   int k_out = 0, t = 0;
+  const bool plain = d.face == -2;
   if (d.face < 0) {
     const long i = line;
     k_out = d.axis == 0 ? (int)(i % nxd) : (int)(i / nxd);
@@ -448,7 +450,9 @@ __global__ void __launch_bounds__(kF2Block) f2_dense_kernel(const F2Const* __res
     w.y = (T)(d.coef * v.y);
     sR[(size_t)m * (n + 1) + k] = w;
   }
-  if (d.face < 0) {
+  if (plain) {
+    for (int r = threadIdx.x; r < nrel; r += blockDim.x) sW[r] = u[(long)line * nrel + r];
+  } else if (d.face < 0) {
     const int e = k_out / d.nl, a = k_out - e * d.nl;
     const int sb = d.nl * d.nl + 2 * d.nl;
     const double* B = d.blocks + (size_t)e * sb;
@@ -491,7 +495,9 @@ __global__ void __launch_bounds__(kF2Block) f2_dense_kernel(const F2Const* __res
         const int cc = cb * 4 + j;
         if (cc >= ncol) continue;
         const long r = (long)cc * nrx + m;
-        if (d.face < 0) {
+        if (plain) {
+          y[(long)line * nrel + r] = acc[j];
+        } else if (d.face < 0) {
           const long o = dof(k_out, t) * nrel + r;
           T2 v = y[o];
           v.x += acc[j].x;
@@ -528,7 +534,9 @@ __global__ void __launch_bounds__(kF2Block) f2_dense_kernel(const F2Const* __res
         const int m = mb * 4 + j;
         if (m >= n) continue;
         const long r = (long)m * nrx + cc;
-        if (d.face < 0) {
+        if (plain) {
+          y[(long)line * nrel + r] = acc[j];
+        } else if (d.face < 0) {
           const long o = dof(k_out, t) * nrel + r;
           T2 v = y[o];
           v.x += acc[j].x;
@@ -650,6 +658,122 @@ __global__ void f2_precond_build_kernel(const double2* __restrict__ d, long n, i
     }
     p64[i] = o;
     p32[i] = make_float2((float)o.x, (float)o.y);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Line-transverse preconditioner (get_LineTransversePreconditioner_full2D.m
+// :199-239) in the eigenbasis of -i D_rhoX.  Vectors keep the global layout
+// with the rho_x index replaced by the mode index k.
+// ---------------------------------------------------------------------------
+
+// Per (mode k, transverse t = rho_y + nry*Ydof) line: the upwind X sweep
+// prev = Xinv_e (w_e - Xcoup_e prev) over the X elements in the direction of
+// the mode's speed, in place.
+template <int NL, typename T>
+__global__ void __launch_bounds__(256) f2_line_sweep_kernel(const F2Const* __restrict__ cp,
+                                                            typename F2Vec<T>::T2* __restrict__ w,
+                                                            const T* __restrict__ xinv, const T* __restrict__ xcoup,
+                                                            const int* __restrict__ fwd, const GmresState* st) {
+  using T2 = typename F2Vec<T>::T2;
+  if (skip(st)) return;
+  const F2Const& c = *cp;
+  const int nrx = c.nrx, nry = c.nry;
+  const long lines = (long)nrx * nry * c.nyd;
+  const long id = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (id >= lines) return;
+  const int k = (int)(id % nrx);
+  const long rest = id / nrx;
+  const int ry = (int)(rest % nry), iyd = (int)(rest / nry);
+  const long sx = (long)nrx * nry;                            // stride of one X DOF
+  T2* base = w + k + (long)nrx * ry + sx * (long)c.nxd * iyd;
+  const bool forward = fwd[k] != 0;
+  T2 prev[NL];
+#pragma unroll
+  for (int a = 0; a < NL; ++a) prev[a] = f2_zero<T2>();
+  for (int step = 0; step < c.nex; ++step) {
+    const int e = forward ? step : c.nex - 1 - step;
+    const T* Xi = xinv + ((size_t)k * c.nex + e) * NL * NL;
+    const T* Xc = xcoup + ((size_t)k * c.nex + e) * NL * NL;
+    T2 blk[NL];
+#pragma unroll
+    for (int a = 0; a < NL; ++a) {
+      T2 v = base[sx * (e * NL + a)];
+#pragma unroll
+      for (int b = 0; b < NL; ++b) {
+        v.x -= Xc[a * NL + b] * prev[b].x;
+        v.y -= Xc[a * NL + b] * prev[b].y;
+      }
+      blk[a] = v;
+    }
+#pragma unroll
+    for (int a = 0; a < NL; ++a) {
+      T2 v = f2_zero<T2>();
+#pragma unroll
+      for (int b = 0; b < NL; ++b) f2_fma(v, Xi[a * NL + b], blk[b]);
+      prev[a] = v;
+      base[sx * (e * NL + a)] = v;
+    }
+  }
+}
+
+// Per mode k: out(t, X) = sum_t' Tinv_k(t, t') v(t', X) (the dense transverse
+// solve as its explicit inverse).  Tile 64 t x 16 X per workgroup, 4 t per
+// thread, t' staged through LDS in chunks of 16.
+template <typename T>
+__global__ void __launch_bounds__(256) f2_transverse_kernel(const F2Const* __restrict__ cp,
+                                                            const typename F2Vec<T>::T2* __restrict__ v,
+                                                            typename F2Vec<T>::T2* __restrict__ out,
+                                                            const typename F2Vec<T>::T2* __restrict__ tinv,
+                                                            const GmresState* st) {
+  using T2 = typename F2Vec<T>::T2;
+  if (skip(st)) return;
+  const F2Const& c = *cp;
+  const int nrx = c.nrx, nry = c.nry, nxd = c.nxd;
+  const int nT = nry * c.nyd;
+  const int k = blockIdx.z;
+  const int t0 = blockIdx.x * 64, x0 = blockIdx.y * 16;
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;   // X column, group of 4 t rows
+  __shared__ T2 sB[64][17];
+  __shared__ T2 sV[16][17];
+  const T2* Bk = tinv + (size_t)k * nT * nT;
+  auto addr = [&](int t, int ixd) -> long {
+    const int ry = t % nry, iyd = t / nry;
+    return k + (long)nrx * (ry + (long)nry * (ixd + (long)nxd * iyd));
+  };
+  T2 acc[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) acc[j] = f2_zero<T2>();
+  for (int p0 = 0; p0 < nT; p0 += 16) {
+    for (int idx = threadIdx.x; idx < 64 * 16; idx += 256) {
+      const int rr = idx >> 4, cc = idx & 15;
+      const int t = t0 + rr, tp = p0 + cc;
+      sB[rr][cc] = (t < nT && tp < nT) ? Bk[(size_t)t * nT + tp] : f2_zero<T2>();
+    }
+    {
+      const int rr = threadIdx.x >> 4, cc = threadIdx.x & 15;
+      const int tp = p0 + rr, ixd = x0 + cc;
+      sV[rr][cc] = (tp < nT && ixd < nxd) ? v[addr(tp, ixd)] : f2_zero<T2>();
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int q = 0; q < 16; ++q) {
+      const T2 vv = sV[q][tx];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const T2 b = sB[ty * 4 + j][q];
+        acc[j].x += b.x * vv.x - b.y * vv.y;
+        acc[j].y += b.x * vv.y + b.y * vv.x;
+      }
+    }
+    __syncthreads();
+  }
+  const int ixd = x0 + tx;
+  if (ixd >= nxd) return;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int t = t0 + ty * 4 + j;
+    if (t < nT) out[addr(t, ixd)] = acc[j];
   }
 }
 

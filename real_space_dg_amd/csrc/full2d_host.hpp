@@ -40,6 +40,15 @@ struct F2Data {
   DevBuf<float2> d_p32;
   DevBuf<unsigned long long> d_count;
   bool have_rhs = false;
+  // line-transverse preconditioner (mode transforms, X sweeps, transverse inverses)
+  bool have_line = false;
+  int nT = 0;
+  DevBuf<int> d_lfwd;
+  DevBuf<double> d_lxi64, d_lxc64;
+  DevBuf<float> d_lxi32, d_lxc32;
+  DevBuf<double2> d_ltinv64, d_la64, d_lb64;
+  DevBuf<float2> d_ltinv32, d_la32, d_lb32;
+  Dense lt_fwd64, lt_bwd64, lt_fwd32, lt_bwd32;   // U^H and U sA/sigma along rho_x
 };
 
 namespace {
@@ -122,9 +131,56 @@ void f2_apply(wdg_ctx* c, const typename F2Vec<T>::T2* u, typename F2Vec<T>::T2*
   }
 }
 
+template <int NL, typename T>
+void f2_line_sweep(wdg_ctx* c, typename F2Vec<T>::T2* a, const GmresState* st) {
+  F2Data& f = *c->f2;
+  const long lines = (long)f.d.n_rho_x * f.d.n_rho_y * f.nyd;
+  const T* xi = std::is_same<T, double>::value ? (const T*)f.d_lxi64.p : (const T*)f.d_lxi32.p;
+  const T* xc = std::is_same<T, double>::value ? (const T*)f.d_lxc64.p : (const T*)f.d_lxc32.p;
+  hipLaunchKernelGGL((f2_line_sweep_kernel<NL, T>), dim3((unsigned)((lines + 255) / 256)), dim3(256), 0, c->stream,
+                     f.d_c.p, a, xi, xc, f.d_lfwd.p, st);
+}
+
+// z = s_A M^-1 r (device units) for the line-transverse M.
+template <typename T>
+void f2_line_apply(wdg_ctx* c, const typename F2Vec<T>::T2* r, typename F2Vec<T>::T2* z, const GmresState* st) {
+  using T2 = typename F2Vec<T>::T2;
+  F2Data& f = *c->f2;
+  const bool dbl = std::is_same<T, double>::value;
+  T2* a = dbl ? (T2*)f.d_la64.p : (T2*)f.d_la32.p;
+  T2* b = dbl ? (T2*)f.d_lb64.p : (T2*)f.d_lb32.p;
+  const T2* tinv = dbl ? (const T2*)f.d_ltinv64.p : (const T2*)f.d_ltinv32.p;
+  const double vb = (double)c->N * sizeof(T2);
+  {
+    ProfScope ps(c, P_F2_LINE, 2.0 * vb);
+    f2_launch_dense<T>(c, dbl ? f.lt_fwd64 : f.lt_fwd32, r, a, st);
+  }
+  {
+    ProfScope ps(c, P_F2_LINE, 2.0 * vb);
+    switch (f.d.nl_x) {
+      case 2: f2_line_sweep<2, T>(c, a, st); break;
+      case 3: f2_line_sweep<3, T>(c, a, st); break;
+      default: f2_line_sweep<4, T>(c, a, st); break;
+    }
+  }
+  {
+    ProfScope ps(c, P_F2_LINE, 2.0 * vb + (double)f.d.n_rho_x * f.nT * f.nT * sizeof(T2));
+    const dim3 grid((f.nT + 63) / 64, (f.nxd + 15) / 16, f.d.n_rho_x);
+    hipLaunchKernelGGL(f2_transverse_kernel<T>, grid, dim3(256), 0, c->stream, f.d_c.p, a, b, tinv, st);
+  }
+  {
+    ProfScope ps(c, P_F2_LINE, 2.0 * vb);
+    f2_launch_dense<T>(c, dbl ? f.lt_bwd64 : f.lt_bwd32, b, z, st);
+  }
+}
+
 template <typename T>
 void f2_precond(wdg_ctx* c, const typename F2Vec<T>::T2* r, typename F2Vec<T>::T2* z, const GmresState* st) {
   F2Data& f = *c->f2;
+  if (f.precond == WDG2_PRECOND_LINE) {
+    f2_line_apply<T>(c, r, z, st);
+    return;
+  }
   const void* p;
   if (f.precond == WDG2_PRECOND_NONE) {
     // identity: the solve skips the preconditioner; the host entry point copies
@@ -453,7 +509,7 @@ void f2_set_potential_impl(wdg_ctx* c, const wdg2_potential& p) {
   HIPCHK(hipStreamSynchronize(c->stream));
   clear_graphs(c);
   // the preconditioner depends on the potential: rebuild on the next request
-  if (f.precond > 0) f.precond = -f.precond;
+  if (f.precond == WDG2_PRECOND_JACOBI || f.precond == WDG2_PRECOND_ROWABS) f.precond = -f.precond;
   c->have_potential = true;
 }
 
@@ -470,8 +526,10 @@ void f2_diag_into(wdg_ctx* c, int mode, double2* dst) {
 
 void f2_build_precond(wdg_ctx* c, int kind, double rel_floor, bool fallback, int* used) {
   F2Data& f = *c->f2;
-  if (kind == WDG2_PRECOND_NONE) {
-    f.precond = WDG2_PRECOND_NONE;
+  if (kind == WDG2_PRECOND_NONE || kind == WDG2_PRECOND_LINE) {
+    if (kind == WDG2_PRECOND_LINE && !f.have_line)
+      throw std::logic_error("line-transverse factors have not been set (wdg2_set_line_precond)");
+    f.precond = kind;
     *used = kind;
     clear_graphs(c);
     return;
@@ -515,6 +573,82 @@ void f2_prepare_solve(wdg_ctx* c) {
     int used = 0;
     f2_build_precond(c, -f.precond, 1e-12, true, &used);
   }
+}
+
+void f2_set_line_impl(wdg_ctx* c, const wdg2_line_factors& lf) {
+  F2Data& f = *c->f2;
+  const int nrx = f.d.n_rho_x, nl = f.d.nl_x, ne = f.d.ne_x;
+  const int nT = f.d.n_rho_y * f.nyd;
+  if (!(lf.sigma > 0) || !std::isfinite(lf.sigma)) throw std::invalid_argument("sigma must be positive and finite");
+  if (!lf.modes || !lf.forward || !lf.x_inverse || !lf.x_coupling || !lf.t_inverse)
+    throw std::invalid_argument("line factor arrays missing");
+  const size_t nxf = (size_t)nrx * ne * nl * nl;
+  const size_t ntf = (size_t)nrx * nT * nT;
+  for (size_t i = 0; i < nxf; ++i)
+    if (!std::isfinite(lf.x_inverse[i]) || !std::isfinite(lf.x_coupling[i]))
+      throw std::invalid_argument("X factors must be finite");
+  for (size_t i = 0; i < 2 * ntf; ++i)
+    if (!std::isfinite(lf.t_inverse[i])) throw std::invalid_argument("transverse factors must be finite");
+  f.nT = nT;
+  auto upd = [&](DevBuf<double>& b, const double* v, size_t n) {
+    b.alloc(n);
+    HIPCHK(hipMemcpyAsync(b.p, v, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  };
+  std::vector<float> x32(nxf), c32(nxf);
+  for (size_t i = 0; i < nxf; ++i) {
+    x32[i] = (float)lf.x_inverse[i];
+    c32[i] = (float)lf.x_coupling[i];
+  }
+  upd(f.d_lxi64, lf.x_inverse, nxf);
+  upd(f.d_lxc64, lf.x_coupling, nxf);
+  f.d_lxi32.alloc(nxf);
+  f.d_lxc32.alloc(nxf);
+  HIPCHK(hipMemcpyAsync(f.d_lxi32.p, x32.data(), nxf * 4, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(f.d_lxc32.p, c32.data(), nxf * 4, hipMemcpyHostToDevice, c->stream));
+  std::vector<int> fw(lf.forward, lf.forward + nrx);
+  f.d_lfwd.alloc(nrx);
+  HIPCHK(hipMemcpyAsync(f.d_lfwd.p, fw.data(), nrx * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  std::vector<float2> t32(ntf);
+  for (size_t i = 0; i < ntf; ++i) t32[i] = make_float2((float)lf.t_inverse[2 * i], (float)lf.t_inverse[2 * i + 1]);
+  f.d_ltinv64.alloc(ntf);
+  f.d_ltinv32.alloc(ntf);
+  HIPCHK(hipMemcpyAsync(f.d_ltinv64.p, lf.t_inverse, ntf * sizeof(double2), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(f.d_ltinv32.p, t32.data(), ntf * sizeof(float2), hipMemcpyHostToDevice, c->stream));
+  // mode transforms: forward U^H, backward U s_A / sigma (z in device units)
+  std::vector<double2> uf((size_t)nrx * nrx), ub((size_t)nrx * nrx);
+  const double sb = f.sA / lf.sigma;
+  for (int m = 0; m < nrx; ++m)
+    for (int k = 0; k < nrx; ++k) {
+      const double re = lf.modes[2 * ((size_t)k * nrx + m)], im = lf.modes[2 * ((size_t)k * nrx + m) + 1];
+      uf[(size_t)m * nrx + k] = make_double2(re, -im);      // (U^H)[m][k] = conj(U[k][m])
+      const double re2 = lf.modes[2 * ((size_t)m * nrx + k)], im2 = lf.modes[2 * ((size_t)m * nrx + k) + 1];
+      ub[(size_t)m * nrx + k] = make_double2(sb * re2, sb * im2);
+    }
+  auto setup_dense = [&](F2Data::Dense& dp, const std::vector<double2>& R) {
+    dp.n = nrx;
+    dp.lines = f.ncen;
+    dp.k = F2Dense{};
+    dp.k.axis = 0;
+    dp.k.face = -2;
+    dp.k.nl = nl;
+    dp.k.ne = ne;
+    dp.k.coef = 1.0;
+    dp.R.alloc(R.size());
+    HIPCHK(hipMemcpyAsync(dp.R.p, R.data(), R.size() * sizeof(double2), hipMemcpyHostToDevice, c->stream));
+    dp.k.R = dp.R.p;
+  };
+  setup_dense(f.lt_fwd64, uf);
+  setup_dense(f.lt_bwd64, ub);
+  setup_dense(f.lt_fwd32, uf);
+  setup_dense(f.lt_bwd32, ub);
+  f.d_la64.alloc(c->N);
+  f.d_lb64.alloc(c->N);
+  f.d_la32.alloc(c->N);
+  f.d_lb32.alloc(c->N);
+  HIPCHK(hipStreamSynchronize(c->stream));
+  f.have_line = true;
+  f.precond = WDG2_PRECOND_LINE;
+  clear_graphs(c);
 }
 
 }  // namespace

@@ -70,12 +70,13 @@ struct DevBuf {
 
 enum ProfId {
   P_TRANSPORT = 0, P_GEMM1, P_COMBINE, P_GEMM2, P_LINE_AGG, P_LINE_APPLY, P_DOT, P_UPDATE, P_THETA_FFT, P_LINE_CARRY,
-  P_F2_APPLY, P_F2_DENSE,
+  P_F2_APPLY, P_F2_DENSE, P_F2_LINE,
   P_COUNT
 };
 const char* kProfNames[P_COUNT] = {"transport",   "theta_gemm1",    "theta_combine", "theta_gemm2",
                                    "line_aggregate", "line_apply",  "arnoldi_dot",   "arnoldi_update",
-                                   "theta_fft",   "line_carry",     "f2_apply",      "f2_dense"};
+                                   "theta_fft",   "line_carry",     "f2_apply",      "f2_dense",
+                                   "f2_line"};
 
 }  // namespace
 
@@ -1778,6 +1779,16 @@ int wdg2_set_precond(wdg_ctx* ctx, int32_t kind, int32_t* used) {
     int u = 0;
     f2_build_precond(ctx, kind, 1e-12, true, &u);
     if (used) *used = u;
+    return WDG_OK;
+  });
+}
+
+int wdg2_set_line_precond(wdg_ctx* ctx, const wdg2_line_factors* lf) {
+  if (!ctx || !lf) return fail(ctx, WDG_ERR_ARG, "null argument");
+  if (!ctx->f2) return fail(ctx, WDG_ERR_UNSUPPORTED, "not a full-2D context");
+  if (!ctx->have_potential) return fail(ctx, WDG_ERR_STATE, "wdg2_set_potential has not been called");
+  return guarded(ctx, [&]() {
+    f2_set_line_impl(ctx, *lf);
     return WDG_OK;
   });
 }

@@ -39,7 +39,7 @@ QE = 1.602176634e-19
 KB = 1.38064852e-23         # get_DefaultReservoirRho_full2D.m
 
 PRECONDITIONERS = {"none": _lib.WDG2_PRECOND_NONE, "jacobi": _lib.WDG2_PRECOND_JACOBI,
-                   "rowabs": _lib.WDG2_PRECOND_ROWABS}
+                   "rowabs": _lib.WDG2_PRECOND_ROWABS, "line-transverse": _lib.WDG2_PRECOND_LINE}
 
 
 # ---------------------------------------------------------------------------
@@ -225,6 +225,15 @@ class RelAxis:
         return e[1] - e[0]
 
 
+def rho_modes(n: int, delta: float):
+    """Eigenpairs of the Hermitian Toeplitz A = -i D, D = tridiag(-1, 0, 1)/(2 delta):
+    lambda_k = cos(k pi/(n+1))/delta with orthonormal eigenvector
+    sqrt(2/(n+1)) i^j sin(j k pi/(n+1)), j, k = 1..n (columns)."""
+    j = np.arange(1, n + 1)
+    Phi = np.sqrt(2.0 / (n + 1)) * (1j ** (j % 4))[:, None] * np.sin(np.pi * np.outer(j, j) / (n + 1))
+    return Phi, np.cos(np.pi * j / (n + 1)) / delta
+
+
 def characteristic_split(n: int, delta: float, tol: float = 1e-12):
     """A+, A-, eigenvalues of A = -i D, D = tridiag(-1, 0, 1)/(2 delta).
 
@@ -234,10 +243,7 @@ def characteristic_split(n: int, delta: float, tol: float = 1e-12):
     1e-12 are zeroed, as in splitCharacteristicMatrix
     (get_Diff_full2D.m:718-730).
     """
-    j = np.arange(1, n + 1)
-    theta = np.pi * np.outer(j, j) / (n + 1)
-    Phi = np.sqrt(2.0 / (n + 1)) * (1j ** (j % 4))[:, None] * np.sin(theta)
-    lam = np.cos(np.pi * j / (n + 1)) / delta
+    Phi, lam = rho_modes(n, delta)
     lp = np.where(lam > tol, lam, 0.0)
     lm = np.where(lam < -tol, lam, 0.0)
     Ap = (Phi * lp) @ Phi.conj().T
@@ -307,6 +313,7 @@ class Full2DSetup:
         params = dg_params(mat)
         self.mat, self.params = mat, params
         self.EfL, self.EfR = EfL, EfR
+        self._vxy = Vxy
         ytype = y_boundary_type(params)
         if ytype in ("periodic", "hard-wall"):
             raise NotImplementedError(f"DG:Full2D:Unsupported: Y boundary '{ytype}' is not built on the device")
@@ -656,14 +663,16 @@ class Full2DOperator:
     plus ``solve`` (runGmresWithProgress, solve_transport_DG_full2D.m:422-506)
     and ``observables``."""
 
-    def __init__(self, setup: Full2DSetup):
+    def __init__(self, setup: Full2DSetup, transport: bool = True, drift: bool = True):
         lib = _lib.load()
         self._lib = lib
         self.setup = s = setup
+        self.drift_on = drift
         self.desc = _lib.wdg2_desc(s.RX.n, s.RY.n, s.X.nl, s.Y.nl, s.X.ne, s.Y.ne, s.RX.delta, s.RY.delta)
         keep = []
-        cparts = (_lib.wdg2_part * len(s.parts))()
-        for i, p in enumerate(s.parts):
+        parts = s.parts if transport else []
+        cparts = (_lib.wdg2_part * max(len(parts), 1))()
+        for i, p in enumerate(parts):
             cp = cparts[i]
             cp.axis, cp.center, cp.rel, cp.face, cp.coef = p.axis, p.center, p.rel, p.face, float(p.coef)
             if p.center == _lib.WDG2_CENTER_BANDED:
@@ -682,7 +691,7 @@ class Full2DOperator:
                 cp.r_dense = dptr(d)
             keep.extend(arrs)
         ctx = C.c_void_p()
-        check(lib.wdg2_create(C.byref(self.desc), cparts, len(s.parts), C.byref(ctx)))
+        check(lib.wdg2_create(C.byref(self.desc), cparts, len(parts), C.byref(ctx)))
         self.ctx = ctx
         self.n = s.n_total
         self.set_potential()
@@ -710,6 +719,7 @@ class Full2DOperator:
         (get_Drift_full2D.m:133-209); Vxy replaces the field if given."""
         s = self.setup
         if Vxy is not None:
+            s._vxy = Vxy
             s._potential(Vxy)
         fv = s.pot_mode == "fv-consistent"
         keep = [_f64(s.vgrid_x), _f64(s.vgrid_y), _f64(s.vfield.T), _f64(s.quad[0]), _f64(s.quad[1]),
@@ -724,7 +734,7 @@ class Full2DOperator:
          pot.cap_x, pot.cap_y) = (dptr(a) for a in keep[:9])
         if fv:
             pot.interp_x, pot.interp_y, pot.proj_x, pot.proj_y = (dptr(a) for a in keep[9:])
-        pot.pot_coef, pot.cap_coef = s.pot_coef, s.cap_coef
+        pot.pot_coef, pot.cap_coef = (s.pot_coef, s.cap_coef) if self.drift_on else (0.0, 0.0)
         check(self._lib.wdg2_set_potential(self.ctx, C.byref(pot)), self.ctx)
 
     def assemble_rhs(self) -> np.ndarray:
@@ -764,6 +774,21 @@ class Full2DOperator:
         used = C.c_int32(0)
         check(self._lib.wdg2_set_precond(self.ctx, PRECONDITIONERS[kind], C.byref(used)), self.ctx)
         return {v: k for k, v in PRECONDITIONERS.items()}[used.value]
+
+    def set_line_transverse(self) -> dict:
+        """Build the line-transverse factors on the host (the reference's CPU
+        setup) and select them for wdg_solve; returns their summary."""
+        lf = line_transverse_factors(self)
+        self._line = lf   # keep the arrays alive while the library copies them
+        f = _lib.wdg2_line_factors()
+        f.sigma = lf["sigma"]
+        f.modes = dptr(lf["modes"])
+        f.forward = lf["forward"].ctypes.data_as(C.POINTER(C.c_int32))
+        f.x_inverse = dptr(lf["x_inverse"])
+        f.x_coupling = dptr(lf["x_coupling"])
+        f.t_inverse = dptr(lf["t_inverse"])
+        check(self._lib.wdg2_set_line_precond(self.ctx, C.byref(f)), self.ctx)
+        return {k: lf[k] for k in ("sigma", "factor_bytes", "transverse_dof", "modes_count")}
 
     def precond(self, r) -> np.ndarray:
         r = _c128(r).reshape(-1)
@@ -811,6 +836,116 @@ class Full2DOperator:
         jxDG = cx * out[1].imag.T
         jyDG = cy * out[2].imag.T
         return nDG, jxDG, jyDG
+
+
+def _probe_block(op: "Full2DOperator", ids: np.ndarray) -> np.ndarray:
+    """A[ids][:, ids] of a device operator by applying it to unit vectors."""
+    out = np.zeros((ids.size, ids.size), complex)
+    e = np.zeros(op.n, complex)
+    for j, col in enumerate(ids):
+        e[col] = 1.0
+        out[:, j] = op.apply(e)[ids]
+        e[col] = 0.0
+    return out
+
+
+def transverse_reference(setup: Full2DSetup, ix_ref: int):
+    """transverseReference (get_LineTransversePreconditioner_full2D.m:128-172):
+    the (rho_y, Y) blocks K (transport) and V (potential) of a two-cell rho_x,
+    two-element X strip carrying the potential of material column ix_ref, with
+    no X transport, no CAP and zero reservoirs, at rho_x cell 1 and X DOF 1.
+    The blocks are read off the device operators of that strip."""
+    import copy
+    mat = setup.mat
+    ref = copy.deepcopy(mat)
+    Vf = setup._field_xy(mat.get("V") if setup._vxy is None else setup._vxy, 0.0)
+    ref["V"] = np.repeat(Vf[ix_ref][None, :], int(mat["Nx"]), axis=0)
+    q = dict(dg_params(mat))
+    nyd, nry = setup.Y.ndof, setup.RY.n
+    q.update({"N_rho_x": 2, "N_K_X": 2, "N_el_X": 2, "full2D_Q_diff_x": 0, "full2D_cap_scale": 0,
+              "full2D_reservoirModel": "zero", "full2D_sourceRho": np.zeros((2 * nry, nyd)),
+              "full2D_drainRho": np.zeros((2 * nry, nyd))})
+    for key in ("full2D_sourceContactMask", "full2D_drainContactMask"):
+        q.pop(key, None)
+    ref.setdefault("dg", {})
+    ref["dg"] = dict(ref["dg"])
+    ref["dg"]["params"] = q
+    strip = Full2DSetup(ref, ref["V"], None, None)
+    ids = np.array([2 * (ry + nry * (strip.X.ndof * iyd)) for iyd in range(nyd) for ry in range(nry)])
+    with Full2DOperator(strip, transport=True, drift=False) as opk:
+        K = _probe_block(opk, ids)
+    with Full2DOperator(strip, transport=False, drift=True) as opv:
+        V = _probe_block(opv, ids)
+    if strip.pot_mode == "fv-consistent":
+        V = 2 * V       # the rho_x diagonal of P_x^T P_x / 4 on two cells is 1/2
+    return K, V
+
+
+def line_transverse_factors(op: "Full2DOperator") -> dict:
+    """Host setup of get_LineTransversePreconditioner_full2D.m:17-111: memory
+    guard, modes of -i D_rhoX, frozen-X transverse blocks, CAP and potential
+    mode weights, shift sigma, X factors and the per-mode transverse inverses."""
+    s = op.setup
+    params = s.params
+    nrx, nl, ne = s.RX.n, s.X.nl, s.X.ne
+    nT = s.RY.n * s.Y.ndof
+    factor_bytes = 32 * nT ** 2 * nrx + 32 * nl ** 2 * nrx * ne + 16 * nrx ** 2 + 8 * nT * nrx
+    max_bytes = read_param(params, "full2D_lineMaxFactorBytes", 4 * 1024 ** 3)
+    if not (np.isscalar(max_bytes) and np.isfinite(max_bytes) and max_bytes > 0):
+        raise ValueError("DG:Full2D:LineMemoryLimit: lineMaxFactorBytes must be positive and finite")
+    if factor_bytes > max_bytes:
+        raise MemoryError(f"DG:Full2D:LineMemoryLimit: estimated factors {factor_bytes / 2**30:.3g} GiB exceed "
+                          f"full2D_lineMaxFactorBytes {max_bytes / 2**30:.3g} GiB")
+    ix_ref = read_param(params, "full2D_lineReferenceXIndex", 1)
+    if not (float(ix_ref).is_integer() and 1 <= ix_ref <= int(s.mat["Nx"])):
+        raise ValueError("DG:Full2D:LineReferenceIndex: reference X index must be an integer in 1:mat.Nx")
+    U, lam = rho_modes(nrx, s.RX.delta)
+    lam = lam.copy()
+    lam[np.abs(lam) < 100 * np.spacing(np.max(np.abs(lam)))] = 0
+    K, V = transverse_reference(s, int(ix_ref) - 1)
+    capY = np.tile(s.cap_y, s.Y.ndof) * s.cap_coef
+    capX = np.real(np.sum(np.conj(U) * (s.cap_x[:, None] * U), axis=0)) * s.cap_coef
+    if s.pot_mode == "fv-consistent":
+        Px = np.zeros((nrx + 1, nrx))
+        Px[np.arange(nrx), np.arange(nrx)] = 1
+        Px[np.arange(1, nrx + 1), np.arange(nrx)] = 1
+        weight = np.real(np.sum(np.conj(U) * ((Px.T @ Px / 4) @ U), axis=0))
+    else:
+        weight = np.ones(nrx)
+    Qx = s.scales[0]
+    xrate = abs(Qx) * np.max(np.abs(lam)) / np.min(s.X.jac)
+    trate = (np.abs(K).sum(axis=1).max() + np.abs(V).sum(axis=1).max() + np.max(np.abs(capY))
+             + np.max(np.abs(capX)))
+    rel_shift = read_param(params, "full2D_lineRelativeShift", 0.003)
+    sigma = float(read_param(params, "full2D_lineShift", rel_shift * max(xrate, trate, np.finfo(float).tiny)))
+    if not (np.isfinite(sigma) and sigma > 0):
+        raise ValueError("DG:Full2D:LineShift: the preconditioner shift must be positive, real and finite")
+    # xFactors: per mode k and element e, (I + a K1 + |a| K2)^-1 and the upwind coupling
+    Mi = np.linalg.inv(s.X.M)
+    e0 = np.zeros((nl, nl)); e0[0, 0] = 1
+    en = np.zeros((nl, nl)); en[-1, -1] = 1
+    K1 = Mi @ (s.X.S + 0.5 * e0 - 0.5 * en)
+    K2 = Mi @ (0.5 * e0 + 0.5 * en)
+    speeds = Qx * lam
+    forward = (speeds >= 0).astype(np.int32)
+    xinv = np.zeros((nrx, ne, nl, nl))
+    xcoup = np.zeros((nrx, ne, nl, nl))
+    for k in range(nrx):
+        for e in range(ne):
+            a = speeds[k] / (sigma * s.X.jac[e])
+            xinv[k, e] = np.linalg.inv(np.eye(nl) + a * K1 + abs(a) * K2)
+            if forward[k]:
+                xcoup[k, e][:, nl - 1] = -a * Mi[:, 0]
+            else:
+                xcoup[k, e][:, 0] = a * Mi[:, nl - 1]
+    tinv = np.empty((nrx, nT, nT), complex)
+    for k in range(nrx):
+        B = (K + weight[k] * V + np.diag(capY + capX[k])) / sigma + np.eye(nT)
+        tinv[k] = np.linalg.inv(B)
+    return {"sigma": sigma, "modes": np.ascontiguousarray(U), "forward": forward,
+            "x_inverse": np.ascontiguousarray(xinv), "x_coupling": np.ascontiguousarray(xcoup),
+            "t_inverse": np.ascontiguousarray(tinv), "factor_bytes": factor_bytes, "transverse_dof": nT,
+            "modes_count": nrx}
 
 
 def project_to_material_y(values_dg, y_dg, target_y):
@@ -861,7 +996,11 @@ def solve_transport_DG_full2D(mat: dict, Vxy=None, EfL=None, EfR=None, rho0=None
         solver = SolveOptions(tol=float(read_param(params, "full2D_solverTol", 1e-4)), restart=restart,
                               max_iter=int(read_param(params, "full2D_solverMaxIt", 1400)) * restart)
     with Full2DOperator(setup) as op:
-        used = op.set_preconditioner(pmode)
+        if pmode == "line-transverse":
+            op.set_line_transverse()
+            used = pmode
+        else:
+            used = op.set_preconditioner(pmode)
         solver.precondition = used != "none"
         op.upload_rhs()
         rho, info = op.solve(solver, x0=rho0)

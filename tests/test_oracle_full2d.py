@@ -290,3 +290,70 @@ def test_drift_diagonal_and_row_bound(mode):
         assert np.allclose(bound, rs, rtol=1e-13, atol=0)
     else:
         assert np.max(bound - rs) > 0   # the triangle inequality is strict somewhere
+
+
+def _line_setup(mode="fv-consistent", **over):
+    m = small_mat()
+    m["dg"]["params"].update({"full2D_potentialDiscretization": mode, **over})
+    p = F.init_params_full2d(m, m["V"], 0.2, 0.1)
+    b = F.boundary_full2d(m, p, 0.2, 0.1, m["V"])
+    parts, _, info = F.get_diff_full2d(m, p, b)
+    drift = F.DriftFull2D(m, p, m["V"])
+    return m, p, parts, info, drift, F.LineTransverse(m, p, m["V"], info, drift)
+
+
+@pytest.mark.parametrize("mode", ["fv-consistent", "collocated"])
+def test_line_transverse_inverts_its_factored_operator(mode):
+    """apply(r) = M^-1 r with M = (sigma I + Xref)(sigma I + Tref)/sigma
+    (get_LineTransversePreconditioner_full2D.m:4-6) assembled independently:
+    Xref per mode = s_k H1 + |s_k| H2 (the DG upwind X operator of speed
+    s_k = Q_x lambda_k), Tref per mode = K + w_k V + diag(capY + capX_k)."""
+    m, p, parts, info, drift, lt = _line_setup(mode)
+    nrx, nry, nxd, nyd = lt.sz
+    ops = F.one_dimensional_dg_operators(p["dg"]["X"])
+    H1, H2 = ops["H1"].toarray(), ops["H2"].toarray() if "H2" in ops else None
+    if H2 is None:
+        H2 = (ops["H2Interior"] + ops["H2Boundary"]).toarray()
+    N = p["nTotal"]
+    nT = lt.nT
+    # mode-space index (k fastest, rho_y, X, Y)
+    Xm = np.zeros((N, N), complex)
+    Tm = np.zeros((N, N), complex)
+    idx = np.arange(N).reshape(nyd, nxd, nry, nrx)
+    for k in range(nrx):
+        s = lt.Qx * lt.lam[k]
+        Hk = s * H1 + abs(s) * H2
+        Bt = lt.K + lt.weight[k] * lt.V + np.diag(lt.capY + lt.capX[k])
+        for iyd in range(nyd):
+            for ry in range(nry):
+                rows = idx[iyd, :, ry, k]
+                Xm[np.ix_(rows, rows)] += Hk
+        for ixd in range(nxd):
+            tt = idx[:, ixd, :, k].reshape(-1)        # (Y, rho_y) -> t = rho_y + nry*Y
+            Tm[np.ix_(tt, tt)] += Bt
+    sig = lt.sigma
+    Mm = (sig * np.eye(N) + Xm) @ (sig * np.eye(N) + Tm) / sig
+    G = np.kron(np.eye(N // nrx), lt.U)
+    M = G @ Mm @ G.conj().T
+    r = np.sin(np.arange(N) * 0.7) + 1j * np.cos(np.arange(N) * 0.3)
+    z = lt.apply(M @ r)
+    assert np.max(np.abs(z - r)) < 1e-9 * np.max(np.abs(r))
+    assert lt.sigma > 0 and lt.factor_bytes > 0
+
+
+def test_line_transverse_reference_block_is_the_y_transport():
+    """K (transverseReference) equals the Y-part of the full operator at rho_x
+    cell 1, X DOF 1: the Y parts do not depend on the X strip used to build it."""
+    m, p, parts, info, drift, lt = _line_setup("fv-consistent")
+    Ay = F.assemble_parts([q for q in parts if q[0].startswith("Y")], p)
+    nrx, nry, nxd, nyd = lt.sz
+    ids = np.array([nrx * (ry + nry * (nxd * iyd)) for iyd in range(nyd) for ry in range(nry)])
+    assert np.allclose(Ay[ids][:, ids].toarray(), lt.K, rtol=0, atol=1e-12 * np.abs(lt.K).max())
+    # the rho_x diagonal of P_x^T P_x / 4 on two cells is 1/2 (hence V is doubled)
+    Px = sp.diags([np.ones(2), np.ones(2)], [0, -1], shape=(3, 2)).toarray()
+    assert np.allclose(np.diag(Px.T @ Px / 4), 0.5)
+
+
+def test_line_transverse_memory_guard():
+    with pytest.raises(MemoryError, match="LineMemoryLimit"):
+        _line_setup("fv-consistent", full2D_lineMaxFactorBytes=1000)
