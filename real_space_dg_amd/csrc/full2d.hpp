@@ -40,6 +40,7 @@ struct F2Const {
   int pot_mode;                 // 0 fv-consistent, 1 collocated
   int nqx, nqy, nxg, nyg;       // quadrature points, material grid size
   int npx, npy;                 // relative sample points per axis (vertices or cells)
+  int v_lds;                    // 1: the material grid is staged in LDS
   double inv2dx, inv2dy;        // 1/(2 delta) of the central difference
   double pot_coef, cap_coef;    // drift coefficients / operator scale (pot multiplies i*DeltaV)
   // banded centre parts (coefficient and 1/s_A folded in); null when absent
@@ -87,6 +88,18 @@ __device__ __forceinline__ double f2_delta_v(const F2Const& c, int kx, int ky) {
   return vp - vm;
 }
 
+// The same from LDS-staged sample tables (index pairs and weights for +/-)
+// and the LDS or global material grid.
+__device__ __forceinline__ double f2_bilin_s(const double* __restrict__ vl, const double* __restrict__ vg, bool lds,
+                                             int nxg, int ix, double tx, int iy, double ty) {
+  const int i0 = iy * nxg + ix, i1 = i0 + nxg;
+  const double v00 = lds ? vl[i0] : vg[i0], v10 = lds ? vl[i0 + 1] : vg[i0 + 1];
+  const double v01 = lds ? vl[i1] : vg[i1], v11 = lds ? vl[i1 + 1] : vg[i1 + 1];
+  const double a = v00 + tx * (v10 - v00);
+  const double b = v01 + tx * (v11 - v01);
+  return a + ty * (b - a);
+}
+
 // Neighbour value with zero padding outside [0, n).
 template <typename T2>
 __device__ __forceinline__ T2 f2_ld(const T2* p, long off, bool ok) {
@@ -94,11 +107,36 @@ __device__ __forceinline__ T2 f2_ld(const T2* p, long off, bool ok) {
 }
 
 // LDS layout of the apply kernel: su[NL][rows+2][nrx] (own element, one halo
-// row each side) followed by sz[NL][rows+1][nrx+1] (vertex values).
+// row each side), then for the FV-consistent potential sz[NL][rows+1][nrx+1]
+// (vertex values), the element's x sample table (nqx*npx*2 cells, weights),
+// the tile's y sample table (nqy*(rows+1)*2) and, when small enough, the
+// material grid.  Offsets are 16-byte aligned.
+struct F2Lds {
+  size_t sz, xi, xt, yi, yt, v, total;
+};
+__host__ __device__ inline size_t f2_al(size_t b) { return (b + 15) & ~(size_t)15; }
+__host__ __device__ inline F2Lds f2_lds_layout(int nl, int rows, int nrx, size_t t2, bool fv, int nqx, int nqy, int npx, int nv) {
+  F2Lds L{};
+  size_t o = f2_al((size_t)nl * (rows + 2) * nrx * t2);
+  if (fv) {
+    L.sz = o;
+    o = f2_al(o + (size_t)nl * (rows + 1) * (nrx + 1) * t2);
+    L.xi = o;
+    o = f2_al(o + (size_t)nqx * npx * 2 * sizeof(int));
+    L.xt = o;
+    o = f2_al(o + (size_t)nqx * npx * 2 * sizeof(double));
+    L.yi = o;
+    o = f2_al(o + (size_t)nqy * (rows + 1) * 2 * sizeof(int));
+    L.yt = o;
+    o = f2_al(o + (size_t)nqy * (rows + 1) * 2 * sizeof(double));
+  }
+  L.v = o;
+  o = f2_al(o + (size_t)nv * sizeof(double));
+  L.total = o;
+  return L;
+}
 inline size_t f2_lds_bytes(int nl, int rows, int nrx, size_t t2, bool fv) {
-  size_t b = (size_t)nl * (rows + 2) * nrx * t2;
-  if (fv) b += (size_t)nl * (rows + 1) * (nrx + 1) * t2;
-  return b;
+  return f2_lds_layout(nl, rows, nrx, t2, fv, 0, 0, 0, 0).total;
 }
 
 template <int NLX, int NLY, typename T, int MODE>
@@ -118,9 +156,34 @@ __global__ void __launch_bounds__(kF2Block) f2_apply_kernel(const F2Const* __res
   const int R2 = c.rows + 2;
   const bool fv = c.pot_mode == 0;
   extern __shared__ __align__(16) unsigned char f2_smem[];
+  const F2Lds L = f2_lds_layout(NL, c.rows, nrx, sizeof(T2), fv, c.nqx, c.nqy, c.npx, c.v_lds ? c.nxg * c.nyg : 0);
   T2* su = reinterpret_cast<T2*>(f2_smem);
-  T2* sz = su + (size_t)NL * R2 * nrx;
+  T2* sz = reinterpret_cast<T2*>(f2_smem + L.sz);
+  int* sxi = reinterpret_cast<int*>(f2_smem + L.xi);
+  double* sxt = reinterpret_cast<double*>(f2_smem + L.xt);
+  int* syi = reinterpret_cast<int*>(f2_smem + L.yi);
+  double* syt = reinterpret_cast<double*>(f2_smem + L.yt);
+  double* sv = reinterpret_cast<double*>(f2_smem + L.v);
+  const bool vl = c.v_lds != 0;
   auto cdof = [&](int ax, int ay) { return (long)(ex * NLX + ax) + (long)c.nxd * (ey * NLY + ay); };
+  if (vl)
+    for (int i = threadIdx.x; i < c.nxg * c.nyg; i += blockDim.x) sv[i] = c.v[i];
+  if (fv) {
+    const int nxt = c.nqx * c.npx * 2;
+    const size_t xo = (size_t)ex * nxt;
+    for (int i = threadIdx.x; i < nxt; i += blockDim.x) {
+      sxi[i] = c.tix[xo + i];
+      sxt[i] = c.ttx[xo + i];
+    }
+    const int nyr = (nrows + 1) * 2;
+    for (int i = threadIdx.x; i < c.nqy * nyr; i += blockDim.x) {
+      const int qy = i / nyr, rem = i - qy * nyr;
+      const size_t g = ((size_t)(ey * c.nqy + qy) * c.npy + ry0) * 2 + rem;
+      syi[qy * (c.rows + 1) * 2 + rem] = c.tiy[g];
+      syt[qy * (c.rows + 1) * 2 + rem] = c.tty[g];
+    }
+  }
+  if (MODE != F2_APPLY) __syncthreads();
 
   if (MODE == F2_APPLY) {
     for (int idx = threadIdx.x; idx < NL * R2 * nrx; idx += blockDim.x) {
@@ -165,7 +228,9 @@ __global__ void __launch_bounds__(kF2Block) f2_apply_kernel(const F2Const* __res
 #pragma unroll
       for (int a = 0; a < NL; ++a) Z[a] = f2_zero<T2>();
       for (int qx = 0; qx < c.nqx; ++qx) {
-        const int kx = (ex * c.nqx + qx) * c.npx + vx;
+        const int kx = 2 * (qx * c.npx + vx);
+        const int ixp = sxi[kx], ixm = sxi[kx + 1];
+        const double txp = sxt[kx], txm = sxt[kx + 1];
         T2 t1[NLY];
         if (MODE == F2_APPLY) {
 #pragma unroll
@@ -180,8 +245,10 @@ __global__ void __launch_bounds__(kF2Block) f2_apply_kernel(const F2Const* __res
 #pragma unroll
         for (int ay = 0; ay < NLY; ++ay) acc[ay] = f2_zero<T2>();
         for (int qy = 0; qy < c.nqy; ++qy) {
-          const int ky = (ey * c.nqy + qy) * c.npy + vy;
-          const T dv = (T)f2_delta_v(c, kx, ky);
+          const int ky = 2 * (qy * (c.rows + 1) + vyl);
+          const double vp = f2_bilin_s(sv, c.v, vl, c.nxg, ixp, txp, syi[ky], syt[ky]);
+          const double vm = f2_bilin_s(sv, c.v, vl, c.nxg, ixm, txm, syi[ky + 1], syt[ky + 1]);
+          const T dv = (T)(vp - vm);
           if (MODE == F2_APPLY) {
             T2 wq = f2_zero<T2>();
 #pragma unroll
@@ -269,7 +336,10 @@ __global__ void __launch_bounds__(kF2Block) f2_apply_kernel(const F2Const* __res
         // collocated: i * pot * DeltaV(node, cell) + cap, diagonal
         const int kx = (ex * NLX + ax) * c.npx + rx;
         const int ky = (ey * NLY + ay) * c.npy + ry;
-        const T dv = (T)f2_delta_v(c, kx, ky);
+        const double vp = f2_bilin_s(sv, c.v, vl, c.nxg, c.tix[2 * kx], c.ttx[2 * kx], c.tiy[2 * ky], c.tty[2 * ky]);
+        const double vm = f2_bilin_s(sv, c.v, vl, c.nxg, c.tix[2 * kx + 1], c.ttx[2 * kx + 1], c.tiy[2 * ky + 1],
+                                     c.tty[2 * ky + 1]);
+        const T dv = (T)(vp - vm);
         const T dre = cc * cap, dim = pc * dv;
         if (MODE == F2_APPLY) {
           const T2 uu = su[((size_t)a * R2 + ryl + 1) * nrx + rx];
@@ -402,6 +472,7 @@ struct F2Dense {
   int face;          // -1: banded centre factor; 0 / 1: face lift at the low / high end;
                      // -2: plain y = R u along the axis (overwrites y)
   int nl, ne;        // nodes per element and elements of the centre axis
+  int chunk;         // relative cells across the axis per workgroup (gridDim.y chunks)
   double coef;       // part coefficient / operator scale
   const double* blocks;   // banded: per element diag nl*nl, lower nl, upper nl
   const double* lift;     // face: nl
@@ -409,11 +480,14 @@ struct F2Dense {
   const double2* R;       // n x n row-major, n = n_rho along the axis
 };
 
-inline size_t f2_dense_lds(int n, int nrel, size_t t2) { return ((size_t)n * (n + 1) + nrel) * t2; }
+// LDS: R (n x (n+1), padded) and the input block (n along the axis x chunk across).
+inline size_t f2_dense_lds(int n, int chunk, size_t t2) { return ((size_t)n * (n + 1) + (size_t)n * chunk) * t2; }
 
-// One workgroup per output line: a centre DOF (banded) or a transverse DOF of
-// the face (lift).  W = C-combination of u in LDS, then R along the axis,
-// 4 outputs per thread sharing one R (or W) load.
+// Workgroup (line, chunk): line = a centre DOF (banded, plain) or a
+// transverse DOF of the face (lift); chunk = a range of relative cells across
+// the axis (rho_y for axis 0, rho_x for axis 1), which the dense factor does
+// not couple.  W = C-combination of u in LDS, then out(m, c) = sum_k R(m, k)
+// W(k, c), 4 outputs per thread sharing one R (or W) load.
 template <typename T>
 __global__ void __launch_bounds__(kF2Block) f2_dense_kernel(const F2Const* __restrict__ cp, F2Dense d,
                                                             const typename F2Vec<T>::T2* __restrict__ u,
@@ -423,21 +497,25 @@ __global__ void __launch_bounds__(kF2Block) f2_dense_kernel(const F2Const* __res
   if (skip(st)) return;
   const F2Const& c = *cp;
   const int nrx = c.nrx, nry = c.nry, nrel = c.nrel;
-  const int n = d.axis == 0 ? nrx : nry;
+  const int n = d.axis == 0 ? nrx : nry;          // along the axis
+  const int ncol = d.axis == 0 ? nry : nrx;       // across
+  const int c0 = blockIdx.y * d.chunk;
+  const int cn = min(d.chunk, ncol - c0);
+  if (cn <= 0) return;
   const int line = blockIdx.x;
   extern __shared__ __align__(16) unsigned char f2_smem[];
   T2* sR = reinterpret_cast<T2*>(f2_smem);          // n x (n+1)
-  T2* sW = sR + (size_t)n * (n + 1);                // nrel
-  // centre DOF helpers: along-axis index k and transverse index t
+  T2* sW = sR + (size_t)n * (n + 1);                // [k][c - c0], n x chunk
   const int nxd = c.nxd;
   auto dof = [&](int k, int t) -> long { return d.axis == 0 ? (long)k + (long)nxd * t : (long)t + (long)nxd * k; };
+  // relative index of (k along, cc across)
+  auto rel = [&](int k, int cc) -> long { return d.axis == 0 ? (long)cc * nrx + k : (long)k * nrx + cc; };
   const int nalong = d.nl * d.ne;
-  int k_out = 0, t = 0;
   const bool plain = d.face == -2;
+  int k_out = 0, t = 0;
   if (d.face < 0) {
-    const long i = line;
-    k_out = d.axis == 0 ? (int)(i % nxd) : (int)(i / nxd);
-    t = d.axis == 0 ? (int)(i / nxd) : (int)(i % nxd);
+    k_out = d.axis == 0 ? line % nxd : line / nxd;
+    t = d.axis == 0 ? line / nxd : line % nxd;
   } else {
     t = line;
     if (d.mask && !d.mask[t]) return;
@@ -450,108 +528,93 @@ __global__ void __launch_bounds__(kF2Block) f2_dense_kernel(const F2Const* __res
     w.y = (T)(d.coef * v.y);
     sR[(size_t)m * (n + 1) + k] = w;
   }
+  const int nw = n * cn;
   if (plain) {
-    for (int r = threadIdx.x; r < nrel; r += blockDim.x) sW[r] = u[(long)line * nrel + r];
+    for (int idx = threadIdx.x; idx < nw; idx += blockDim.x) {
+      const int k = idx % n, cc = idx / n;
+      sW[(size_t)k * d.chunk + cc] = u[(long)line * nrel + rel(k, c0 + cc)];
+    }
   } else if (d.face < 0) {
     const int e = k_out / d.nl, a = k_out - e * d.nl;
     const int sb = d.nl * d.nl + 2 * d.nl;
     const double* B = d.blocks + (size_t)e * sb;
-    for (int r = threadIdx.x; r < nrel; r += blockDim.x) {
+    for (int idx = threadIdx.x; idx < nw; idx += blockDim.x) {
+      const int k = idx % n, cc = idx / n;
+      const long r = rel(k, c0 + cc);
       T2 w = f2_zero<T2>();
       for (int b = 0; b < d.nl; ++b) f2_fma(w, (T)B[a * d.nl + b], u[dof(e * d.nl + b, t) * nrel + r]);
       if (e > 0) f2_fma(w, (T)B[d.nl * d.nl + a], u[dof(e * d.nl - 1, t) * nrel + r]);
       if (e < d.ne - 1) f2_fma(w, (T)B[d.nl * d.nl + d.nl + a], u[dof((e + 1) * d.nl, t) * nrel + r]);
-      sW[r] = w;
+      sW[(size_t)k * d.chunk + cc] = w;
     }
   } else {
     const int ktr = d.face == 0 ? 0 : nalong - 1;
-    for (int r = threadIdx.x; r < nrel; r += blockDim.x) sW[r] = u[dof(ktr, t) * nrel + r];
+    const long base = dof(ktr, t) * nrel;
+    for (int idx = threadIdx.x; idx < nw; idx += blockDim.x) {
+      const int k = idx % n, cc = idx / n;
+      sW[(size_t)k * d.chunk + cc] = u[base + rel(k, c0 + cc)];
+    }
   }
   __syncthreads();
-  // out[m][cidx]: axis 0: m = rx, cidx = ry; axis 1: m = ry, cidx = rx
-  const int ncol = d.axis == 0 ? nry : nrx;
-  const int nblk = d.axis == 0 ? (ncol + 3) / 4 : (n + 3) / 4;
-  const int items = d.axis == 0 ? n * nblk : ncol * nblk;
+  // items: m fastest (axis 0: m = rx is contiguous in memory) with 4 c per
+  // thread, or c fastest (axis 1: c = rx) with 4 m per thread
+  const int nb = d.axis == 0 ? (cn + 3) / 4 : (n + 3) / 4;
+  const int items = d.axis == 0 ? n * nb : cn * nb;
   for (int it = threadIdx.x; it < items; it += blockDim.x) {
     T2 acc[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[j] = f2_zero<T2>();
+    int mm[4], cc[4];
     if (d.axis == 0) {
       const int m = it % n, cb = it / n;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) { mm[j] = m; cc[j] = cb * 4 + j; }
       for (int k = 0; k < n; ++k) {
         const T2 rv = sR[(size_t)m * (n + 1) + k];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          const int cc = cb * 4 + j;
-          if (cc < ncol) {
-            const T2 w = sW[(size_t)cc * nrx + k];
+        for (int j = 0; j < 4; ++j)
+          if (cc[j] < cn) {
+            const T2 w = sW[(size_t)k * d.chunk + cc[j]];
             acc[j].x += rv.x * w.x - rv.y * w.y;
             acc[j].y += rv.x * w.y + rv.y * w.x;
           }
-        }
-      }
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int cc = cb * 4 + j;
-        if (cc >= ncol) continue;
-        const long r = (long)cc * nrx + m;
-        if (plain) {
-          y[(long)line * nrel + r] = acc[j];
-        } else if (d.face < 0) {
-          const long o = dof(k_out, t) * nrel + r;
-          T2 v = y[o];
-          v.x += acc[j].x;
-          v.y += acc[j].y;
-          y[o] = v;
-        } else {
-          for (int a = 0; a < d.nl; ++a) {
-            const int k = d.face == 0 ? a : nalong - d.nl + a;
-            const T l = (T)d.lift[a];
-            const long o = dof(k, t) * nrel + r;
-            T2 v = y[o];
-            v.x += l * acc[j].x;
-            v.y += l * acc[j].y;
-            y[o] = v;
-          }
-        }
       }
     } else {
-      const int cc = it % ncol, mb = it / ncol;
-      for (int k = 0; k < n; ++k) {
-        const T2 w = sW[(size_t)k * nrx + cc];
+      const int c1 = it % cn, mb = it / cn;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          const int m = mb * 4 + j;
-          if (m < n) {
-            const T2 rv = sR[(size_t)m * (n + 1) + k];
+      for (int j = 0; j < 4; ++j) { mm[j] = mb * 4 + j; cc[j] = c1; }
+      for (int k = 0; k < n; ++k) {
+        const T2 w = sW[(size_t)k * d.chunk + c1];
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          if (mm[j] < n) {
+            const T2 rv = sR[(size_t)mm[j] * (n + 1) + k];
             acc[j].x += rv.x * w.x - rv.y * w.y;
             acc[j].y += rv.x * w.y + rv.y * w.x;
           }
-        }
       }
+    }
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int m = mb * 4 + j;
-        if (m >= n) continue;
-        const long r = (long)m * nrx + cc;
-        if (plain) {
-          y[(long)line * nrel + r] = acc[j];
-        } else if (d.face < 0) {
-          const long o = dof(k_out, t) * nrel + r;
+    for (int j = 0; j < 4; ++j) {
+      if (cc[j] >= cn || mm[j] >= n) continue;
+      const long r = rel(mm[j], c0 + cc[j]);
+      if (plain) {
+        y[(long)line * nrel + r] = acc[j];
+      } else if (d.face < 0) {
+        const long o = dof(k_out, t) * nrel + r;
+        T2 v = y[o];
+        v.x += acc[j].x;
+        v.y += acc[j].y;
+        y[o] = v;
+      } else {
+        for (int a = 0; a < d.nl; ++a) {
+          const int k = d.face == 0 ? a : nalong - d.nl + a;
+          const T l = (T)d.lift[a];
+          const long o = dof(k, t) * nrel + r;
           T2 v = y[o];
-          v.x += acc[j].x;
-          v.y += acc[j].y;
+          v.x += l * acc[j].x;
+          v.y += l * acc[j].y;
           y[o] = v;
-        } else {
-          for (int a = 0; a < d.nl; ++a) {
-            const int k = d.face == 0 ? a : nalong - d.nl + a;
-            const T l = (T)d.lift[a];
-            const long o = dof(k, t) * nrel + r;
-            T2 v = y[o];
-            v.x += l * acc[j].x;
-            v.y += l * acc[j].y;
-            y[o] = v;
-          }
         }
       }
     }

@@ -100,6 +100,18 @@ void f2_launch_apply(wdg_ctx* c, const typename F2Vec<T>::T2* u, typename F2Vec<
   }
 }
 
+// Cells across the axis per workgroup of a dense part: enough workgroups to
+// fill the chip, within the LDS budget.
+int f2_dense_chunk(int n, int ncol, int lines) {
+  const int want = std::max(1, (2048 + lines - 1) / lines);
+  int chunk = std::max(1, (ncol + want - 1) / want);
+  chunk = std::max(chunk, std::min(ncol, 4));
+  while (chunk > 1 && f2_dense_lds(n, chunk, sizeof(double2)) > kF2MaxLds) chunk = (chunk + 1) / 2;
+  if (f2_dense_lds(n, chunk, sizeof(double2)) > kF2MaxLds)
+    throw std::invalid_argument("n_rho too large for the dense relative factor in one workgroup's LDS");
+  return chunk;
+}
+
 template <typename T>
 void f2_launch_dense(wdg_ctx* c, const F2Data::Dense& dp, const typename F2Vec<T>::T2* u, typename F2Vec<T>::T2* y,
                      const GmresState* st) {
@@ -110,8 +122,10 @@ void f2_launch_dense(wdg_ctx* c, const F2Data::Dense& dp, const typename F2Vec<T
                                (int)kF2MaxLds));
     attr = true;
   }
-  const size_t lds = f2_dense_lds(dp.n, f.nrel, sizeof(typename F2Vec<T>::T2));
-  hipLaunchKernelGGL(f2_dense_kernel<T>, dim3(dp.lines), dim3(kF2Block), lds, c->stream, f.d_c.p, dp.k, u, y, st);
+  const int ncol = dp.k.axis == 0 ? f.d.n_rho_y : f.d.n_rho_x;
+  const size_t lds = f2_dense_lds(dp.n, dp.k.chunk, sizeof(typename F2Vec<T>::T2));
+  const dim3 grid(dp.lines, (ncol + dp.k.chunk - 1) / dp.k.chunk);
+  hipLaunchKernelGGL(f2_dense_kernel<T>, grid, dim3(kF2Block), lds, c->stream, f.d_c.p, dp.k, u, y, st);
 }
 
 // y = A_hat u
@@ -225,8 +239,8 @@ void f2_create_impl(wdg_ctx* c, const wdg2_desc& d, const wdg2_part* parts, int 
   if (N > (1L << 40)) throw std::invalid_argument("problem too large");
   // the dense kernel keeps one relative block and the n x n factor in LDS
   const int nmax = std::max(d.n_rho_x, d.n_rho_y);
-  if (f2_dense_lds(nmax, f->nrel, sizeof(double2)) > kF2MaxLds)
-    throw std::invalid_argument("relative grid too large: n_rho_x*n_rho_y + n_rho^2 exceeds one workgroup's LDS");
+  if (f2_dense_lds(nmax, 1, sizeof(double2)) > kF2MaxLds)
+    throw std::invalid_argument("n_rho too large: the dense n_rho x n_rho factor exceeds one workgroup's LDS");
   // operator scale: the largest |coef| * max|C| * max|R| over the parts
   double sA = 0;
   std::vector<const wdg2_part*> xc, xi, yc, yi;
@@ -280,6 +294,34 @@ void f2_create_impl(wdg_ctx* c, const wdg2_desc& d, const wdg2_part* parts, int 
   if (!yi.empty()) { f->hyi = f2_merge_banded(yi, d.nl_y, d.ne_y, sA); upd(f->d_byi, f->hyi); }
   // dense parts and the Kronecker diagonal / row-abs factors of every part
   const int nrx = d.n_rho_x, nry = d.n_rho_y, nrel = f->nrel, ncen = f->ncen, nxd = f->nxd;
+  // dense face parts with the same axis, face, lift and mask run as one launch
+  // with the sum of their (scaled) relative factors
+  std::vector<int> owner(np, -1);
+  std::vector<std::vector<double2>> mergedR(np);
+  for (int i = 0; i < np; ++i) {
+    const wdg2_part& p = parts[i];
+    if (p.rel != WDG2_REL_DENSE) continue;
+    const int nl = p.axis == 0 ? d.nl_x : d.nl_y, nr = p.axis == 0 ? nrx : nry;
+    const int ntr = p.axis == 0 ? f->nyd : f->nxd;
+    owner[i] = i;
+    if (p.center == WDG2_CENTER_FACE)
+      for (int j = 0; j < i; ++j) {
+        const wdg2_part& q = parts[j];
+        if (owner[j] != j || q.center != WDG2_CENTER_FACE || q.axis != p.axis || q.face != p.face) continue;
+        if (!std::equal(p.c_diag, p.c_diag + nl, q.c_diag)) continue;
+        if ((p.mask == nullptr) != (q.mask == nullptr)) continue;
+        if (p.mask && !std::equal(p.mask, p.mask + ntr, q.mask)) continue;
+        owner[i] = j;
+        break;
+      }
+    auto& R = mergedR[owner[i]];
+    R.resize((size_t)nr * nr, make_double2(0, 0));
+    const double cs = p.coef / sA;
+    for (size_t k = 0; k < R.size(); ++k) {
+      R[k].x += cs * p.r_dense[2 * k];
+      R[k].y += cs * p.r_dense[2 * k + 1];
+    }
+  }
   std::vector<double> dc, ac;
   std::vector<double2> dr, ar;
   for (int i = 0; i < np; ++i) {
@@ -288,16 +330,15 @@ void f2_create_impl(wdg_ctx* c, const wdg2_desc& d, const wdg2_part* parts, int 
     const int nl = ax == 0 ? d.nl_x : d.nl_y, ne = ax == 0 ? d.ne_x : d.ne_y;
     const int nr = ax == 0 ? nrx : nry;
     const int ntr = ax == 0 ? f->nyd : f->nxd;     // transverse centre DOFs
-    if (p.rel == WDG2_REL_DENSE) {
+    if (p.rel == WDG2_REL_DENSE && owner[i] == i) {
       auto dp = std::make_unique<F2Data::Dense>();
       dp->n = nr;
       dp->k.axis = ax;
       dp->k.face = p.center == WDG2_CENTER_FACE ? p.face : -1;
       dp->k.nl = nl;
       dp->k.ne = ne;
-      dp->k.coef = p.coef / sA;
-      std::vector<double2> R((size_t)nr * nr);
-      for (size_t k = 0; k < R.size(); ++k) R[k] = make_double2(p.r_dense[2 * k], p.r_dense[2 * k + 1]);
+      dp->k.coef = 1.0;
+      const std::vector<double2>& R = mergedR[i];
       dp->R.alloc(R.size());
       HIPCHK(hipMemcpyAsync(dp->R.p, R.data(), R.size() * sizeof(double2), hipMemcpyHostToDevice, c->stream));
       dp->k.R = dp->R.p;
@@ -314,6 +355,7 @@ void f2_create_impl(wdg_ctx* c, const wdg2_desc& d, const wdg2_part* parts, int 
         upd(dp->blocks, b);
         dp->k.blocks = dp->blocks.p;
         dp->lines = ncen;
+        dp->k.chunk = f2_dense_chunk(nr, ax == 0 ? nry : nrx, dp->lines);
       } else {
         upd(dp->lift, std::vector<double>(p.c_diag, p.c_diag + nl));
         dp->k.lift = dp->lift.p;
@@ -324,6 +366,7 @@ void f2_create_impl(wdg_ctx* c, const wdg2_desc& d, const wdg2_part* parts, int 
           dp->k.mask = dp->mask.p;
         }
         dp->lines = ntr;
+        dp->k.chunk = f2_dense_chunk(nr, ax == 0 ? nry : nrx, dp->lines);
       }
       HIPCHK(hipStreamSynchronize(c->stream));   // host vectors above go out of scope
       f->dense.push_back(std::move(dp));
@@ -386,8 +429,6 @@ void f2_create_impl(wdg_ctx* c, const wdg2_desc& d, const wdg2_part* parts, int 
     HIPCHK(hipMemcpyAsync(f->d_dr.p, dr.data(), dr.size() * sizeof(double2), hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(f->d_ar.p, ar.data(), ar.size() * sizeof(double2), hipMemcpyHostToDevice, c->stream));
   }
-  f->lds_dense64 = f2_dense_lds(nmax, nrel, sizeof(double2));
-  f->lds_dense32 = f2_dense_lds(nmax, nrel, sizeof(float2));
   // solve vectors
   c->N = N;
   c->ncols = ncen;
@@ -494,17 +535,22 @@ void f2_set_potential_impl(wdg_ctx* c, const wdg2_potential& p) {
         h.ipy[k] = (q < p.nq_y && a < d.nl_y) ? p.interp_y[q * d.nl_y + a] : 0.0;
         h.pry[k] = (q < p.nq_y && a < d.nl_y) ? p.proj_y[q * d.nl_y + a] : 0.0;
       }
-  // rho_y rows per apply tile: as many as fit 256 threads and the LDS budget
+  // rho_y rows per apply tile: as many as fit 256 threads and the LDS budget;
+  // the material grid joins the tile in LDS when it is small
   const int nl = d.nl_x * d.nl_y;
+  const int nvl = (size_t)nv * sizeof(double) <= 48 * 1024 ? (int)nv : 0;
+  h.v_lds = nvl > 0 ? 1 : 0;
+  auto lds = [&](int rows, size_t t2) {
+    return f2_lds_layout(nl, rows, nrx, t2, fv, h.nqx, h.nqy, npx, nvl).total;
+  };
   int rows = std::max(1, std::min(nry, kF2Block / nrx));
-  while (rows > 1 && f2_lds_bytes(nl, rows, nrx, sizeof(double2), fv) > 96 * 1024) --rows;
-  if (f2_lds_bytes(nl, rows, nrx, sizeof(double2), fv) > kF2MaxLds)
-    throw std::invalid_argument("n_rho_x too large for one apply tile");
+  while (rows > 1 && lds(rows, sizeof(double2)) > 96 * 1024) --rows;
+  if (lds(rows, sizeof(double2)) > kF2MaxLds) throw std::invalid_argument("n_rho_x too large for one apply tile");
   h.rows = rows;
   h.tiles = (nry + rows - 1) / rows;
   f.fv = fv;
-  f.lds_apply64 = f2_lds_bytes(nl, rows, nrx, sizeof(double2), fv);
-  f.lds_apply32 = f2_lds_bytes(nl, rows, nrx, sizeof(float2), fv);
+  f.lds_apply64 = lds(rows, sizeof(double2));
+  f.lds_apply32 = lds(rows, sizeof(float2));
   HIPCHK(hipMemcpyAsync(f.d_c.p, &h, sizeof(F2Const), hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   clear_graphs(c);
@@ -628,6 +674,7 @@ void f2_set_line_impl(wdg_ctx* c, const wdg2_line_factors& lf) {
     dp.n = nrx;
     dp.lines = f.ncen;
     dp.k = F2Dense{};
+    dp.k.chunk = f2_dense_chunk(nrx, f.d.n_rho_y, dp.lines);
     dp.k.axis = 0;
     dp.k.face = -2;
     dp.k.nl = nl;

@@ -113,3 +113,25 @@ def rtd_self_consistent(n_elem: int = 256, degree: int = 3, nk: int = 512, n_bia
     mat["Vdd"] = np.linspace(0.0, bias_max, n_bias)
     mat["fermi_offset"] = fermi_offset_ev
     return mat
+
+
+def fet_slab(length_nm: float = 40.0, width_nm: float = 20.0, n_el_x: int = 64, n_el_y: int = 32, degree: int = 1,
+             n_rho: int = 64, barrier_ev: float = 0.2, sigma_nm: float = 5.0, mass: float = 0.067,
+             temp: float = 300.0, flux: str = "rusanov"):
+    """Full-2D FET slab (the ``mat`` of solve_transport_DG_full2D,
+    DG_code/initParams_full2D.m:42-115): ``length_nm`` x ``width_nm`` with
+    n_el_x x n_el_y rectangular DG elements of ``degree`` (material grid
+    n_el*degree+1 points per axis, nm), n_rho x n_rho relative FV cells over
+    the default relative lengths (the Y extent), and a Gaussian gate barrier
+    V = barrier * exp(-((x-L/2)^2 + (y-W/2)^2)/(2 sigma^2)) [eV].
+    Returns (mat, V)."""
+    nx, ny = n_el_x * degree + 1, n_el_y * degree + 1
+    x = np.linspace(0.0, length_nm, nx)
+    y = np.linspace(0.0, width_nm, ny)
+    X, Y = np.meshgrid(x, y, indexing="ij")
+    V = barrier_ev * np.exp(-((X - length_nm / 2) ** 2 + (Y - width_nm / 2) ** 2) / (2 * sigma_nm ** 2))
+    mat = {"type": "FET-slab", "Nx": nx, "Ny": ny, "x": x, "y": y, "dx": x[1] - x[0], "dy": y[1] - y[0],
+           "Temp": temp, "deg_factor": 1, "me_x_ch": mass, "me_y_ch": mass, "V": V,
+           "dg": {"params": {"N_K_X": degree + 1, "N_K_Y": degree + 1, "N_rho_x": n_rho, "N_rho_y": n_rho,
+                             "full2D_fluxType": flux}}}
+    return mat, V
