@@ -224,6 +224,37 @@ typedef struct wdg_transient_opts {
 int wdg_transient_rk4(wdg_ctx* ctx, const wdg_transient_opts* opts, const double* rho0, double* rho_out,
                       const double* m_nodes, double* n_t, double* j_t, int32_t max_saves);
 
+/* Crank-Nicolson (trapezoidal) time stepping of the same ODE,
+ * (I + dt/2 a_scale A) rho+ = (I - dt/2 a_scale A) rho + dt b_scale rhs,
+ * with one device GMRES solve per step warm-started from rho (tolerance tol
+ * relative to the step's right-hand side, precision / krylov_basis as in
+ * wdg_solve_opts).  No counterpart in the reference, whose full-2D transient
+ * entry points are stubs (DG_code/solve_transport_DG_full2D_transient_flatband.m:8-11):
+ * the implicit step that lets the time step follow the physics instead of
+ * the operator's stiffness.  Starts from rho0 (host) or, when NULL, from the
+ * context's last solution; the final state becomes that solution (read it
+ * with wdg_observables / wdg2_observables, rho = NULL).  Works on 1D and
+ * full-2D contexts; full-2D Jacobi / row-sum preconditioners are rebuilt for
+ * I + dt/2 a_scale A (use_precond), the 1D line and the line-transverse
+ * preconditioners are not used for the shifted system. */
+typedef struct wdg_cn_opts {
+  double  dt;            /* time step [s]                                   */
+  int32_t nsteps;
+  double  a_scale, b_scale;
+  double  tol;           /* per-step GMRES tolerance                        */
+  int32_t restart, max_iter, precision, krylov_basis, use_precond;
+} wdg_cn_opts;
+typedef struct wdg_cn_info {
+  int32_t steps;               /* steps taken                               */
+  int32_t iterations;          /* GMRES iterations, all steps               */
+  int32_t max_step_iterations;
+  int32_t flag;                /* 0: every step converged                   */
+  double  max_relres;          /* largest per-step true relative residual   */
+  double  seconds;             /* device time of the run                    */
+} wdg_cn_info;
+int wdg_transient_cn(wdg_ctx* ctx, const wdg_cn_opts* opts, const double* rho0, double* rho_out,
+                     wdg_cn_info* info);
+
 /* Formulation of the Theta[V] apply.  WDG_THETA_FFT evaluates phi U and
  * i phi^H Z as chirp-z transforms (FFTs of length 2^p 3^q >= nk+n_xi-1) in
  * one fused kernel per element; WDG_THETA_GEMM runs them as dense batched

@@ -12,9 +12,16 @@ with A prescaled by a_scale (2e-10 in the reference, :148) and rhs by
 b_scale (1 there).  Observables after each saved step as
 solve_transport_DG_2D_transient_SC_new.m:172-178 (n_xi even branch).
 
+Also the Crank-Nicolson (trapezoidal) step of the same ODE,
+(I + dt/2 a A) rho+ = (I - dt/2 a A) rho + dt b rhs            -> crank_nicolson
+which has no counterpart in the reference (its full-2D transient entry points
+are stubs, DG_code/solve_transport_DG_full2D_transient_flatband.m:8-11).
+
 Pinning (tests/test_oracle_transient.py): against the exact solution of the
 linear ODE, rho(t) = rho_inf + expm(-a A t)(rho0 - rho_inf), with the global
-error falling as dt^4; the stationary state is a fixed point.
+error falling as dt^4 (RK4) and dt^2 (Crank-Nicolson); the stationary state
+is a fixed point of both; on a diagonal A each Crank-Nicolson step multiplies
+a mode by (1 - z/2)/(1 + z/2), z = dt a lambda.
 """
 from __future__ import annotations
 
@@ -43,3 +50,21 @@ def rk4(A, rhs, rho0, dt, nsteps, a_scale=2e-10, b_scale=1.0, save_every=1, p=No
     if p is None:
         return rho, None, None
     return rho, np.array(n_t).T, np.array(j_t).T
+
+
+def crank_nicolson(A, rhs, rho0, dt, nsteps, a_scale=1.0, b_scale=1.0):
+    """nsteps trapezoidal steps of d rho/dt = b_scale rhs - a_scale A rho,
+    each solved directly (sparse LU of I + dt/2 a A, factored once)."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    n = A.shape[0]
+    As = sp.csc_matrix(A) * a_scale
+    I = sp.identity(n, format="csc")
+    lu = spla.splu((I + 0.5 * dt * As).tocsc())
+    minus = (I - 0.5 * dt * As).tocsr()
+    g = dt * b_scale * np.asarray(rhs, complex)
+    rho = np.asarray(rho0, complex).copy()
+    for _ in range(nsteps):
+        rho = lu.solve(minus @ rho + g)
+    return rho
+

@@ -91,6 +91,17 @@ class wdg_transient_opts(C.Structure):
                 ("a_scale", C.c_double), ("b_scale", C.c_double)]
 
 
+class wdg_cn_opts(C.Structure):
+    _fields_ = [("dt", C.c_double), ("nsteps", C.c_int32), ("a_scale", C.c_double), ("b_scale", C.c_double),
+                ("tol", C.c_double), ("restart", C.c_int32), ("max_iter", C.c_int32), ("precision", C.c_int32),
+                ("krylov_basis", C.c_int32), ("use_precond", C.c_int32)]
+
+
+class wdg_cn_info(C.Structure):
+    _fields_ = [("steps", C.c_int32), ("iterations", C.c_int32), ("max_step_iterations", C.c_int32),
+                ("flag", C.c_int32), ("max_relres", C.c_double), ("seconds", C.c_double)]
+
+
 class wdg_poisson_desc(C.Structure):
     _fields_ = [("n_mat", C.c_int32), ("method", C.c_int32), ("x_mat", C.POINTER(C.c_double)),
                 ("eps_r", C.POINTER(C.c_double)), ("net_doping", C.POINTER(C.c_double)), ("temp", C.c_double)]
@@ -149,6 +160,7 @@ SIGNATURES = {
     "wdg_precond_device": (C.c_int, [_P, _P, _P, C.c_int32]),
     "wdg_host_setup": (C.c_int, [C.POINTER(wdg_desc), C.POINTER(wdg_potential), _D, _D]),
     "wdg_transient_rk4": (C.c_int, [_P, C.POINTER(wdg_transient_opts), _D, _D, _D, _D, _D, C.c_int32]),
+    "wdg_transient_cn": (C.c_int, [_P, C.POINTER(wdg_cn_opts), _D, _D, C.POINTER(wdg_cn_info)]),
     "wdg_set_theta_method": (C.c_int, [_P, C.c_int32]),
     "wdg_theta_method": (C.c_int, [_P]),
     "wdg2_create": (C.c_int, [C.POINTER(wdg2_desc), C.POINTER(wdg2_part), C.c_int32, C.POINTER(_P)]),

@@ -41,6 +41,10 @@ struct F2Const {
   int nqx, nqy, nxg, nyg;       // quadrature points, material grid size
   int npx, npy;                 // relative sample points per axis (vertices or cells)
   int v_lds;                    // 1: the material grid is staged in LDS
+  // stored potential differences DeltaV(element, q = qx*nqy + qy, vertex), vertex
+  // x fastest (storedPotentialDifference, get_Drift_full2D.m:609-618); null: on the fly
+  const double* dv64;
+  const float* dv32;
   double inv2dx, inv2dy;        // 1/(2 delta) of the central difference
   double pot_coef, cap_coef;    // drift coefficients / operator scale (pot multiplies i*DeltaV)
   // banded centre parts (coefficient and 1/s_A folded in); null when absent
@@ -115,12 +119,15 @@ struct F2Lds {
   size_t sz, xi, xt, yi, yt, v, total;
 };
 __host__ __device__ inline size_t f2_al(size_t b) { return (b + 15) & ~(size_t)15; }
-__host__ __device__ inline F2Lds f2_lds_layout(int nl, int rows, int nrx, size_t t2, bool fv, int nqx, int nqy, int npx, int nv) {
+__host__ __device__ inline F2Lds f2_lds_layout(int nl, int rows, int nrx, size_t t2, bool fv, bool tables, int nqx,
+                                                int nqy, int npx, int nv) {
   F2Lds L{};
   size_t o = f2_al((size_t)nl * (rows + 2) * nrx * t2);
   if (fv) {
     L.sz = o;
     o = f2_al(o + (size_t)nl * (rows + 1) * (nrx + 1) * t2);
+  }
+  if (tables) {
     L.xi = o;
     o = f2_al(o + (size_t)nqx * npx * 2 * sizeof(int));
     L.xt = o;
@@ -135,9 +142,7 @@ __host__ __device__ inline F2Lds f2_lds_layout(int nl, int rows, int nrx, size_t
   L.total = o;
   return L;
 }
-inline size_t f2_lds_bytes(int nl, int rows, int nrx, size_t t2, bool fv) {
-  return f2_lds_layout(nl, rows, nrx, t2, fv, 0, 0, 0, 0).total;
-}
+
 
 template <int NLX, int NLY, typename T, int MODE>
 __global__ void __launch_bounds__(kF2Block) f2_apply_kernel(const F2Const* __restrict__ cp,
@@ -156,7 +161,9 @@ __global__ void __launch_bounds__(kF2Block) f2_apply_kernel(const F2Const* __res
   const int R2 = c.rows + 2;
   const bool fv = c.pot_mode == 0;
   extern __shared__ __align__(16) unsigned char f2_smem[];
-  const F2Lds L = f2_lds_layout(NL, c.rows, nrx, sizeof(T2), fv, c.nqx, c.nqy, c.npx, c.v_lds ? c.nxg * c.nyg : 0);
+  const bool tab = std::is_same<T, double>::value ? c.dv64 != nullptr : c.dv32 != nullptr;
+  const bool vl = c.v_lds != 0 && !tab;
+  const F2Lds L = f2_lds_layout(NL, c.rows, nrx, sizeof(T2), fv, fv && !tab, c.nqx, c.nqy, c.npx, vl ? c.nxg * c.nyg : 0);
   T2* su = reinterpret_cast<T2*>(f2_smem);
   T2* sz = reinterpret_cast<T2*>(f2_smem + L.sz);
   int* sxi = reinterpret_cast<int*>(f2_smem + L.xi);
@@ -164,11 +171,10 @@ __global__ void __launch_bounds__(kF2Block) f2_apply_kernel(const F2Const* __res
   int* syi = reinterpret_cast<int*>(f2_smem + L.yi);
   double* syt = reinterpret_cast<double*>(f2_smem + L.yt);
   double* sv = reinterpret_cast<double*>(f2_smem + L.v);
-  const bool vl = c.v_lds != 0;
   auto cdof = [&](int ax, int ay) { return (long)(ex * NLX + ax) + (long)c.nxd * (ey * NLY + ay); };
   if (vl)
     for (int i = threadIdx.x; i < c.nxg * c.nyg; i += blockDim.x) sv[i] = c.v[i];
-  if (fv) {
+  if (fv && !tab) {
     const int nxt = c.nqx * c.npx * 2;
     const size_t xo = (size_t)ex * nxt;
     for (int i = threadIdx.x; i < nxt; i += blockDim.x) {
@@ -185,14 +191,21 @@ __global__ void __launch_bounds__(kF2Block) f2_apply_kernel(const F2Const* __res
   }
   if (MODE != F2_APPLY) __syncthreads();
 
+  // thread -> (relative x cell tx, tile row ty); rows*nrx <= blockDim
+  const int tx = threadIdx.x % nrx, ty = threadIdx.x / nrx;
+  const bool active = ty < c.rows;
   if (MODE == F2_APPLY) {
-    for (int idx = threadIdx.x; idx < NL * R2 * nrx; idx += blockDim.x) {
-      const int a = idx / (R2 * nrx);
-      const int rem = idx - a * R2 * nrx;
-      const int rr = rem / nrx, rx = rem - rr * nrx;
-      const int ry = ry0 - 1 + rr;
-      const bool ok = ry >= 0 && ry < nry && rr <= nrows + 1;
-      su[idx] = ok ? u[cdof(a % NLX, a / NLX) * nrel + (long)ry * nrx + rx] : f2_zero<T2>();
+    if (active) {
+#pragma unroll
+      for (int a = 0; a < NL; ++a) {
+        const T2* src = u + cdof(a % NLX, a / NLX) * nrel + tx;
+        T2* dst = su + (size_t)a * R2 * nrx + tx;
+        for (int rr = ty; rr < R2; rr += c.rows) {
+          const int ry = ry0 - 1 + rr;
+          const bool ok = ry >= 0 && ry < nry && rr <= nrows + 1;
+          dst[(size_t)rr * nrx] = ok ? src[(long)ry * nrx] : f2_zero<T2>();
+        }
+      }
     }
     __syncthreads();
   }
@@ -227,10 +240,25 @@ __global__ void __launch_bounds__(kF2Block) f2_apply_kernel(const F2Const* __res
       }
 #pragma unroll
       for (int a = 0; a < NL; ++a) Z[a] = f2_zero<T2>();
+      // stored DeltaV: one quadrature row (nqy values) in flight ahead of its use
+      const size_t qs = (size_t)(nry + 1) * (nrx + 1);
+      const size_t tb = (size_t)el * c.nqx * c.nqy * qs + (size_t)vy * (nrx + 1) + vx;
+      T dcur[kF2MaxNq], dnext[kF2MaxNq];
+      if (tab) {
+#pragma unroll
+        for (int qy = 0; qy < kF2MaxNq; ++qy)
+          if (qy < c.nqy) dcur[qy] = std::is_same<T, double>::value ? (T)c.dv64[tb + qy * qs] : (T)c.dv32[tb + qy * qs];
+      }
       for (int qx = 0; qx < c.nqx; ++qx) {
         const int kx = 2 * (qx * c.npx + vx);
-        const int ixp = sxi[kx], ixm = sxi[kx + 1];
-        const double txp = sxt[kx], txm = sxt[kx + 1];
+        const int ixp = tab ? 0 : sxi[kx], ixm = tab ? 0 : sxi[kx + 1];
+        const double txp = tab ? 0.0 : sxt[kx], txm = tab ? 0.0 : sxt[kx + 1];
+        if (tab && qx + 1 < c.nqx) {
+          const size_t o = tb + (size_t)(qx + 1) * c.nqy * qs;
+#pragma unroll
+          for (int qy = 0; qy < kF2MaxNq; ++qy)
+            if (qy < c.nqy) dnext[qy] = std::is_same<T, double>::value ? (T)c.dv64[o + qy * qs] : (T)c.dv32[o + qy * qs];
+        }
         T2 t1[NLY];
         if (MODE == F2_APPLY) {
 #pragma unroll
@@ -244,11 +272,18 @@ __global__ void __launch_bounds__(kF2Block) f2_apply_kernel(const F2Const* __res
         T2 acc[NLY];
 #pragma unroll
         for (int ay = 0; ay < NLY; ++ay) acc[ay] = f2_zero<T2>();
-        for (int qy = 0; qy < c.nqy; ++qy) {
-          const int ky = 2 * (qy * (c.rows + 1) + vyl);
-          const double vp = f2_bilin_s(sv, c.v, vl, c.nxg, ixp, txp, syi[ky], syt[ky]);
-          const double vm = f2_bilin_s(sv, c.v, vl, c.nxg, ixm, txm, syi[ky + 1], syt[ky + 1]);
-          const T dv = (T)(vp - vm);
+#pragma unroll
+        for (int qy = 0; qy < kF2MaxNq; ++qy) {
+          if (qy >= c.nqy) break;
+          T dv;
+          if (tab) {
+            dv = dcur[qy];
+          } else {
+            const int ky = 2 * (qy * (c.rows + 1) + vyl);
+            const double vp = f2_bilin_s(sv, c.v, vl, c.nxg, ixp, txp, syi[ky], syt[ky]);
+            const double vm = f2_bilin_s(sv, c.v, vl, c.nxg, ixm, txm, syi[ky + 1], syt[ky + 1]);
+            dv = (T)(vp - vm);
+          }
           if (MODE == F2_APPLY) {
             T2 wq = f2_zero<T2>();
 #pragma unroll
@@ -287,6 +322,10 @@ __global__ void __launch_bounds__(kF2Block) f2_apply_kernel(const F2Const* __res
         for (int ay = 0; ay < NLY; ++ay)
 #pragma unroll
           for (int ax = 0; ax < NLX; ++ax) f2_fma(Z[ax + NLX * ay], wx[ax], acc[ay]);
+        if (tab) {
+#pragma unroll
+          for (int qy = 0; qy < kF2MaxNq; ++qy) dcur[qy] = dnext[qy];
+        }
       }
       if (MODE == F2_ROWABS) {
 #pragma unroll
@@ -299,10 +338,9 @@ __global__ void __launch_bounds__(kF2Block) f2_apply_kernel(const F2Const* __res
   }
 
   // ---- cell phase ----
-  const int ncell = nrows * nrx;
   const T pc = (T)c.pot_coef, cc = (T)c.cap_coef;
-  for (int t = threadIdx.x; t < ncell; t += blockDim.x) {
-    const int ryl = t / nrx, rx = t - ryl * nrx;
+  if (active && ty < nrows) {
+    const int ryl = ty, rx = tx;
     const int ry = ry0 + ryl;
     const long r = (long)ry * nrx + rx;
     const T cap = (T)(c.capx[rx] + c.capy[ry]);
@@ -721,6 +759,28 @@ __global__ void f2_precond_build_kernel(const double2* __restrict__ d, long n, i
     }
     p64[i] = o;
     p32[i] = make_float2((float)o.x, (float)o.y);
+  }
+}
+
+// DeltaV(element, q, vertex) for every element of the FV-consistent potential,
+// once per potential (FP64 and FP32 copies).
+__global__ void f2_dv_table_kernel(const F2Const* __restrict__ cp, double* __restrict__ t64, float* __restrict__ t32) {
+  const F2Const& c = *cp;
+  const long nv = (long)(c.nrx + 1) * (c.nry + 1);
+  const long nq = (long)c.nqx * c.nqy;
+  const long total = (long)c.nex * c.ney * nq * nv;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
+    const long v = i % nv, rest = i / nv;
+    const int q = (int)(rest % nq);
+    const long el = rest / nq;
+    const int vx = (int)(v % (c.nrx + 1)), vy = (int)(v / (c.nrx + 1));
+    const int qx = q / c.nqy, qy = q - qx * c.nqy;
+    const int ex = (int)(el % c.nex), ey = (int)(el / c.nex);
+    const int kx = (ex * c.nqx + qx) * c.npx + vx;
+    const int ky = (ey * c.nqy + qy) * c.npy + vy;
+    const double dv = f2_delta_v(c, kx, ky);
+    t64[i] = dv;
+    t32[i] = (float)dv;
   }
 }
 

@@ -32,6 +32,8 @@ struct F2Data {
   DevBuf<F2Const> d_c;
   DevBuf<int> d_tix, d_tiy;
   DevBuf<double> d_ttx, d_tty, d_v, d_capx, d_capy;
+  DevBuf<double> d_dv64;                // stored potential differences (FV-consistent)
+  DevBuf<float> d_dv32;
   bool fv = true;
   size_t lds_apply64 = 0, lds_apply32 = 0, lds_dense64 = 0, lds_dense32 = 0;
   // preconditioner
@@ -489,6 +491,7 @@ void f2_set_potential_impl(wdg_ctx* c, const wdg2_potential& p) {
     if (!std::isfinite(p.v[i])) throw std::invalid_argument("potential must be finite");
   if (!std::isfinite(p.pot_coef) || !std::isfinite(p.cap_coef)) throw std::invalid_argument("coefficients must be finite");
   const int nrx = d.n_rho_x, nry = d.n_rho_y;
+  if (nrx > kF2Block) throw std::invalid_argument("n_rho_x above 256 is not supported by the apply tiles");
   const int ncx = fv ? d.ne_x * p.nq_x : f.nxd, ncy = fv ? d.ne_y * p.nq_y : f.nyd;
   const int npx = fv ? nrx + 1 : nrx, npy = fv ? nry + 1 : nry;
   std::vector<int> tix, tiy;
@@ -535,13 +538,27 @@ void f2_set_potential_impl(wdg_ctx* c, const wdg2_potential& p) {
         h.ipy[k] = (q < p.nq_y && a < d.nl_y) ? p.interp_y[q * d.nl_y + a] : 0.0;
         h.pry[k] = (q < p.nq_y && a < d.nl_y) ? p.proj_y[q * d.nl_y + a] : 0.0;
       }
+  // Store DeltaV per (element, quadrature point, relative vertex) when it fits
+  // a quarter of free device memory: the apply then streams it instead of
+  // interpolating the material grid twice per sample.
+  bool tab = false;
+  if (fv) {
+    const size_t cnt = (size_t)d.ne_x * d.ne_y * p.nq_x * p.nq_y * (nrx + 1) * (nry + 1);
+    size_t free_b = 0, total_b = 0;
+    HIPCHK(hipMemGetInfo(&free_b, &total_b));
+    if (cnt * 12 <= free_b / 4 + f.d_dv64.n * 8 + f.d_dv32.n * 4) {
+      f.d_dv64.alloc(cnt);
+      f.d_dv32.alloc(cnt);
+      tab = true;
+    }
+  }
   // rho_y rows per apply tile: as many as fit 256 threads and the LDS budget;
   // the material grid joins the tile in LDS when it is small
   const int nl = d.nl_x * d.nl_y;
   const int nvl = (size_t)nv * sizeof(double) <= 48 * 1024 ? (int)nv : 0;
   h.v_lds = nvl > 0 ? 1 : 0;
   auto lds = [&](int rows, size_t t2) {
-    return f2_lds_layout(nl, rows, nrx, t2, fv, h.nqx, h.nqy, npx, nvl).total;
+    return f2_lds_layout(nl, rows, nrx, t2, fv, fv && !tab, h.nqx, h.nqy, npx, tab ? 0 : nvl).total;
   };
   int rows = std::max(1, std::min(nry, kF2Block / nrx));
   while (rows > 1 && lds(rows, sizeof(double2)) > 96 * 1024) --rows;
@@ -551,7 +568,18 @@ void f2_set_potential_impl(wdg_ctx* c, const wdg2_potential& p) {
   f.fv = fv;
   f.lds_apply64 = lds(rows, sizeof(double2));
   f.lds_apply32 = lds(rows, sizeof(float2));
+  h.dv64 = nullptr;
+  h.dv32 = nullptr;
   HIPCHK(hipMemcpyAsync(f.d_c.p, &h, sizeof(F2Const), hipMemcpyHostToDevice, c->stream));
+  if (tab) {
+    const size_t cnt = f.d_dv64.n;
+    const unsigned grid = (unsigned)std::min<size_t>(65536, (cnt + 255) / 256);
+    hipLaunchKernelGGL(f2_dv_table_kernel, dim3(grid), dim3(256), 0, c->stream, f.d_c.p, f.d_dv64.p, f.d_dv32.p);
+    HIPCHK(hipGetLastError());
+    h.dv64 = f.d_dv64.p;
+    h.dv32 = f.d_dv32.p;
+    HIPCHK(hipMemcpyAsync(f.d_c.p, &h, sizeof(F2Const), hipMemcpyHostToDevice, c->stream));
+  }
   HIPCHK(hipStreamSynchronize(c->stream));
   clear_graphs(c);
   // the preconditioner depends on the potential: rebuild on the next request
@@ -585,6 +613,9 @@ void f2_build_precond(wdg_ctx* c, int kind, double rel_floor, bool fallback, int
   f.d_p32.alloc(c->N);
   for (int attempt = 0; attempt < 2; ++attempt) {
     f2_diag_into(c, kind == WDG2_PRECOND_JACOBI ? F2_DIAG : F2_ROWABS, c->d_y.p);
+    if (c->shift_on)   // of alpha I + beta A_hat
+      hipLaunchKernelGGL(shift_diag_kernel, dim3(c->red_blocks), dim3(256), 0, c->stream, c->d_y.p, c->N, c->shift_a,
+                         c->shift_b, kind == WDG2_PRECOND_ROWABS ? 1 : 0);
     const int nb = c->red_blocks;
     hipLaunchKernelGGL(f2_absmax_kernel, dim3(nb), dim3(256), 0, c->stream, c->d_y.p, c->N, (double*)c->d_part.p);
     std::vector<double> part(nb);

@@ -1424,6 +1424,55 @@ __global__ void scale_complex_kernel(double2* __restrict__ x, double s, long N) 
 }
 
 // ---------------------------------------------------------------------------
+// Shifted operator alpha I + beta A (Crank-Nicolson steps): y = alpha u + beta y.
+template <typename T>
+__global__ void shift_kernel(const typename cvec<T>::type* __restrict__ u, typename cvec<T>::type* __restrict__ y,
+                             double alpha, double beta, long N, const GmresState* st) {
+  if (skip(st)) return;
+  const T a = (T)alpha, b = (T)beta;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) {
+    const auto v = u[i];
+    auto w = y[i];
+    w.x = a * v.x + b * w.x;
+    w.y = a * v.y + b * w.y;
+    y[i] = w;
+  }
+}
+
+// Crank-Nicolson right-hand side c = 2 x - (I + beta A) x + gamma b, with
+// per-block partial sums of |c|^2.
+template <int BS>
+__global__ void cn_rhs_kernel(const double2* __restrict__ x, const double2* __restrict__ ax,
+                              const double2* __restrict__ b, double gamma, double2* __restrict__ c, long N,
+                              double* __restrict__ part) {
+  __shared__ double sm[BS];
+  double s = 0;
+  for (long i = (long)blockIdx.x * BS + threadIdx.x; i < N; i += (long)gridDim.x * BS) {
+    const double2 xv = x[i], av = ax[i], bv = b[i];
+    const double2 cv = make_double2(2 * xv.x - av.x + gamma * bv.x, 2 * xv.y - av.y + gamma * bv.y);
+    c[i] = cv;
+    s += cv.x * cv.x + cv.y * cv.y;
+  }
+  sm[threadIdx.x] = s;
+  __syncthreads();
+  for (int k = BS / 2; k > 0; k >>= 1) {
+    if (threadIdx.x < k) sm[threadIdx.x] += sm[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sm[0];
+}
+
+// Diagonal / row-abs bound of alpha I + beta A from those of A.
+__global__ void shift_diag_kernel(double2* __restrict__ d, long N, double alpha, double beta, int rowabs) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) {
+    double2 v = d[i];
+    if (rowabs) v = make_double2(fabs(alpha) + fabs(beta) * v.x, 0.0);
+    else v = make_double2(alpha + beta * v.x, beta * v.y);
+    d[i] = v;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Explicit RK4 of d rho/dt = c_b rhs - c_A A rho (DG_code/core/transient.m:12-16,
 // DG_code/solve_transport_DG_2D_transient_SC_new.m:145-158) in device units:
 // with rho = sigma x, sigma = ||b||/s_A, the ODE is dx/dt' = c_b b^ - c_A A^ x

@@ -84,6 +84,10 @@ struct F2Data;   // full-2D operator data (full2d_host.hpp)
 
 struct wdg_ctx {
   std::shared_ptr<F2Data> f2;         // set for full-2D contexts
+  // shifted operator alpha I + beta A_hat (Crank-Nicolson steps)
+  bool shift_on = false;
+  double shift_a = 1.0, shift_b = 0.0;
+  DevBuf<double2> d_b0;               // the normalised inflow RHS while b holds a step RHS
   Geometry g;
   std::string err;
   hipStream_t stream = nullptr;
@@ -428,9 +432,20 @@ void theta_fft_attrs() {
                              hipFuncAttributeMaxDynamicSharedMemorySize, kMaxLdsBytes));
 }
 
-// y = A_hat u in precision T.
+template <typename T>
+void op_apply_base(wdg_ctx* c, const typename cvec<T>::type* u, typename cvec<T>::type* y, const GmresState* st);
+
+// y = A_hat u in precision T, or (alpha I + beta A_hat) u while a shift is set.
 template <typename T>
 void op_apply(wdg_ctx* c, const typename cvec<T>::type* u, typename cvec<T>::type* y, const GmresState* st) {
+  op_apply_base<T>(c, u, y, st);
+  if (c->shift_on)
+    hipLaunchKernelGGL(shift_kernel<T>, dim3(c->red_blocks), dim3(256), 0, c->stream, u, y, c->shift_a, c->shift_b,
+                       c->N, st);
+}
+
+template <typename T>
+void op_apply_base(wdg_ctx* c, const typename cvec<T>::type* u, typename cvec<T>::type* y, const GmresState* st) {
   if (c->f2) {
     f2_apply<T>(c, u, y, st);
     return;
@@ -645,8 +660,8 @@ void run_cycle(wdg_ctx* c, bool fp64, int basis, int m, bool precond, bool singl
     enqueue();
     return;
   }
-  const long key = ((long)m << 5) | (single_pass ? 16 : 0) | ((long)(fp64 ? 0 : basis) << 2) | (fp64 ? 2 : 0) |
-                   (precond ? 1 : 0);
+  const long key = ((long)m << 6) | (c->shift_on ? 32 : 0) | (single_pass ? 16 : 0) | ((long)(fp64 ? 0 : basis) << 2) |
+                   (fp64 ? 2 : 0) | (precond ? 1 : 0);
   auto it = c->graphs.find(key);
   if (it == c->graphs.end()) {
     HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
@@ -1646,6 +1661,133 @@ int wdg_transient_rk4(wdg_ctx* ctx, const wdg_transient_opts* o, const double* r
       }
     return WDG_OK;
   });
+}
+
+int wdg_transient_cn(wdg_ctx* ctx, const wdg_cn_opts* o, const double* rho0, double* rho_out, wdg_cn_info* info) {
+  if (!ctx || !o || !info) return fail(ctx, WDG_ERR_ARG, "null argument");
+  if (!ctx->have_potential) return fail(ctx, WDG_ERR_STATE, "the operator has no potential yet");
+  std::memset(info, 0, sizeof(*info));
+  auto restore = [&]() {
+    if (ctx->shift_on) {
+      ctx->shift_on = false;
+      clear_graphs(ctx);
+      if (ctx->f2 && (ctx->f2->precond == WDG2_PRECOND_JACOBI || ctx->f2->precond == WDG2_PRECOND_ROWABS))
+        ctx->f2->precond = -ctx->f2->precond;   // rebuilt for the unshifted operator on the next solve
+    }
+  };
+  const int rc = guarded(ctx, [&]() {
+    if (!(o->dt > 0) || o->nsteps < 0 || !std::isfinite(o->a_scale) || !std::isfinite(o->b_scale) || !(o->tol > 0))
+      throw std::invalid_argument("dt and tol must be positive, nsteps >= 0, finite scales");
+    const long N = ctx->N;
+    const double sA = ctx->g.scale;
+    // device units: rho = sigma x with sigma = ||b|| / s_A; b_hat = b / ||b||
+    const double sigma = (ctx->rhs_norm > 0) ? ctx->rhs_norm / sA : 1.0 / sA;
+    if (rho0) {
+      ctx->stage.resize(2 * N);
+      for (long i = 0; i < 2 * N; ++i) {
+        if (!std::isfinite(rho0[i])) throw std::invalid_argument("rho0 must contain only finite values");
+        ctx->stage[i] = rho0[i] / sigma;
+      }
+      HIPCHK(hipMemcpy(ctx->d_x.p, ctx->stage.data(), 2 * N * 8, hipMemcpyHostToDevice));
+    } else if (!ctx->have_solution) {
+      throw std::logic_error("no initial state: pass rho0 or solve first");
+    } else if (ctx->sol_scale != sigma) {
+      hipLaunchKernelGGL(scale_complex_kernel, dim3(ctx->red_blocks), dim3(256), 0, ctx->stream, ctx->d_x.p,
+                         ctx->sol_scale / sigma, N);
+    }
+    ctx->sol_scale = sigma;
+    ctx->have_solution = true;
+    ctx->d_b0.alloc(N);
+    HIPCHK(hipMemcpyAsync(ctx->d_b0.p, ctx->d_b.p, N * 16, hipMemcpyDeviceToDevice, ctx->stream));
+    // (I + beta A_hat) x+ = (I - beta A_hat) x + gamma b_hat: the trapezoidal
+    // rule for d rho/dt = b_scale rhs - a_scale A rho in device units
+    const double beta = 0.5 * o->dt * o->a_scale * sA;
+    const double gamma = (ctx->rhs_norm > 0) ? o->dt * o->b_scale * sA : 0.0;
+    ctx->shift_on = true;
+    ctx->shift_a = 1.0;
+    ctx->shift_b = beta;
+    clear_graphs(ctx);
+    wdg_solve_opts so{};
+    so.tol = o->tol;
+    so.restart = o->restart > 0 ? o->restart : 30;
+    so.max_iter = o->max_iter > 0 ? o->max_iter : 20000;
+    so.precision = o->precision;
+    so.inner_tol = 1e-4;
+    so.use_graph = 1;
+    so.orthogonalization = 1;
+    so.warm_device = 1;
+    so.krylov_basis = o->krylov_basis;
+    // the 1D line preconditioner inverts A_hat's sweep, not the shifted operator's
+    so.use_precond = (ctx->f2 && o->use_precond) ? 1 : 0;
+    if (ctx->f2 && (ctx->f2->precond == WDG2_PRECOND_JACOBI || ctx->f2->precond == WDG2_PRECOND_ROWABS)) {
+      int used = 0;
+      f2_build_precond(ctx, ctx->f2->precond, 1e-12, true, &used);
+    } else if (ctx->f2 && ctx->f2->precond == WDG2_PRECOND_LINE) {
+      so.use_precond = 0;   // the line-transverse factors approximate A, not I + beta A
+    }
+    const int nb = ctx->red_blocks;
+    std::vector<double> part(nb);
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventRecord(e0, ctx->stream));
+    double max_rel = 0;
+    int flag = 0;
+    for (int step = 0; step < o->nsteps; ++step) {
+      op_apply<double>(ctx, ctx->d_x.p, ctx->d_y.p, nullptr);
+      hipLaunchKernelGGL((cn_rhs_kernel<256>), dim3(nb), dim3(256), 0, ctx->stream, ctx->d_x.p, ctx->d_y.p,
+                         ctx->d_b0.p, gamma, ctx->d_b.p, N, (double*)ctx->d_part.p);
+      HIPCHK(hipMemcpyAsync(part.data(), ctx->d_part.p, nb * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+      HIPCHK(hipStreamSynchronize(ctx->stream));
+      double s2 = 0;
+      for (double v : part) s2 += v;
+      const double cn = std::sqrt(s2);
+      if (!(cn > 0) || !std::isfinite(cn)) throw std::runtime_error("Crank-Nicolson right-hand side vanished or is not finite");
+      hipLaunchKernelGGL(scale_complex_kernel, dim3(nb), dim3(256), 0, ctx->stream, ctx->d_b.p, 1.0 / cn, N);
+      hipLaunchKernelGGL(scale_complex_kernel, dim3(nb), dim3(256), 0, ctx->stream, ctx->d_x.p, 1.0 / cn, N);
+      wdg_solve_info si;
+      {
+        SolveRun r(ctx, so, &si);
+        r.begin(nullptr);
+        r.sync_poll();
+        while (r.active) {
+          r.step();
+          r.sync_poll();
+        }
+        r.end(nullptr);
+      }
+      hipLaunchKernelGGL(scale_complex_kernel, dim3(nb), dim3(256), 0, ctx->stream, ctx->d_x.p, cn, N);
+      info->iterations += si.iterations;
+      info->max_step_iterations = std::max(info->max_step_iterations, si.iterations);
+      max_rel = std::max(max_rel, si.true_relres);
+      if (si.flag != 0) flag = 1;
+      info->steps = step + 1;
+    }
+    HIPCHK(hipEventRecord(e1, ctx->stream));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    info->seconds = ms * 1e-3;
+    info->max_relres = max_rel;
+    info->flag = flag;
+    // restore the inflow RHS and the solution scale
+    HIPCHK(hipMemcpyAsync(ctx->d_b.p, ctx->d_b0.p, N * 16, hipMemcpyDeviceToDevice, ctx->stream));
+    ctx->sol_scale = sigma;
+    if (rho_out) {
+      HIPCHK(hipMemcpyAsync(rho_out, ctx->d_x.p, N * 16, hipMemcpyDeviceToHost, ctx->stream));
+      HIPCHK(hipStreamSynchronize(ctx->stream));
+      for (long i = 0; i < 2 * N; ++i) rho_out[i] *= sigma;
+    }
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return WDG_OK;
+  });
+  try {
+    restore();
+  } catch (...) {
+  }
+  return rc;
 }
 
 int wdg_set_theta_method(wdg_ctx* ctx, int32_t method) {

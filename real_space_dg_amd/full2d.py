@@ -790,6 +790,15 @@ class Full2DOperator:
         check(self._lib.wdg2_set_line_precond(self.ctx, C.byref(f)), self.ctx)
         return {k: lf[k] for k in ("sigma", "factor_bytes", "transverse_dof", "modes_count")}
 
+    def transient_cn(self, dt: float, nsteps: int, a_scale: float = 1.0, b_scale: float = 1.0, tol: float = 1e-10,
+                     restart: int = 30, precision: str = "fp64", rho0=None, return_rho: bool = True):
+        """Crank-Nicolson steps of d rho/dt = b_scale*rhs - a_scale*A rho
+        (wdg_transient_cn); the Jacobi / row-sum preconditioner in use is
+        rebuilt for the shifted operator.  Returns (rho, info)."""
+        from .solver import transient_cn
+        return transient_cn(self._lib, self.ctx, self.n, dt, nsteps, a_scale, b_scale, tol, restart, precision,
+                            rho0, return_rho)
+
     def precond(self, r) -> np.ndarray:
         r = _c128(r).reshape(-1)
         z = np.empty_like(r)

@@ -202,6 +202,13 @@ class WignerOperator:
                                           dptr(j_t), max(nsave, 1)), self._h)
         return rho, n_t[:nsave].T, j_t[:nsave].T
 
+    def transient_cn(self, dt: float, nsteps: int, a_scale: float = 1.0, b_scale: float = 1.0, tol: float = 1e-10,
+                     restart: int = 30, precision: str = "fp64", rho0=None, return_rho: bool = True):
+        """Crank-Nicolson steps of d rho/dt = b_scale*rhs - a_scale*A rho (wdg_transient_cn);
+        returns (rho, info)."""
+        return transient_cn(self._lib, self._h, self.N, dt, nsteps, a_scale, b_scale, tol, restart, precision,
+                            rho0, return_rho, use_precond=False)
+
     # ---- electrostatics (Gummel loop) ----
     def poisson_setup(self, x_mat_m, eps_r, net_doping, temp: float = 300.0, method: str = "newton-raphson"):
         x = np.ascontiguousarray(np.asarray(x_mat_m, float))
@@ -252,6 +259,19 @@ class WignerOperator:
         keys = names.value.decode().split(",")
         return {k: {"avg_ms": avg[i], "launches": cnt[i], "units": units[i]}
                 for i, k in enumerate(keys[:ne.value])}
+
+
+def transient_cn(lib, ctx, n: int, dt: float, nsteps: int, a_scale: float, b_scale: float, tol: float, restart: int,
+                 precision: str, rho0, return_rho: bool, use_precond: bool = True, krylov_basis: str = "fp16"):
+    """Shared ctypes call of wdg_transient_cn for 1D and full-2D contexts."""
+    r0 = None if rho0 is None else _as_c128(rho0, n)
+    rho = np.empty(n, np.complex128) if return_rho else None
+    o = _lib.wdg_cn_opts(float(dt), int(nsteps), float(a_scale), float(b_scale), float(tol), int(restart), 20000,
+                         PRECISIONS[precision], {"fp32": _lib.WDG_BASIS_WORKING, "bf16": _lib.WDG_BASIS_BF16,
+                                                 "fp16": _lib.WDG_BASIS_FP16}[krylov_basis], 1 if use_precond else 0)
+    info = _lib.wdg_cn_info()
+    check(lib.wdg_transient_cn(ctx, C.byref(o), dptr(r0), dptr(rho), C.byref(info)), ctx)
+    return rho, {k: getattr(info, k) for k, _ in info._fields_}
 
 
 def solve_batch(ops, opts: SolveOptions | None = None):

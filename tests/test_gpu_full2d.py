@@ -153,3 +153,32 @@ def test_line_transverse_solve(precision):
         print(f"line-transverse {precision}: iterations {info.iterations} relres {info.true_relres:.2e}", flush=True)
         assert info.flag == 0 and rel(x, rho_or) < 1e-6
         assert info.iterations < 1000
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_crank_nicolson_full2d_matches_oracle(precision):
+    """Crank-Nicolson steps of the full-2D operator (rowabs rebuilt for the
+    shifted operator) against the oracle's directly solved steps; the state
+    moves towards the stationary solution."""
+    from oracle import transient as TR
+    m, EfL, EfR = ALL["rusanov-specular-fv"]
+    p, A, rhs, rho_inf = F.solve_direct_full2d(m, m["V"], EfL, EfR)
+    lam = abs(A).sum(axis=1).max()
+    dt = 20.0 / lam
+    rho0 = np.zeros(p["nTotal"], complex)
+    ref = TR.crank_nicolson(A, rhs, rho0, dt, 8)
+    with P.Full2DOperator(P.Full2DSetup(m, m["V"], EfL, EfR)) as op:
+        op.upload_rhs()
+        assert op.set_preconditioner("rowabs") == "rowabs"
+        tol = 1e-12 if precision == "fp64" else 1e-10
+        rho, info = op.transient_cn(dt, 8, tol=tol, restart=40, precision=precision, rho0=rho0)
+        print(f"CN full2D {precision}: {info}", flush=True)
+        assert info["flag"] == 0 and info["steps"] == 8
+        assert rel(rho, ref) < 1e-8, rel(rho, ref)
+        assert np.linalg.norm(rho - rho_inf) < np.linalg.norm(rho0 - rho_inf)
+        nDG, _, _ = op.observables()
+        n_ref, _, _ = F.observables_full2d(ref, p, m)
+        assert rel(nDG, n_ref) < 1e-7
+        # the stationary solve still works after the transient (preconditioner restored)
+        x, sinfo = op.solve(SolveOptions(tol=1e-8, restart=64, max_iter=60000))
+        assert sinfo.flag == 0

@@ -60,3 +60,20 @@ def test_rk4_rejects_bad_input(case):
     from real_space_dg_amd._lib import WdgError
     with pytest.raises(WdgError):
         case["op"].transient_rk4(-1.0, 3, case["m_nodes"])
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_crank_nicolson_1d_matches_oracle(case, precision):
+    """wdg_transient_cn against the oracle's directly solved trapezoidal steps."""
+    a = 2e-10
+    x_stat = spla.spsolve(case["K"], case["rhs"])
+    rho0 = x_stat + 0.05 * np.linalg.norm(x_stat) / np.sqrt(x_stat.size) * rand_c(x_stat.size, 11)
+    dt = 4.0 / (a * case["lam"])      # 8x the explicit stability limit of the RK4 step above
+    ref = TR.crank_nicolson(case["K"], case["rhs"], rho0, dt, 6, a_scale=a, b_scale=a)
+    tol = 1e-12 if precision == "fp64" else 1e-10
+    rho, info = case["op"].transient_cn(dt, 6, a_scale=a, b_scale=a, tol=tol, precision=precision, rho0=rho0)
+    assert info["steps"] == 6 and info["flag"] == 0 and info["max_relres"] <= tol
+    assert np.linalg.norm(rho - ref) <= 1e-9 * np.linalg.norm(ref)
+    n_last, _, _ = case["op"].observables(case["m_nodes"])
+    n_ref, _, _ = TR.W.observables(case["p"], ref, case["m_nodes"])
+    assert np.max(np.abs(n_last - n_ref)) <= 1e-8 * np.max(np.abs(n_ref))

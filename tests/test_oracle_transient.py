@@ -45,3 +45,31 @@ def test_stationary_state_is_a_fixed_point():
     assert np.linalg.norm(rho - rho_inf) <= 1e-9 * np.linalg.norm(rho_inf)
     assert n_t.shape == (c["p"]["N_chi"] * c["p"]["N_K_chi"], 5)
     assert np.allclose(n_t[:, 0], n_t[:, -1], rtol=1e-9)
+
+
+def test_crank_nicolson_second_order_and_amplification_factor():
+    c, K, rhs = _small()
+    a = 2e-10
+    Kd = K.toarray() * a
+    lam = np.max(np.abs(np.linalg.eigvals(Kd)))
+    rho_inf = np.linalg.solve(Kd, rhs)
+    rng = np.random.default_rng(1)
+    rho0 = rho_inf + 0.1 * np.linalg.norm(rho_inf) / np.sqrt(rho_inf.size) * (
+        rng.standard_normal(rho_inf.size) + 1j * rng.standard_normal(rho_inf.size))
+    T = 8.0 / lam
+    exact = rho_inf + sla.expm(-Kd * T) @ (rho0 - rho_inf)
+    errs = []
+    for nsteps in (200, 400):
+        rho = TR.crank_nicolson(K, rhs, rho0, T / nsteps, nsteps, a_scale=a)
+        errs.append(np.linalg.norm(rho - exact) / np.linalg.norm(exact))
+    ratio = errs[0] / errs[1]
+    assert 3.5 < ratio < 4.5, (errs, ratio)
+    # stationary state is a fixed point
+    rho = TR.crank_nicolson(K, rhs, rho_inf, 3.0 / lam, 4, a_scale=a)
+    assert np.linalg.norm(rho - rho_inf) <= 1e-10 * np.linalg.norm(rho_inf)
+    # diagonal A: per-mode amplification (1 - z/2)/(1 + z/2)
+    import scipy.sparse as sp
+    d = np.array([1.0, 2.0 + 1j, 0.3j, 5.0])
+    z = 0.7 * d
+    r = TR.crank_nicolson(sp.diags(d), np.zeros(4), np.ones(4), 0.7, 3)
+    assert np.allclose(r, ((1 - z / 2) / (1 + z / 2)) ** 3, rtol=1e-14, atol=0)
