@@ -17,8 +17,9 @@ solver.  Kernel durations are HIP events on the context stream recorded
 inside the timed region.  Every library call synchronises its stream before
 returning, so the wall clock brackets finished GPU work.
 
-Beside the metric the line reports the batched 64-instance bias sweep and
-the 16-bias self-consistent (Gummel) workload, each in both precisions.
+Beside the metric the line reports the batched 64-instance bias sweep, the
+16-bias self-consistent (Gummel) workload and the full-2D FET-slab transient
+(Crank-Nicolson steps of 33.5M unknowns), each in both precisions.
 """
 from __future__ import annotations
 
@@ -196,6 +197,57 @@ def self_consistent(n_bias: int = 16) -> dict:
     return out
 
 
+def transient_full2d(steps: int = 2, dt: float = 1e-15) -> dict:
+    """Full-2D FET slab of the transient workload: 40x20 nm, 64x32 DG elements
+    p=1, 64x64 relative cells (N = 33,554,432), Gaussian 0.2 eV gate barrier
+    (sigma 5 nm); Crank-Nicolson steps of dt from the zero state with one
+    row-scaled GMRES(30) solve to 1e-8 per step, in FP64 and mixed precision.
+    Reports steps/s, iterations per step, the density difference of the two
+    precisions and the full-2D apply kernel against the HBM roofline."""
+    from real_space_dg_amd import full2d as F2
+    from real_space_dg_amd.devices import fet_slab
+    from real_space_dg_amd.solver import PROFILE_BITS
+    mat, V = fet_slab()
+    setup = F2.Full2DSetup(mat, V, 0.1, 0.05)
+    out = {"workload": "FET slab 40x20 nm, 64x32 DG elements p=1 x 64x64 relative cells, Gaussian gate "
+                       f"0.2 eV sigma 5 nm; Crank-Nicolson dt {dt:g} s x {steps} steps from zero, GMRES(30) + "
+                       "row scaling to 1e-8 per step", "unknowns": setup.n_total}
+    dens = {}
+    with F2.Full2DOperator(setup) as op:
+        op.upload_rhs()
+        op.set_preconditioner("rowabs")
+        op.transient_cn(dt, 1, tol=1e-8, precision="mixed", rho0=np.zeros(setup.n_total, complex),
+                        return_rho=False)   # graphs / tables warm
+        mask = (1 << PROFILE_BITS["f2_apply"]) | (1 << PROFILE_BITS["f2_dense"])
+        for prec in ("mixed", "fp64"):
+            op.set_profiling(mask)
+            t0 = time.perf_counter()
+            _, info = op.transient_cn(dt, steps, tol=1e-8, precision=prec, rho0=np.zeros(setup.n_total, complex),
+                                      return_rho=False)
+            wall = time.perf_counter() - t0
+            prof = op.profile()
+            op.set_profiling(0)
+            nDG, _, _ = op.observables()
+            dens[prec] = nDG
+            out[prec] = {"steps_per_s": steps / wall, "seconds": wall, "gmres_iterations_per_step":
+                         info["iterations"] / max(info["steps"], 1), "max_step_relres": info["max_relres"],
+                         "flag": info["flag"]}
+            k = prof["f2_apply"]
+            if k["launches"]:
+                ach = k["units"] / k["launches"] / (k["avg_ms"] * 1e-3) / 1e9
+                out[prec]["f2_apply"] = {"avg_ms": k["avg_ms"], "launches": k["launches"], "bound": "hbm",
+                                         "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                         "frac": ach / HBM_PEAK_GBS}
+            d = prof["f2_dense"]
+            if d["launches"]:
+                out[prec]["f2_dense_avg_ms"] = d["avg_ms"]
+            print(f"[bench] transient full2D {prec}: {wall:.2f} s", file=sys.stderr, flush=True)
+    out["mixed_over_fp64"] = out["mixed"]["steps_per_s"] / out["fp64"]["steps_per_s"]
+    out["mixed_vs_fp64_density_rel_diff"] = float(np.max(np.abs(dens["mixed"] - dens["fp64"])) /
+                                                  np.max(np.abs(dens["fp64"])))
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -292,6 +344,7 @@ def main():
     if not args.no_extras:
         line["batched_sweep"] = batched_sweep()
         line["self_consistent"] = self_consistent()
+        line["transient_full2d"] = transient_full2d()
     if not args.no_cpu_baseline:
         print("[bench] CPU baseline sample", file=sys.stderr, flush=True)
         line["cpu_baseline"] = cpu_baseline(mat, E, efl, efr, info_d["iterations"], args.cpu_sample_iters)

@@ -799,6 +799,13 @@ class Full2DOperator:
         return transient_cn(self._lib, self.ctx, self.n, dt, nsteps, a_scale, b_scale, tol, restart, precision,
                             rho0, return_rho)
 
+    def set_profiling(self, mask: int):
+        check(self._lib.wdg_set_profiling(self.ctx, int(mask)), self.ctx)
+
+    def profile(self) -> dict:
+        from .solver import read_profile
+        return read_profile(self._lib, self.ctx)
+
     def precond(self, r) -> np.ndarray:
         r = _c128(r).reshape(-1)
         z = np.empty_like(r)

@@ -250,15 +250,19 @@ class WignerOperator:
         check(self._lib.wdg_set_profiling(self._h, C.c_uint32(mask)), self._h)
 
     def profile(self) -> dict:
-        names = C.create_string_buffer(512)
-        avg = (C.c_double * 32)()
-        cnt = (C.c_int64 * 32)()
-        units = (C.c_double * 32)()
-        ne = C.c_int32()
-        check(self._lib.wdg_profile(self._h, names, 512, avg, cnt, units, 32, C.byref(ne)), self._h)
-        keys = names.value.decode().split(",")
-        return {k: {"avg_ms": avg[i], "launches": cnt[i], "units": units[i]}
-                for i, k in enumerate(keys[:ne.value])}
+        return read_profile(self._lib, self._h)
+
+
+def read_profile(lib, ctx) -> dict:
+    """Per-kernel HIP-event statistics of a context (wdg_profile)."""
+    names = C.create_string_buffer(512)
+    avg = (C.c_double * 32)()
+    cnt = (C.c_int64 * 32)()
+    units = (C.c_double * 32)()
+    ne = C.c_int32()
+    check(lib.wdg_profile(ctx, names, 512, avg, cnt, units, 32, C.byref(ne)), ctx)
+    keys = names.value.decode().split(",")
+    return {k: {"avg_ms": avg[i], "launches": cnt[i], "units": units[i]} for i, k in enumerate(keys[:ne.value])}
 
 
 def transient_cn(lib, ctx, n: int, dt: float, nsteps: int, a_scale: float, b_scale: float, tol: float, restart: int,
@@ -301,7 +305,7 @@ def solve_batch(ops, opts: SolveOptions | None = None):
 
 PROFILE_BITS = {"transport": 0, "theta_gemm1": 1, "theta_combine": 2, "theta_gemm2": 3,
                 "line_aggregate": 4, "line_apply": 5, "arnoldi_dot": 6, "arnoldi_update": 7, "theta_fft": 8,
-                "line_carry": 9}
+                "line_carry": 9, "f2_apply": 10, "f2_dense": 11, "f2_line": 12}
 
 
 # ---------------------------------------------------------------------------
