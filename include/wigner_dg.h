@@ -300,6 +300,11 @@ typedef struct wdg2_desc {
   int32_t ne_x, ne_y;         /* DG elements (N_el_X, N_el_Y)                                */
   double  delta_x, delta_y;   /* FV cell widths; central difference D = (-1, 0, +1)/(2 delta)
                                  (initParams_full2D.m:199-214)                                */
+  int32_t periodic_y;         /* 1: periodic Y (initParams_full2D.m:65-87): the central
+                                 difference along rho_y, the banded Y parts and the
+                                 relative vertex map wrap around, the potential is
+                                 sampled periodically in Y (get_Drift_full2D.m:180-184,
+                                 650-666); Y face parts must not be given               */
 } wdg2_desc;
 
 /* One operator part coef * kron(C, R) (get_Diff_full2D.m:463-474 addOperatorPart),

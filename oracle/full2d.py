@@ -7,7 +7,7 @@ X-DG DOF and the Y-DG DOF (DG_code/initParams_full2D.m:4-8,105-107); an
 operator part coefficient * kron(C_center, R_relative) acts as
 Y = c R U C^T on U = reshape(u, nRelative, nCenter).
 
-Restates, for non-periodic Y ('specular' and 'zero-inflow' Y boundaries):
+Restates the 'specular', 'zero-inflow' and 'periodic' Y boundaries:
   DG_code/initParams_full2D.m:1-243                 -> init_params_full2d
   DG_code/core/get_Diff_full2D.m:1-813              -> get_diff_full2d (parts, rhs parts)
   DG_code/core/get_Drift_full2D.m:1-880             -> DriftFull2D ('fv-consistent', 'collocated')
@@ -24,9 +24,11 @@ and A+ A- = 0 for the characteristic split; constant potential without CAP
 gives a zero drift; zero separation has no potential difference; the
 FV-consistent relative block is a 3x3 stencil; a flat-band constant state
 is transported without a residual in the interior; the direct solve has a
-vanishing residual.  Parity unpinned: 'hard-wall' and 'periodic' Y
-boundaries, contact-mode reservoirs, position-dependent masses (not
-restated).
+vanishing residual; with periodic Y the operator commutes with the
+double-cover half-period shift rho(Y+W/2, rho_y+W) (get_PeriodicYStateError_full2D.m)
+and a Y-constant state has no Y-transport residual anywhere.  Parity
+unpinned: the 'hard-wall' Y boundary, contact-mode reservoirs,
+position-dependent masses (not restated).
 """
 from __future__ import annotations
 
@@ -126,10 +128,20 @@ def init_params_full2d(mat, Vxy=None, EfL=None, EfR=None):
     l_def = _param(params, "Ly", max(abs(y[-1] - y[0]), np.finfo(float).eps))
     lrx = float(_param(params, "L_rho_x", l_def))
     lry = float(_param(params, "L_rho_y", l_def))
-    if y_boundary_type(mat) == "periodic":
-        raise NotImplementedError("periodic Y is not restated")
+    periodic = y_boundary_type(mat) == "periodic"
+    if periodic:                                    # initParams_full2D.m:65-76
+        W_y = y[-1] - y[0]
+        lry = float(_param(params, "L_rho_y", 2 * W_y))
+        if abs(lry - 2 * W_y) > 1e-10 * W_y:
+            raise ValueError("DG:Full2D:PeriodicRelativeLength")
+        if Y["nElements"] % 2 or nry % 2 or nry < 4:
+            raise ValueError("DG:Full2D:PeriodicGrid")
     rx = make_fv_relative_axis(nrx, lrx)
     ry = make_fv_relative_axis(nry, lry)
+    ry["periodic"] = periodic
+    if periodic:                                    # :82-87
+        ry["D"] = (ry["D"] + sp.csr_matrix(([-1.0, 1.0], ([0, nry - 1], [nry - 1, 0])), shape=(nry, nry))
+                   / (2 * ry["delta"])).tocsr()
     Ix, Iy = sp.identity(nrx, format="csr"), sp.identity(nry, format="csr")
     rel = {"rhoX": rx, "rhoY": ry, "NrhoX": nrx, "NrhoY": nry, "nDof": nrx * nry,
            "Ax1D": (-1j * rx["D"]).tocsr(), "Ay1D": (-1j * ry["D"]).tocsr()}
@@ -139,7 +151,38 @@ def init_params_full2d(mat, Vxy=None, EfL=None, EfR=None):
     p["arraySize"] = (nrx, nry, X["nDof"], Y["nDof"])
     p["nTotal"] = nrx * nry * X["nDof"] * Y["nDof"]
     p["domain"] = {"X": (x[0], x[-1]), "Y": (y[0], y[-1])}
+    p["periodicY"] = {"enabled": periodic}
+    if periodic:                                    # :109-115
+        p["periodicY"].update({"width": y[-1] - y[0],
+                               "centerShift": np.roll(np.arange(Y["nDof"]), Y["nDof"] // 2),
+                               "relativeShift": np.roll(np.arange(nry), nry // 2)})
     return p
+
+
+def close_periodic_axis(ops):
+    """close_PeriodicDGAxis_full2D.m: the last/first DG faces become one
+    interior interface with the interior central and dissipative fluxes."""
+    n = ops["nDof"]
+    left = sp.csr_matrix(([1.0], ([0], [0])), shape=(1, n))
+    right = sp.csr_matrix(([1.0], ([0], [n - 1])), shape=(1, n))
+    ll = sp.csr_matrix(ops["leftLift"][:, None])
+    rl = sp.csr_matrix(ops["rightLift"][:, None])
+    out = dict(ops)
+    out["H1"] = (ops["H1"] - 0.5 * ll @ right + 0.5 * rl @ left).tocsr()
+    seam = 0.5 * (ll - rl) @ (left - right)
+    out["H2Interior"] = (ops["H2Interior"] + seam).tocsr()
+    out["H2Boundary"] = sp.csr_matrix((n, n))
+    out["H2"] = out["H2Interior"]
+    return out
+
+
+def periodic_y_shift(u, p):
+    """The double-cover half-period shift rho(Y+W/2, rho_y+W) of
+    get_PeriodicYStateError_full2D.m (relative and centre Y index shifts)."""
+    nrx, nry, nxd, nyd = p["arraySize"]
+    R4 = np.asarray(u).reshape(nyd, nxd, nry, nrx)
+    R4 = R4[p["periodicY"]["centerShift"]][:, :, p["periodicY"]["relativeShift"], :]
+    return R4.reshape(-1)
 
 
 def y_boundary_type(mat):
@@ -278,10 +321,12 @@ def get_diff_full2d(mat, p, boundary):
     params = _params(mat)
     xf, yf = flux_types(params)
     ytype = y_boundary_type(mat)
-    if ytype in ("hard-wall", "periodic"):
+    if ytype == "hard-wall":
         raise NotImplementedError(ytype)
     ox = one_dimensional_dg_operators(p["dg"]["X"])
     oy = one_dimensional_dg_operators(p["dg"]["Y"])
+    if ytype == "periodic":
+        oy = close_periodic_axis(oy)
     nxd, nyd = p["dg"]["X"]["nDof"], p["dg"]["Y"]["nDof"]
     IX, IY = sp.identity(nxd), sp.identity(nyd)
     C = {"X": {"H1": sp.kron(IY, ox["H1"]), "H2I": sp.kron(IY, ox["H2Interior"]),
@@ -322,7 +367,8 @@ def get_diff_full2d(mat, p, boundary):
         add("Y interior characteristic upwind", sy, C["Y"]["H2I"], R["Y"]["AabsChar"])
     elif R["Y"]["betaLF"] != 0:
         add("Y interior Rusanov", sy, C["Y"]["H2I"], R["Y"]["AabsLF"])
-    add("Y physical boundary characteristic penalty", sy, C["Y"]["H2B"], R["Y"]["AabsChar"])
+    if ytype != "periodic":
+        add("Y physical boundary characteristic penalty", sy, C["Y"]["H2B"], R["Y"]["AabsChar"])
     if ytype == "specular":
         add("Y-bottom specular reflection correction", -sy, C["Y"]["bottom"], R["Y"]["AplusChar"] @ Ry)
         add("Y-top specular reflection correction", sy, C["Y"]["top"], R["Y"]["AminusChar"] @ Ry)
@@ -516,6 +562,8 @@ def cap_full2d(p):
     rx, ry = p["relative"]["rhoX"], p["relative"]["rhoY"]
     wx = W.get_cap(rx["cells"], rx["length"], rx["delta"])
     wy = W.get_cap(ry["cells"], ry["length"], ry["delta"])
+    if ry.get("periodic"):
+        wy = np.zeros_like(wy)       # no absorbing edge on a periodic coherence coordinate
     return (wx[None, :] + wy[:, None]).reshape(-1)   # rho_x fastest
 
 
@@ -537,6 +585,9 @@ class DriftFull2D:
         ix, iy = np.argsort(gx), np.argsort(gy)
         self.gx, self.gy = gx[ix], gy[iy]
         self.Vf = Vf[np.ix_(ix, iy)]
+        self.periodic = bool(p.get("periodicY", {}).get("enabled"))
+        if self.periodic and np.max(np.abs(self.Vf[:, 0] - self.Vf[:, -1])) > 1e-10 * max(1, np.abs(self.Vf).max()):
+            raise ValueError("DG:Full2D:NonperiodicPotential")
         self.interp = RegularGridInterpolator((self.gx, self.gy), self.Vf, method="linear")
         q_def = QE / HBAR
         self.Qdrift = float(_param(params, "full2D_Q_drift", q_def))
@@ -558,7 +609,10 @@ class DriftFull2D:
     def V(self, qx, qy):
         """evaluateExtendedPotential: clamp X and Y independently, then bilinear."""
         qx = np.clip(qx, self.gx[0], self.gx[-1])
-        qy = np.clip(qy, self.gy[0], self.gy[-1])
+        if self.periodic:
+            qy = self.gy[0] + np.mod(qy - self.gy[0], self.gy[-1] - self.gy[0])
+        else:
+            qy = np.clip(qy, self.gy[0], self.gy[-1])
         return self.interp(np.stack([np.ravel(qx), np.ravel(qy)], axis=-1)).reshape(np.shape(qx))
 
     def _fv_setup(self, params):
@@ -567,8 +621,13 @@ class DriftFull2D:
         nx, ny = rel["NrhoX"], rel["NrhoY"]
         Ex = sp.diags([np.ones(nx), np.ones(nx)], [0, -1], shape=(nx + 1, nx))
         Ey = sp.diags([np.ones(ny), np.ones(ny)], [0, -1], shape=(ny + 1, ny))
+        edges_y = rel["rhoY"]["edges"]
+        if self.periodic:                           # unique periodic vertices, get_Drift_full2D.m:180-184
+            Ey = sp.identity(ny) + sp.csr_matrix((np.ones(ny), (np.arange(ny), np.r_[ny - 1, np.arange(ny - 1)])),
+                                                 shape=(ny, ny))
+            edges_y = edges_y[:-1]
         self.P = sp.kron(Ey, Ex).tocsr()
-        vx, vy = np.meshgrid(rel["rhoX"]["edges"], rel["rhoY"]["edges"], indexing="ij")
+        vx, vy = np.meshgrid(rel["rhoX"]["edges"], edges_y, indexing="ij")
         self.vx, self.vy = vx.reshape(-1, order="F"), vy.reshape(-1, order="F")
         X, Y = p["dg"]["X"], p["dg"]["Y"]
         nlx, nly = X["nLocal"], Y["nLocal"]
@@ -840,8 +899,10 @@ class LineTransverse:
         self.U, self.lam = U, lam
         K, V = _transverse_reference(mat, p, Vxy, ix_ref)
         self.K, self.V = K, V
-        capY = np.tile(W.get_cap(p["relative"]["rhoY"]["cells"], p["relative"]["rhoY"]["length"],
-                                 p["relative"]["rhoY"]["delta"]), Y["nDof"]) * drift.cap_coef
+        wy = W.get_cap(p["relative"]["rhoY"]["cells"], p["relative"]["rhoY"]["length"], p["relative"]["rhoY"]["delta"])
+        if p["relative"]["rhoY"].get("periodic"):
+            wy = np.zeros_like(wy)          # sys.drift.CAP.profileY is zero for a periodic rho_y
+        capY = np.tile(wy, Y["nDof"]) * drift.cap_coef
         wx = W.get_cap(p["relative"]["rhoX"]["cells"], p["relative"]["rhoX"]["length"], p["relative"]["rhoX"]["delta"])
         capX = np.real(np.sum(np.conj(U) * (wx[:, None] * U), axis=0)) * drift.cap_coef
         if drift.mode == "fv-consistent":

@@ -114,7 +114,8 @@ class wdg_poisson_info(C.Structure):
 
 class wdg2_desc(C.Structure):
     _fields_ = [("n_rho_x", C.c_int32), ("n_rho_y", C.c_int32), ("nl_x", C.c_int32), ("nl_y", C.c_int32),
-                ("ne_x", C.c_int32), ("ne_y", C.c_int32), ("delta_x", C.c_double), ("delta_y", C.c_double)]
+                ("ne_x", C.c_int32), ("ne_y", C.c_int32), ("delta_x", C.c_double), ("delta_y", C.c_double),
+                ("periodic_y", C.c_int32)]
 
 
 class wdg2_part(C.Structure):

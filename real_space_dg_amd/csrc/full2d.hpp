@@ -41,6 +41,7 @@ struct F2Const {
   int nqx, nqy, nxg, nyg;       // quadrature points, material grid size
   int npx, npy;                 // relative sample points per axis (vertices or cells)
   int v_lds;                    // 1: the material grid is staged in LDS
+  int periodic_y;               // 1: rho_y, Y elements and the rho_y vertex map wrap around
   // stored potential differences DeltaV(element, q = qx*nqy + qy, vertex), vertex
   // x fastest (storedPotentialDifference, get_Drift_full2D.m:609-618); null: on the fly
   const double* dv64;
@@ -201,7 +202,8 @@ __global__ void __launch_bounds__(kF2Block) f2_apply_kernel(const F2Const* __res
         const T2* src = u + cdof(a % NLX, a / NLX) * nrel + tx;
         T2* dst = su + (size_t)a * R2 * nrx + tx;
         for (int rr = ty; rr < R2; rr += c.rows) {
-          const int ry = ry0 - 1 + rr;
+          int ry = ry0 - 1 + rr;
+          if (c.periodic_y) ry = ry < 0 ? ry + nry : (ry >= nry ? ry - nry : ry);
           const bool ok = ry >= 0 && ry < nry && rr <= nrows + 1;
           dst[(size_t)rr * nrx] = ok ? src[(long)ry * nrx] : f2_zero<T2>();
         }
@@ -235,7 +237,7 @@ __global__ void __launch_bounds__(kF2Block) f2_apply_kernel(const F2Const* __res
           W[b] = w;
         }
       } else {
-        const int cxn = (vx >= 1) + (vx < nrx), cyn = (vy >= 1) + (vy < nry);
+        const int cxn = (vx >= 1) + (vx < nrx), cyn = c.periodic_y ? 2 : (vy >= 1) + (vy < nry);
         deg = (T)(cxn * cyn);
       }
 #pragma unroll
@@ -394,7 +396,11 @@ __global__ void __launch_bounds__(kF2Block) f2_apply_kernel(const F2Const* __res
     }
     if (MODE == F2_APPLY) {
       // ---- transport: X parts along rho_x, Y parts along rho_y ----
-      const bool rxl = rx > 0, rxr = rx < nrx - 1, ryd = ry > 0, ryu = ry < nry - 1;
+      const bool rxl = rx > 0, rxr = rx < nrx - 1;
+      const bool pY = c.periodic_y != 0;
+      const bool ryd = ry > 0 || pY, ryu = ry < nry - 1 || pY;
+      const long offd = ry > 0 ? -(long)nrx : (long)(nry - 1) * nrx;    // rho_y - 1 (wrapped when periodic)
+      const long offu = ry < nry - 1 ? (long)nrx : -(long)(nry - 1) * nrx;
       const int sX = NLX * NLX + 2 * NLX, sY = NLY * NLY + 2 * NLY;
 #pragma unroll
       for (int ay = 0; ay < NLY; ++ay) {
@@ -445,13 +451,15 @@ __global__ void __launch_bounds__(kF2Block) f2_apply_kernel(const F2Const* __res
 #pragma unroll
       for (int ax = 0; ax < NLX; ++ax) {
         T2 dm = f2_zero<T2>(), d0 = dm, dp = dm, tm = dm, t0 = dm, tp = dm;
-        if (ey > 0) {
-          const T2* p = u + ((long)(ex * NLX + ax) + (long)c.nxd * (ey * NLY - 1)) * nrel + r;
-          dm = f2_ld(p, -(long)nrx, ryd); d0 = p[0]; dp = f2_ld(p, (long)nrx, ryu);
+        if (ey > 0 || pY) {
+          const int eb = ey > 0 ? ey - 1 : c.ney - 1;
+          const T2* p = u + ((long)(ex * NLX + ax) + (long)c.nxd * (eb * NLY + NLY - 1)) * nrel + r;
+          dm = f2_ld(p, offd, ryd); d0 = p[0]; dp = f2_ld(p, offu, ryu);
         }
-        if (ey < c.ney - 1) {
-          const T2* p = u + ((long)(ex * NLX + ax) + (long)c.nxd * ((ey + 1) * NLY)) * nrel + r;
-          tm = f2_ld(p, -(long)nrx, ryd); t0 = p[0]; tp = f2_ld(p, (long)nrx, ryu);
+        if (ey < c.ney - 1 || pY) {
+          const int et = ey < c.ney - 1 ? ey + 1 : 0;
+          const T2* p = u + ((long)(ex * NLX + ax) + (long)c.nxd * (et * NLY)) * nrel + r;
+          tm = f2_ld(p, offd, ryd); t0 = p[0]; tp = f2_ld(p, offu, ryu);
         }
 #pragma unroll
         for (int ay = 0; ay < NLY; ++ay) {
@@ -511,6 +519,7 @@ struct F2Dense {
                      // -2: plain y = R u along the axis (overwrites y)
   int nl, ne;        // nodes per element and elements of the centre axis
   int chunk;         // relative cells across the axis per workgroup (gridDim.y chunks)
+  int periodic;      // banded: the end elements couple across the seam
   double coef;       // part coefficient / operator scale
   const double* blocks;   // banded: per element diag nl*nl, lower nl, upper nl
   const double* lift;     // face: nl
@@ -581,8 +590,10 @@ __global__ void __launch_bounds__(kF2Block) f2_dense_kernel(const F2Const* __res
       const long r = rel(k, c0 + cc);
       T2 w = f2_zero<T2>();
       for (int b = 0; b < d.nl; ++b) f2_fma(w, (T)B[a * d.nl + b], u[dof(e * d.nl + b, t) * nrel + r]);
-      if (e > 0) f2_fma(w, (T)B[d.nl * d.nl + a], u[dof(e * d.nl - 1, t) * nrel + r]);
-      if (e < d.ne - 1) f2_fma(w, (T)B[d.nl * d.nl + d.nl + a], u[dof((e + 1) * d.nl, t) * nrel + r]);
+      if (e > 0 || d.periodic)
+        f2_fma(w, (T)B[d.nl * d.nl + a], u[dof((e > 0 ? e : d.ne) * d.nl - 1, t) * nrel + r]);
+      if (e < d.ne - 1 || d.periodic)
+        f2_fma(w, (T)B[d.nl * d.nl + d.nl + a], u[dof(e < d.ne - 1 ? (e + 1) * d.nl : 0, t) * nrel + r]);
       sW[(size_t)k * d.chunk + cc] = w;
     }
   } else {

@@ -230,6 +230,8 @@ void f2_create_impl(wdg_ctx* c, const wdg2_desc& d, const wdg2_part* parts, int 
     throw std::invalid_argument("nl_x and nl_y must be in 2..4");
   if (d.ne_x < 1 || d.ne_y < 1) throw std::invalid_argument("ne_x and ne_y must be positive");
   if (!(d.delta_x > 0) || !(d.delta_y > 0)) throw std::invalid_argument("delta_x and delta_y must be positive");
+  if (d.periodic_y && (d.ne_y < 2 || d.n_rho_y < 3))
+    throw std::invalid_argument("periodic Y needs at least two Y elements and three rho_y cells");
   if (np < 0 || (np > 0 && !parts)) throw std::invalid_argument("parts missing");
   auto f = std::make_shared<F2Data>();
   f->d = d;
@@ -340,6 +342,9 @@ void f2_create_impl(wdg_ctx* c, const wdg2_desc& d, const wdg2_part* parts, int 
       dp->k.nl = nl;
       dp->k.ne = ne;
       dp->k.coef = 1.0;
+      dp->k.periodic = (ax == 1 && d.periodic_y) ? 1 : 0;
+      if (d.periodic_y && ax == 1 && p.center == WDG2_CENTER_FACE)
+        throw std::invalid_argument("a periodic Y axis has no faces");
       const std::vector<double2>& R = mergedR[i];
       dp->R.alloc(R.size());
       HIPCHK(hipMemcpyAsync(dp->R.p, R.data(), R.size() * sizeof(double2), hipMemcpyHostToDevice, c->stream));
@@ -383,8 +388,9 @@ void f2_create_impl(wdg_ctx* c, const wdg2_desc& d, const wdg2_part* parts, int 
         vdc[i_c] = p.c_diag[(size_t)e * nl * nl + a * nl + a];
         double s = 0;
         for (int b = 0; b < nl; ++b) s += std::abs(p.c_diag[(size_t)e * nl * nl + a * nl + b]);
-        if (e > 0) s += std::abs(p.c_lower[(size_t)e * nl + a]);
-        if (e < ne - 1) s += std::abs(p.c_upper[(size_t)e * nl + a]);
+        const bool wrap = ax == 1 && d.periodic_y;
+        if (e > 0 || wrap) s += std::abs(p.c_lower[(size_t)e * nl + a]);
+        if (e < ne - 1 || wrap) s += std::abs(p.c_upper[(size_t)e * nl + a]);
         vac[i_c] = s;
       } else {
         const bool in_face = p.face == 0 ? e == 0 : e == ne - 1;
@@ -405,7 +411,7 @@ void f2_create_impl(wdg_ctx* c, const wdg2_desc& d, const wdg2_part* parts, int 
       double2 dv = make_double2(0, 0);
       double av = 0;
       if (p.rel == WDG2_REL_CENTRAL) {
-        av = 0.5 / (ax == 0 ? d.delta_x : d.delta_y) * ((k > 0) + (k < nr - 1));
+        av = 0.5 / (ax == 0 ? d.delta_x : d.delta_y) * ((ax == 1 && d.periodic_y) ? 2 : (k > 0) + (k < nr - 1));
       } else if (p.rel == WDG2_REL_IDENTITY) {
         dv = make_double2(1, 0);
         av = 1;
@@ -451,7 +457,7 @@ void f2_create_impl(wdg_ctx* c, const wdg2_desc& d, const wdg2_part* parts, int 
 // Per (centre sample, relative sample, sign): the material cell and the
 // linear weight of coordinate centre +- rel/2 after clamping to the grid.
 void f2_sample_table(const double* centre, int nc, const double* rel, int nr, const std::vector<double>& g,
-                     std::vector<int>& idx, std::vector<double>& t) {
+                     std::vector<int>& idx, std::vector<double>& t, bool periodic = false) {
   const int ng = (int)g.size();
   idx.resize((size_t)nc * nr * 2);
   t.resize(idx.size());
@@ -459,6 +465,11 @@ void f2_sample_table(const double* centre, int nc, const double* rel, int nr, co
     for (int j = 0; j < nr; ++j)
       for (int s = 0; s < 2; ++s) {
         double q = centre[i] + (s == 0 ? 0.5 : -0.5) * rel[j];
+        if (periodic) {   // g0 + mod(q - g0, W), mod as floor-based remainder in [0, W)
+          const double w = g[ng - 1] - g[0];
+          const double r = q - g[0];
+          q = g[0] + (r - std::floor(r / w) * w);
+        }
         q = std::min(std::max(q, g[0]), g[ng - 1]);
         int k = (int)(std::upper_bound(g.begin(), g.end(), q) - g.begin()) - 1;
         k = std::min(std::max(k, 0), ng - 2);
@@ -497,7 +508,7 @@ void f2_set_potential_impl(wdg_ctx* c, const wdg2_potential& p) {
   std::vector<int> tix, tiy;
   std::vector<double> ttx, tty;
   f2_sample_table(p.center_x, ncx, p.rel_x, npx, gx, tix, ttx);
-  f2_sample_table(p.center_y, ncy, p.rel_y, npy, gy, tiy, tty);
+  f2_sample_table(p.center_y, ncy, p.rel_y, npy, gy, tiy, tty, d.periodic_y != 0);
   auto upi = [&](DevBuf<int>& b, const std::vector<int>& v) {
     b.alloc(v.size());
     HIPCHK(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
@@ -522,6 +533,7 @@ void f2_set_potential_impl(wdg_ctx* c, const wdg2_potential& p) {
   h.nqy = fv ? p.nq_y : 0;
   h.nxg = p.nx_grid; h.nyg = p.ny_grid;
   h.npx = npx; h.npy = npy;
+  h.periodic_y = d.periodic_y ? 1 : 0;
   h.inv2dx = 0.5 / d.delta_x;
   h.inv2dy = 0.5 / d.delta_y;
   h.pot_coef = p.pot_coef / f.sA;

@@ -10,7 +10,7 @@ operator to structured pieces:
 * per-element DG blocks of each centre axis (H1, interior H2) and the face
   lifts of its two ends;
 * the relative-axis operators: central difference -i D, identity (Rusanov),
-  dense characteristic matrices |A|, A+ R, A- R;
+  dense characteristic matrices |A|, A+ R, A- R (periodic rho_y: circulant);
 * the potential on the material grid with the quadrature/projection factors
   of the FV-consistent (or collocated) discretisation and the CAP profiles;
 * the inflow right-hand side and the sparse zero-separation weights of the
@@ -166,7 +166,7 @@ class AxisBlocks:
     upper: np.ndarray   # (ne, nl)
 
 
-def axis_operators(ax: DGAxis) -> dict:
+def axis_operators(ax: DGAxis, periodic: bool = False) -> dict:
     """H1 (central), interior H2 and the face lifts of get_Diff_full2D.m:206-266.
 
     K1 = M^-1 (S + P1/2 - P3/2) on the element, the central-flux coupling
@@ -184,19 +184,26 @@ def axis_operators(ax: DGAxis) -> dict:
     K2 = 0.5 * Mi @ (np.outer(e0, e0) + np.outer(en, en))
     from_left = -0.5 * Mi[:, 0]           # column n-1 of -M^-1 P2 / 2
     from_right = 0.5 * Mi[:, n - 1]       # column 0 of M^-1 P4 / 2
+    # a periodic axis closes the last/first faces into one interior interface
+    # (close_PeriodicDGAxis_full2D.m): the end elements couple across the seam
+    # with the interior fluxes, scaled by their own jacobian
+    def up_j(e):
+        return J[e + 1] if e < ne - 1 else J[e]
+    lower_on = [e > 0 or periodic for e in range(ne)]
+    upper_on = [e < ne - 1 or periodic for e in range(ne)]
     h1 = AxisBlocks(np.stack([K1 / J[e] for e in range(ne)]),
-                    np.stack([from_left / J[e] if e > 0 else np.zeros(n) for e in range(ne)]),
-                    np.stack([from_right / J[e + 1] if e < ne - 1 else np.zeros(n) for e in range(ne)]))
+                    np.stack([from_left / J[e] if lower_on[e] else np.zeros(n) for e in range(ne)]),
+                    np.stack([from_right / up_j(e) if upper_on[e] else np.zeros(n) for e in range(ne)]))
     d2 = []
     for e in range(ne):
         blk = K2.copy()
-        if e == 0:
+        if e == 0 and not periodic:
             blk -= 0.5 * np.outer(Mi[:, 0], e0)
-        if e == ne - 1:
+        if e == ne - 1 and not periodic:
             blk -= 0.5 * np.outer(Mi[:, n - 1], en)
         d2.append(blk / J[e])
-    h2 = AxisBlocks(np.stack(d2), np.stack([from_left / J[e] if e > 0 else np.zeros(n) for e in range(ne)]),
-                    np.stack([-from_right / J[e + 1] if e < ne - 1 else np.zeros(n) for e in range(ne)]))
+    h2 = AxisBlocks(np.stack(d2), np.stack([from_left / J[e] if lower_on[e] else np.zeros(n) for e in range(ne)]),
+                    np.stack([-from_right / up_j(e) if upper_on[e] else np.zeros(n) for e in range(ne)]))
     return {"h1": h1, "h2i": h2, "lift_low": Mi[:, 0] / J[0], "lift_high": Mi[:, n - 1] / J[-1]}
 
 
@@ -234,7 +241,16 @@ def rho_modes(n: int, delta: float):
     return Phi, np.cos(np.pi * j / (n + 1)) / delta
 
 
-def characteristic_split(n: int, delta: float, tol: float = 1e-12):
+def rho_modes_periodic(n: int, delta: float):
+    """Eigenpairs of the Hermitian circulant -i D with the periodic central
+    difference (initParams_full2D.m:82-87): lambda_k = sin(2 pi k/n)/delta,
+    eigenvectors exp(2 pi i j k/n)/sqrt(n)."""
+    j = np.arange(n)
+    Phi = np.exp(2j * np.pi * np.outer(j, j) / n) / np.sqrt(n)
+    return Phi, np.sin(2 * np.pi * j / n) / delta
+
+
+def characteristic_split(n: int, delta: float, tol: float = 1e-12, periodic: bool = False):
     """A+, A-, eigenvalues of A = -i D, D = tridiag(-1, 0, 1)/(2 delta).
 
     A is Hermitian Toeplitz; its eigenpairs are closed form:
@@ -243,7 +259,7 @@ def characteristic_split(n: int, delta: float, tol: float = 1e-12):
     1e-12 are zeroed, as in splitCharacteristicMatrix
     (get_Diff_full2D.m:718-730).
     """
-    Phi, lam = rho_modes(n, delta)
+    Phi, lam = rho_modes_periodic(n, delta) if periodic else rho_modes(n, delta)
     lp = np.where(lam > tol, lam, 0.0)
     lm = np.where(lam < -tol, lam, 0.0)
     Ap = (Phi * lp) @ Phi.conj().T
@@ -315,8 +331,9 @@ class Full2DSetup:
         self.EfL, self.EfR = EfL, EfR
         self._vxy = Vxy
         ytype = y_boundary_type(params)
-        if ytype in ("periodic", "hard-wall"):
+        if ytype == "hard-wall":
             raise NotImplementedError(f"DG:Full2D:Unsupported: Y boundary '{ytype}' is not built on the device")
+        self.periodic = ytype == "periodic"
         mass_model = str(read_param(params, "full2D_massModel", "constant")).strip().lower()
         if mass_model not in ("constant", "scalar", "legacy", "reference"):
             raise NotImplementedError("DG:Full2D:Unsupported: position-dependent mass is not built on the device")
@@ -334,6 +351,13 @@ class Full2DSetup:
         l_def = read_param(params, "Ly", max(abs(gy[-1] - gy[0]), np.finfo(float).eps))
         self.RX = RelAxis(nrx, float(read_param(params, "L_rho_x", l_def)))
         self.RY = RelAxis(nry, float(read_param(params, "L_rho_y", l_def)))
+        if self.periodic:     # initParams_full2D.m:65-76
+            width = gy[-1] - gy[0]
+            self.RY = RelAxis(nry, float(read_param(params, "L_rho_y", 2 * width)))
+            if abs(self.RY.length - 2 * width) > 1e-10 * width:
+                raise ValueError("DG:Full2D:PeriodicRelativeLength: periodic Y requires L_rho_y = 2*(Ymax-Ymin)")
+            if self.Y.ne % 2 or nry % 2 or nry < 4:
+                raise ValueError("DG:Full2D:PeriodicGrid: periodic Y requires even N_el_Y and even N_rho_y >= 4")
         self.nrel = nrx * nry
         self.ncen = self.X.ndof * self.Y.ndof
         self.n_total = self.nrel * self.ncen
@@ -358,12 +382,12 @@ class Full2DSetup:
         sy = f * float(read_param(params, "full2D_Q_diff_y",
                                   read_param(params, "full2D_transportScaleY", HBAR / float(mass_kg(me_y)))))
         self.scales = (sx, sy)
-        ops = {0: axis_operators(self.X), 1: axis_operators(self.Y)}
+        ops = {0: axis_operators(self.X), 1: axis_operators(self.Y, self.periodic)}
         self.axis_ops = ops
         self.char = {}
         parts = []
         for ax, R, flux, s, key in ((0, self.RX, self.flux_x, sx, "X"), (1, self.RY, self.flux_y, sy, "Y")):
-            Ap, Am, lam = characteristic_split(R.n, R.delta, tol)
+            Ap, Am, lam = characteristic_split(R.n, R.delta, tol, periodic=(ax == 1 and self.periodic))
             theta = 0.0
             if flux == "rusanov":
                 theta = float(read_param(params, f"full2D_thetaLF_{key}",
@@ -381,6 +405,8 @@ class Full2DSetup:
                 parts.append(Part(f"{key} interior characteristic upwind", ax, C, D, s, blocks=o["h2i"], dense=Aabs))
             elif beta != 0:
                 parts.append(Part(f"{key} interior Rusanov", ax, C, I, s * beta, blocks=o["h2i"]))
+            if ax == 1 and self.periodic:
+                continue      # no physical Y faces
             # physical boundary penalty H2B (x) |A| = (lift_low e_first^T + lift_high e_last^T)/2 (x) |A|
             parts.append(Part(f"{key}-low physical boundary penalty", ax, F, D, 0.5 * s, face=0,
                               lift=o["lift_low"], dense=Aabs))
@@ -559,6 +585,11 @@ class Full2DSetup:
         self.cap_coef = ds * cs * qd
         self.cap_x = cap_profile(self.RX.cells, self.RX.length, self.RX.delta)
         self.cap_y = cap_profile(self.RY.cells, self.RY.length, self.RY.delta)
+        if self.periodic:
+            self.cap_y = np.zeros_like(self.cap_y)     # get_CAP_full2D.m:14-16
+            if np.max(np.abs(self.vfield[:, 0] - self.vfield[:, -1])) > 1e-10 * max(1.0, np.abs(self.vfield).max()):
+                raise ValueError("DG:Full2D:NonperiodicPotential: the first and last material Y columns must "
+                                 "represent the same periodic potential")
         X, Y = self.X, self.Y
         if mode == "fv-consistent":
             order = read_param(params, "full2D_potentialQuadratureOrder")
@@ -577,7 +608,10 @@ class Full2DSetup:
                 self.proj.append((ip * wq[:, None]) @ Mi.T)         # (nq, nl)
                 mid = 0.5 * (ax.vertices[:-1] + ax.vertices[1:])
                 self.quad.append((mid[:, None] + ax.jac[:, None] * rq[None, :]).ravel())
-            self.rel_pts = (self.RX.edges, self.RY.edges)
+            ey_ = self.RY.edges.copy()
+            if self.periodic:
+                ey_[-1] = ey_[0]    # the last vertex row is the first one (unique periodic vertices)
+            self.rel_pts = (self.RX.edges, ey_)
         else:
             self.nq = (0, 0)
             self.quad = [X.nodes, Y.nodes]
@@ -668,7 +702,8 @@ class Full2DOperator:
         self._lib = lib
         self.setup = s = setup
         self.drift_on = drift
-        self.desc = _lib.wdg2_desc(s.RX.n, s.RY.n, s.X.nl, s.Y.nl, s.X.ne, s.Y.ne, s.RX.delta, s.RY.delta)
+        self.desc = _lib.wdg2_desc(s.RX.n, s.RY.n, s.X.nl, s.Y.nl, s.X.ne, s.Y.ne, s.RX.delta, s.RY.delta,
+                                   1 if s.periodic else 0)
         keep = []
         parts = s.parts if transport else []
         cparts = (_lib.wdg2_part * max(len(parts), 1))()

@@ -10,7 +10,7 @@ from oracle import full2d as F
 from real_space_dg_amd import _lib
 from real_space_dg_amd import full2d as P
 
-from test_oracle_full2d import small_mat
+from test_oracle_full2d import periodic_mat, small_mat
 
 
 def configs():
@@ -27,6 +27,8 @@ def configs():
     m = small_mat()
     m["dg"]["params"].update({"N_K_X": 2, "N_K_Y": 2, "N_rho_x": 6, "N_rho_y": 4, "full2D_potentialQuadratureOrder": 5})
     out["p1-even-rho"] = (m, 0.2, 0.15)
+    out["periodic-rusanov-fv"] = (periodic_mat("rusanov", "fv-consistent"), 0.2, 0.1)
+    out["periodic-upwind-collocated"] = (periodic_mat("upwind", "collocated"), 0.22, 0.12)
     return out
 
 
@@ -40,10 +42,8 @@ def _axis_matrix(part, nl, ne):
         for e in range(ne):
             r = slice(e * nl, (e + 1) * nl)
             H[r, r] = part.blocks.diag[e]
-            if e > 0:
-                H[r, e * nl - 1] = part.blocks.lower[e]
-            if e < ne - 1:
-                H[r, (e + 1) * nl] = part.blocks.upper[e]
+            H[r, (e * nl - 1) % nd] += part.blocks.lower[e]       # zero at the ends unless periodic
+            H[r, ((e + 1) * nl) % nd] += part.blocks.upper[e]
     else:
         rows = slice(0, nl) if part.face == 0 else slice(nd - nl, nd)
         H[rows, 0 if part.face == 0 else nd - 1] = part.lift
@@ -64,7 +64,11 @@ def expand(setup):
         R1 = RX if ax == 0 else RY
         if part.rel == _lib.WDG2_REL_CENTRAL:
             n = R1.n
-            base = -1j * sp.diags([-np.ones(n - 1), np.ones(n - 1)], [-1, 1]) / (2 * R1.delta)
+            D = sp.diags([-np.ones(n - 1), np.ones(n - 1)], [-1, 1]).tolil()
+            if ax == 1 and setup.periodic:
+                D[0, n - 1] = -1
+                D[n - 1, 0] = 1
+            base = -1j * sp.csr_matrix(D) / (2 * R1.delta)
         elif part.rel == _lib.WDG2_REL_IDENTITY:
             base = sp.identity(R1.n)
         else:
@@ -144,8 +148,11 @@ def test_closed_form_characteristic_split_matches_eig():
 
 
 def test_unsupported_options_raise():
-    m = small_mat(); m["dg"]["params"]["full2D_Y_boundary"] = "periodic"
+    m = small_mat(); m["dg"]["params"]["full2D_Y_boundary"] = "hard-wall"
     with pytest.raises(NotImplementedError, match="Unsupported"):
+        P.Full2DSetup(m, m["V"], 0.2, 0.1)
+    m = periodic_mat(); m["V"] = m["V"] + 0.01 * np.arange(5)[None, :]
+    with pytest.raises(ValueError, match="NonperiodicPotential"):
         P.Full2DSetup(m, m["V"], 0.2, 0.1)
     m = small_mat(); m["dg"]["params"]["full2D_massModel"] = "position-dependent"
     with pytest.raises(NotImplementedError, match="Unsupported"):

@@ -26,6 +26,17 @@ ALL["tiled-64x10"] = (tiled_mat(), 0.2, 0.1)
 ALL["tiled-80x7-upwind-p2"] = (tiled_mat(80, 7, 3, "upwind", "zero-inflow"), 0.2, 0.1)
 
 
+def tiled_periodic():
+    """Periodic Y with several rho_y tiles per element (wrapped halo rows)."""
+    from test_oracle_full2d import periodic_mat
+    m = periodic_mat("upwind", "fv-consistent")
+    m["dg"]["params"].update({"N_K_X": 2, "N_K_Y": 2, "N_el_Y": 4, "N_rho_x": 64, "N_rho_y": 10})
+    return m
+
+
+ALL["tiled-periodic-64x10"] = (tiled_periodic(), 0.2, 0.1)
+
+
 def oracle_parts(m, EfL, EfR):
     p = F.init_params_full2d(m, m["V"], EfL, EfR)
     b = F.boundary_full2d(m, p, EfL, EfR, m["V"])
@@ -58,7 +69,7 @@ def test_apply_diagonal_rowabs(name):
 
 
 @pytest.mark.parametrize("name", ["rusanov-specular-fv", "upwind-zero-inflow", "central-closed-faces-collocated",
-                                  "p1-even-rho"])
+                                  "p1-even-rho", "periodic-rusanov-fv"])
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_solve_matches_direct_and_observables(name, precision):
     m, EfL, EfR = ALL[name]
@@ -115,7 +126,8 @@ def test_reference_signature_entry_point():
 
 
 @pytest.mark.parametrize("name", ["rusanov-specular-fv", "upwind-zero-inflow", "central-closed-faces-collocated",
-                                  "p1-even-rho", "tiled-80x7-upwind-p2"])
+                                  "p1-even-rho", "tiled-80x7-upwind-p2", "periodic-rusanov-fv",
+                                  "periodic-upwind-collocated"])
 def test_line_transverse_matches_oracle(name):
     """pre.apply(r) of get_LineTransversePreconditioner_full2D.m on the device
     (modes, X sweeps, transverse inverses) against the oracle's LU form."""

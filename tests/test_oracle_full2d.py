@@ -357,3 +357,51 @@ def test_line_transverse_reference_block_is_the_y_transport():
 def test_line_transverse_memory_guard():
     with pytest.raises(MemoryError, match="LineMemoryLimit"):
         _line_setup("fv-consistent", full2D_lineMaxFactorBytes=1000)
+
+
+def periodic_mat(flux="rusanov", mode="fv-consistent"):
+    """makeSmallMat with periodic Y: V periodic in y (period W), L_rho_y = 2W."""
+    m = small_mat()
+    X, Y = np.meshgrid(m["x"], m["y"], indexing="ij")
+    W = m["y"][-1] - m["y"][0]
+    m["V"] = 0.1 * X + 0.03 * np.cos(2 * np.pi * (Y - m["y"][0]) / W)
+    m["dg"]["params"].pop("L_rho_y")
+    m["dg"]["params"].update({"full2D_Y_boundary": "periodic", "N_rho_y": 6, "N_el_Y": 2,
+                              "full2D_X_flux": flux, "full2D_Y_flux": flux,
+                              "full2D_potentialDiscretization": mode})
+    return m
+
+
+@pytest.mark.parametrize("flux", ["rusanov", "upwind"])
+@pytest.mark.parametrize("mode", ["fv-consistent", "collocated"])
+def test_periodic_y_commutes_with_half_period_shift(flux, mode):
+    """Physical states satisfy rho(Y+W/2, rho_y+W) = rho(Y, rho_y)
+    (get_PeriodicYStateError_full2D.m); the periodic operator (transport and
+    potential) commutes with that shift, and a Y-constant state has no
+    Y-transport residual anywhere."""
+    m = periodic_mat(flux, mode)
+    p, b, parts, rparts, info, A, rhs = diff_setup(m)
+    assert p["relative"]["rhoY"]["length"] == pytest.approx(2 * (m["y"][-1] - m["y"][0]))
+    assert not any("Y physical" in q[0] or "specular" in q[0] for q in parts)
+    G = F.DriftFull2D(m, p, m["V"]).assemble()
+    u = det_vec(p["nTotal"])
+    for Op in (A, G):
+        lhs = Op @ F.periodic_y_shift(u, p)
+        rhs_ = F.periodic_y_shift(Op @ u, p)
+        assert np.max(np.abs(lhs - rhs_)) <= 1e-12 * np.max(np.abs(lhs))
+    assert np.allclose(F.apply_parts(parts, u, p), A @ u, rtol=0, atol=1e-10 * abs(A).max())
+    Ay = F.assemble_parts([q for q in parts if q[0].startswith("Y")], p)
+    nr, nc = p["relative"]["nDof"], p["dg"]["nCenterDof"]
+    f = det_vec(nr)
+    assert np.max(np.abs(Ay @ np.tile(f, nc))) <= 1e-12 * abs(Ay).max() * np.max(np.abs(f))
+    assert np.all(F.cap_full2d(p).reshape(p["relative"]["NrhoY"], -1)[:, 0] ==
+                  F.cap_full2d(p).reshape(p["relative"]["NrhoY"], -1)[0, 0])   # no rho_y CAP
+
+
+def test_periodic_y_grid_checks():
+    m = periodic_mat(); m["dg"]["params"]["N_rho_y"] = 5
+    with pytest.raises(ValueError, match="PeriodicGrid"):
+        F.init_params_full2d(m)
+    m = periodic_mat(); m["dg"]["params"]["L_rho_y"] = 0.7
+    with pytest.raises(ValueError, match="PeriodicRelativeLength"):
+        F.init_params_full2d(m)
