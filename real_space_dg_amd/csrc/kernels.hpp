@@ -1023,25 +1023,23 @@ __global__ __launch_bounds__(NT) void arnoldi_dot_kernel(GmresBufs gb, const VP*
   block_reduce_store<NT>(acc, gb.part + blin * stride);
 }
 
-// Finalize of arnoldi_dot_kernel (one block): sums the per-block partials
-// of every group into h1 (+ nrm0) or h2.
+// Finalize of arnoldi_dot_kernel (one block per group of kDotGroup basis
+// vectors): sums the group's per-block partials into h1 (+ nrm0) or h2.
 template <int NT>
 __global__ __launch_bounds__(NT) void arnoldi_dot_finalize_kernel(GmresBufs gb, int nblocks, int nv, int mode) {
   if (gb.st->done || (mode == 1 && !gb.st->reorth)) return;
   const int stride = kDotGroup + 1;
   double2* out = (mode == 0) ? gb.h1 : gb.h2;
-  const int ngy = (nv + kDotGroup - 1) / kDotGroup;
-  for (int gy = 0; gy < ngy; ++gy) {
-    const int gi0 = gy * kDotGroup;
-    const int gn = min(kDotGroup, nv - gi0);
-    double2 res[kDotGroup + 1];
-    finalize_group<NT>(gb.part, (long)gy * nblocks, stride, nblocks, res);
-    if (threadIdx.x == 0) {
+  const int gy = blockIdx.x;
+  const int gi0 = gy * kDotGroup;
+  const int gn = min(kDotGroup, nv - gi0);
+  double2 res[kDotGroup + 1];
+  finalize_group<NT>(gb.part, (long)gy * nblocks, stride, nblocks, res);
+  if (threadIdx.x == 0) {
 #pragma unroll
-      for (int i = 0; i < kDotGroup; ++i)
-        if (i < gn) out[gi0 + i] = res[i];
-      if (mode == 0 && gy == 0) gb.st->nrm0 = sqrt(res[kDotGroup].x);
-    }
+    for (int i = 0; i < kDotGroup; ++i)
+      if (i < gn) out[gi0 + i] = res[i];
+    if (mode == 0 && gy == 0) gb.st->nrm0 = sqrt(res[kDotGroup].x);
   }
 }
 
@@ -1117,6 +1115,51 @@ __global__ __launch_bounds__(NT) void arnoldi_update_finalize_kernel(GmresBufs g
     if (threadIdx.x == 0) col[nv] = mk(nrm, 0);
     __syncthreads();
     givens_column(gb, j, col);
+  }
+}
+
+// Single-pass Gram-Schmidt: the finalize of arnoldi_update_kernel fused
+// with the normalisation.  Every block sums the per-block norm partials in
+// the same order, so all blocks hold the same ||w||; block 0 also finishes
+// the Hessenberg column with its Givens rotation (st->done becomes visible to
+// the kernels after this one), and every block scales its slice of w into
+// V_{j+1} (and vcur for a narrowed basis).
+template <typename T, typename VP, int NT>
+__global__ __launch_bounds__(NT) void arnoldi_finish_kernel(GmresBufs gb, int nblocks, int j,
+                                                            const typename Pk<T>::type* __restrict__ w,
+                                                            VP* __restrict__ vnext,
+                                                            typename Pk<T>::type* __restrict__ vcur, long NP) {
+  GmresState* st = gb.st;
+  if (st->done) return;
+  const int stride = kDotGroup + 1;
+  __shared__ double wsum[NT / 64];
+  __shared__ double2 col[kMaxRestart + 2];
+  double acc = 0.0;
+  for (int b = threadIdx.x; b < nblocks; b += NT) acc += gb.part[(long)b * stride].x;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+  if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  double tot = wsum[0];
+#pragma unroll
+  for (int i = 1; i < NT / 64; ++i) tot += wsum[i];
+  const double nrm = sqrt(tot);
+  if (blockIdx.x == 0) {
+    const int nv = j + 1;
+    if (threadIdx.x == 0) {
+      st->reorth = 0;
+      st->hnext = nrm;
+      col[nv] = mk(nrm, 0);
+    }
+    for (int i = threadIdx.x; i < nv; i += NT) col[i] = gb.h1[i];
+    __syncthreads();
+    givens_column(gb, j, col);
+  }
+  const double sc = (nrm > 0) ? 1.0 / nrm : 0.0;
+  for (long x = (long)blockIdx.x * NT + threadIdx.x; x < NP; x += (long)gridDim.x * NT) {
+    const typename Pk<T>::type v = Pk<T>::scale(w[x], sc);
+    vnext[x] = BasisIO<VP>::store(v);
+    if (vcur) vcur[x] = v;
   }
 }
 

@@ -16,9 +16,9 @@
 // theta_fft_kernel fuses, per element e (one workgroup):
 //   T_b = phi U_{e,b}            for the n nodes b    (n forward chirp-z)
 //   Z_a = sum_b Cbar_ab (*) T_b  tridiagonal in l     (the Theta combine)
-//   Y_{e,a} += i phi^H Z_a       for the n nodes a    (n backward chirp-z)
-// with T held in LDS, so HBM sees one read of u, one read-modify-write of y
-// and the edge table.  FFTs are Stockham autosort passes whose first and
+//   Y_{e,a} = (transport u)_{e,a} + i phi^H Z_a       (n backward chirp-z)
+// with T held in LDS, so HBM sees one read of u (the transport rows re-read
+// it from L2), one write of y and the edge table: the whole operator apply.  FFTs are Stockham autosort passes whose first and
 // last passes keep the data in registers (see fft_regs), so the chirp and
 // spectrum multiplies and the HBM traffic need no extra LDS round trips.
 #pragma once
@@ -281,18 +281,81 @@ __device__ __forceinline__ void fft_regs(T2 (&v)[8], LdsP<T2> buf0, LdsP<T2> buf
   cur = (X == buf0) ? 1 : 0;
 }
 
-// y += Theta u for the elements of the grid (one workgroup per element).
-// blockDim = G * TT (G groups of TT = P/8 threads, each group transforms one
-// DG node's vector at a time).  Dynamic LDS: n*n_xi (T, then Z), 2 G padded
+// Every node of an element is transformed in one round when the workgroup
+// holds at least n groups; T and Z (n*n_xi values) then share LDS with the
+// groups' work buffers, which are idle between the forward and backward
+// transforms.  Otherwise T/Z get their own region.
+__host__ __device__ constexpr bool theta_lds_alias(int n, int nx, int P, int G) {
+  return G >= n && (long)n * nx <= 2L * G * padded_len(P);
+}
+// LDS words (complex values) of theta_fft_kernel for G groups.
+__host__ __device__ constexpr long theta_lds_words(int n, int nx, int P, int G, int ntw) {
+  return 2L * G * padded_len(P) + (theta_lds_alias(n, nx, P, G) ? 0 : (long)n * nx) + ntw;
+}
+
+// Row a of the transport at wave-number index j of element e, with the
+// element's own values from LDS (us[b * nk + j]) and the upstream neighbour
+// node from u: the sum of transport_kernel for one output value (FP64
+// arithmetic).
+template <int NN, typename T2>
+__device__ __forceinline__ T2 transport_row(const OpConst* __restrict__ oc, const double* __restrict__ kvec,
+                                            LdsP<T2> us, const T2* __restrict__ u, int e, int a, int j) {
+  using S = decltype(T2{}.x);
+  const int nk = oc->nk, ne = oc->n_chi;
+  const double k = kvec[j], ak = fabs(k);
+  double sr = 0, si = 0;
+#pragma unroll
+  for (int b = 0; b < NN; ++b) {
+    const T2 v = lds_ld<T2>(us, b * nk + j);
+    const double c = k * oc->K1[a * kMaxN + b] + ak * oc->K2[a * kMaxN + b];
+    sr += c * v.x;
+    si += c * v.y;
+  }
+  if (k > 0 && e > 0) {               // -(k+|k|)/2 invM(:,0) u_{e-1,n-1}
+    const T2 v = u[((long)(e - 1) * NN + (NN - 1)) * nk + j];
+    const double cf = -0.5 * (k + ak) * oc->invM0[a];
+    sr += cf * v.x;
+    si += cf * v.y;
+  } else if (k < 0 && e < ne - 1) {   // (k-|k|)/2 invM(:,n-1) u_{e+1,0}
+    const T2 v = u[((long)(e + 1) * NN) * nk + j];
+    const double cf = 0.5 * (k - ak) * oc->invMn[a];
+    sr += cf * v.x;
+    si += cf * v.y;
+  }
+  T2 o;
+  o.x = (S)(oc->tq * sr);
+  o.y = (S)(oc->tq * si);
+  return o;
+}
+
+// An opaque copy of a pointer (uniform: SGPRs, per lane: VGPRs).  Loads
+// through the copy stay below this point: the scheduler would otherwise
+// hoist the independent table loads of a straight-line wide block above the
+// transforms and keep them live through them, past a 128-VGPR budget.
+template <typename Q>
+__device__ __forceinline__ Q* pin_uniform(Q* p) {
+  asm volatile("" : "+s"(p));
+  return p;
+}
+template <typename Q>
+__device__ __forceinline__ Q* pin_lane(Q* p) {
+  asm volatile("" : "+v"(p));
+  return p;
+}
+
+// y = (transport + Theta) u for the elements of the grid (one workgroup per
+// element).
+// blockDim = G * TT <= NT (G groups of TT = P/8 threads, each group
+// transforms one DG node's vector at a time).  Dynamic LDS: n*n_xi (T, then
+// Z; aliased with the work buffers when theta_lds_alias holds), 2 G padded
 // work buffers and the twiddle table, in complex values of type T2.
-template <int NN, typename T, typename TS>
-__global__ __launch_bounds__(kFftThreads) void theta_fft_kernel(const OpConst* __restrict__ oc, FftPlan pl,
-                                                                ThetaTabs<typename cvec<T>::type> tb,
-                                                                const TS* __restrict__ sbar,
-                                                                const double* __restrict__ cap,
-                                                                const typename cvec<T>::type* __restrict__ u,
-                                                                typename cvec<T>::type* __restrict__ y,
-                                                                const GmresState* st) {
+template <int NN, typename T, typename TS, int NT>
+__global__ __launch_bounds__(NT) void theta_fft_kernel(const OpConst* __restrict__ oc, FftPlan pl,
+                                                       ThetaTabs<typename cvec<T>::type> tb,
+                                                       const TS* __restrict__ sbar, const double* __restrict__ cap,
+                                                       const double* __restrict__ kvec,
+                                                       const typename cvec<T>::type* __restrict__ u,
+                                                       typename cvec<T>::type* __restrict__ y, const GmresState* st) {
   using T2 = typename cvec<T>::type;
   if (skip(st)) return;
   extern __shared__ __align__(16) unsigned char theta_smem[];
@@ -302,46 +365,77 @@ __global__ __launch_bounds__(kFftThreads) void theta_fft_kernel(const OpConst* _
   const int g = threadIdx.x / TT, t = threadIdx.x % TT;
   const int e = blockIdx.x;
   const int PL = padded_len(P);
+  const bool alias = theta_lds_alias(NN, nx, P, G);
   LdsP<T2> Tst = (LdsP<T2>)theta_smem;
-  LdsP<T2> buf0 = Tst + NN * nx + 2 * g * PL;
+  LdsP<T2> work = alias ? Tst : Tst + NN * nx;
+  LdsP<T2> buf0 = work + 2 * g * PL;
   LdsP<T2> buf1 = buf0 + PL;
-  LdsP<T2> twl = Tst + NN * nx + 2 * G * PL;
+  LdsP<T2> twl = work + 2 * G * PL;
   for (int i = threadIdx.x; i < pl.ntw; i += blockDim.x) lds_st<T2>(twl, i, tb.tw[i]);
   int cur = 0;
   T2 zero;
   zero.x = 0;
   zero.y = 0;
-  // this thread's table entries (elements t + TT r)
+  // this thread's table entries (elements t + TT r): held in registers by
+  // 256-thread blocks, which loop over the nodes; wide blocks (one round, at
+  // most 128 VGPRs) read them from the L2-resident tables where they are used
+  constexpr bool kPre = NT <= 256;
   T2 a_pre[8], g1[8], w_post[8];
+  if constexpr (kPre) {
 #pragma unroll
-  for (int r = 0; r < 8; ++r) {
-    const int i = t + TT * r;
-    a_pre[r] = (i < nk) ? tb.a_pre[i] : zero;
-    g1[r] = tb.g1[i];
-    w_post[r] = (i < nx) ? tb.w_post[i] : zero;
+    for (int r = 0; r < 8; ++r) {
+      const int i = t + TT * r;
+      a_pre[r] = (i < nk) ? tb.a_pre[i] : zero;
+      g1[r] = tb.g1[i];
+      w_post[r] = (i < nx) ? tb.w_post[i] : zero;
+    }
   }
   __syncthreads();
+
+  // u of the element, staged in the T region (n*n_xi = n*nk values, free
+  // until the forward transforms write T) by one coalesced sweep
+  {
+    const T2* __restrict__ ue = u + (long)e * NN * nk;
+    for (int i = threadIdx.x; i < NN * nk; i += blockDim.x) lds_st<T2>(Tst, i, ue[i]);
+  }
+  __syncthreads();
+
+  // transport rows first: Y_a = (transport u)_a, written by the thread that
+  // adds i phi^H Z_a to the same values below (no other thread reads them)
+  for (int a0 = 0; a0 < NN; a0 += G) {
+    const int a = a0 + g;
+    if (a < NN) {
+      T2* __restrict__ yb = y + ((long)e * NN + a) * nk;
+#pragma unroll
+      for (int r = 0; r < 8; ++r) {
+        const int j = t + TT * r;
+        if (j < nk) yb[j] = transport_row<NN>(oc, kvec, Tst, u, e, a, j);
+      }
+    }
+  }
 
   // forward: T_b = w o IFFT(G1 o FFT(a o U_b)) for b = g, g + G, ...
   for (int b0 = 0; b0 < NN; b0 += G) {
     const int b = b0 + g;
     const bool act = b < NN;
-    const T2* __restrict__ ub = u + ((long)e * NN + (act ? b : 0)) * nk;
     T2 v[8];
 #pragma unroll
     for (int r = 0; r < 8; ++r) {
       const int i = t + TT * r;
-      v[r] = (act && i < nk) ? c2mul(a_pre[r], ub[i]) : zero;
+      v[r] = (act && i < nk) ? c2mul(kPre ? a_pre[r] : tb.a_pre[i], lds_ld<T2>(Tst, b * nk + i)) : zero;
     }
+    if (alias) __syncthreads();   // u is read before the work buffers overwrite it
     fft_regs<false, T2>(v, buf0, buf1, cur, twl, pl, t, TT);
 #pragma unroll
-    for (int r = 0; r < 8; ++r) v[r] = c2mul(v[r], g1[r]);
+    for (int r = 0; r < 8; ++r) v[r] = c2mul(v[r], kPre ? g1[r] : pin_uniform(tb.g1)[t + TT * r]);
     fft_regs<true, T2>(v, buf0, buf1, cur, twl, pl, t, TT);
+    if (alias) __syncthreads();   // every group's last pass has read its buffer
+    const T2* wpp = kPre ? tb.w_post : pin_uniform(tb.w_post);
     if (act) {
 #pragma unroll
       for (int r = 0; r < 8; ++r) {
         const int i = t + TT * r;
-        if (i < nx) lds_st<T2>(Tst, b * nx + i, c2mul(w_post[r], v[r]));
+        if (i < nx) lds_st<T2>(Tst, b * nx + i, c2mul(kPre ? w_post[r] : wpp[i], v[r]));
       }
     }
   }
@@ -352,7 +446,9 @@ __global__ __launch_bounds__(kFftThreads) void theta_fft_kernel(const OpConst* _
   // edge sums A_b = T_b(l) + T_b(l-1), C_b = T_b(l) + T_b(l+1):
   //   Z_a = sum_b c_ab (S_ab(l) A_b + S_ab(l+1) C_b) - i sum_b c_ab Mq_ab (CAP(l) A_b + CAP(l+1) C_b).
   // All a at once per l (in the kernel's precision T), then Z overwrites T.
-  constexpr int kMaxL = 4;   // l values per thread (the host guarantees n_xi <= 4 blockDim)
+  // l values per thread: n_xi <= (P+1)/2 means n_xi <= 4 TT, and the host
+  // launches NT = 1024 only with G >= 4 groups
+  constexpr int kMaxL = NT >= 1024 ? 1 : 1024 / NT;
   T2 zreg[kMaxL][NN];
   const TS* sb = sbar + (long)e * np * nl;
 #pragma unroll
@@ -406,12 +502,14 @@ __global__ __launch_bounds__(kFftThreads) void theta_fft_kernel(const OpConst* _
   }
   // this thread's backward table entries
   T2 b_pre[8], g2[8], a_post[8];
+  if constexpr (kPre) {
 #pragma unroll
-  for (int r = 0; r < 8; ++r) {
-    const int i = t + TT * r;
-    b_pre[r] = (i < nx) ? tb.b_pre[i] : zero;
-    g2[r] = tb.g2[i];
-    a_post[r] = (i < nk) ? tb.a_post[i] : zero;
+    for (int r = 0; r < 8; ++r) {
+      const int i = t + TT * r;
+      b_pre[r] = (i < nx) ? tb.b_pre[i] : zero;
+      g2[r] = tb.g2[i];
+      a_post[r] = (i < nk) ? tb.a_post[i] : zero;
+    }
   }
   __syncthreads();
 
@@ -424,18 +522,21 @@ __global__ __launch_bounds__(kFftThreads) void theta_fft_kernel(const OpConst* _
 #pragma unroll
     for (int r = 0; r < 8; ++r) {
       const int i = t + TT * r;
-      v[r] = (act && i < nx) ? c2mul(b_pre[r], lds_ld<T2>(Tst, a * nx + i)) : zero;
-      yo[r] = (act && i < nk) ? yb[i] : zero;
+      v[r] = (act && i < nx) ? c2mul(kPre ? b_pre[r] : tb.b_pre[i], lds_ld<T2>(Tst, a * nx + i)) : zero;
+      if constexpr (kPre) yo[r] = (act && i < nk) ? yb[i] : zero;   // early, to hide its latency
     }
+    if (alias) __syncthreads();   // Z is read before the work buffers overwrite it
     fft_regs<false, T2>(v, buf0, buf1, cur, twl, pl, t, TT);
 #pragma unroll
-    for (int r = 0; r < 8; ++r) v[r] = c2mul(v[r], g2[r]);
+    for (int r = 0; r < 8; ++r) v[r] = c2mul(v[r], kPre ? g2[r] : pin_uniform(tb.g2)[t + TT * r]);
     fft_regs<true, T2>(v, buf0, buf1, cur, twl, pl, t, TT);
+    const T2* app = kPre ? tb.a_post : pin_uniform(tb.a_post);
+    if constexpr (!kPre) yb = pin_lane(yb);
     if (act) {
 #pragma unroll
       for (int r = 0; r < 8; ++r) {
         const int j = t + TT * r;
-        if (j < nk) yb[j] = c2add(yo[r], c2mul(a_post[r], v[r]));
+        if (j < nk) yb[j] = c2add(kPre ? yo[r] : yb[j], c2mul(kPre ? a_post[r] : app[j], v[r]));
       }
     }
   }

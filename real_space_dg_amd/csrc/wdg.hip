@@ -112,7 +112,8 @@ struct wdg_ctx {
   ThetaTabs<double2> tt64{};
   ThetaTabs<float2> tt32{};
   size_t lds64 = 0, lds32 = 0, lds_phi = 0;
-  int fft_block = 0, fft_groups = 0;  // threads per workgroup, transforms per workgroup
+  int fft_block = 0, fft_groups = 0;  // phi_fft_kernel: threads per workgroup, transforms per workgroup
+  int theta_block64 = 0, theta_block32 = 0;   // theta_fft_kernel threads per workgroup (256 or 1024)
   DevBuf<double2> d_b, d_x, d_r, d_y; // FP64 solve vectors (normalised units)
   DevBuf<double2> d_u64, d_v64;       // scratch for host-buffer entry points
   DevBuf<float2> d_u32, d_v32;
@@ -396,17 +397,20 @@ template <typename T>
 void op_theta_fft(wdg_ctx* c, const typename cvec<T>::type* u, typename cvec<T>::type* y, const GmresState* st) {
   constexpr bool f64 = std::is_same<T, double>::value;
   const size_t s = sizeof(typename cvec<T>::type), ts = f64 ? 8 : 4;
-  // algorithmic HBM bytes: read u, read and write y, read the edge table
-  ProfScope ps(c, P_THETA_FFT, 3.0 * c->N * s + (double)c->g.n_chi * c->oc.n_pairs * (c->g.n_xi + 1) * ts);
-  const dim3 grid(c->g.n_chi), block(c->fft_block);
+  // algorithmic HBM bytes: read u, write y, read the edge table
+  ProfScope ps(c, P_THETA_FFT, 2.0 * c->N * s + (double)c->g.n_chi * c->oc.n_pairs * (c->g.n_xi + 1) * ts);
+  const dim3 grid(c->g.n_chi), block(f64 ? c->theta_block64 : c->theta_block32);
   const size_t lds = f64 ? c->lds64 : c->lds32;
-#define WDG_THETA(NN)                                                                                         \
-  if constexpr (f64)                                                                                          \
-    hipLaunchKernelGGL((theta_fft_kernel<NN, double, double>), grid, block, lds, c->stream, c->d_oc.p, c->fplan, \
-                       c->tt64, c->d_sbar64.p, c->d_cap.p, u, y, st);                                          \
-  else                                                                                                        \
-    hipLaunchKernelGGL((theta_fft_kernel<NN, float, float>), grid, block, lds, c->stream, c->d_oc.p, c->fplan,  \
-                       c->tt32, c->d_sbar32.p, c->d_cap.p, u, y, st);                                          \
+#define WDG_THETA(NN)                                                                                              \
+  if constexpr (f64)                                                                                               \
+    hipLaunchKernelGGL((theta_fft_kernel<NN, double, double, 256>), grid, block, lds, c->stream, c->d_oc.p,         \
+                       c->fplan, c->tt64, c->d_sbar64.p, c->d_cap.p, c->d_k.p, u, y, st);                                     \
+  else if (c->theta_block32 > kFftThreads)                                                                         \
+    hipLaunchKernelGGL((theta_fft_kernel<NN, float, float, 1024>), grid, block, lds, c->stream, c->d_oc.p,          \
+                       c->fplan, c->tt32, c->d_sbar32.p, c->d_cap.p, c->d_k.p, u, y, st);                                     \
+  else                                                                                                             \
+    hipLaunchKernelGGL((theta_fft_kernel<NN, float, float, 256>), grid, block, lds, c->stream, c->d_oc.p,           \
+                       c->fplan, c->tt32, c->d_sbar32.p, c->d_cap.p, c->d_k.p, u, y, st);                                     \
   break;
   switch (c->g.n) {
     case 2: WDG_THETA(2)
@@ -426,10 +430,25 @@ void op_theta_fft(wdg_ctx* c, const typename cvec<T>::type* u, typename cvec<T>:
 // the kernel).
 template <int NN>
 void theta_fft_attrs() {
-  HIPCHK(hipFuncSetAttribute((const void*)theta_fft_kernel<NN, double, double>,
+  HIPCHK(hipFuncSetAttribute((const void*)theta_fft_kernel<NN, double, double, 256>,
                              hipFuncAttributeMaxDynamicSharedMemorySize, kMaxLdsBytes));
-  HIPCHK(hipFuncSetAttribute((const void*)theta_fft_kernel<NN, float, float>,
+  HIPCHK(hipFuncSetAttribute((const void*)theta_fft_kernel<NN, float, float, 256>,
                              hipFuncAttributeMaxDynamicSharedMemorySize, kMaxLdsBytes));
+  HIPCHK(hipFuncSetAttribute((const void*)theta_fft_kernel<NN, float, float, 1024>,
+                             hipFuncAttributeMaxDynamicSharedMemorySize, kMaxLdsBytes));
+}
+
+// Resident workgroups per CU of the FP32 Theta kernel at the 256-thread and
+// the wide block (0 where the calculator fails).
+template <int NN>
+void theta32_occupancy(int block, size_t lds, int wide, size_t lds_wide, int& nb_base, int& nb_wide) {
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb_base, (const void*)theta_fft_kernel<NN, float, float, 256>,
+                                                   block, lds) != hipSuccess)
+    nb_base = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb_wide, (const void*)theta_fft_kernel<NN, float, float, 1024>,
+                                                   wide, lds_wide) != hipSuccess)
+    nb_wide = 0;
+  (void)hipGetLastError();
 }
 
 template <typename T>
@@ -450,11 +469,11 @@ void op_apply_base(wdg_ctx* c, const typename cvec<T>::type* u, typename cvec<T>
     f2_apply<T>(c, u, y, st);
     return;
   }
-  op_transport<T>(c, u, y, st);
-  if (c->use_fft) {
+  if (c->use_fft) {   // the chirp-z kernel writes transport + Theta
     op_theta_fft<T>(c, u, y, st);
     return;
   }
+  op_transport<T>(c, u, y, st);
   const int M1 = 2 * c->g.n_xi, K1 = 2 * c->g.nk;
   const long work = (long)c->g.n_chi * c->g.n_xi;
   if constexpr (std::is_same<T, double>::value) {
@@ -617,20 +636,23 @@ void gmres_cycle(wdg_ctx* c, int m, bool precond, bool single_pass) {
       ProfScope ps(c, P_DOT, nv * vbytes + N * sizeof(T2));
       hipLaunchKernelGGL((arnoldi_dot_kernel<T, VP, 256>), dg, dim3(256), 0, c->stream, gb, V, wp, NP, nv, 0);
     }
-    hipLaunchKernelGGL((arnoldi_dot_finalize_kernel<256>), dim3(1), dim3(256), 0, c->stream, gb, nb, nv, 0);
+    hipLaunchKernelGGL((arnoldi_dot_finalize_kernel<256>), dim3(dg.y), dim3(256), 0, c->stream, gb, nb, nv, 0);
     {
       ProfScope ps(c, P_UPDATE, nv * vbytes + 2.0 * N * sizeof(T2));
       hipLaunchKernelGGL((arnoldi_update_kernel<T, VP, 256>), dim3(nb), dim3(256), 0, c->stream, gb, V, wp, NP, j,
                          0);
     }
-    hipLaunchKernelGGL((arnoldi_update_finalize_kernel<256>), dim3(1), dim3(256), 0, c->stream, gb, nb, j, 0);
-    if (!single_pass) {  // DGKS second pass, taken on the device only when requested (st->reorth)
-      hipLaunchKernelGGL((arnoldi_dot_kernel<T, VP, 256>), dg, dim3(256), 0, c->stream, gb, V, wp, NP, nv, 1);
-      hipLaunchKernelGGL((arnoldi_dot_finalize_kernel<256>), dim3(1), dim3(256), 0, c->stream, gb, nb, nv, 1);
-      hipLaunchKernelGGL((arnoldi_update_kernel<T, VP, 256>), dim3(nb), dim3(256), 0, c->stream, gb, V, wp, NP, j,
-                         1);
-      hipLaunchKernelGGL((arnoldi_update_finalize_kernel<256>), dim3(1), dim3(256), 0, c->stream, gb, nb, j, 1);
+    if (single_pass) {
+      hipLaunchKernelGGL((arnoldi_finish_kernel<T, VP, 256>), dim3(nb), dim3(256), 0, c->stream, gb, nb, j, wp,
+                         V + (size_t)(j + 1) * NP, vcp, NP);
+      continue;
     }
+    hipLaunchKernelGGL((arnoldi_update_finalize_kernel<256>), dim3(1), dim3(256), 0, c->stream, gb, nb, j, 0);
+    // DGKS second pass, taken on the device only when requested (st->reorth)
+    hipLaunchKernelGGL((arnoldi_dot_kernel<T, VP, 256>), dg, dim3(256), 0, c->stream, gb, V, wp, NP, nv, 1);
+    hipLaunchKernelGGL((arnoldi_dot_finalize_kernel<256>), dim3(dg.y), dim3(256), 0, c->stream, gb, nb, nv, 1);
+    hipLaunchKernelGGL((arnoldi_update_kernel<T, VP, 256>), dim3(nb), dim3(256), 0, c->stream, gb, V, wp, NP, j, 1);
+    hipLaunchKernelGGL((arnoldi_update_finalize_kernel<256>), dim3(1), dim3(256), 0, c->stream, gb, nb, j, 1);
     hipLaunchKernelGGL((arnoldi_normalize_kernel<T, VP>), dim3(c->red_blocks), dim3(256), 0, c->stream, gb, wp,
                        V + (size_t)(j + 1) * NP, vcp, NP);
   }
@@ -746,13 +768,14 @@ void setup_theta(wdg_ctx* c) {
   c->fft_ok = false;
   if (cp.ok && (int)cp.rad.size() <= kFftMaxPasses) {
     const int tt = cp.P / 8;
+    const int ntw = (int)cp.tw.size();
     c->fft_groups = std::max(1, kFftThreads / tt);
     c->fft_block = tt * c->fft_groups;
-    const size_t words =
-        (size_t)g.n * g.n_xi + 2 * (size_t)c->fft_groups * padded_len(cp.P) + cp.tw.size();
-    c->lds64 = words * sizeof(double2);
-    c->lds32 = words * sizeof(float2);
     c->lds_phi = (2 * (size_t)c->fft_groups * padded_len(cp.P) + cp.tw.size()) * sizeof(double2);
+    c->theta_block64 = c->fft_block;
+    c->lds64 = (size_t)theta_lds_words(g.n, g.n_xi, cp.P, c->fft_groups, ntw) * sizeof(double2);
+    c->theta_block32 = c->fft_block;
+    c->lds32 = (size_t)theta_lds_words(g.n, g.n_xi, cp.P, c->fft_groups, ntw) * sizeof(float2);
     c->fft_ok = tt <= kFftThreads && (cp.P % 64) == 0 && c->lds64 <= (size_t)kMaxLdsBytes;
   }
   if (c->fft_ok) {
@@ -798,6 +821,26 @@ void setup_theta(wdg_ctx* c) {
       case 7: theta_fft_attrs<7>(); break;
       case 8: theta_fft_attrs<8>(); break;
       default: break;
+    }
+    // FP32 Theta: 256 threads, or a wide block of ~1024 threads (four or
+    // more transforms in flight), whichever keeps more waves resident per CU
+    // (LDS and VGPRs both counted by the occupancy calculator).  n <= 5: the
+    // wide kernel's combine of more nodes would spill past 128 VGPRs.
+    {
+      const int tt = cp.P / 8, wide = tt * (1024 / tt);
+      const size_t lw = (size_t)theta_lds_words(g.n, g.n_xi, cp.P, wide / tt, (int)cp.tw.size()) * sizeof(float2);
+      int nb_base = 0, nb_wide = 0;
+      switch (g.n) {
+        case 2: theta32_occupancy<2>(c->fft_block, c->lds32, wide, lw, nb_base, nb_wide); break;
+        case 3: theta32_occupancy<3>(c->fft_block, c->lds32, wide, lw, nb_base, nb_wide); break;
+        case 4: theta32_occupancy<4>(c->fft_block, c->lds32, wide, lw, nb_base, nb_wide); break;
+        case 5: theta32_occupancy<5>(c->fft_block, c->lds32, wide, lw, nb_base, nb_wide); break;
+        default: break;
+      }
+      if (lw <= (size_t)kMaxLdsBytes && nb_wide * ((wide + 63) / 64) > nb_base * ((c->fft_block + 63) / 64)) {
+        c->theta_block32 = wide;
+        c->lds32 = lw;
+      }
     }
     HIPCHK(hipFuncSetAttribute((const void*)phi_fft_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                kMaxLdsBytes));

@@ -6,7 +6,7 @@ from tests._cases import build_case, gpu_operator, line_operator, oracle_system,
 
 pytestmark = pytest.mark.gpu
 
-CASES = [(16, 2, 64, 0.0), (8, 3, 64, 0.2), (12, 1, 96, 0.1)]
+CASES = [(16, 2, 64, 0.0), (8, 3, 64, 0.2), (12, 1, 96, 0.1), (6, 5, 64, 0.1)]
 
 
 @pytest.fixture(scope="module", params=CASES, ids=lambda c: "ne%d_p%d_nk%d_b%.1f" % c)
