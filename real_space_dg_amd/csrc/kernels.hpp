@@ -767,6 +767,132 @@ __global__ __launch_bounds__(256) void line_apply_kernel(const OpConst* __restri
 }
 
 // ---------------------------------------------------------------------------
+// Line preconditioner of the FP32 solver from stored block inverses.  Each
+// (element, k) block D and the coupling column c are fixed for a potential,
+// so line_inverse_kernel factors every block once (FP64 Gaussian elimination
+// with partial pivoting, as in the sweeps above) and stores D^-1 and D^-1 c
+// in FP32 as rows of nk values: linv[(e * (n*n + n) + q) * nk + j] with
+// q = a*n + b for (D^-1)_ab and q = n*n + a for (D^-1 c)_a.  The sweeps then
+// compose s_out = (D^-1 r)_out - (D^-1 c)_out s_in and z = D^-1 r - (D^-1 c) s_in.
+// ---------------------------------------------------------------------------
+template <int NN>
+__global__ __launch_bounds__(256) void line_inverse_kernel(const OpConst* __restrict__ oc,
+                                                           const double* __restrict__ kvec,
+                                                           const double2* __restrict__ s0s1,
+                                                           float2* __restrict__ linv) {
+  const int nk = oc->nk, ne = oc->n_chi;
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)ne * nk) return;
+  const int e = (int)(t / nk), j = (int)(t % nk);
+  const double k = kvec[j];
+  Aug<NN, NN + 1> B;
+  build_aug<NN, NN + 1>(oc, s0s1, e, k, cos(k * oc->hprime), B);
+#pragma unroll
+  for (int a = 0; a < NN; ++a) {
+#pragma unroll
+    for (int q = 0; q < NN; ++q) {
+      B.r[a][NN + q] = (a == q) ? 1.0 : 0.0;
+      B.i[a][NN + q] = 0.0;
+    }
+    B.r[a][2 * NN] = line_coupling(oc, a, k);
+    B.i[a][2 * NN] = 0.0;
+  }
+  gauss_solve<NN, NN + 1>(B);
+  float2* out = linv + (long)e * (NN * NN + NN) * nk + j;
+#pragma unroll
+  for (int a = 0; a < NN; ++a) {
+#pragma unroll
+    for (int b = 0; b < NN; ++b)
+      out[(long)(a * NN + b) * nk] = make_float2((float)B.r[a][NN + b], (float)B.i[a][NN + b]);
+    out[(long)(NN * NN + a) * nk] = make_float2((float)B.r[a][2 * NN], (float)B.i[a][2 * NN]);
+  }
+}
+
+// line_aggregate_kernel from the stored inverses (FP32 products, the chunk
+// map composed in FP64).
+template <int NN>
+__global__ __launch_bounds__(256) void line_aggregate_inv_kernel(const OpConst* __restrict__ oc,
+                                                                 const double* __restrict__ kvec,
+                                                                 const float2* __restrict__ linv,
+                                                                 const float2* __restrict__ r,
+                                                                 double2* __restrict__ agg, const GmresState* st) {
+  if (skip(st)) return;
+  const int nk = oc->nk, ne = oc->n_chi, S = oc->chunk, nc = oc->n_chunks;
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)nc * nk) return;
+  const int c = (int)(t / nk), j = (int)(t % nk);
+  const bool fwd = kvec[j] > 0;
+  const int out = fwd ? NN - 1 : 0;
+  const int e_lo = c * S, e_hi = min(ne, e_lo + S);
+  double Ar = 0, Ai = 0, Br = 1, Bi = 0;
+  for (int step = 0; step < e_hi - e_lo; ++step) {
+    const int e = fwd ? e_lo + step : e_hi - 1 - step;
+    const float2* li = linv + (long)e * (NN * NN + NN) * nk + j;
+    const float2* re = r + (long)e * NN * nk + j;
+    float al_r = 0, al_i = 0;
+#pragma unroll
+    for (int b = 0; b < NN; ++b) {
+      const float2 m = li[(long)(out * NN + b) * nk], v = re[(long)b * nk];
+      al_r += m.x * v.x - m.y * v.y;
+      al_i += m.x * v.y + m.y * v.x;
+    }
+    const float2 be = li[(long)(NN * NN + out) * nk];
+    // s_out = alpha - beta * s_in, composed with the running map s_in = A + B s0
+    const double nAr = al_r - (be.x * Ar - be.y * Ai), nAi = al_i - (be.x * Ai + be.y * Ar);
+    const double nBr = -(be.x * Br - be.y * Bi), nBi = -(be.x * Bi + be.y * Br);
+    Ar = nAr; Ai = nAi; Br = nBr; Bi = nBi;
+  }
+  agg[((long)c * nk + j) * 2 + 0] = mk(Ar, Ai);
+  agg[((long)c * nk + j) * 2 + 1] = mk(Br, Bi);
+}
+
+// line_apply_kernel from the stored inverses: z_e = D^-1 r_e - (D^-1 c) s_in.
+template <int NN>
+__global__ __launch_bounds__(256) void line_apply_inv_kernel(const OpConst* __restrict__ oc,
+                                                             const double* __restrict__ kvec,
+                                                             const float2* __restrict__ linv,
+                                                             const double2* __restrict__ agg,
+                                                             const float2* __restrict__ r, float2* __restrict__ z,
+                                                             const GmresState* st) {
+  if (skip(st)) return;
+  const int nk = oc->nk, ne = oc->n_chi, S = oc->chunk, nc = oc->n_chunks;
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)nc * nk) return;
+  const int c = (int)(t / nk), j = (int)(t % nk);
+  const bool fwd = kvec[j] > 0;
+  const int out = fwd ? NN - 1 : 0;
+  const int e_lo = c * S, e_hi = min(ne, e_lo + S);
+  const double2 s_in = agg[((long)c * nk + j) * 2 + 0];
+  float sr = (float)s_in.x, si = (float)s_in.y;
+  for (int step = 0; step < e_hi - e_lo; ++step) {
+    const int e = fwd ? e_lo + step : e_hi - 1 - step;
+    const float2* li = linv + (long)e * (NN * NN + NN) * nk + j;
+    const float2* re = r + (long)e * NN * nk + j;
+    float2 v[NN];
+#pragma unroll
+    for (int b = 0; b < NN; ++b) v[b] = re[(long)b * nk];
+    float2 zo[NN];
+#pragma unroll
+    for (int a = 0; a < NN; ++a) {
+      const float2 bc = li[(long)(NN * NN + a) * nk];
+      float zr = -(bc.x * sr - bc.y * si), zi = -(bc.x * si + bc.y * sr);
+#pragma unroll
+      for (int b = 0; b < NN; ++b) {
+        const float2 m = li[(long)(a * NN + b) * nk];
+        zr += m.x * v[b].x - m.y * v[b].y;
+        zi += m.x * v[b].y + m.y * v[b].x;
+      }
+      zo[a] = make_float2(zr, zi);
+    }
+#pragma unroll
+    for (int a = 0; a < NN; ++a) {
+      z[((long)e * NN + a) * nk + j] = zo[a];
+      if (a == out) { sr = zo[a].x; si = zo[a].y; }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // GMRES device state and reductions.  Reductions write per-block partials
 // that a single-block finalize kernel sums in block order, so results are
 // deterministic and no device-scope fence (an L2 write-back per block on a
@@ -908,6 +1034,8 @@ template <typename T> struct Pk;
 template <> struct Pk<double> {
   using type = double2;
   static constexpr int per = 1;
+  __device__ static __forceinline__ void dot_acc(double& ar, double& ai, type v, type w) { dot(ar, ai, v, w); }
+  __device__ static __forceinline__ double nrm2_w(type w) { return w.x * w.x + w.y * w.y; }
   __device__ static __forceinline__ void dot(double& ar, double& ai, type v, type w) {
     ar += v.x * w.x + v.y * w.y;
     ai += v.x * w.y - v.y * w.x;
@@ -922,6 +1050,11 @@ template <> struct Pk<double> {
 template <> struct Pk<float> {
   using type = float4;
   static constexpr int per = 2;
+  __device__ static __forceinline__ void dot_acc(float& ar, float& ai, type v, type w) {
+    ar += v.x * w.x + v.y * w.y + v.z * w.z + v.w * w.w;
+    ai += v.x * w.y - v.y * w.x + v.z * w.w - v.w * w.z;
+  }
+  __device__ static __forceinline__ float nrm2_w(type w) { return w.x * w.x + w.y * w.y + w.z * w.z + w.w * w.w; }
   __device__ static __forceinline__ void dot(double& ar, double& ai, type v, type w) {
     ar += (double)v.x * w.x + (double)v.y * w.y + (double)v.z * w.z + (double)v.w * w.w;
     ai += (double)v.x * w.y - (double)v.y * w.x + (double)v.z * w.w - (double)v.w * w.z;
@@ -956,12 +1089,16 @@ __device__ __forceinline__ uint32_t float_to_bf16_bits(float f) {
 template <typename VP>
 struct BasisIO {
   using work = VP;
+  static constexpr double raw_scale = 1.0;   // load() == raw_scale * load_raw()
   __device__ static __forceinline__ work load(VP v) { return v; }
+  __device__ static __forceinline__ work load_raw(VP v) { return v; }
   __device__ static __forceinline__ VP store(work v) { return v; }
 };
 template <>
 struct BasisIO<uint2> {
   using work = float4;
+  static constexpr double raw_scale = 1.0;
+  __device__ static __forceinline__ float4 load_raw(uint2 p) { return load(p); }
   __device__ static __forceinline__ float4 load(uint2 p) {
     return make_float4(bf16_bits_to_float(p.x & 0xFFFFu), bf16_bits_to_float(p.x >> 16),
                        bf16_bits_to_float(p.y & 0xFFFFu), bf16_bits_to_float(p.y >> 16));
@@ -979,6 +1116,13 @@ struct BasisIO<uint2> {
 template <>
 struct BasisIO<ushort4> {
   using work = float4;
+  static constexpr double raw_scale = 1.0 / 256.0;
+  // the stored values times 256, unscaled: the multi-dot and the update fold
+  // the 1/256 into their sums and coefficients
+  __device__ static __forceinline__ float4 load_raw(ushort4 p) {
+    return make_float4((float)__builtin_bit_cast(_Float16, p.x), (float)__builtin_bit_cast(_Float16, p.y),
+                       (float)__builtin_bit_cast(_Float16, p.z), (float)__builtin_bit_cast(_Float16, p.w));
+  }
   __device__ static __forceinline__ float4 load(ushort4 p) {
     const float s = 1.0f / 256.0f;
     return make_float4((float)__builtin_bit_cast(_Float16, p.x) * s, (float)__builtin_bit_cast(_Float16, p.y) * s,
@@ -1007,17 +1151,23 @@ __global__ __launch_bounds__(NT) void arnoldi_dot_kernel(GmresBufs gb, const VP*
   if (gb.st->done || (mode == 1 && !gb.st->reorth)) return;
   const int i0 = blockIdx.y * kDotGroup;
   const int ng = min(kDotGroup, nv - i0);
-  double2 acc[kDotGroup + 1];
+  // per-thread sums in the working precision (a thread adds a few dozen
+  // products), the block and grid sums in FP64
+  T ar[kDotGroup + 1], ai[kDotGroup + 1];
 #pragma unroll
-  for (int i = 0; i < kDotGroup + 1; ++i) acc[i] = mk(0, 0);
+  for (int i = 0; i < kDotGroup + 1; ++i) ar[i] = ai[i] = 0;
   const VP* V0 = V + (long)i0 * NP;
   for (long x = (long)blockIdx.x * NT + threadIdx.x; x < NP; x += (long)gridDim.x * NT) {
     const typename P::type wv = w[x];
 #pragma unroll
     for (int i = 0; i < kDotGroup; ++i)
-      if (i < ng) P::dot(acc[i].x, acc[i].y, IO::load(V0[(long)i * NP + x]), wv);
-    acc[kDotGroup].x += P::nrm2(wv);
+      if (i < ng) P::dot_acc(ar[i], ai[i], IO::load_raw(V0[(long)i * NP + x]), wv);
+    ar[kDotGroup] += P::nrm2_w(wv);
   }
+  double2 acc[kDotGroup + 1];
+#pragma unroll
+  for (int i = 0; i < kDotGroup; ++i) acc[i] = mk(IO::raw_scale * ar[i], IO::raw_scale * ai[i]);
+  acc[kDotGroup] = mk(ar[kDotGroup], 0);
   const int stride = kDotGroup + 1;
   const long blin = (long)blockIdx.y * gridDim.x + blockIdx.x;
   block_reduce_store<NT>(acc, gb.part + blin * stride);
@@ -1057,7 +1207,7 @@ __global__ __launch_bounds__(NT) void arnoldi_update_kernel(GmresBufs gb, const 
   const int nv = j + 1;
   __shared__ double2 hs[kMaxRestart + 1];
   const double2* h = (mode == 0) ? gb.h1 : gb.h2;
-  for (int i = threadIdx.x; i < nv; i += NT) hs[i] = h[i];
+  for (int i = threadIdx.x; i < nv; i += NT) hs[i] = mk(IO::raw_scale * h[i].x, IO::raw_scale * h[i].y);
   __syncthreads();
   double2 acc[kDotGroup + 1];
 #pragma unroll
@@ -1066,16 +1216,16 @@ __global__ __launch_bounds__(NT) void arnoldi_update_kernel(GmresBufs gb, const 
     typename P::type wv = w[x];
     int i = 0;
     for (; i + 4 <= nv; i += 4) {
-      const typename P::type v0 = IO::load(V[(long)(i + 0) * NP + x]);
-      const typename P::type v1 = IO::load(V[(long)(i + 1) * NP + x]);
-      const typename P::type v2 = IO::load(V[(long)(i + 2) * NP + x]);
-      const typename P::type v3 = IO::load(V[(long)(i + 3) * NP + x]);
+      const typename P::type v0 = IO::load_raw(V[(long)(i + 0) * NP + x]);
+      const typename P::type v1 = IO::load_raw(V[(long)(i + 1) * NP + x]);
+      const typename P::type v2 = IO::load_raw(V[(long)(i + 2) * NP + x]);
+      const typename P::type v3 = IO::load_raw(V[(long)(i + 3) * NP + x]);
       P::axpy(wv, hs[i + 0], v0);
       P::axpy(wv, hs[i + 1], v1);
       P::axpy(wv, hs[i + 2], v2);
       P::axpy(wv, hs[i + 3], v3);
     }
-    for (; i < nv; ++i) P::axpy(wv, hs[i], IO::load(V[(long)i * NP + x]));
+    for (; i < nv; ++i) P::axpy(wv, hs[i], IO::load_raw(V[(long)i * NP + x]));
     w[x] = wv;
     acc[0].x += P::nrm2(wv);
   }

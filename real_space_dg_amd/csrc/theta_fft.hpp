@@ -295,36 +295,37 @@ __host__ __device__ constexpr long theta_lds_words(int n, int nx, int P, int G, 
 
 // Row a of the transport at wave-number index j of element e, with the
 // element's own values from LDS (us[b * nk + j]) and the upstream neighbour
-// node from u: the sum of transport_kernel for one output value (FP64
-// arithmetic).
+// node from u: the sum of transport_kernel for one output value, in the
+// kernel's precision.
 template <int NN, typename T2>
 __device__ __forceinline__ T2 transport_row(const OpConst* __restrict__ oc, const double* __restrict__ kvec,
                                             LdsP<T2> us, const T2* __restrict__ u, int e, int a, int j) {
   using S = decltype(T2{}.x);
   const int nk = oc->nk, ne = oc->n_chi;
-  const double k = kvec[j], ak = fabs(k);
-  double sr = 0, si = 0;
+  const S k = (S)kvec[j], ak = k < 0 ? -k : k;
+  S sr = 0, si = 0;
 #pragma unroll
   for (int b = 0; b < NN; ++b) {
     const T2 v = lds_ld<T2>(us, b * nk + j);
-    const double c = k * oc->K1[a * kMaxN + b] + ak * oc->K2[a * kMaxN + b];
+    const S c = k * (S)oc->K1[a * kMaxN + b] + ak * (S)oc->K2[a * kMaxN + b];
     sr += c * v.x;
     si += c * v.y;
   }
   if (k > 0 && e > 0) {               // -(k+|k|)/2 invM(:,0) u_{e-1,n-1}
     const T2 v = u[((long)(e - 1) * NN + (NN - 1)) * nk + j];
-    const double cf = -0.5 * (k + ak) * oc->invM0[a];
+    const S cf = (S)-0.5 * (k + ak) * (S)oc->invM0[a];
     sr += cf * v.x;
     si += cf * v.y;
   } else if (k < 0 && e < ne - 1) {   // (k-|k|)/2 invM(:,n-1) u_{e+1,0}
     const T2 v = u[((long)(e + 1) * NN) * nk + j];
-    const double cf = 0.5 * (k - ak) * oc->invMn[a];
+    const S cf = (S)0.5 * (k - ak) * (S)oc->invMn[a];
     sr += cf * v.x;
     si += cf * v.y;
   }
+  const S tq = (S)oc->tq;
   T2 o;
-  o.x = (S)(oc->tq * sr);
-  o.y = (S)(oc->tq * si);
+  o.x = tq * sr;
+  o.y = tq * si;
   return o;
 }
 
@@ -343,13 +344,137 @@ __device__ __forceinline__ Q* pin_lane(Q* p) {
   return p;
 }
 
+// ---------------------------------------------------------------------------
+// The same transform with its length P fixed at compile time (P a power of
+// two, 128 <= P <= 2048): the plan (radix 8 first and last, the radices of
+// P/64 between, as the host builds it) is unrolled, so every pass's LDS
+// index arithmetic folds into per-thread bases plus immediate offsets.
+// ---------------------------------------------------------------------------
+__host__ __device__ constexpr int fftc_npass(int P) {
+  int m = P / 64, n = 0;
+  while (m % 8 == 0) { m /= 8; ++n; }
+  while (m % 4 == 0) { m /= 4; ++n; }
+  while (m % 2 == 0) { m /= 2; ++n; }
+  return (P == 64) ? 2 : n + 2;
+}
+__host__ __device__ constexpr int fftc_rad(int P, int p) {
+  if (p == 0 || p == fftc_npass(P) - 1) return 8;
+  int m = P / 64, i = 1;
+  while (m % 8 == 0) { if (i == p) return 8; m /= 8; ++i; }
+  while (m % 4 == 0) { if (i == p) return 4; m /= 4; ++i; }
+  while (m % 2 == 0) { if (i == p) return 2; m /= 2; ++i; }
+  return 0;
+}
+__host__ __device__ constexpr int fftc_ns(int P, int p) {   // product of the radices before pass p
+  int ns = 1;
+  for (int i = 0; i < p; ++i) ns *= fftc_rad(P, i);
+  return ns;
+}
+__host__ __device__ constexpr int fftc_toff(int P, int p) {   // twiddle-table offset of pass p
+  int o = 0;
+  for (int i = 0; i < p; ++i) o += fftc_ns(P, i);
+  return o;
+}
+// padded index of i + k for a compile-time k that is a multiple of 16
+template <int K>
+__device__ __forceinline__ int pad_plus(int padded_i) {
+  static_assert(K % 16 == 0, "offset must keep the low four bits");
+  return padded_i + K + K / 16;
+}
+
+template <int P, int p, bool INV, typename T2>
+__device__ __forceinline__ void fftc_mid_pass(LdsP<T2> X, LdsP<T2> Y, LdsP<T2> tw, int t) {
+  constexpr int R = fftc_rad(P, p), Ns = fftc_ns(P, p), nb = P / R, TT = P / 8;
+  static_assert(nb % TT == 0, "whole butterflies per thread");
+  static_assert(nb % 16 == 0, "strided loads keep the padding linear");
+#pragma unroll
+  for (int it = 0; it < nb / TT; ++it) {
+    const int b = t + it * TT;
+    const int q = b / Ns, jm = b % Ns;   // Ns a power of two: shift and mask
+    const int lb = lds_pad(b);
+    T2 v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[r] = lds_ld<T2>(X, lb + r * (nb + nb / 16));
+    T2 w = lds_ld<T2>(tw, fftc_toff(P, p) + jm);
+    if (INV) w.y = -w.y;
+    T2 wr = w;
+    v[1] = c2mul(v[1], w);
+#pragma unroll
+    for (int r = 2; r < R; ++r) {
+      wr = c2mul(wr, w);
+      v[r] = c2mul(v[r], wr);
+    }
+    SmallDft<R, INV, T2>::run(v);
+    const int base = q * Ns * R + jm;
+    if constexpr (Ns % 16 == 0) {
+      const int sb = lds_pad(base);
+#pragma unroll
+      for (int r = 0; r < R; ++r) lds_st<T2>(Y, sb + r * (Ns + Ns / 16), v[r]);
+    } else {
+#pragma unroll
+      for (int r = 0; r < R; ++r) lds_st<T2>(Y, lds_pad(base + r * Ns), v[r]);
+    }
+  }
+}
+
+template <int P, int p, bool INV, typename T2>
+__device__ __forceinline__ void fftc_mid_passes(LdsP<T2>& X, LdsP<T2>& Y, LdsP<T2> tw, int t) {
+  if constexpr (p < fftc_npass(P) - 1) {
+    fftc_mid_pass<P, p, INV, T2>(X, Y, tw, t);
+    __syncthreads();
+    LdsP<T2> tmp = X;
+    X = Y;
+    Y = tmp;
+    fftc_mid_passes<P, p + 1, INV, T2>(X, Y, tw, t);
+  }
+}
+
+template <int P, bool INV, typename T2>
+__device__ __forceinline__ void fftc_regs(T2 (&v)[8], LdsP<T2> buf0, LdsP<T2> buf1, int& cur, LdsP<T2> tw, int t) {
+  constexpr int TT = P / 8, NP = fftc_npass(P);
+  static_assert(TT % 16 == 0, "last-pass loads keep the padding linear");
+  SmallDft<8, INV, T2>::run(v);
+  LdsP<T2> X = cur ? buf1 : buf0;
+  LdsP<T2> Y = cur ? buf0 : buf1;
+  {
+    const int sb = 8 * t + (t >> 1);   // lds_pad(8 t + r) = 8 t + t/2 + r for r < 8
+#pragma unroll
+    for (int r = 0; r < 8; ++r) lds_st<T2>(X, sb + r, v[r]);
+  }
+  __syncthreads();
+  fftc_mid_passes<P, 1, INV, T2>(X, Y, tw, t);
+  const int lt = lds_pad(t);
+#pragma unroll
+  for (int r = 0; r < 8; ++r) v[r] = lds_ld<T2>(X, lt + r * (TT + TT / 16));
+  T2 w = lds_ld<T2>(tw, fftc_toff(P, NP - 1) + t);
+  if (INV) w.y = -w.y;
+  T2 wr = w;
+  v[1] = c2mul(v[1], w);
+#pragma unroll
+  for (int r = 2; r < 8; ++r) {
+    wr = c2mul(wr, w);
+    v[r] = c2mul(v[r], wr);
+  }
+  SmallDft<8, INV, T2>::run(v);
+  cur = (X == buf0) ? 1 : 0;
+}
+
+// One transform: the compile-time plan when PC > 0 (PC == pl.P), else the
+// plan read at run time.
+template <int PC, bool INV, typename T2>
+__device__ __forceinline__ void fft_any(T2 (&v)[8], LdsP<T2> buf0, LdsP<T2> buf1, int& cur, LdsP<T2> tw,
+                                        const FftPlan& pl, int t, int TT) {
+  if constexpr (PC > 0) fftc_regs<PC, INV, T2>(v, buf0, buf1, cur, tw, t);
+  else fft_regs<INV, T2>(v, buf0, buf1, cur, tw, pl, t, TT);
+}
+
 // y = (transport + Theta) u for the elements of the grid (one workgroup per
 // element).
 // blockDim = G * TT <= NT (G groups of TT = P/8 threads, each group
 // transforms one DG node's vector at a time).  Dynamic LDS: n*n_xi (T, then
 // Z; aliased with the work buffers when theta_lds_alias holds), 2 G padded
 // work buffers and the twiddle table, in complex values of type T2.
-template <int NN, typename T, typename TS, int NT>
+template <int NN, typename T, typename TS, int NT, int PC>
 __global__ __launch_bounds__(NT) void theta_fft_kernel(const OpConst* __restrict__ oc, FftPlan pl,
                                                        ThetaTabs<typename cvec<T>::type> tb,
                                                        const TS* __restrict__ sbar, const double* __restrict__ cap,
@@ -360,7 +485,7 @@ __global__ __launch_bounds__(NT) void theta_fft_kernel(const OpConst* __restrict
   if (skip(st)) return;
   extern __shared__ __align__(16) unsigned char theta_smem[];
   constexpr int np = NN * (NN + 1) / 2;
-  const int nk = oc->nk, nx = oc->n_xi, nl = nx + 1, P = pl.P;
+  const int nk = oc->nk, nx = oc->n_xi, nl = nx + 1, P = PC > 0 ? PC : pl.P;
   const int TT = P >> 3, G = blockDim.x / TT;
   const int g = threadIdx.x / TT, t = threadIdx.x % TT;
   const int e = blockIdx.x;
@@ -425,10 +550,10 @@ __global__ __launch_bounds__(NT) void theta_fft_kernel(const OpConst* __restrict
       v[r] = (act && i < nk) ? c2mul(kPre ? a_pre[r] : tb.a_pre[i], lds_ld<T2>(Tst, b * nk + i)) : zero;
     }
     if (alias) __syncthreads();   // u is read before the work buffers overwrite it
-    fft_regs<false, T2>(v, buf0, buf1, cur, twl, pl, t, TT);
+    fft_any<PC, false, T2>(v, buf0, buf1, cur, twl, pl, t, TT);
 #pragma unroll
     for (int r = 0; r < 8; ++r) v[r] = c2mul(v[r], kPre ? g1[r] : pin_uniform(tb.g1)[t + TT * r]);
-    fft_regs<true, T2>(v, buf0, buf1, cur, twl, pl, t, TT);
+    fft_any<PC, true, T2>(v, buf0, buf1, cur, twl, pl, t, TT);
     if (alias) __syncthreads();   // every group's last pass has read its buffer
     const T2* wpp = kPre ? tb.w_post : pin_uniform(tb.w_post);
     if (act) {
@@ -526,10 +651,10 @@ __global__ __launch_bounds__(NT) void theta_fft_kernel(const OpConst* __restrict
       if constexpr (kPre) yo[r] = (act && i < nk) ? yb[i] : zero;   // early, to hide its latency
     }
     if (alias) __syncthreads();   // Z is read before the work buffers overwrite it
-    fft_regs<false, T2>(v, buf0, buf1, cur, twl, pl, t, TT);
+    fft_any<PC, false, T2>(v, buf0, buf1, cur, twl, pl, t, TT);
 #pragma unroll
     for (int r = 0; r < 8; ++r) v[r] = c2mul(v[r], kPre ? g2[r] : pin_uniform(tb.g2)[t + TT * r]);
-    fft_regs<true, T2>(v, buf0, buf1, cur, twl, pl, t, TT);
+    fft_any<PC, true, T2>(v, buf0, buf1, cur, twl, pl, t, TT);
     const T2* app = kPre ? tb.a_post : pin_uniform(tb.a_post);
     if constexpr (!kPre) yb = pin_lane(yb);
     if (act) {

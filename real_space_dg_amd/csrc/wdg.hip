@@ -97,6 +97,7 @@ struct wdg_ctx {
   DevBuf<double> d_sbar64;
   DevBuf<float> d_sbar32;
   DevBuf<double2> d_s0s1, d_agg;
+  DevBuf<float2> d_linv32;            // FP32 line-preconditioner block inverses (line_inverse_kernel)
   DevBuf<double> d_A1, d_A2;          // realified phi / i*phi^H, FP64 (GEMM formulation)
   DevBuf<uint16_t> d_A1b, d_A2b;      // same in BF16
   DevBuf<double2> d_T64, d_Z64;       // Theta intermediates (GEMM formulation)
@@ -401,16 +402,29 @@ void op_theta_fft(wdg_ctx* c, const typename cvec<T>::type* u, typename cvec<T>:
   ProfScope ps(c, P_THETA_FFT, 2.0 * c->N * s + (double)c->g.n_chi * c->oc.n_pairs * (c->g.n_xi + 1) * ts);
   const dim3 grid(c->g.n_chi), block(f64 ? c->theta_block64 : c->theta_block32);
   const size_t lds = f64 ? c->lds64 : c->lds32;
-#define WDG_THETA(NN)                                                                                              \
-  if constexpr (f64)                                                                                               \
-    hipLaunchKernelGGL((theta_fft_kernel<NN, double, double, 256>), grid, block, lds, c->stream, c->d_oc.p,         \
-                       c->fplan, c->tt64, c->d_sbar64.p, c->d_cap.p, c->d_k.p, u, y, st);                                     \
-  else if (c->theta_block32 > kFftThreads)                                                                         \
-    hipLaunchKernelGGL((theta_fft_kernel<NN, float, float, 1024>), grid, block, lds, c->stream, c->d_oc.p,          \
-                       c->fplan, c->tt32, c->d_sbar32.p, c->d_cap.p, c->d_k.p, u, y, st);                                     \
-  else                                                                                                             \
-    hipLaunchKernelGGL((theta_fft_kernel<NN, float, float, 256>), grid, block, lds, c->stream, c->d_oc.p,           \
-                       c->fplan, c->tt32, c->d_sbar32.p, c->d_cap.p, c->d_k.p, u, y, st);                                     \
+#define WDG_THETA_P(NN, PC)                                                                                    \
+  if constexpr (f64)                                                                                           \
+    hipLaunchKernelGGL((theta_fft_kernel<NN, double, double, 256, PC>), grid, block, lds, c->stream, c->d_oc.p, \
+                       c->fplan, c->tt64, c->d_sbar64.p, c->d_cap.p, c->d_k.p, u, y, st);                      \
+  else if (c->theta_block32 > kFftThreads)                                                                     \
+    hipLaunchKernelGGL((theta_fft_kernel<NN, float, float, 1024, PC>), grid, block, lds, c->stream, c->d_oc.p,  \
+                       c->fplan, c->tt32, c->d_sbar32.p, c->d_cap.p, c->d_k.p, u, y, st);                      \
+  else                                                                                                         \
+    hipLaunchKernelGGL((theta_fft_kernel<NN, float, float, 256, PC>), grid, block, lds, c->stream, c->d_oc.p,   \
+                       c->fplan, c->tt32, c->d_sbar32.p, c->d_cap.p, c->d_k.p, u, y, st);
+// transform lengths with a compile-time plan (n <= 5), else the run-time plan
+#define WDG_THETA(NN)                              \
+  if (NN <= 5 && c->fplan.P == 2048) {             \
+    WDG_THETA_P(NN, (NN <= 5 ? 2048 : 0))          \
+  } else if (NN <= 5 && c->fplan.P == 1024) {      \
+    WDG_THETA_P(NN, (NN <= 5 ? 1024 : 0))          \
+  } else if (NN <= 5 && c->fplan.P == 512) {       \
+    WDG_THETA_P(NN, (NN <= 5 ? 512 : 0))           \
+  } else if (NN <= 5 && c->fplan.P == 128) {       \
+    WDG_THETA_P(NN, (NN <= 5 ? 128 : 0))           \
+  } else {                                         \
+    WDG_THETA_P(NN, 0)                             \
+  }                                                \
   break;
   switch (c->g.n) {
     case 2: WDG_THETA(2)
@@ -423,30 +437,42 @@ void op_theta_fft(wdg_ctx* c, const typename cvec<T>::type* u, typename cvec<T>:
     default: throw std::invalid_argument("n_k_chi outside 2..8");
   }
 #undef WDG_THETA
+#undef WDG_THETA_P
 }
 
 // The attribute is process-wide per kernel, so it is set to the hardware
 // maximum rather than to this context's plan (contexts of other grids share
 // the kernel).
+template <int NN, int PC>
+void theta_fft_attrs_p() {
+  HIPCHK(hipFuncSetAttribute((const void*)theta_fft_kernel<NN, double, double, 256, PC>,
+                             hipFuncAttributeMaxDynamicSharedMemorySize, kMaxLdsBytes));
+  HIPCHK(hipFuncSetAttribute((const void*)theta_fft_kernel<NN, float, float, 256, PC>,
+                             hipFuncAttributeMaxDynamicSharedMemorySize, kMaxLdsBytes));
+  HIPCHK(hipFuncSetAttribute((const void*)theta_fft_kernel<NN, float, float, 1024, PC>,
+                             hipFuncAttributeMaxDynamicSharedMemorySize, kMaxLdsBytes));
+}
 template <int NN>
 void theta_fft_attrs() {
-  HIPCHK(hipFuncSetAttribute((const void*)theta_fft_kernel<NN, double, double, 256>,
-                             hipFuncAttributeMaxDynamicSharedMemorySize, kMaxLdsBytes));
-  HIPCHK(hipFuncSetAttribute((const void*)theta_fft_kernel<NN, float, float, 256>,
-                             hipFuncAttributeMaxDynamicSharedMemorySize, kMaxLdsBytes));
-  HIPCHK(hipFuncSetAttribute((const void*)theta_fft_kernel<NN, float, float, 1024>,
-                             hipFuncAttributeMaxDynamicSharedMemorySize, kMaxLdsBytes));
+  theta_fft_attrs_p<NN, 0>();
+  if constexpr (NN <= 5) {
+    theta_fft_attrs_p<NN, 128>();
+    theta_fft_attrs_p<NN, 512>();
+    theta_fft_attrs_p<NN, 1024>();
+    theta_fft_attrs_p<NN, 2048>();
+  }
 }
 
 // Resident workgroups per CU of the FP32 Theta kernel at the 256-thread and
 // the wide block (0 where the calculator fails).
 template <int NN>
 void theta32_occupancy(int block, size_t lds, int wide, size_t lds_wide, int& nb_base, int& nb_wide) {
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb_base, (const void*)theta_fft_kernel<NN, float, float, 256>,
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb_base, (const void*)theta_fft_kernel<NN, float, float, 256, 0>,
                                                    block, lds) != hipSuccess)
     nb_base = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb_wide, (const void*)theta_fft_kernel<NN, float, float, 1024>,
-                                                   wide, lds_wide) != hipSuccess)
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb_wide,
+                                                   (const void*)theta_fft_kernel<NN, float, float, 1024, 0>, wide,
+                                                   lds_wide) != hipSuccess)
     nb_wide = 0;
   (void)hipGetLastError();
 }
@@ -496,6 +522,26 @@ void op_apply_base(wdg_ctx* c, const typename cvec<T>::type* u, typename cvec<T>
 template <int NN, typename T>
 void op_precond_n(wdg_ctx* c, const typename cvec<T>::type* r, typename cvec<T>::type* z, const GmresState* st) {
   const long work = (long)c->oc.n_chunks * c->g.nk;
+  if constexpr (std::is_same<T, float>::value) {
+    // FP32: stream the stored block inverses
+    const double rows = (double)c->N / NN;   // (element, k) blocks
+    {
+      ProfScope ps(c, P_LINE_AGG, 8.0 * (c->N + rows * (NN + 1)));
+      hipLaunchKernelGGL((line_aggregate_inv_kernel<NN>), dim3(grid1(work, 256)), dim3(256), 0, c->stream,
+                         c->d_oc.p, c->d_k.p, c->d_linv32.p, r, c->d_agg.p, st);
+    }
+    {
+      ProfScope ps(c, P_LINE_CARRY, 3.0 * c->oc.n_chunks * c->g.nk * 2 * sizeof(double2));
+      hipLaunchKernelGGL(line_carry_kernel, dim3(grid1(64L * c->g.nk, 256)), dim3(256), 0, c->stream, c->d_oc.p,
+                         c->d_k.p, c->d_agg.p, st);
+    }
+    {
+      ProfScope ps(c, P_LINE_APPLY, 8.0 * (2.0 * c->N + rows * (NN * NN + NN)));
+      hipLaunchKernelGGL((line_apply_inv_kernel<NN>), dim3(grid1(work, 256)), dim3(256), 0, c->stream, c->d_oc.p,
+                         c->d_k.p, c->d_linv32.p, c->d_agg.p, r, z, st);
+    }
+    return;
+  }
   {
     ProfScope ps(c, P_LINE_AGG, 1.0 * c->N * sizeof(typename cvec<T>::type));
     hipLaunchKernelGGL((line_aggregate_kernel<NN, T>), dim3(grid1(work, 256)), dim3(256), 0, c->stream, c->d_oc.p,
@@ -922,6 +968,7 @@ void create_impl(wdg_ctx* c, const wdg_desc& d) {
   HIPCHK(hipMemcpyAsync(c->d_phic.p, ph.data(), ph.size() * sizeof(double2), hipMemcpyHostToDevice, c->stream));
   c->d_s0s1.alloc((size_t)g.n_chi * n * n * 2);
   c->d_agg.alloc((size_t)oc.n_chunks * nk * 2);
+  c->d_linv32.alloc((size_t)g.n_chi * nk * (n * n + n));
   c->d_sbar64.alloc((size_t)g.n_chi * oc.n_pairs * (nx + 1));
   c->d_sbar32.alloc((size_t)g.n_chi * oc.n_pairs * (nx + 1));
   c->d_b.alloc(c->N);
@@ -987,6 +1034,20 @@ void set_potential_impl(wdg_ctx* c, const wdg_potential& pot) {
                      c->d_edges.p, c->d_xg.p, c->d_ug.p, pot.n_mat, g.l_chi, umin, c->d_sbar64.p, c->d_sbar32.p);
   hipLaunchKernelGGL(precond_sums_kernel, dim3(g.n_chi * g.n * g.n), dim3(256), 0, c->stream, c->d_oc.p,
                      c->d_sbar64.p, c->d_cap.p, c->d_s0s1.p);
+  // FP32 line-preconditioner block inverses (FP64 elimination, once per potential)
+  {
+    const dim3 gi(grid1((long)g.n_chi * g.nk, 256));
+    switch (g.n) {
+#define WDG_LINV(NN)                                                                                   \
+  case NN:                                                                                             \
+    hipLaunchKernelGGL((line_inverse_kernel<NN>), gi, dim3(256), 0, c->stream, c->d_oc.p, c->d_k.p,    \
+                       c->d_s0s1.p, c->d_linv32.p);                                                    \
+    break;
+      WDG_LINV(2) WDG_LINV(3) WDG_LINV(4) WDG_LINV(5) WDG_LINV(6) WDG_LINV(7) WDG_LINV(8)
+#undef WDG_LINV
+      default: throw std::invalid_argument("n_k_chi outside 2..8");
+    }
+  }
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(c->stream));
   // A previous solution stays usable as a warm start: keep its physical value

@@ -103,3 +103,24 @@ def test_precond_inverts_line_operator(case):
     res = np.linalg.norm(P @ z - r) / np.linalg.norm(r)
     print("precond residual", res)
     assert res < 1e-11
+
+
+def test_precond_fp32_matches_fp64(case):
+    """The FP32 line preconditioner (stored FP32 block inverses) against the
+    FP64 sweeps, both through wdg_precond_device in device units."""
+    import torch
+    from real_space_dg_amd import _lib as L
+    from real_space_dg_amd.solver import check
+    op = case["op"]
+    r = rand_c(op.N, 5)
+    r64 = torch.tensor(r, dtype=torch.complex128, device="cuda")
+    r32 = r64.to(torch.complex64)
+    z64 = torch.empty_like(r64)
+    z32 = torch.empty_like(r32)
+    torch.cuda.synchronize()
+    check(op._lib.wdg_precond_device(op._h, r64.data_ptr(), z64.data_ptr(), L.WDG_PREC_FP64), op._h)
+    check(op._lib.wdg_precond_device(op._h, r32.data_ptr(), z32.data_ptr(), L.WDG_PREC_MIXED), op._h)
+    a, b = z32.cpu().numpy().astype(np.complex128), z64.cpu().numpy()
+    err = np.linalg.norm(a - b) / np.linalg.norm(b)
+    print("precond fp32 vs fp64", err)
+    assert err < 1e-5
