@@ -440,7 +440,7 @@ void f2_create_impl(wdg_ctx* c, const wdg2_desc& d, const wdg2_part* parts, int 
   // solve vectors
   c->N = N;
   c->ncols = ncen;
-  c->red_blocks = (int)std::min<long>(512, std::max<long>(1, N / (256 * 8)));
+  c->red_blocks = (int)std::min<long>(1024, std::max<long>(1, N / (256 * 8)));
   c->d_b.alloc(N);
   c->d_x.alloc(N);
   c->d_r.alloc(N);

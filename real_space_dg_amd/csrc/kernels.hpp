@@ -1140,7 +1140,7 @@ struct BasisIO<ushort4> {
 
 // Multi-dot: h[i] = <V_i, w> for i < nv (plus ||w||^2 when mode 0).
 // Grid (blocks, ceil(nv/kDotGroup)); block row y handles basis vectors
-// [8y, 8y+8).  mode 0 writes h1 and nrm0; mode 1 (only when st->reorth)
+// [G y, G y + G), G = kDotGroup.  mode 0 writes h1 and nrm0; mode 1 (only when st->reorth)
 // writes h2.  NP = number of 16-byte packets per vector.
 template <typename T, typename VP, int NT>
 __global__ __launch_bounds__(NT) void arnoldi_dot_kernel(GmresBufs gb, const VP* __restrict__ V,
@@ -1157,12 +1157,25 @@ __global__ __launch_bounds__(NT) void arnoldi_dot_kernel(GmresBufs gb, const VP*
 #pragma unroll
   for (int i = 0; i < kDotGroup + 1; ++i) ar[i] = ai[i] = 0;
   const VP* V0 = V + (long)i0 * NP;
-  for (long x = (long)blockIdx.x * NT + threadIdx.x; x < NP; x += (long)gridDim.x * NT) {
-    const typename P::type wv = w[x];
+  if (ng == kDotGroup) {
+    // full group: the group's basis loads are issued together
+    for (long x = (long)blockIdx.x * NT + threadIdx.x; x < NP; x += (long)gridDim.x * NT) {
+      const typename P::type wv = w[x];
+      VP raw[kDotGroup];
 #pragma unroll
-    for (int i = 0; i < kDotGroup; ++i)
-      if (i < ng) P::dot_acc(ar[i], ai[i], IO::load_raw(V0[(long)i * NP + x]), wv);
-    ar[kDotGroup] += P::nrm2_w(wv);
+      for (int i = 0; i < kDotGroup; ++i) raw[i] = V0[(long)i * NP + x];
+#pragma unroll
+      for (int i = 0; i < kDotGroup; ++i) P::dot_acc(ar[i], ai[i], IO::load_raw(raw[i]), wv);
+      ar[kDotGroup] += P::nrm2_w(wv);
+    }
+  } else {
+    for (long x = (long)blockIdx.x * NT + threadIdx.x; x < NP; x += (long)gridDim.x * NT) {
+      const typename P::type wv = w[x];
+#pragma unroll
+      for (int i = 0; i < kDotGroup; ++i)
+        if (i < ng) P::dot_acc(ar[i], ai[i], IO::load_raw(V0[(long)i * NP + x]), wv);
+      ar[kDotGroup] += P::nrm2_w(wv);
+    }
   }
   double2 acc[kDotGroup + 1];
 #pragma unroll
@@ -1215,6 +1228,13 @@ __global__ __launch_bounds__(NT) void arnoldi_update_kernel(GmresBufs gb, const 
   for (long x = (long)blockIdx.x * NT + threadIdx.x; x < NP; x += (long)gridDim.x * NT) {
     typename P::type wv = w[x];
     int i = 0;
+    for (; i + 8 <= nv; i += 8) {   // eight basis loads in flight
+      VP raw[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) raw[q] = V[(long)(i + q) * NP + x];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) P::axpy(wv, hs[i + q], IO::load_raw(raw[q]));
+    }
     for (; i + 4 <= nv; i += 4) {
       const typename P::type v0 = IO::load_raw(V[(long)(i + 0) * NP + x]);
       const typename P::type v1 = IO::load_raw(V[(long)(i + 1) * NP + x]);

@@ -903,7 +903,7 @@ void create_impl(wdg_ctx* c, const wdg_desc& d) {
   const int n = g.n;
   c->N = (long)g.n_chi * n * g.nk;
   c->ncols = g.n_chi * n;
-  c->red_blocks = (int)std::min<long>(512, std::max<long>(1, c->N / (256 * 8)));
+  c->red_blocks = (int)std::min<long>(1024, std::max<long>(1, c->N / (256 * 8)));
   HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   OpConst& oc = c->oc;
   std::memset(&oc, 0, sizeof(oc));
