@@ -243,6 +243,7 @@ typedef struct wdg_cn_opts {
   double  a_scale, b_scale;
   double  tol;           /* per-step GMRES tolerance                        */
   int32_t restart, max_iter, precision, krylov_basis, use_precond;
+  int32_t use_graph;     /* 1: replay the restart cycles from captured HIP graphs */
 } wdg_cn_opts;
 typedef struct wdg_cn_info {
   int32_t steps;               /* steps taken                               */

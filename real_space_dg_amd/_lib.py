@@ -94,7 +94,7 @@ class wdg_transient_opts(C.Structure):
 class wdg_cn_opts(C.Structure):
     _fields_ = [("dt", C.c_double), ("nsteps", C.c_int32), ("a_scale", C.c_double), ("b_scale", C.c_double),
                 ("tol", C.c_double), ("restart", C.c_int32), ("max_iter", C.c_int32), ("precision", C.c_int32),
-                ("krylov_basis", C.c_int32), ("use_precond", C.c_int32)]
+                ("krylov_basis", C.c_int32), ("use_precond", C.c_int32), ("use_graph", C.c_int32)]
 
 
 class wdg_cn_info(C.Structure):

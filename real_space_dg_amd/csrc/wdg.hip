@@ -1817,7 +1817,7 @@ int wdg_transient_cn(wdg_ctx* ctx, const wdg_cn_opts* o, const double* rho0, dou
     so.max_iter = o->max_iter > 0 ? o->max_iter : 20000;
     so.precision = o->precision;
     so.inner_tol = 1e-4;
-    so.use_graph = 1;
+    so.use_graph = o->use_graph ? 1 : 0;
     so.orthogonalization = 1;
     so.warm_device = 1;
     so.krylov_basis = o->krylov_basis;

@@ -826,13 +826,16 @@ class Full2DOperator:
         return {k: lf[k] for k in ("sigma", "factor_bytes", "transverse_dof", "modes_count")}
 
     def transient_cn(self, dt: float, nsteps: int, a_scale: float = 1.0, b_scale: float = 1.0, tol: float = 1e-10,
-                     restart: int = 30, precision: str = "fp64", rho0=None, return_rho: bool = True):
+                     restart: int = 30, precision: str = "fp64", rho0=None, return_rho: bool = True,
+                     use_graph: bool = True):
         """Crank-Nicolson steps of d rho/dt = b_scale*rhs - a_scale*A rho
         (wdg_transient_cn); the Jacobi / row-sum preconditioner in use is
-        rebuilt for the shifted operator.  Returns (rho, info)."""
+        rebuilt for the shifted operator.  use_graph=False launches the restart
+        cycles kernel by kernel instead of replaying captured graphs.
+        Returns (rho, info)."""
         from .solver import transient_cn
         return transient_cn(self._lib, self.ctx, self.n, dt, nsteps, a_scale, b_scale, tol, restart, precision,
-                            rho0, return_rho)
+                            rho0, return_rho, use_graph=use_graph)
 
     def set_profiling(self, mask: int):
         check(self._lib.wdg_set_profiling(self.ctx, int(mask)), self.ctx)

@@ -266,13 +266,15 @@ def read_profile(lib, ctx) -> dict:
 
 
 def transient_cn(lib, ctx, n: int, dt: float, nsteps: int, a_scale: float, b_scale: float, tol: float, restart: int,
-                 precision: str, rho0, return_rho: bool, use_precond: bool = True, krylov_basis: str = "fp16"):
+                 precision: str, rho0, return_rho: bool, use_precond: bool = True, krylov_basis: str = "fp16",
+                 use_graph: bool = True):
     """Shared ctypes call of wdg_transient_cn for 1D and full-2D contexts."""
     r0 = None if rho0 is None else _as_c128(rho0, n)
     rho = np.empty(n, np.complex128) if return_rho else None
     o = _lib.wdg_cn_opts(float(dt), int(nsteps), float(a_scale), float(b_scale), float(tol), int(restart), 20000,
                          PRECISIONS[precision], {"fp32": _lib.WDG_BASIS_WORKING, "bf16": _lib.WDG_BASIS_BF16,
-                                                 "fp16": _lib.WDG_BASIS_FP16}[krylov_basis], 1 if use_precond else 0)
+                                                 "fp16": _lib.WDG_BASIS_FP16}[krylov_basis], 1 if use_precond else 0,
+                             1 if use_graph else 0)
     info = _lib.wdg_cn_info()
     check(lib.wdg_transient_cn(ctx, C.byref(o), dptr(r0), dptr(rho), C.byref(info)), ctx)
     return rho, {k: getattr(info, k) for k, _ in info._fields_}
