@@ -525,17 +525,21 @@ struct F2Dense {
   const double* lift;     // face: nl
   const int* mask;        // face: per transverse centre DOF (null = all)
   const double2* R;       // n x n row-major, n = n_rho along the axis
+  int rglobal;            // 1: R is read from global memory (L2-resident) instead of LDS
 };
 
-// LDS: R (n x (n+1), padded) and the input block (n along the axis x chunk across).
-inline size_t f2_dense_lds(int n, int chunk, size_t t2) { return ((size_t)n * (n + 1) + (size_t)n * chunk) * t2; }
+// LDS: R (n x (n+1), padded, unless it is read from global memory) and the
+// input block (n along the axis x chunk across).
+inline size_t f2_dense_lds(int n, int chunk, size_t t2, bool rglobal = false) {
+  return ((rglobal ? (size_t)0 : (size_t)n * (n + 1)) + (size_t)n * chunk) * t2;
+}
 
 // Workgroup (line, chunk): line = a centre DOF (banded, plain) or a
 // transverse DOF of the face (lift); chunk = a range of relative cells across
 // the axis (rho_y for axis 0, rho_x for axis 1), which the dense factor does
 // not couple.  W = C-combination of u in LDS, then out(m, c) = sum_k R(m, k)
 // W(k, c), 4 outputs per thread sharing one R (or W) load.
-template <typename T>
+template <typename T, bool RG>
 __global__ void __launch_bounds__(kF2Block) f2_dense_kernel(const F2Const* __restrict__ cp, F2Dense d,
                                                             const typename F2Vec<T>::T2* __restrict__ u,
                                                             typename F2Vec<T>::T2* __restrict__ y,
@@ -551,8 +555,20 @@ __global__ void __launch_bounds__(kF2Block) f2_dense_kernel(const F2Const* __res
   if (cn <= 0) return;
   const int line = blockIdx.x;
   extern __shared__ __align__(16) unsigned char f2_smem[];
-  T2* sR = reinterpret_cast<T2*>(f2_smem);          // n x (n+1)
-  T2* sW = sR + (size_t)n * (n + 1);                // [k][c - c0], n x chunk
+  T2* sR = reinterpret_cast<T2*>(f2_smem);          // n x (n+1) (RG: not staged)
+  T2* sW = RG ? sR : sR + (size_t)n * (n + 1);      // [k][c - c0], n x chunk
+  // R(m, k) scaled by the part coefficient, from LDS or (RG) from global memory
+  auto rval = [&](int m, int k) -> T2 {
+    if constexpr (RG) {
+      const double2 v = d.R[(size_t)m * n + k];
+      T2 w;
+      w.x = (T)(d.coef * v.x);
+      w.y = (T)(d.coef * v.y);
+      return w;
+    } else {
+      return sR[(size_t)m * (n + 1) + k];
+    }
+  };
   const int nxd = c.nxd;
   auto dof = [&](int k, int t) -> long { return d.axis == 0 ? (long)k + (long)nxd * t : (long)t + (long)nxd * k; };
   // relative index of (k along, cc across)
@@ -567,13 +583,15 @@ __global__ void __launch_bounds__(kF2Block) f2_dense_kernel(const F2Const* __res
     t = line;
     if (d.mask && !d.mask[t]) return;
   }
-  for (int idx = threadIdx.x; idx < n * n; idx += blockDim.x) {
-    const int m = idx / n, k = idx - m * n;
-    const double2 v = d.R[idx];
-    T2 w;
-    w.x = (T)(d.coef * v.x);
-    w.y = (T)(d.coef * v.y);
-    sR[(size_t)m * (n + 1) + k] = w;
+  if constexpr (!RG) {
+    for (int idx = threadIdx.x; idx < n * n; idx += blockDim.x) {
+      const int m = idx / n, k = idx - m * n;
+      const double2 v = d.R[idx];
+      T2 w;
+      w.x = (T)(d.coef * v.x);
+      w.y = (T)(d.coef * v.y);
+      sR[(size_t)m * (n + 1) + k] = w;
+    }
   }
   const int nw = n * cn;
   if (plain) {
@@ -619,7 +637,7 @@ __global__ void __launch_bounds__(kF2Block) f2_dense_kernel(const F2Const* __res
 #pragma unroll
       for (int j = 0; j < 4; ++j) { mm[j] = m; cc[j] = cb * 4 + j; }
       for (int k = 0; k < n; ++k) {
-        const T2 rv = sR[(size_t)m * (n + 1) + k];
+        const T2 rv = rval(m, k);
 #pragma unroll
         for (int j = 0; j < 4; ++j)
           if (cc[j] < cn) {
@@ -637,7 +655,7 @@ __global__ void __launch_bounds__(kF2Block) f2_dense_kernel(const F2Const* __res
 #pragma unroll
         for (int j = 0; j < 4; ++j)
           if (mm[j] < n) {
-            const T2 rv = sR[(size_t)mm[j] * (n + 1) + k];
+            const T2 rv = rval(mm[j], k);
             acc[j].x += rv.x * w.x - rv.y * w.y;
             acc[j].y += rv.x * w.y + rv.y * w.x;
           }

@@ -104,13 +104,16 @@ void f2_launch_apply(wdg_ctx* c, const typename F2Vec<T>::T2* u, typename F2Vec<
 
 // Cells across the axis per workgroup of a dense part: enough workgroups to
 // fill the chip, within the LDS budget.
-int f2_dense_chunk(int n, int ncol, int lines) {
+// Relative cells across the axis per workgroup; *rglobal = 1 when the n x n
+// factor does not fit in LDS beside one column and is read from global memory.
+int f2_dense_chunk(int n, int ncol, int lines, int* rglobal) {
   const int want = std::max(1, (2048 + lines - 1) / lines);
   int chunk = std::max(1, (ncol + want - 1) / want);
   chunk = std::max(chunk, std::min(ncol, 4));
-  while (chunk > 1 && f2_dense_lds(n, chunk, sizeof(double2)) > kF2MaxLds) chunk = (chunk + 1) / 2;
-  if (f2_dense_lds(n, chunk, sizeof(double2)) > kF2MaxLds)
-    throw std::invalid_argument("n_rho too large for the dense relative factor in one workgroup's LDS");
+  *rglobal = f2_dense_lds(n, 1, sizeof(double2)) > kF2MaxLds ? 1 : 0;
+  while (chunk > 1 && f2_dense_lds(n, chunk, sizeof(double2), *rglobal) > kF2MaxLds) chunk = (chunk + 1) / 2;
+  if (f2_dense_lds(n, chunk, sizeof(double2), *rglobal) > kF2MaxLds)
+    throw std::invalid_argument("n_rho too large for one relative line in one workgroup's LDS");
   return chunk;
 }
 
@@ -120,14 +123,19 @@ void f2_launch_dense(wdg_ctx* c, const F2Data::Dense& dp, const typename F2Vec<T
   F2Data& f = *c->f2;
   static bool attr = false;
   if (!attr) {
-    HIPCHK(hipFuncSetAttribute((const void*)f2_dense_kernel<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+    HIPCHK(hipFuncSetAttribute((const void*)f2_dense_kernel<T, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)kF2MaxLds));
+    HIPCHK(hipFuncSetAttribute((const void*)f2_dense_kernel<T, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)kF2MaxLds));
     attr = true;
   }
   const int ncol = dp.k.axis == 0 ? f.d.n_rho_y : f.d.n_rho_x;
-  const size_t lds = f2_dense_lds(dp.n, dp.k.chunk, sizeof(typename F2Vec<T>::T2));
+  const size_t lds = f2_dense_lds(dp.n, dp.k.chunk, sizeof(typename F2Vec<T>::T2), dp.k.rglobal != 0);
   const dim3 grid(dp.lines, (ncol + dp.k.chunk - 1) / dp.k.chunk);
-  hipLaunchKernelGGL(f2_dense_kernel<T>, grid, dim3(kF2Block), lds, c->stream, f.d_c.p, dp.k, u, y, st);
+  if (dp.k.rglobal)
+    hipLaunchKernelGGL((f2_dense_kernel<T, true>), grid, dim3(kF2Block), lds, c->stream, f.d_c.p, dp.k, u, y, st);
+  else
+    hipLaunchKernelGGL((f2_dense_kernel<T, false>), grid, dim3(kF2Block), lds, c->stream, f.d_c.p, dp.k, u, y, st);
 }
 
 // y = A_hat u
@@ -241,10 +249,10 @@ void f2_create_impl(wdg_ctx* c, const wdg2_desc& d, const wdg2_part* parts, int 
   f->ncen = f->nxd * f->nyd;
   const long N = (long)f->nrel * f->ncen;
   if (N > (1L << 40)) throw std::invalid_argument("problem too large");
-  // the dense kernel keeps one relative block and the n x n factor in LDS
+  // the dense kernel keeps one relative line in LDS (and the n x n factor when it fits)
   const int nmax = std::max(d.n_rho_x, d.n_rho_y);
-  if (f2_dense_lds(nmax, 1, sizeof(double2)) > kF2MaxLds)
-    throw std::invalid_argument("n_rho too large: the dense n_rho x n_rho factor exceeds one workgroup's LDS");
+  if (f2_dense_lds(nmax, 1, sizeof(double2), true) > kF2MaxLds)
+    throw std::invalid_argument("n_rho too large: one relative line exceeds one workgroup's LDS");
   // operator scale: the largest |coef| * max|C| * max|R| over the parts
   double sA = 0;
   std::vector<const wdg2_part*> xc, xi, yc, yi;
@@ -362,7 +370,7 @@ void f2_create_impl(wdg_ctx* c, const wdg2_desc& d, const wdg2_part* parts, int 
         upd(dp->blocks, b);
         dp->k.blocks = dp->blocks.p;
         dp->lines = ncen;
-        dp->k.chunk = f2_dense_chunk(nr, ax == 0 ? nry : nrx, dp->lines);
+        dp->k.chunk = f2_dense_chunk(nr, ax == 0 ? nry : nrx, dp->lines, &dp->k.rglobal);
       } else {
         upd(dp->lift, std::vector<double>(p.c_diag, p.c_diag + nl));
         dp->k.lift = dp->lift.p;
@@ -373,7 +381,7 @@ void f2_create_impl(wdg_ctx* c, const wdg2_desc& d, const wdg2_part* parts, int 
           dp->k.mask = dp->mask.p;
         }
         dp->lines = ntr;
-        dp->k.chunk = f2_dense_chunk(nr, ax == 0 ? nry : nrx, dp->lines);
+        dp->k.chunk = f2_dense_chunk(nr, ax == 0 ? nry : nrx, dp->lines, &dp->k.rglobal);
       }
       HIPCHK(hipStreamSynchronize(c->stream));   // host vectors above go out of scope
       f->dense.push_back(std::move(dp));
@@ -717,7 +725,7 @@ void f2_set_line_impl(wdg_ctx* c, const wdg2_line_factors& lf) {
     dp.n = nrx;
     dp.lines = f.ncen;
     dp.k = F2Dense{};
-    dp.k.chunk = f2_dense_chunk(nrx, f.d.n_rho_y, dp.lines);
+    dp.k.chunk = f2_dense_chunk(nrx, f.d.n_rho_y, dp.lines, &dp.k.rglobal);
     dp.k.axis = 0;
     dp.k.face = -2;
     dp.k.nl = nl;

@@ -30,6 +30,7 @@ import numpy as np
 from numpy.polynomial import legendre as npleg
 
 from . import _lib
+from .contact_modes import contact_mode_reservoir
 from ._lib import check, dptr
 from .solver import SolveOptions
 
@@ -510,6 +511,7 @@ class Full2DSetup:
         params = self.params
         nyd = self.Y.ndof
         self.boundary = {}
+        self.reservoir_info = {}
         for side, Ef in (("source", self.EfL), ("drain", self.EfR)):
             direct = read_param(params, f"full2D_{side}Rho")
             if direct is not None:
@@ -521,6 +523,8 @@ class Full2DSetup:
                 model = str(read_param(params, "full2D_reservoirModel", "material-default")).lower().replace("_", "-")
                 if model in ("material", "material-default", "bulk", "bulk-local", "local-bulk"):
                     rho = self._reservoir(side, Ef, Vxy)
+                elif model in ("contact-mode", "contact-modes", "modes", "transverse-modes"):
+                    rho, self.reservoir_info[side] = contact_mode_reservoir(self, side, Ef, Vxy)
                 elif model in ("zero", "none", "off", "false"):
                     rho = np.zeros((self.nrel, nyd))
                 else:
