@@ -1031,6 +1031,7 @@ class Full2DResult:
     jxDG: np.ndarray
     jyDG: np.ndarray
     info: dict = field(default_factory=dict)
+    setup: "Full2DSetup | None" = None    # the host setup of the solve (grids, DG axes, reservoirs)
 
 
 def solve_transport_DG_full2D(mat: dict, Vxy=None, EfL=None, EfR=None, rho0=None,
@@ -1072,4 +1073,4 @@ def solve_transport_DG_full2D(mat: dict, Vxy=None, EfL=None, EfR=None, rho0=None
     return Full2DResult(status, rho, n, jx, jy, j, nDG, jxDG, jyDG,
                         {"solve": {"flag": info.flag, "relres": info.true_relres, "trueRelres": info.true_relres,
                                    "iterations": info.iterations, "preconditioner": used,
-                                   "seconds": info.seconds}})
+                                   "seconds": info.seconds}}, setup)

@@ -14,7 +14,7 @@ def load():
     for k in ("full2D_sourceContactMask", "full2D_drainContactMask"):
         params[k] = np.asarray(params[k])
     mat = dict(json.loads(str(d["scalars_json"])))
-    mat.update({k: np.asarray(d[k]) for k in ("x", "y", "Vc", "me_x", "me_y", "me_z", "Nd")})
+    mat.update({k: np.asarray(d[k]) for k in ("x", "y", "Vc", "me_x", "me_y", "me_z", "Nd", "eps")})
     mat["dg"] = {"params": params}
     cases = []
     for i in range(d["bias"].size):
@@ -23,5 +23,7 @@ def load():
         efl, efr = mat["Vc"][0, 0] + 0.05, mat["Vc"][-1, 0] + 0.05 - bias
         cases.append({"bias": bias, "V": np.asarray(d["potential"][:, :, i]), "EfL": efl, "EfR": efr,
                       "n": np.asarray(d["density"][:, :, i]), "j": np.asarray(d["current"][:, i]),
-                      "relres": float(d["relres"][i])})
+                      "relres": float(d["relres"][i]), "phi": np.asarray(d["phi"][:, :, i]),
+                      "last_potential_defect": float(d["last_potential_defect"][i]),
+                      "last_poisson_relres": float(d["last_poisson_relres"][i])})
     return mat, cases

@@ -2,8 +2,9 @@
 full-2D RTD sweep (2026-10-28_08-47/Sim_Res.mat, MAT v5 data read with
 scipy.io.loadmat): the material description and dg.params of the run, and
 for three biases the potential V = Vc - phi handed to the transport solve,
-the converged density n (X DOF x material y) and current j (X DOF), and
-the solver's relative residual.  Run from the repository root, in a
+the converged density n (X DOF x material y) and current j (X DOF), the
+electrostatic potential phi, the solver's relative residual and the last
+self-consistency iteration's potential defect and Poisson residual.  Run from the repository root, in a
 container where the reference tree is present:
 
     python tests/golden/make_sim_res_slices.py
@@ -32,15 +33,23 @@ def main():
     mat, sweep = d["mat"], d["sweep"]
     params = {f: plain(getattr(mat.dg.params, f)) for f in mat.dg.params._fieldnames}
     scal = {f: plain(getattr(mat, f)) for f in ("Nx", "Ny", "Temp", "deg_factor", "me_x_ch", "me_y_ch", "me_z_ch",
-                                                "dx", "dy")}
+                                                "dx", "dy", "Lc1", "Lc2")}
     arrays = {"x": np.asarray(mat.x, float), "y": np.asarray(mat.y, float), "Vc": np.asarray(mat.Vc, float),
               "me_x": np.asarray(mat.me_x, float), "me_y": np.asarray(mat.me_y, float),
-              "me_z": np.asarray(mat.me_z, float), "Nd": np.asarray(mat.Nd, float)}
+              "me_z": np.asarray(mat.me_z, float), "Nd": np.asarray(mat.Nd, float),
+              "eps": np.asarray(mat.eps, float)}
     ib = np.array(BIASES)
+    # last self-consistency iteration of each bias: unmixed potential defect
+    # [V] and the Poisson physical residual at the stored state
+    hist = d["selfConsistency"].history
+    last = [np.atleast_1d(hist[i])[-1] for i in BIASES]
+    arrays["last_potential_defect"] = np.array([float(h.potentialDefect) for h in last])
+    arrays["last_poisson_relres"] = np.array([float(h.poissonPhysicalRelres) for h in last])
     np.savez_compressed(
         OUT, params_json=np.array(json.dumps(params)), scalars_json=np.array(json.dumps(scal)), **arrays,
         bias_index=ib, bias=np.asarray(sweep.bias, float)[ib],
-        potential=np.asarray(sweep.potential, float)[:, :, ib], density=np.asarray(sweep.density, float)[:, :, ib],
+        potential=np.asarray(sweep.potential, float)[:, :, ib], phi=np.asarray(sweep.phi, float)[:, :, ib],
+        density=np.asarray(sweep.density, float)[:, :, ib],
         current=np.asarray(sweep.current, float)[:, ib], relres=np.asarray(sweep.relres, float)[ib],
         flag=np.asarray(sweep.flag, float)[ib])
     print("wrote", OUT, OUT.stat().st_size, "bytes")

@@ -5,11 +5,14 @@ settings (periodic Y, contact-mode reservoirs with neutrality Fermi levels,
 upwind fluxes, FV-consistent potential, line-transverse preconditioner,
 GMRES(40) to relres 5e-6), reproduces the stored density and current.  The
 stored results carry the run's GMRES tolerance (relres ~5e-6), so agreement
-is checked to 1e-3 in the density, not to 1e-10."""
+is checked to 1e-3 in the density, not to 1e-10.  The stored potential is
+also a fixed point of the transport + 2D Poisson map within the run's
+self-consistency tolerances."""
 import numpy as np
 import pytest
 
 from real_space_dg_amd import full2d as P
+from real_space_dg_amd import poisson2d as PS
 from tests._sim_res import load
 
 pytestmark = pytest.mark.gpu
@@ -34,3 +37,34 @@ def test_stored_sweep_density_and_current(case):
         assert jm < 1e-3 and ej < 1e-2
     else:   # zero bias: a small residual current, matched in the mean
         assert jm < 1e-2
+    # self-consistency: the stored potential is a fixed point of transport +
+    # Poisson (RTD/solve2D_rtd_self_consistent.m:143-176, Newton-Raphson
+    # Poisson as in the run) within the run's tolerances 5e-4 V and 8e-2
+    M, types, periodic = PS.eval_poisson2d(MAT)
+    n_pois, tinfo = PS.project_dg_density(np.real(r.nDG), r.setup, periodic)
+    cand, pinfo = PS.poisson_step(M, types, n_pois, MAT, case["phi"], case["bias"], "newton-raphson")
+    defect = float(np.max(np.abs(cand - case["phi"])))
+    print(f"fixed point: defect {defect:.3e} V (run {case['last_potential_defect']:.3e}), Poisson relres "
+          f"{pinfo['physicalRelresBefore']:.3e} (run {case['last_poisson_relres']:.3e}), charge transfer "
+          f"{tinfo['relativeConservationError']:.1e}", flush=True)
+    assert tinfo["relativeConservationError"] < 1e-12 and pinfo["converged"]
+    assert defect <= 5e-4 and pinfo["physicalRelresBefore"] <= 8e-2
+    # and it reproduces the run's own last-iteration diagnostics (the
+    # transport states differ within the GMRES tolerance)
+    assert abs(defect - case["last_potential_defect"]) <= 0.1 * case["last_potential_defect"]
+    assert abs(pinfo["physicalRelresBefore"] - case["last_poisson_relres"]) <= 0.05 * case["last_poisson_relres"]
+
+
+def test_self_consistent_loop_converges_at_the_stored_state():
+    """The warm-started loop, started from the stored 0.2 V potential,
+    stops by the run's criteria within three iterations and stays at the
+    stored state."""
+    case = CASES[1]
+    res = PS.solve2d_rtd_self_consistent(MAT, initial_phi=case["phi"], biases=[case["bias"]],
+                                         poisson_method="newton-raphson", max_iterations=6)
+    sw, hist = res["sweep"], res["histories"][0]
+    for h in hist:
+        print(h, flush=True)
+    assert res["converged"] and sw["status"][0] == "fixed-point-converged" and len(hist) <= 3
+    assert np.max(np.abs(sw["phi"][0] - case["phi"])) < 1e-4
+    assert abs(np.mean(sw["current"][0]) - np.mean(case["j"])) <= 1e-3 * abs(np.mean(case["j"]))
