@@ -12,6 +12,8 @@ around the device transport solve).
 * `solve2d_rtd_self_consistent` is the warm-started transport/Poisson loop
   with linear mixing and the unmixed-defect, density-change and Poisson
   residual tests (RTD/solve2D_rtd_self_consistent.m:88-205).
+* `transient_cn_self_consistent` interleaves Crank-Nicolson steps of the
+  device operator with a Poisson update every few steps.
 
 The Poisson grid holds Nx*Ny (C5: 4,693) unknowns, so its sparse LU runs on
 the host; every transport solve runs on the device.
@@ -325,3 +327,36 @@ def solve2d_rtd_self_consistent(mat: dict, max_iterations: int = 100, mixing: fl
         prev_phi = phi
         rho_guess = tr.rho
     return {"sweep": sweep, "histories": histories, "converged": all(sweep["converged"]) and len(hist) > 0}
+
+
+def transient_cn_self_consistent(op, phi0, bias: float, dt: float, steps: int, poisson_every: int = 10,
+                                 precision: str = "mixed", tol: float = 1e-8, restart: int = 30,
+                                 poisson_method: str = "newton-raphson", rho0=None, use_graph: bool = True):
+    """Crank-Nicolson steps of a full-2D operator (`Full2DOperator`) with a
+    2D Poisson update of the potential every `poisson_every` steps: after
+    each block of steps the device density is projected onto the Poisson
+    dual volumes, one Poisson step from the current phi gives the new phi,
+    and the operator's potential becomes Vc - phi (the contacts keep
+    phi = 0 and bias, so the inflow data stay valid).  rho0=None continues
+    from the operator's device state (e.g. a stationary solve).  Returns
+    (phi, history) with one entry per block."""
+    mat = op.setup.mat
+    Vc = np.asarray(mat["Vc"], float)
+    M, types, periodic = eval_poisson2d(mat)
+    phi = np.asarray(phi0, float).copy()
+    done, hist = 0, []
+    while done < steps:
+        k = min(int(poisson_every), steps - done)
+        _, info = op.transient_cn(dt, k, tol=tol, restart=restart, precision=precision,
+                                  rho0=rho0 if done == 0 else None, return_rho=False, use_graph=use_graph)
+        done += k
+        nDG, _, _ = op.observables()
+        n, tinfo = project_dg_density(np.real(nDG), op.setup, periodic)
+        new, pinfo = poisson_step(M, types, n, mat, phi, bias, poisson_method)
+        hist.append({"steps": done, "gmresIterations": info["iterations"], "maxStepRelres": info["max_relres"],
+                     "flag": info["flag"], "potentialUpdate": float(np.max(np.abs(new - phi))),
+                     "poissonPhysicalRelres": pinfo["physicalRelresBefore"], "poissonConverged": pinfo["converged"],
+                     "chargeTransferError": tinfo["relativeConservationError"]})
+        phi = new
+        op.set_potential(Vc - phi)
+    return phi, hist

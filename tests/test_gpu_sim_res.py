@@ -68,3 +68,29 @@ def test_self_consistent_loop_converges_at_the_stored_state():
     assert res["converged"] and sw["status"][0] == "fixed-point-converged" and len(hist) <= 3
     assert np.max(np.abs(sw["phi"][0] - case["phi"])) < 1e-4
     assert abs(np.mean(sw["current"][0]) - np.mean(case["j"])) <= 1e-3 * abs(np.mean(case["j"]))
+
+
+def test_self_consistent_transient_stays_at_the_stationary_state():
+    """Crank-Nicolson steps with a 2D Poisson update every 10 steps, started
+    from the stationary solution at the stored 0.2 V state: a self-consistent
+    stationary state barely moves (potential updates at the fixed-point
+    defect, density within 1e-3)."""
+    from real_space_dg_amd.solver import SolveOptions
+    case = CASES[1]
+    setup = P.Full2DSetup(MAT, case["V"], case["EfL"], case["EfR"])
+    with P.Full2DOperator(setup) as op:
+        op.upload_rhs()
+        op.set_line_transverse()
+        _, info = op.solve(SolveOptions(tol=5e-6, restart=40, max_iter=6000))
+        assert info.flag == 0
+        n0, _, _ = op.observables()
+        op.set_preconditioner("rowabs")
+        phi, hist = PS.transient_cn_self_consistent(op, case["phi"], case["bias"], 1e-15, 20, poisson_every=10,
+                                                    precision="mixed", tol=1e-8)
+        for h in hist:
+            print(h, flush=True)
+        n1, _, _ = op.observables()
+    assert len(hist) == 2 and all(h["flag"] == 0 and h["poissonConverged"] for h in hist)
+    assert max(h["potentialUpdate"] for h in hist) < 1e-3
+    assert np.max(np.abs(n1 - n0)) <= 1e-3 * np.max(np.abs(n0))
+    assert np.max(np.abs(phi - case["phi"])) < 1e-3
