@@ -58,9 +58,12 @@ def parse():
     ap.add_argument("--cpu-sample-iters", type=int, default=6)
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the batched-sweep and self-consistent workloads reported beside the metric")
-    ap.add_argument("--only", choices=["batched_sweep", "self_consistent", "transient_full2d"], default=None,
+    ap.add_argument("--only", choices=["batched_sweep", "self_consistent", "transient_full2d", "transient_full2d_batch"],
+                    default=None,
                     help="run just this side workload and print its JSON (profiling runs)")
     ap.add_argument("--full2d-steps", type=int, default=2, help="Crank-Nicolson steps of the full-2D transient")
+    ap.add_argument("--full2d-instances", type=int, default=8,
+                    help="transverse-mode instances of the batched full-2D transient (--only transient_full2d_batch)")
     ap.add_argument("--full2d-precisions", default="mixed,fp64", help="precisions of the full-2D transient (--only)")
     ap.add_argument("--calibrate-copy", type=float, default=0.0, metavar="BYTES",
                     help="first run one plain device copy of BYTES bytes (HBM-counter calibration)")
@@ -289,6 +292,56 @@ def transient_full2d(steps: int = 2, dt: float = 1e-15, precisions=("mixed", "fp
     return out
 
 
+def transient_full2d_batch(instances: int = 8, steps: int = 1, dt: float = 1e-15, precision: str = "mixed") -> dict:
+    """The full-2D FET slab transient with several transverse-mode instances
+    (subband offsets of 5 meV each on the gate potential) stepped together by
+    wdg_transient_cn_batch, against the same instances stepped one after
+    another.  Reports instance-steps/s of both."""
+    from real_space_dg_amd import full2d as F2
+    from real_space_dg_amd.devices import fet_slab
+    from real_space_dg_amd.solver import transient_cn_batch
+    mat, V = fet_slab()
+    setups = [F2.Full2DSetup(mat, V + 0.005 * i, 0.1, 0.05) for i in range(instances)]
+    n = setups[0].n_total
+    out = {"workload": f"FET slab 40x20 nm, 64x32 DG elements p=1 x 64x64 relative cells, {instances} transverse-mode "
+                       f"instances (gate potential + 5 meV x mode); Crank-Nicolson dt {dt:g} s x {steps} steps from "
+                       "zero, GMRES(30) + row scaling to 1e-8 per step", "unknowns_per_instance": n,
+           "instances": instances, "precision": precision}
+    ops = []
+    try:
+        for s in setups:
+            op = F2.Full2DOperator(s)
+            op.upload_rhs()
+            op.set_preconditioner("rowabs")
+            ops.append(op)
+        zeros = [np.zeros(n, complex)] * instances
+        transient_cn_batch(ops, dt, 1, tol=1e-8, precision=precision, rho0=zeros, return_rho=False)   # warm
+        t0 = time.perf_counter()
+        _, infos = transient_cn_batch(ops, dt, steps, tol=1e-8, precision=precision, rho0=zeros, return_rho=False)
+        batched = time.perf_counter() - t0
+        dens_b = [op.observables()[0] for op in ops]
+        print(f"[bench] transient full2D batch of {instances}: {batched:.2f} s", file=sys.stderr, flush=True)
+        t0 = time.perf_counter()
+        its = []
+        for op in ops:
+            _, info = op.transient_cn(dt, steps, tol=1e-8, precision=precision, rho0=zeros[0], return_rho=False)
+            its.append(info["iterations"])
+        serial = time.perf_counter() - t0
+        print(f"[bench] transient full2D serial {instances}: {serial:.2f} s", file=sys.stderr, flush=True)
+        diff = max(float(np.max(np.abs(op.observables()[0] - d)) / np.max(np.abs(d))) for op, d in zip(ops, dens_b))
+    finally:
+        for op in ops:
+            op.close()
+    out["batched"] = {"instance_steps_per_s": instances * steps / batched, "seconds": batched,
+                      "gmres_iterations_per_step": [i["iterations"] / max(i["steps"], 1) for i in infos],
+                      "flags": [i["flag"] for i in infos]}
+    out["serial"] = {"instance_steps_per_s": instances * steps / serial, "seconds": serial,
+                     "gmres_iterations_per_step": [i / max(steps, 1) for i in its]}
+    out["batched_over_serial"] = serial / batched
+    out["batched_vs_serial_density_rel_diff"] = diff
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -300,7 +353,9 @@ def main():
     if args.only:
         fn = {"batched_sweep": batched_sweep, "self_consistent": self_consistent,
               "transient_full2d": lambda: transient_full2d(steps=args.full2d_steps,
-                                                           precisions=tuple(args.full2d_precisions.split(",")))
+                                                           precisions=tuple(args.full2d_precisions.split(","))),
+              "transient_full2d_batch": lambda: transient_full2d_batch(instances=args.full2d_instances,
+                                                                       steps=args.full2d_steps)
               }[args.only]
         print(json.dumps({args.only: fn()}))
         return

@@ -151,6 +151,16 @@ int wdg_diagonal(wdg_ctx* ctx, double* d);
 int wdg_solve(wdg_ctx* ctx, const wdg_solve_opts* opts, const double* x0, double* x,
               wdg_solve_info* info);
 
+/* wdg_solve (zero start, solution left on the device) on n independent
+ * instances together: every instance runs its restart cycles on its own
+ * stream and the host polls each one once per cycle, so all unconverged
+ * instances are in flight at once.  Replaces the serial loop over
+ * transverse modes and valleys of DG_code/solve_transport_DG_2D.m:62-209
+ * (one stationary solve per mode).  Instances may differ in grid and
+ * potential, 1D and full-2D alike; infos has n entries.  On failure the
+ * message of wdg_last_error(NULL) names the instance. */
+int wdg_solve_batch(wdg_ctx* const* ctxs, int32_t n, const wdg_solve_opts* opts, wdg_solve_info* infos);
+
 /* Observables of rho (or of the last solution when rho is NULL):
  * rho_u = phi*rho per node, n = Re rho_u at xi=0, j = q*hbar/m*Im d rho_u/dxi
  * (DG_code/solve_transport_DG_2D.m:114-156).  m_nodes: x-mass per DG node
@@ -255,6 +265,18 @@ typedef struct wdg_cn_info {
 } wdg_cn_info;
 int wdg_transient_cn(wdg_ctx* ctx, const wdg_cn_opts* opts, const double* rho0, double* rho_out,
                      wdg_cn_info* info);
+/* wdg_transient_cn on n independent instances together: each step launches
+ * every instance's right-hand side, then runs all the step solves with their
+ * restart cycles interleaved on the instances' own streams (as
+ * wdg_solve_batch), so the GPU keeps every unconverged instance in flight.
+ * rho0 / rho_out are arrays of n host pointers (the arrays themselves may be
+ * NULL: all instances start from their last solution / no copy back); infos
+ * has n entries.  All instances take the same options.  No counterpart in the
+ * reference, which has no transient for these instances (its full-2D
+ * transient entry points are stubs); on failure the message of
+ * wdg_last_error(NULL) names the instance. */
+int wdg_transient_cn_batch(wdg_ctx* const* ctxs, int32_t n, const wdg_cn_opts* opts, const double* const* rho0,
+                           double* const* rho_out, wdg_cn_info* infos);
 
 /* Formulation of the Theta[V] apply.  WDG_THETA_FFT evaluates phi U and
  * i phi^H Z as chirp-z transforms (FFTs of length 2^p 3^q >= nk+n_xi-1) in

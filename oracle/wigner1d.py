@@ -259,8 +259,9 @@ def verteilung(mat, p, N_xi, phi, EfL, EfR, V, m):
     EL = V[0] + hb ** 2 * k_fermi ** 2 / 2 / m[0] / q
     ER = V[Nx - 1] + hb ** 2 * k_fermi ** 2 / 2 / m[Nx - 1] / q
     if p["choose_fermi"] == "DG_MSA_3D":                                   # :77-114
-        fL = 2 * p["deg_factor_temp"] * (1.0 / (1.0 + np.exp(q * (EL - EfL) / (kB * Temp))))
-        fR = 2 * p["deg_factor_temp"] * (1.0 / (1.0 + np.exp(q * (ER - EfR) / (kB * Temp))))
+        with np.errstate(over="ignore"):      # exp -> inf far above EF: occupation 0, as in MATLAB
+            fL = 2 * p["deg_factor_temp"] * (1.0 / (1.0 + np.exp(q * (EL - EfL) / (kB * Temp))))
+            fR = 2 * p["deg_factor_temp"] * (1.0 / (1.0 + np.exp(q * (ER - EfR) / (kB * Temp))))
     elif p["choose_fermi"] == "TSU_ESAKI":
         # transverse-integrated contacts: the form of verteilung.m:64-65 ('LS')
         # m kT/(pi hbar^2) ln(1+exp(q(EF-E)/kT)) with the device's EF, band

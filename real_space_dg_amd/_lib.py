@@ -162,6 +162,9 @@ SIGNATURES = {
     "wdg_host_setup": (C.c_int, [C.POINTER(wdg_desc), C.POINTER(wdg_potential), _D, _D]),
     "wdg_transient_rk4": (C.c_int, [_P, C.POINTER(wdg_transient_opts), _D, _D, _D, _D, _D, C.c_int32]),
     "wdg_transient_cn": (C.c_int, [_P, C.POINTER(wdg_cn_opts), _D, _D, C.POINTER(wdg_cn_info)]),
+    "wdg_solve_batch": (C.c_int, [C.POINTER(_P), C.c_int32, C.POINTER(wdg_solve_opts), C.POINTER(wdg_solve_info)]),
+    "wdg_transient_cn_batch": (C.c_int, [C.POINTER(_P), C.c_int32, C.POINTER(wdg_cn_opts), C.POINTER(_D),
+                                         C.POINTER(_D), C.POINTER(wdg_cn_info)]),
     "wdg_set_theta_method": (C.c_int, [_P, C.c_int32]),
     "wdg_theta_method": (C.c_int, [_P]),
     "wdg2_create": (C.c_int, [C.POINTER(wdg2_desc), C.POINTER(wdg2_part), C.c_int32, C.POINTER(_P)]),

@@ -171,6 +171,14 @@ int fail(wdg_ctx* c, int code, const std::string& msg) {
   return code;
 }
 
+// Failure of instance i of a batched call: the instance's own message and,
+// for wdg_last_error(NULL), the same message naming the instance.
+int batch_fail(wdg_ctx* c, int i, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  g_create_error = "instance " + std::to_string(i) + ": " + msg;
+  return code;
+}
+
 template <typename F>
 int guarded(wdg_ctx* c, F&& f) {
   try {
@@ -1509,10 +1517,10 @@ int wdg_solve(wdg_ctx* ctx, const wdg_solve_opts* opts, const double* x0, double
 int wdg_solve_batch(wdg_ctx* const* ctxs, int32_t n, const wdg_solve_opts* opts, wdg_solve_info* infos) {
   if (!ctxs || n < 0 || !opts || !infos) return fail(nullptr, WDG_ERR_ARG, "null argument");
   for (int i = 0; i < n; ++i) {
-    if (!ctxs[i]) return fail(nullptr, WDG_ERR_ARG, "null context in batch");
-    if (!ctxs[i]->have_potential) return fail(ctxs[i], WDG_ERR_STATE, "wdg_set_potential has not been called");
+    if (!ctxs[i]) return batch_fail(nullptr, i, WDG_ERR_ARG, "null context in batch");
+    if (!ctxs[i]->have_potential) return batch_fail(ctxs[i], i, WDG_ERR_STATE, "wdg_set_potential has not been called");
     for (int j = 0; j < i; ++j)
-      if (ctxs[j] == ctxs[i]) return fail(ctxs[i], WDG_ERR_ARG, "a context appears twice in the batch");
+      if (ctxs[j] == ctxs[i]) return batch_fail(ctxs[i], i, WDG_ERR_ARG, "a context appears twice in the batch");
   }
   std::vector<std::unique_ptr<SolveRun>> runs;
   runs.reserve(n);
@@ -1552,7 +1560,7 @@ int wdg_solve_batch(wdg_ctx* const* ctxs, int32_t n, const wdg_solve_opts* opts,
     });
     if (rc != WDG_OK) failed = i;
   }
-  if (rc != WDG_OK && failed >= 0) g_create_error = std::string("instance ") + std::to_string(failed) + ": " + ctxs[failed]->err;
+  if (rc != WDG_OK && failed >= 0) batch_fail(ctxs[failed], failed, rc, ctxs[failed]->err);
   return rc;
 }
 
@@ -1767,25 +1775,37 @@ int wdg_transient_rk4(wdg_ctx* ctx, const wdg_transient_opts* o, const double* r
   });
 }
 
-int wdg_transient_cn(wdg_ctx* ctx, const wdg_cn_opts* o, const double* rho0, double* rho_out, wdg_cn_info* info) {
-  if (!ctx || !o || !info) return fail(ctx, WDG_ERR_ARG, "null argument");
-  if (!ctx->have_potential) return fail(ctx, WDG_ERR_STATE, "the operator has no potential yet");
-  std::memset(info, 0, sizeof(*info));
-  auto restore = [&]() {
-    if (ctx->shift_on) {
-      ctx->shift_on = false;
-      clear_graphs(ctx);
-      if (ctx->f2 && (ctx->f2->precond == WDG2_PRECOND_JACOBI || ctx->f2->precond == WDG2_PRECOND_ROWABS))
-        ctx->f2->precond = -ctx->f2->precond;   // rebuilt for the unshifted operator on the next solve
-    }
-  };
-  const int rc = guarded(ctx, [&]() {
+namespace {
+
+// One instance of a Crank-Nicolson run: the shifted operator, the per-step
+// right-hand side and the restore of the unshifted state.  Steps of several
+// instances interleave in wdg_transient_cn_batch.
+struct CnRun {
+  wdg_ctx* ctx;
+  const wdg_cn_opts* o;
+  wdg_cn_info* info;
+  wdg_solve_opts so{};
+  double sigma = 1, beta = 0, gamma = 0, cn = 1, max_rel = 0;
+  int flag = 0;
+  std::vector<double> part;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  std::unique_ptr<SolveRun> run;
+  wdg_solve_info si{};
+
+  CnRun(wdg_ctx* c, const wdg_cn_opts* opts, wdg_cn_info* inf) : ctx(c), o(opts), info(inf) {}
+  ~CnRun() {
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+  }
+  // Initial state, shift and preconditioner (device units: rho = sigma x,
+  // sigma = ||b|| / s_A; b_hat = b / ||b||).
+  void begin(const double* rho0) {
+    std::memset(info, 0, sizeof(*info));
     if (!(o->dt > 0) || o->nsteps < 0 || !std::isfinite(o->a_scale) || !std::isfinite(o->b_scale) || !(o->tol > 0))
       throw std::invalid_argument("dt and tol must be positive, nsteps >= 0, finite scales");
     const long N = ctx->N;
     const double sA = ctx->g.scale;
-    // device units: rho = sigma x with sigma = ||b|| / s_A; b_hat = b / ||b||
-    const double sigma = (ctx->rhs_norm > 0) ? ctx->rhs_norm / sA : 1.0 / sA;
+    sigma = (ctx->rhs_norm > 0) ? ctx->rhs_norm / sA : 1.0 / sA;
     if (rho0) {
       ctx->stage.resize(2 * N);
       for (long i = 0; i < 2 * N; ++i) {
@@ -1805,13 +1825,12 @@ int wdg_transient_cn(wdg_ctx* ctx, const wdg_cn_opts* o, const double* rho0, dou
     HIPCHK(hipMemcpyAsync(ctx->d_b0.p, ctx->d_b.p, N * 16, hipMemcpyDeviceToDevice, ctx->stream));
     // (I + beta A_hat) x+ = (I - beta A_hat) x + gamma b_hat: the trapezoidal
     // rule for d rho/dt = b_scale rhs - a_scale A rho in device units
-    const double beta = 0.5 * o->dt * o->a_scale * sA;
-    const double gamma = (ctx->rhs_norm > 0) ? o->dt * o->b_scale * sA : 0.0;
+    beta = 0.5 * o->dt * o->a_scale * sA;
+    gamma = (ctx->rhs_norm > 0) ? o->dt * o->b_scale * sA : 0.0;
     ctx->shift_on = true;
     ctx->shift_a = 1.0;
     ctx->shift_b = beta;
     clear_graphs(ctx);
-    wdg_solve_opts so{};
     so.tol = o->tol;
     so.restart = o->restart > 0 ? o->restart : 30;
     so.max_iter = o->max_iter > 0 ? o->max_iter : 20000;
@@ -1829,50 +1848,52 @@ int wdg_transient_cn(wdg_ctx* ctx, const wdg_cn_opts* o, const double* rho0, dou
     } else if (ctx->f2 && ctx->f2->precond == WDG2_PRECOND_LINE) {
       so.use_precond = 0;   // the line-transverse factors approximate A, not I + beta A
     }
-    const int nb = ctx->red_blocks;
-    std::vector<double> part(nb);
-    hipEvent_t e0, e1;
+    part.assign(ctx->red_blocks, 0.0);
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
     HIPCHK(hipEventRecord(e0, ctx->stream));
-    double max_rel = 0;
-    int flag = 0;
-    for (int step = 0; step < o->nsteps; ++step) {
-      op_apply<double>(ctx, ctx->d_x.p, ctx->d_y.p, nullptr);
-      hipLaunchKernelGGL((cn_rhs_kernel<256>), dim3(nb), dim3(256), 0, ctx->stream, ctx->d_x.p, ctx->d_y.p,
-                         ctx->d_b0.p, gamma, ctx->d_b.p, N, (double*)ctx->d_part.p);
-      HIPCHK(hipMemcpyAsync(part.data(), ctx->d_part.p, nb * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-      HIPCHK(hipStreamSynchronize(ctx->stream));
-      double s2 = 0;
-      for (double v : part) s2 += v;
-      const double cn = std::sqrt(s2);
-      if (!(cn > 0) || !std::isfinite(cn)) throw std::runtime_error("Crank-Nicolson right-hand side vanished or is not finite");
-      hipLaunchKernelGGL(scale_complex_kernel, dim3(nb), dim3(256), 0, ctx->stream, ctx->d_b.p, 1.0 / cn, N);
-      hipLaunchKernelGGL(scale_complex_kernel, dim3(nb), dim3(256), 0, ctx->stream, ctx->d_x.p, 1.0 / cn, N);
-      wdg_solve_info si;
-      {
-        SolveRun r(ctx, so, &si);
-        r.begin(nullptr);
-        r.sync_poll();
-        while (r.active) {
-          r.step();
-          r.sync_poll();
-        }
-        r.end(nullptr);
-      }
-      hipLaunchKernelGGL(scale_complex_kernel, dim3(nb), dim3(256), 0, ctx->stream, ctx->d_x.p, cn, N);
-      info->iterations += si.iterations;
-      info->max_step_iterations = std::max(info->max_step_iterations, si.iterations);
-      max_rel = std::max(max_rel, si.true_relres);
-      if (si.flag != 0) flag = 1;
-      info->steps = step + 1;
-    }
+  }
+  // Step right-hand side c = 2x - (I + beta A_hat) x + gamma b_hat and its
+  // norm partials (asynchronous on the instance's stream).
+  void rhs_launch() {
+    const long N = ctx->N;
+    const int nb = ctx->red_blocks;
+    op_apply<double>(ctx, ctx->d_x.p, ctx->d_y.p, nullptr);
+    hipLaunchKernelGGL((cn_rhs_kernel<256>), dim3(nb), dim3(256), 0, ctx->stream, ctx->d_x.p, ctx->d_y.p,
+                       ctx->d_b0.p, gamma, ctx->d_b.p, N, (double*)ctx->d_part.p);
+    HIPCHK(hipMemcpyAsync(part.data(), ctx->d_part.p, nb * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  }
+  // After rhs_launch: normalise the step system and start its solve.
+  void rhs_finish() {
+    const long N = ctx->N;
+    const int nb = ctx->red_blocks;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    double s2 = 0;
+    for (double v : part) s2 += v;
+    cn = std::sqrt(s2);
+    if (!(cn > 0) || !std::isfinite(cn)) throw std::runtime_error("Crank-Nicolson right-hand side vanished or is not finite");
+    hipLaunchKernelGGL(scale_complex_kernel, dim3(nb), dim3(256), 0, ctx->stream, ctx->d_b.p, 1.0 / cn, N);
+    hipLaunchKernelGGL(scale_complex_kernel, dim3(nb), dim3(256), 0, ctx->stream, ctx->d_x.p, 1.0 / cn, N);
+    run.reset(new SolveRun(ctx, so, &si));
+    run->begin(nullptr);
+  }
+  void step_end(int step) {
+    run->end(nullptr);
+    run.reset();
+    hipLaunchKernelGGL(scale_complex_kernel, dim3(ctx->red_blocks), dim3(256), 0, ctx->stream, ctx->d_x.p, cn,
+                       ctx->N);
+    info->iterations += si.iterations;
+    info->max_step_iterations = std::max(info->max_step_iterations, si.iterations);
+    max_rel = std::max(max_rel, si.true_relres);
+    if (si.flag != 0) flag = 1;
+    info->steps = step + 1;
+  }
+  void finish(double* rho_out) {
+    const long N = ctx->N;
     HIPCHK(hipEventRecord(e1, ctx->stream));
     HIPCHK(hipEventSynchronize(e1));
     float ms = 0;
     HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     info->seconds = ms * 1e-3;
     info->max_relres = max_rel;
     info->flag = flag;
@@ -1885,13 +1906,76 @@ int wdg_transient_cn(wdg_ctx* ctx, const wdg_cn_opts* o, const double* rho0, dou
       for (long i = 0; i < 2 * N; ++i) rho_out[i] *= sigma;
     }
     HIPCHK(hipStreamSynchronize(ctx->stream));
-    return WDG_OK;
-  });
-  try {
-    restore();
-  } catch (...) {
   }
+  // The unshifted operator again (also after a failure).
+  void restore() {
+    if (ctx->shift_on) {
+      ctx->shift_on = false;
+      clear_graphs(ctx);
+      if (ctx->f2 && (ctx->f2->precond == WDG2_PRECOND_JACOBI || ctx->f2->precond == WDG2_PRECOND_ROWABS))
+        ctx->f2->precond = -ctx->f2->precond;   // rebuilt for the unshifted operator on the next solve
+    }
+  }
+};
+
+}  // namespace
+
+int wdg_transient_cn_batch(wdg_ctx* const* ctxs, int32_t n, const wdg_cn_opts* o, const double* const* rho0,
+                           double* const* rho_out, wdg_cn_info* infos) {
+  if (!ctxs || n < 0 || !o || !infos) return fail(nullptr, WDG_ERR_ARG, "null argument");
+  for (int i = 0; i < n; ++i) {
+    if (!ctxs[i]) return batch_fail(nullptr, i, WDG_ERR_ARG, "null context in batch");
+    if (!ctxs[i]->have_potential) return batch_fail(ctxs[i], i, WDG_ERR_STATE, "the operator has no potential yet");
+    for (int j = 0; j < i; ++j)
+      if (ctxs[j] == ctxs[i]) return batch_fail(ctxs[i], i, WDG_ERR_ARG, "a context appears twice in the batch");
+  }
+  std::vector<std::unique_ptr<CnRun>> runs;
+  for (int i = 0; i < n; ++i) runs.emplace_back(new CnRun(ctxs[i], o, &infos[i]));
+  int rc = WDG_OK, failed = -1;
+  // run f on every instance in order; stop at the first failure
+  auto each = [&](auto&& f) {
+    for (int i = 0; i < n && rc == WDG_OK; ++i) {
+      rc = guarded(ctxs[i], [&]() {
+        f(*runs[i], i);
+        return WDG_OK;
+      });
+      if (rc != WDG_OK) failed = i;
+    }
+  };
+  each([&](CnRun& r, int i) { r.begin(rho0 ? rho0[i] : nullptr); });
+  // per step: every instance's right-hand side, then all step solves with
+  // their restart cycles interleaved on the instances' streams
+  for (int step = 0; step < o->nsteps && rc == WDG_OK; ++step) {
+    each([&](CnRun& r, int) { r.rhs_launch(); });
+    each([&](CnRun& r, int) { r.rhs_finish(); });
+    bool any = true;
+    while (rc == WDG_OK && any) {
+      any = false;
+      each([&](CnRun& cr, int) {
+        SolveRun& r = *cr.run;
+        if (r.pending) r.sync_poll();
+        if (r.active) {
+          r.step();
+          any = true;
+        }
+      });
+    }
+    each([&](CnRun& r, int) { r.step_end(step); });
+  }
+  each([&](CnRun& r, int i) { r.finish(rho_out ? rho_out[i] : nullptr); });
+  for (auto& r : runs) {
+    try {
+      r->restore();
+    } catch (...) {
+    }
+  }
+  if (rc != WDG_OK && failed >= 0) batch_fail(ctxs[failed], failed, rc, ctxs[failed]->err);
   return rc;
+}
+
+int wdg_transient_cn(wdg_ctx* ctx, const wdg_cn_opts* o, const double* rho0, double* rho_out, wdg_cn_info* info) {
+  if (!ctx || !o || !info) return fail(ctx, WDG_ERR_ARG, "null argument");
+  return wdg_transient_cn_batch(&ctx, 1, o, &rho0, &rho_out, info);
 }
 
 int wdg_set_theta_method(wdg_ctx* ctx, int32_t method) {

@@ -265,19 +265,58 @@ def read_profile(lib, ctx) -> dict:
     return {k: {"avg_ms": avg[i], "launches": cnt[i], "units": units[i]} for i, k in enumerate(keys[:ne.value])}
 
 
+def _cn_opts(dt, nsteps, a_scale, b_scale, tol, restart, precision, use_precond, krylov_basis, use_graph):
+    return _lib.wdg_cn_opts(float(dt), int(nsteps), float(a_scale), float(b_scale), float(tol), int(restart), 20000,
+                            PRECISIONS[precision], {"fp32": _lib.WDG_BASIS_WORKING, "bf16": _lib.WDG_BASIS_BF16,
+                                                    "fp16": _lib.WDG_BASIS_FP16}[krylov_basis],
+                            1 if use_precond else 0, 1 if use_graph else 0)
+
+
+def _handle(op):
+    """(library, context handle, unknowns) of a WignerOperator or Full2DOperator."""
+    if hasattr(op, "_h"):
+        return op._lib, op._h, op.N
+    return op._lib, op.ctx, op.n
+
+
 def transient_cn(lib, ctx, n: int, dt: float, nsteps: int, a_scale: float, b_scale: float, tol: float, restart: int,
                  precision: str, rho0, return_rho: bool, use_precond: bool = True, krylov_basis: str = "fp16",
                  use_graph: bool = True):
     """Shared ctypes call of wdg_transient_cn for 1D and full-2D contexts."""
     r0 = None if rho0 is None else _as_c128(rho0, n)
     rho = np.empty(n, np.complex128) if return_rho else None
-    o = _lib.wdg_cn_opts(float(dt), int(nsteps), float(a_scale), float(b_scale), float(tol), int(restart), 20000,
-                         PRECISIONS[precision], {"fp32": _lib.WDG_BASIS_WORKING, "bf16": _lib.WDG_BASIS_BF16,
-                                                 "fp16": _lib.WDG_BASIS_FP16}[krylov_basis], 1 if use_precond else 0,
-                             1 if use_graph else 0)
+    o = _cn_opts(dt, nsteps, a_scale, b_scale, tol, restart, precision, use_precond, krylov_basis, use_graph)
     info = _lib.wdg_cn_info()
     check(lib.wdg_transient_cn(ctx, C.byref(o), dptr(r0), dptr(rho), C.byref(info)), ctx)
     return rho, {k: getattr(info, k) for k, _ in info._fields_}
+
+
+def transient_cn_batch(ops, dt: float, nsteps: int, a_scale: float = 1.0, b_scale: float = 1.0, tol: float = 1e-10,
+                       restart: int = 30, precision: str = "fp64", rho0=None, return_rho: bool = True,
+                       use_precond: bool = True, krylov_basis: str = "fp16", use_graph: bool = True):
+    """Crank-Nicolson steps of several independent instances together
+    (wdg_transient_cn_batch): per step every instance's right-hand side, then
+    all step solves with their restart cycles interleaved on the instances'
+    streams.  ops: WignerOperator and/or Full2DOperator objects; rho0: None
+    (every instance from its last solution) or one initial state (or None)
+    per instance.  Returns (list of rho or None, list of info dicts)."""
+    ops = list(ops)
+    if not ops:
+        return [], []
+    hs = [_handle(op) for op in ops]
+    lib = hs[0][0]
+    n = len(ops)
+    if rho0 is not None and len(rho0) != n:
+        raise ValueError(f"rho0 needs one entry per instance ({n}), got {len(rho0)}")
+    r0 = [None if rho0 is None or rho0[i] is None else _as_c128(rho0[i], hs[i][2]) for i in range(n)]
+    outs = [np.empty(h[2], np.complex128) if return_rho else None for h in hs]
+    ctxs = (C.c_void_p * n)(*[h[1] for h in hs])
+    r0p = (_lib._D * n)(*[dptr(r) for r in r0])
+    outp = (_lib._D * n)(*[dptr(r) for r in outs])
+    infos = (_lib.wdg_cn_info * n)()
+    o = _cn_opts(dt, nsteps, a_scale, b_scale, tol, restart, precision, use_precond, krylov_basis, use_graph)
+    check(lib.wdg_transient_cn_batch(ctxs, n, C.byref(o), r0p, outp, infos), None)
+    return outs, [{k: getattr(info, k) for k, _ in info._fields_} for info in infos]
 
 
 def solve_batch(ops, opts: SolveOptions | None = None):
@@ -290,8 +329,9 @@ def solve_batch(ops, opts: SolveOptions | None = None):
     ops = list(ops)
     if not ops:
         return []
-    lib = ops[0]._lib
-    arr = (C.c_void_p * len(ops))(*[op._h for op in ops])
+    hs = [_handle(op) for op in ops]
+    lib = hs[0][0]
+    arr = (C.c_void_p * len(ops))(*[h[1] for h in hs])
     infos = (_lib.wdg_solve_info * len(ops))()
     co = opts.to_c()
     rc = lib.wdg_solve_batch(arr, len(ops), C.byref(co), infos)

@@ -194,3 +194,58 @@ def test_crank_nicolson_full2d_matches_oracle(precision):
         # the stationary solve still works after the transient (preconditioner restored)
         x, sinfo = op.solve(SolveOptions(tol=1e-8, restart=64, max_iter=60000))
         assert sinfo.flag == 0
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_crank_nicolson_batch_matches_single_runs(precision):
+    """wdg_transient_cn_batch over three full-2D instances of different
+    fluxes and boundaries gives each instance the state its own
+    wdg_transient_cn run gives, and the oracle's directly solved steps
+    (small grids: the oracle's sparse LU of the larger tiled ones takes
+    minutes)."""
+    from oracle import transient as TR
+    from real_space_dg_amd.solver import transient_cn_batch
+    names = ["rusanov-specular-fv", "upwind-zero-inflow", "periodic-rusanov-fv"]
+    tol = 1e-12 if precision == "fp64" else 1e-10
+    ops, refs, rho0s, dts = [], [], [], []
+    try:
+        for k, name in enumerate(names):
+            m, EfL, EfR = ALL[name]
+            p, A, rhs, rho_inf = F.solve_direct_full2d(m, m["V"], EfL, EfR)
+            dt = 20.0 / abs(A).sum(axis=1).max()
+            rng = np.random.default_rng(5 + k)
+            rho0 = 0.1 * np.max(np.abs(rho_inf)) * (rng.standard_normal(p["nTotal"])
+                                                     + 1j * rng.standard_normal(p["nTotal"]))
+            op = P.Full2DOperator(P.Full2DSetup(m, m["V"], EfL, EfR))
+            op.upload_rhs()
+            assert op.set_preconditioner("rowabs") == "rowabs"
+            ops.append(op)
+            print(f"CN batch instance {name}: N = {p['nTotal']}", flush=True)
+            rho0s.append(rho0)
+            dts.append(dt)
+            refs.append((A, rhs))
+        # one time step for all: the smallest of the three stability scales
+        dt = min(dts)
+        single = [op.transient_cn(dt, 5, tol=tol, restart=40, precision=precision, rho0=r0)
+                  for op, r0 in zip(ops, rho0s)]
+        outs, infos = transient_cn_batch(ops, dt, 5, tol=tol, restart=40, precision=precision, rho0=rho0s)
+        print(f"CN batch {precision}: {infos}", flush=True)
+        for (rho_s, info_s), rho_b, info_b, (A, rhs), r0 in zip(single, outs, infos, refs, rho0s):
+            assert info_b["flag"] == 0 and info_b["steps"] == 5
+            assert info_b["iterations"] == info_s["iterations"]
+            assert rel(rho_b, rho_s) < 1e-12
+            ref = TR.crank_nicolson(A, rhs, r0, dt, 5)
+            assert rel(rho_b, ref) < 1e-8, rel(rho_b, ref)
+        # the batch leaves each instance's final state on the device
+        for op, rho_b, name in zip(ops, outs, names):
+            nDG, _, _ = op.observables()
+            n_ref, _, _ = op.observables(rho_b)
+            assert rel(nDG, n_ref) < 1e-12
+        # an empty batch is a no-op, a repeated instance an error
+        assert transient_cn_batch([], dt, 1) == ([], [])
+        from real_space_dg_amd._lib import WdgError
+        with pytest.raises(WdgError, match="twice"):
+            transient_cn_batch([ops[0], ops[0]], dt, 1)
+    finally:
+        for op in ops:
+            op.close()
