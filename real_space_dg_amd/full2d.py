@@ -1072,5 +1072,6 @@ def solve_transport_DG_full2D(mat: dict, Vxy=None, EfL=None, EfR=None, rho0=None
     status = "solved-gmres" if info.flag == 0 else "gmres-not-converged"
     return Full2DResult(status, rho, n, jx, jy, j, nDG, jxDG, jyDG,
                         {"solve": {"flag": info.flag, "relres": info.true_relres, "trueRelres": info.true_relres,
+                                   "initialTrueRelres": info.initial_true_relres, "warmStartUsed": rho0 is not None,
                                    "iterations": info.iterations, "preconditioner": used,
                                    "seconds": info.seconds}}, setup)

@@ -292,6 +292,7 @@ def solve2d_rtd_self_consistent(mat: dict, max_iterations: int = 100, mixing: fl
             tr = solve_transport_DG_full2D(mat, V, efl, efr, rho0=rho_guess, solver=solver)
             s = tr.info["solve"]
             h = {"iteration": it, "transportFlag": s["flag"], "transportRelres": s["relres"],
+                 "warmStartUsed": bool(s["warmStartUsed"]), "initialTrueRelres": s["initialTrueRelres"],
                  "gmresIterations": s["iterations"], "densityChange": np.inf}
             if s["flag"] != 0 or not np.isfinite(s["relres"]):
                 status = "transport-not-converged"

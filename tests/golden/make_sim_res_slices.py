@@ -4,7 +4,8 @@ scipy.io.loadmat): the material description and dg.params of the run, and
 for three biases the potential V = Vc - phi handed to the transport solve,
 the converged density n (X DOF x material y) and current j (X DOF), the
 electrostatic potential phi, the solver's relative residual and the last
-self-consistency iteration's potential defect and Poisson residual.  Run from the repository root, in a
+self-consistency iteration's potential defect and Poisson residual; and the
+file's layout (variables, field names, sweep shapes, history fields).  Run from the repository root, in a
 container where the reference tree is present:
 
     python tests/golden/make_sim_res_slices.py
@@ -45,8 +46,17 @@ def main():
     last = [np.atleast_1d(hist[i])[-1] for i in BIASES]
     arrays["last_potential_defect"] = np.array([float(h.potentialDefect) for h in last])
     arrays["last_poisson_relres"] = np.array([float(h.poissonPhysicalRelres) for h in last])
+    # layout of the file itself: top-level variables, their fields (in the
+    # stored order), the sweep arrays' shapes and the history record fields
+    raw = sio.loadmat(SRC, squeeze_me=False, struct_as_record=True)
+    inventory = {k: list(raw[k].dtype.names) for k in ("sweep", "selfConsistency", "mat", "boundary")}
+    inventory["sweep_shapes"] = {f: list(np.shape(raw["sweep"][f][0, 0])) for f in raw["sweep"].dtype.names}
+    inventory["history_fields"] = list(raw["selfConsistency"]["history"][0, 0][0, 0].dtype.names)
+    inventory["history_cell_shape"] = list(raw["selfConsistency"]["history"][0, 0].shape)
+    inventory["history_first_shape"] = list(raw["selfConsistency"]["history"][0, 0][0, 0].shape)
     np.savez_compressed(
         OUT, params_json=np.array(json.dumps(params)), scalars_json=np.array(json.dumps(scal)), **arrays,
+        inventory_json=np.array(json.dumps(inventory)),
         bias_index=ib, bias=np.asarray(sweep.bias, float)[ib],
         potential=np.asarray(sweep.potential, float)[:, :, ib], phi=np.asarray(sweep.phi, float)[:, :, ib],
         density=np.asarray(sweep.density, float)[:, :, ib],
