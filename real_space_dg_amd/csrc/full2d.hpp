@@ -12,7 +12,8 @@
 //                    both axes, the FV-consistent potential
 //                    (1/16) P^T diag(DeltaV) P projected over the element's
 //                    tensor quadrature (get_Drift_full2D.m:331-364) or the
-//                    collocated diagonal, and the CAP.  Writes y.
+//                    collocated diagonal, and the CAP.  Writes y, or
+//                    sa u + sb y for a shifted operator.
 //                    With MODE_DIAG / MODE_ROWABS the same tiling produces the
 //                    drift diagonal and row-absolute bound instead
 //                    (get_Drift_full2D.m:466-523).
@@ -149,7 +150,7 @@ template <int NLX, int NLY, typename T, int MODE>
 __global__ void __launch_bounds__(kF2Block) f2_apply_kernel(const F2Const* __restrict__ cp,
                                                             const typename F2Vec<T>::T2* __restrict__ u,
                                                             typename F2Vec<T>::T2* __restrict__ y,
-                                                            const GmresState* st) {
+                                                            const GmresState* st, double sa, double sb) {
   using T2 = typename F2Vec<T>::T2;
   constexpr int NL = NLX * NLY;
   if (skip(st)) return;
@@ -494,11 +495,19 @@ __global__ void __launch_bounds__(kF2Block) f2_apply_kernel(const F2Const* __res
         }
       }
     }
+    // (sa I + sb A) u of a shifted operator (Crank-Nicolson): u from the tile
+    const bool shifted = MODE == F2_APPLY && (sa != 0.0 || sb != 1.0);
 #pragma unroll
     for (int a = 0; a < NL; ++a) {
       const long i = cdof(a % NLX, a / NLX);
       if (MODE == F2_APPLY) {
-        y[i * nrel + r] = out[a];
+        T2 o = out[a];
+        if (shifted) {
+          const T2 uu = su[((size_t)a * R2 + ryl + 1) * nrx + rx];
+          o.x = (T)sa * uu.x + (T)sb * o.x;
+          o.y = (T)sa * uu.y + (T)sb * o.y;
+        }
+        y[i * nrel + r] = o;
       } else {
         // diagonal / row sums accumulate onto the transport part
         T2 o = y[i * nrel + r];

@@ -68,7 +68,7 @@ void f2_require(const wdg_ctx* c) {
 // ---- kernel dispatch over the element node counts ----
 template <typename T, int MODE, int NLX, int NLY>
 void f2_launch_apply_n(wdg_ctx* c, const typename F2Vec<T>::T2* u, typename F2Vec<T>::T2* y,
-                       const GmresState* st, size_t lds) {
+                       const GmresState* st, size_t lds, double sa, double sb) {
   F2Data& f = *c->f2;
   auto k = f2_apply_kernel<NLX, NLY, T, MODE>;
   static bool attr = false;   // per instantiation: raise the dynamic LDS limit once
@@ -77,27 +77,29 @@ void f2_launch_apply_n(wdg_ctx* c, const typename F2Vec<T>::T2* u, typename F2Ve
     attr = true;
   }
   const unsigned grid = (unsigned)(f.d.ne_x * f.d.ne_y * f.hc.tiles);
-  hipLaunchKernelGGL(k, dim3(grid), dim3(kF2Block), lds, c->stream, f.d_c.p, u, y, st);
+  hipLaunchKernelGGL(k, dim3(grid), dim3(kF2Block), lds, c->stream, f.d_c.p, u, y, st, sa, sb);
 }
 
 template <typename T, int MODE, int NLX>
 void f2_launch_apply_x(wdg_ctx* c, const typename F2Vec<T>::T2* u, typename F2Vec<T>::T2* y,
-                       const GmresState* st, size_t lds) {
+                       const GmresState* st, size_t lds, double sa, double sb) {
   switch (c->f2->d.nl_y) {
-    case 2: f2_launch_apply_n<T, MODE, NLX, 2>(c, u, y, st, lds); break;
-    case 3: f2_launch_apply_n<T, MODE, NLX, 3>(c, u, y, st, lds); break;
-    case 4: f2_launch_apply_n<T, MODE, NLX, 4>(c, u, y, st, lds); break;
+    case 2: f2_launch_apply_n<T, MODE, NLX, 2>(c, u, y, st, lds, sa, sb); break;
+    case 3: f2_launch_apply_n<T, MODE, NLX, 3>(c, u, y, st, lds, sa, sb); break;
+    case 4: f2_launch_apply_n<T, MODE, NLX, 4>(c, u, y, st, lds, sa, sb); break;
     default: throw std::invalid_argument("nl_y outside 2..4");
   }
 }
 
+// sa, sb: y = sa u + sb (apply part) for a shifted operator (F2_APPLY only)
 template <typename T, int MODE>
-void f2_launch_apply(wdg_ctx* c, const typename F2Vec<T>::T2* u, typename F2Vec<T>::T2* y, const GmresState* st) {
+void f2_launch_apply(wdg_ctx* c, const typename F2Vec<T>::T2* u, typename F2Vec<T>::T2* y, const GmresState* st,
+                     double sa = 0.0, double sb = 1.0) {
   const size_t lds = std::is_same<T, double>::value ? c->f2->lds_apply64 : c->f2->lds_apply32;
   switch (c->f2->d.nl_x) {
-    case 2: f2_launch_apply_x<T, MODE, 2>(c, u, y, st, lds); break;
-    case 3: f2_launch_apply_x<T, MODE, 3>(c, u, y, st, lds); break;
-    case 4: f2_launch_apply_x<T, MODE, 4>(c, u, y, st, lds); break;
+    case 2: f2_launch_apply_x<T, MODE, 2>(c, u, y, st, lds, sa, sb); break;
+    case 3: f2_launch_apply_x<T, MODE, 3>(c, u, y, st, lds, sa, sb); break;
+    case 4: f2_launch_apply_x<T, MODE, 4>(c, u, y, st, lds, sa, sb); break;
     default: throw std::invalid_argument("nl_x outside 2..4");
   }
 }
@@ -119,7 +121,7 @@ int f2_dense_chunk(int n, int ncol, int lines, int* rglobal) {
 
 template <typename T>
 void f2_launch_dense(wdg_ctx* c, const F2Data::Dense& dp, const typename F2Vec<T>::T2* u, typename F2Vec<T>::T2* y,
-                     const GmresState* st) {
+                     const GmresState* st, double scale = 1.0) {
   F2Data& f = *c->f2;
   static bool attr = false;
   if (!attr) {
@@ -132,26 +134,30 @@ void f2_launch_dense(wdg_ctx* c, const F2Data::Dense& dp, const typename F2Vec<T
   const int ncol = dp.k.axis == 0 ? f.d.n_rho_y : f.d.n_rho_x;
   const size_t lds = f2_dense_lds(dp.n, dp.k.chunk, sizeof(typename F2Vec<T>::T2), dp.k.rglobal != 0);
   const dim3 grid(dp.lines, (ncol + dp.k.chunk - 1) / dp.k.chunk);
-  if (dp.k.rglobal)
-    hipLaunchKernelGGL((f2_dense_kernel<T, true>), grid, dim3(kF2Block), lds, c->stream, f.d_c.p, dp.k, u, y, st);
+  F2Dense k = dp.k;
+  k.coef *= scale;
+  if (k.rglobal)
+    hipLaunchKernelGGL((f2_dense_kernel<T, true>), grid, dim3(kF2Block), lds, c->stream, f.d_c.p, k, u, y, st);
   else
-    hipLaunchKernelGGL((f2_dense_kernel<T, false>), grid, dim3(kF2Block), lds, c->stream, f.d_c.p, dp.k, u, y, st);
+    hipLaunchKernelGGL((f2_dense_kernel<T, false>), grid, dim3(kF2Block), lds, c->stream, f.d_c.p, k, u, y, st);
 }
 
-// y = A_hat u
+// y = A_hat u, or (sa I + sb A_hat) u: the shift rides on the apply kernel
+// (which holds u) and on the dense parts' coefficients
 template <typename T>
-void f2_apply(wdg_ctx* c, const typename F2Vec<T>::T2* u, typename F2Vec<T>::T2* y, const GmresState* st) {
+void f2_apply(wdg_ctx* c, const typename F2Vec<T>::T2* u, typename F2Vec<T>::T2* y, const GmresState* st,
+              double sa = 0.0, double sb = 1.0) {
   F2Data& f = *c->f2;
   const double vb = (double)c->N * sizeof(typename F2Vec<T>::T2);
   {
     ProfScope ps(c, P_F2_APPLY, 2.0 * vb);
-    f2_launch_apply<T, F2_APPLY>(c, u, y, st);
+    f2_launch_apply<T, F2_APPLY>(c, u, y, st, sa, sb);
   }
   for (auto& dp : f.dense) {
     // a face part reads one trace line and updates nl lines per transverse DOF
     const double lines = dp->k.face < 0 ? (double)f.ncen * 3 : (double)dp->lines * (1 + 2 * dp->k.nl);
     ProfScope ps(c, P_F2_DENSE, lines * f.nrel * sizeof(typename F2Vec<T>::T2));
-    f2_launch_dense<T>(c, *dp, u, y, st);
+    f2_launch_dense<T>(c, *dp, u, y, st, sb);
   }
 }
 

@@ -1138,11 +1138,19 @@ struct BasisIO<ushort4> {
   }
 };
 
+// Two 8-byte basis packets (FP16 / BF16) read with one 16-byte load.
+template <typename VP>
+struct VPPair {
+  VP a, b;
+};
+
 // Multi-dot: h[i] = <V_i, w> for i < nv (plus ||w||^2 when mode 0).
 // Grid (blocks, ceil(nv/kDotGroup)); block row y handles basis vectors
 // [G y, G y + G), G = kDotGroup.  mode 0 writes h1 and nrm0; mode 1 (only when st->reorth)
-// writes h2.  NP = number of 16-byte packets per vector.
-template <typename T, typename VP, int NT>
+// writes h2.  NP = number of working packets per vector.  W = 2 (8-byte
+// basis packets, NP even): every thread step reads two basis packets per
+// vector with one 16-byte load and the two matching working packets of w.
+template <typename T, typename VP, int NT, int W = 1>
 __global__ __launch_bounds__(NT) void arnoldi_dot_kernel(GmresBufs gb, const VP* __restrict__ V,
                                                          const typename Pk<T>::type* __restrict__ w, long NP, int nv,
                                                          int mode) {
@@ -1157,7 +1165,24 @@ __global__ __launch_bounds__(NT) void arnoldi_dot_kernel(GmresBufs gb, const VP*
 #pragma unroll
   for (int i = 0; i < kDotGroup + 1; ++i) ar[i] = ai[i] = 0;
   const VP* V0 = V + (long)i0 * NP;
-  if (ng == kDotGroup) {
+  if constexpr (W == 2) {
+    static_assert(sizeof(VP) == 8 && sizeof(VPPair<VP>) == 16, "two 8-byte packets per 16-byte load");
+    const long NQ = NP / 2;
+    const uint4* V0q = reinterpret_cast<const uint4*>(V0);
+    for (long x = (long)blockIdx.x * NT + threadIdx.x; x < NQ; x += (long)gridDim.x * NT) {
+      const typename P::type w0 = w[2 * x], w1 = w[2 * x + 1];
+      uint4 raw[kDotGroup];
+#pragma unroll
+      for (int i = 0; i < kDotGroup; ++i) raw[i] = (i < ng) ? V0q[(long)i * NQ + x] : make_uint4(0, 0, 0, 0);
+#pragma unroll
+      for (int i = 0; i < kDotGroup; ++i) {
+        const VPPair<VP> pr = __builtin_bit_cast(VPPair<VP>, raw[i]);
+        P::dot_acc(ar[i], ai[i], IO::load_raw(pr.a), w0);
+        P::dot_acc(ar[i], ai[i], IO::load_raw(pr.b), w1);
+      }
+      ar[kDotGroup] += P::nrm2_w(w0) + P::nrm2_w(w1);
+    }
+  } else if (ng == kDotGroup) {
     // full group: the group's basis loads are issued together
     for (long x = (long)blockIdx.x * NT + threadIdx.x; x < NP; x += (long)gridDim.x * NT) {
       const typename P::type wv = w[x];
@@ -1210,7 +1235,7 @@ __global__ __launch_bounds__(NT) void arnoldi_dot_finalize_kernel(GmresBufs gb, 
 // finishes the Hessenberg column (DGKS test ||w_new|| >= ||w||/sqrt(2)) or
 // requests a second pass; mode 1 (only when st->reorth) uses h2 and finishes
 // the column with h1 + h2.
-template <typename T, typename VP, int NT>
+template <typename T, typename VP, int NT, int W = 1>
 __global__ __launch_bounds__(NT) void arnoldi_update_kernel(GmresBufs gb, const VP* __restrict__ V,
                                                             typename Pk<T>::type* __restrict__ w, long NP, int j,
                                                             int mode) {
@@ -1225,6 +1250,37 @@ __global__ __launch_bounds__(NT) void arnoldi_update_kernel(GmresBufs gb, const 
   double2 acc[kDotGroup + 1];
 #pragma unroll
   for (int i = 0; i < kDotGroup + 1; ++i) acc[i] = mk(0, 0);
+  if constexpr (W == 2) {
+    // two 8-byte basis packets per 16-byte load (the dot kernel's W = 2)
+    static_assert(sizeof(VP) == 8 && sizeof(VPPair<VP>) == 16, "two 8-byte packets per 16-byte load");
+    const long NQ = NP / 2;
+    const uint4* Vq = reinterpret_cast<const uint4*>(V);
+    for (long x = (long)blockIdx.x * NT + threadIdx.x; x < NQ; x += (long)gridDim.x * NT) {
+      typename P::type w0 = w[2 * x], w1 = w[2 * x + 1];
+      int i = 0;
+      for (; i + 8 <= nv; i += 8) {
+        uint4 raw[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) raw[q] = Vq[(long)(i + q) * NQ + x];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          const VPPair<VP> pr = __builtin_bit_cast(VPPair<VP>, raw[q]);
+          P::axpy(w0, hs[i + q], IO::load_raw(pr.a));
+          P::axpy(w1, hs[i + q], IO::load_raw(pr.b));
+        }
+      }
+      for (; i < nv; ++i) {
+        const VPPair<VP> pr = __builtin_bit_cast(VPPair<VP>, Vq[(long)i * NQ + x]);
+        P::axpy(w0, hs[i], IO::load_raw(pr.a));
+        P::axpy(w1, hs[i], IO::load_raw(pr.b));
+      }
+      w[2 * x] = w0;
+      w[2 * x + 1] = w1;
+      acc[0].x += P::nrm2(w0) + P::nrm2(w1);
+    }
+    block_reduce_store<NT>(acc, gb.part + (long)blockIdx.x * (kDotGroup + 1));
+    return;
+  }
   for (long x = (long)blockIdx.x * NT + threadIdx.x; x < NP; x += (long)gridDim.x * NT) {
     typename P::type wv = w[x];
     int i = 0;

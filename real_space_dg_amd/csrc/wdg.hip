@@ -491,6 +491,10 @@ void op_apply_base(wdg_ctx* c, const typename cvec<T>::type* u, typename cvec<T>
 // y = A_hat u in precision T, or (alpha I + beta A_hat) u while a shift is set.
 template <typename T>
 void op_apply(wdg_ctx* c, const typename cvec<T>::type* u, typename cvec<T>::type* y, const GmresState* st) {
+  if (c->f2 && c->shift_on) {
+    f2_apply<T>(c, u, y, st, c->shift_a, c->shift_b);   // shift fused into the apply
+    return;
+  }
   op_apply_base<T>(c, u, y, st);
   if (c->shift_on)
     hipLaunchKernelGGL(shift_kernel<T>, dim3(c->red_blocks), dim3(256), 0, c->stream, u, y, c->shift_a, c->shift_b,
@@ -668,6 +672,7 @@ void gmres_cycle(wdg_ctx* c, int m, bool precond, bool single_pass) {
   const long N = c->N;
   const long NP = N * (long)sizeof(T2) / 16;        // working packets per vector
   const double vbytes = (double)NP * sizeof(VP);    // bytes of one stored basis vector
+  const bool wide = sizeof(VP) == 8 && NP % 2 == 0;
   PT* wp = reinterpret_cast<PT*>(w);
   PT* vcp = reinterpret_cast<PT*>(vcur);
   GmresBufs gb = c->gb();
@@ -684,17 +689,41 @@ void gmres_cycle(wdg_ctx* c, int m, bool precond, bool single_pass) {
     }
     op_apply<T>(c, opin, w, st);
     const int nv = j + 1;
-    const dim3 dg(c->red_blocks, (nv + kDotGroup - 1) / kDotGroup);
+    // 8-byte basis packets with an even packet count: the multi-dot and the
+    // update read two packets per 16-byte load, the multi-dot on half the
+    // blocks (fewer, longer blocks measured faster for it)
+    const int nbd = wide ? std::max(1, c->red_blocks / 2) : c->red_blocks;
+    const dim3 dg(nbd, (nv + kDotGroup - 1) / kDotGroup);
     const int nb = c->red_blocks;
+    auto dot = [&](int mode) {
+      if constexpr (sizeof(VP) == 8) {
+        if (wide) {
+          hipLaunchKernelGGL((arnoldi_dot_kernel<T, VP, 256, 2>), dg, dim3(256), 0, c->stream, gb, V, wp, NP, nv,
+                             mode);
+          return;
+        }
+      }
+      hipLaunchKernelGGL((arnoldi_dot_kernel<T, VP, 256, 1>), dg, dim3(256), 0, c->stream, gb, V, wp, NP, nv, mode);
+    };
+    auto update = [&](int mode) {
+      if constexpr (sizeof(VP) == 8) {
+        if (wide) {
+          hipLaunchKernelGGL((arnoldi_update_kernel<T, VP, 256, 2>), dim3(nb), dim3(256), 0, c->stream, gb, V, wp,
+                             NP, j, mode);
+          return;
+        }
+      }
+      hipLaunchKernelGGL((arnoldi_update_kernel<T, VP, 256, 1>), dim3(nb), dim3(256), 0, c->stream, gb, V, wp, NP,
+                         j, mode);
+    };
     {
       ProfScope ps(c, P_DOT, nv * vbytes + N * sizeof(T2));
-      hipLaunchKernelGGL((arnoldi_dot_kernel<T, VP, 256>), dg, dim3(256), 0, c->stream, gb, V, wp, NP, nv, 0);
+      dot(0);
     }
-    hipLaunchKernelGGL((arnoldi_dot_finalize_kernel<256>), dim3(dg.y), dim3(256), 0, c->stream, gb, nb, nv, 0);
+    hipLaunchKernelGGL((arnoldi_dot_finalize_kernel<256>), dim3(dg.y), dim3(256), 0, c->stream, gb, nbd, nv, 0);
     {
       ProfScope ps(c, P_UPDATE, nv * vbytes + 2.0 * N * sizeof(T2));
-      hipLaunchKernelGGL((arnoldi_update_kernel<T, VP, 256>), dim3(nb), dim3(256), 0, c->stream, gb, V, wp, NP, j,
-                         0);
+      update(0);
     }
     if (single_pass) {
       hipLaunchKernelGGL((arnoldi_finish_kernel<T, VP, 256>), dim3(nb), dim3(256), 0, c->stream, gb, nb, j, wp,
@@ -703,9 +732,9 @@ void gmres_cycle(wdg_ctx* c, int m, bool precond, bool single_pass) {
     }
     hipLaunchKernelGGL((arnoldi_update_finalize_kernel<256>), dim3(1), dim3(256), 0, c->stream, gb, nb, j, 0);
     // DGKS second pass, taken on the device only when requested (st->reorth)
-    hipLaunchKernelGGL((arnoldi_dot_kernel<T, VP, 256>), dg, dim3(256), 0, c->stream, gb, V, wp, NP, nv, 1);
-    hipLaunchKernelGGL((arnoldi_dot_finalize_kernel<256>), dim3(dg.y), dim3(256), 0, c->stream, gb, nb, nv, 1);
-    hipLaunchKernelGGL((arnoldi_update_kernel<T, VP, 256>), dim3(nb), dim3(256), 0, c->stream, gb, V, wp, NP, j, 1);
+    dot(1);
+    hipLaunchKernelGGL((arnoldi_dot_finalize_kernel<256>), dim3(dg.y), dim3(256), 0, c->stream, gb, nbd, nv, 1);
+    update(1);
     hipLaunchKernelGGL((arnoldi_update_finalize_kernel<256>), dim3(1), dim3(256), 0, c->stream, gb, nb, j, 1);
     hipLaunchKernelGGL((arnoldi_normalize_kernel<T, VP>), dim3(c->red_blocks), dim3(256), 0, c->stream, gb, wp,
                        V + (size_t)(j + 1) * NP, vcp, NP);
