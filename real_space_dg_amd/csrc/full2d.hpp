@@ -25,6 +25,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "kernels.hpp"
 
 namespace wdg {
@@ -146,8 +148,19 @@ __host__ __device__ inline F2Lds f2_lds_layout(int nl, int rows, int nrx, size_t
 }
 
 
-template <int NLX, int NLY, typename T, int MODE>
-__global__ void __launch_bounds__(kF2Block) f2_apply_kernel(const F2Const* __restrict__ cp,
+// Resident waves per SIMD asked of the compiler: the FP32 apply with at most
+// 4 quadrature points per axis fits 6 waves (79 VGPRs, no spills); elsewhere
+// the compiler's own choice.
+template <typename T, int MODE, int NQ>
+struct F2ApplyOcc {
+  static constexpr int value = (std::is_same<T, float>::value && MODE == F2_APPLY && NQ <= 4) ? 6 : 1;
+};
+
+// NQ: compile-time bound on the potential quadrature points per axis (nqx,
+// nqy <= NQ <= kF2MaxNq); it sizes the per-vertex DeltaV registers.
+template <int NLX, int NLY, typename T, int MODE, int NQ = kF2MaxNq>
+__global__ void __launch_bounds__(kF2Block) __attribute__((amdgpu_waves_per_eu(F2ApplyOcc<T, MODE, NQ>::value)))
+f2_apply_kernel(const F2Const* __restrict__ cp,
                                                             const typename F2Vec<T>::T2* __restrict__ u,
                                                             typename F2Vec<T>::T2* __restrict__ y,
                                                             const GmresState* st, double sa, double sb) {
@@ -246,10 +259,10 @@ __global__ void __launch_bounds__(kF2Block) f2_apply_kernel(const F2Const* __res
       // stored DeltaV: one quadrature row (nqy values) in flight ahead of its use
       const size_t qs = (size_t)(nry + 1) * (nrx + 1);
       const size_t tb = (size_t)el * c.nqx * c.nqy * qs + (size_t)vy * (nrx + 1) + vx;
-      T dcur[kF2MaxNq], dnext[kF2MaxNq];
+      T dcur[NQ], dnext[NQ];
       if (tab) {
 #pragma unroll
-        for (int qy = 0; qy < kF2MaxNq; ++qy)
+        for (int qy = 0; qy < NQ; ++qy)
           if (qy < c.nqy) dcur[qy] = std::is_same<T, double>::value ? (T)c.dv64[tb + qy * qs] : (T)c.dv32[tb + qy * qs];
       }
       for (int qx = 0; qx < c.nqx; ++qx) {
@@ -259,7 +272,7 @@ __global__ void __launch_bounds__(kF2Block) f2_apply_kernel(const F2Const* __res
         if (tab && qx + 1 < c.nqx) {
           const size_t o = tb + (size_t)(qx + 1) * c.nqy * qs;
 #pragma unroll
-          for (int qy = 0; qy < kF2MaxNq; ++qy)
+          for (int qy = 0; qy < NQ; ++qy)
             if (qy < c.nqy) dnext[qy] = std::is_same<T, double>::value ? (T)c.dv64[o + qy * qs] : (T)c.dv32[o + qy * qs];
         }
         T2 t1[NLY];
@@ -276,7 +289,7 @@ __global__ void __launch_bounds__(kF2Block) f2_apply_kernel(const F2Const* __res
 #pragma unroll
         for (int ay = 0; ay < NLY; ++ay) acc[ay] = f2_zero<T2>();
 #pragma unroll
-        for (int qy = 0; qy < kF2MaxNq; ++qy) {
+        for (int qy = 0; qy < NQ; ++qy) {
           if (qy >= c.nqy) break;
           T dv;
           if (tab) {
@@ -327,7 +340,7 @@ __global__ void __launch_bounds__(kF2Block) f2_apply_kernel(const F2Const* __res
           for (int ax = 0; ax < NLX; ++ax) f2_fma(Z[ax + NLX * ay], wx[ax], acc[ay]);
         if (tab) {
 #pragma unroll
-          for (int qy = 0; qy < kF2MaxNq; ++qy) dcur[qy] = dnext[qy];
+          for (int qy = 0; qy < NQ; ++qy) dcur[qy] = dnext[qy];
         }
       }
       if (MODE == F2_ROWABS) {

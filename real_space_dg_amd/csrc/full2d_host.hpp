@@ -66,11 +66,11 @@ void f2_require(const wdg_ctx* c) {
 }
 
 // ---- kernel dispatch over the element node counts ----
-template <typename T, int MODE, int NLX, int NLY>
-void f2_launch_apply_n(wdg_ctx* c, const typename F2Vec<T>::T2* u, typename F2Vec<T>::T2* y,
+template <typename T, int MODE, int NLX, int NLY, int NQ>
+void f2_launch_apply_q(wdg_ctx* c, const typename F2Vec<T>::T2* u, typename F2Vec<T>::T2* y,
                        const GmresState* st, size_t lds, double sa, double sb) {
   F2Data& f = *c->f2;
-  auto k = f2_apply_kernel<NLX, NLY, T, MODE>;
+  auto k = f2_apply_kernel<NLX, NLY, T, MODE, NQ>;
   static bool attr = false;   // per instantiation: raise the dynamic LDS limit once
   if (!attr) {
     HIPCHK(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kF2MaxLds));
@@ -78,6 +78,21 @@ void f2_launch_apply_n(wdg_ctx* c, const typename F2Vec<T>::T2* u, typename F2Ve
   }
   const unsigned grid = (unsigned)(f.d.ne_x * f.d.ne_y * f.hc.tiles);
   hipLaunchKernelGGL(k, dim3(grid), dim3(kF2Block), lds, c->stream, f.d_c.p, u, y, st, sa, sb);
+}
+
+template <typename T, int MODE, int NLX, int NLY>
+void f2_launch_apply_n(wdg_ctx* c, const typename F2Vec<T>::T2* u, typename F2Vec<T>::T2* y,
+                       const GmresState* st, size_t lds, double sa, double sb) {
+  // at most 4 quadrature points per axis (the default up to p = 2): the
+  // register-lean instance; the setup-time diagonal and row-sum modes use the
+  // general one
+  if constexpr (MODE == F2_APPLY) {
+    if (c->f2->hc.nqx <= 4 && c->f2->hc.nqy <= 4) {
+      f2_launch_apply_q<T, MODE, NLX, NLY, 4>(c, u, y, st, lds, sa, sb);
+      return;
+    }
+  }
+  f2_launch_apply_q<T, MODE, NLX, NLY, kF2MaxNq>(c, u, y, st, lds, sa, sb);
 }
 
 template <typename T, int MODE, int NLX>
