@@ -151,6 +151,16 @@ int wdg_diagonal(wdg_ctx* ctx, double* d);
 int wdg_solve(wdg_ctx* ctx, const wdg_solve_opts* opts, const double* x0, double* x,
               wdg_solve_info* info);
 
+/* Residual history of the context's last solve (wdg_solve, wdg_solve_batch,
+ * or the last step of a Crank-Nicolson run): the resvec of the reference's
+ * GMRES call on the normalised system (DG_code/solve_transport_DG_full2D.m:306-308):
+ * [0] the initial residual, [k] the GMRES estimate after Arnoldi step k, both
+ * relative to ||rhs||.  The reference reports resvec * ||rhs|| as
+ * info.solve.resvec (:324,333); multiply by ||rhs|| for that.  *n receives
+ * the full length (iterations + 1, 0 before any solve; at most
+ * min(max_iter + 1, 2^22)); min(n, cap) values are copied to out. */
+int wdg_solve_resvec(wdg_ctx* ctx, double* out, int32_t cap, int32_t* n);
+
 /* wdg_solve (zero start, solution left on the device) on n independent
  * instances together: every instance runs its restart cycles on its own
  * stream and the host polls each one once per cycle, so all unconverged

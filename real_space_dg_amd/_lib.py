@@ -162,6 +162,7 @@ SIGNATURES = {
     "wdg_host_setup": (C.c_int, [C.POINTER(wdg_desc), C.POINTER(wdg_potential), _D, _D]),
     "wdg_transient_rk4": (C.c_int, [_P, C.POINTER(wdg_transient_opts), _D, _D, _D, _D, _D, C.c_int32]),
     "wdg_transient_cn": (C.c_int, [_P, C.POINTER(wdg_cn_opts), _D, _D, C.POINTER(wdg_cn_info)]),
+    "wdg_solve_resvec": (C.c_int, [_P, _D, C.c_int32, C.POINTER(C.c_int32)]),
     "wdg_solve_batch": (C.c_int, [C.POINTER(_P), C.c_int32, C.POINTER(wdg_solve_opts), C.POINTER(wdg_solve_info)]),
     "wdg_transient_cn_batch": (C.c_int, [C.POINTER(_P), C.c_int32, C.POINTER(wdg_cn_opts), C.POINTER(_D),
                                          C.POINTER(_D), C.POINTER(wdg_cn_info)]),

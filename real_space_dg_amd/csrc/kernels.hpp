@@ -42,6 +42,7 @@ struct GmresState {
   int reorth;        // second CGS pass needed for the current step
   int breakdown;
   unsigned int ticket;   // unused (kept for layout stability)
+  int its_total;     // Arnoldi steps of the whole solve (index into the residual history)
 };
 
 __device__ __forceinline__ bool skip(const GmresState* st) { return st && st->done; }
@@ -914,6 +915,9 @@ struct GmresBufs {
   double2* part;     // partial sums: blocks x (kDotGroup+1)
   GmresState* st;
   int m;
+  double* resv;      // residual history of the solve: [k] after Arnoldi step k (null: none;
+                     // the host writes [0], the initial residual)
+  int resv_cap;
 };
 
 __device__ __forceinline__ double2 cmul(double2 a, double2 b) { return mk(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
@@ -1020,6 +1024,9 @@ __device__ __forceinline__ void givens_column(GmresBufs gb, int j, const double2
     const double gn = sqrt(t.x * t.x + t.y * t.y);
     st->resid = gn;
     st->jlast = j + 1;
+    const int k = st->its_total + 1;
+    st->its_total = k;
+    if (gb.resv && k < gb.resv_cap) gb.resv[k] = gn;
     if (gn <= st->target || r == 0.0) st->done = 1;
     if (r == 0.0) st->breakdown = 1;
   }
@@ -1409,6 +1416,7 @@ __global__ void solve_init_kernel(GmresState* st, double tol_abs, double inner_t
     st->tol_abs = tol_abs;
     st->inner_tol = inner_tol;
     st->single_pass = single_pass;
+    st->its_total = 0;
   }
 }
 

@@ -127,6 +127,8 @@ struct wdg_ctx {
   DevBuf<float2> d_vcur32;            // FP32 copy of the newest BF16 basis vector
   DevBuf<double2> d_H, d_cs, d_sn, d_g, d_h1, d_h2, d_yv, d_part;
   DevBuf<GmresState> d_st;
+  DevBuf<double> d_resv;   // residual history of the last solve (relative norms)
+  int last_its = -1;       // Arnoldi steps of the last solve (-1: none yet)
   std::vector<cplx> fnl, fnr;        // inflow data of the current potential (k basis)
   double rhs_norm = 0;
   bool have_potential = false;
@@ -159,6 +161,7 @@ struct wdg_ctx {
     GmresBufs b;
     b.H = d_H.p; b.cs = d_cs.p; b.sn = d_sn.p; b.g = d_g.p; b.h1 = d_h1.p; b.h2 = d_h2.p;
     b.y = d_yv.p; b.part = d_part.p; b.st = d_st.p; b.m = m_alloc;
+    b.resv = d_resv.p; b.resv_cap = (int)d_resv.n;
     return b;
   }
 };
@@ -1139,6 +1142,14 @@ struct SolveRun {
     basis = fp64 ? WDG_BASIS_WORKING : o.krylov_basis;
     b16 = basis != WDG_BASIS_WORKING;
     ensure_krylov(c, m, fp64, b16);
+    // residual history, one value per Arnoldi step (captured cycles hold its
+    // address: a larger buffer drops them)
+    const size_t resv_cap = std::min<size_t>((size_t)o.max_iter + 1, (size_t)1 << 22);
+    if (c->d_resv.n < resv_cap) {
+      c->d_resv.alloc(resv_cap);
+      clear_graphs(c);
+    }
+    c->last_its = -1;
     if (c->f2) f2_prepare_solve(c);
     else if (!c->use_fft) autotune(c, fp64);
     precond = o.use_precond != 0 && !(c->f2 && c->f2->precond == WDG2_PRECOND_NONE);
@@ -1218,6 +1229,9 @@ struct SolveRun {
     info->reported_relres = reported;
     info->flag = (std::isfinite(relres) && relres <= tol) ? 0 : 1;
     c->have_solution = true;
+    c->last_its = its;
+    if (c->d_resv.n > 0)   // the history starts with the initial residual
+      HIPCHK(hipMemcpy(c->d_resv.p, &info->initial_true_relres, sizeof(double), hipMemcpyHostToDevice));
     if (x) {
       HIPCHK(hipMemcpy(x, c->d_x.p, 2 * N * 8, hipMemcpyDeviceToHost));
       for (long i = 0; i < 2 * N; ++i) x[i] *= xs;
@@ -1539,6 +1553,20 @@ int wdg_solve(wdg_ctx* ctx, const wdg_solve_opts* opts, const double* x0, double
       r.sync_poll();
     }
     r.end(x);
+    return WDG_OK;
+  });
+}
+
+int wdg_solve_resvec(wdg_ctx* ctx, double* out, int32_t cap, int32_t* n_out) {
+  if (!ctx || !n_out || cap < 0 || (cap > 0 && !out)) return fail(ctx, WDG_ERR_ARG, "null argument or negative cap");
+  return guarded(ctx, [&]() {
+    const long n = ctx->last_its < 0 ? 0 : std::min<long>((long)ctx->last_its + 1, (long)ctx->d_resv.n);
+    *n_out = (int32_t)n;
+    const long k = std::min<long>(n, cap);
+    if (k > 0) {
+      HIPCHK(hipMemcpyAsync(out, ctx->d_resv.p, (size_t)k * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+      HIPCHK(hipStreamSynchronize(ctx->stream));
+    }
     return WDG_OK;
   });
 }

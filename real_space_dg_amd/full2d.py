@@ -733,6 +733,7 @@ class Full2DOperator:
         check(lib.wdg2_create(C.byref(self.desc), cparts, len(parts), C.byref(ctx)))
         self.ctx = ctx
         self.n = s.n_total
+        self.rhs_norm = 0.0
         self.set_potential()
         self._rhs = None
 
@@ -786,6 +787,7 @@ class Full2DOperator:
         if r.size != self.n:
             raise ValueError("rhs has the wrong size")
         check(self._lib.wdg2_set_rhs(self.ctx, dptr(r)), self.ctx)
+        self.rhs_norm = float(np.linalg.norm(r))
 
     def apply(self, u, precision: str = "fp64") -> np.ndarray:
         u = _c128(u).reshape(-1)
@@ -865,6 +867,12 @@ class Full2DOperator:
         check(self._lib.wdg_solve(self.ctx, C.byref(opts.to_c()), x0p, dptr(x) if return_x else None,
                                   C.byref(info)), self.ctx)
         return x, info
+
+    def resvec(self) -> np.ndarray:
+        """Residual history of the last solve relative to ||rhs|| ([0] initial,
+        [k] after Arnoldi step k; wdg_solve_resvec)."""
+        from .solver import read_resvec
+        return read_resvec(self._lib, self.ctx)
 
     def observables(self, rho=None):
         """nDG, jxDG, jyDG on the (X dof, Y dof) grid (get_Observables_full2D.m:24-60);
@@ -1063,6 +1071,8 @@ def solve_transport_DG_full2D(mat: dict, Vxy=None, EfL=None, EfR=None, rho0=None
         solver.precondition = used != "none"
         op.upload_rhs()
         rho, info = op.solve(solver, x0=rho0)
+        # resvec * ||rhs|| as the reference reports it (solve_transport_DG_full2D.m:324,333)
+        resvec = op.resvec() * (op.rhs_norm if op.rhs_norm > 0 else 1.0)
         nDG, jxDG, jyDG = op.observables()
     target_y = setup.grid_y if "Ny" in mat else setup.Y.nodes
     n = project_to_material_y(nDG, setup.Y.nodes, target_y)
@@ -1073,5 +1083,5 @@ def solve_transport_DG_full2D(mat: dict, Vxy=None, EfL=None, EfR=None, rho0=None
     return Full2DResult(status, rho, n, jx, jy, j, nDG, jxDG, jyDG,
                         {"solve": {"flag": info.flag, "relres": info.true_relres, "trueRelres": info.true_relres,
                                    "initialTrueRelres": info.initial_true_relres, "warmStartUsed": rho0 is not None,
-                                   "iterations": info.iterations, "preconditioner": used,
+                                   "iterations": info.iterations, "resvec": resvec, "preconditioner": used,
                                    "seconds": info.seconds}}, setup)

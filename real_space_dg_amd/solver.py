@@ -169,6 +169,7 @@ class WignerOperator:
         d = {f: getattr(info, f) for f, _ in info._fields_}
         d["status"] = "solved-gmres" if info.flag == 0 else "iterative-gmres-not-converged"
         d["precision"] = opts.precision
+        d["resvec"] = read_resvec(self._lib, self._h)   # relative to ||rhs||
         return x, d
 
     def observables(self, m_nodes, rho=None, want_rho_u: bool = False):
@@ -251,6 +252,17 @@ class WignerOperator:
 
     def profile(self) -> dict:
         return read_profile(self._lib, self._h)
+
+
+def read_resvec(lib, ctx) -> np.ndarray:
+    """Residual history of the context's last solve (wdg_solve_resvec):
+    [0] initial, [k] after Arnoldi step k, relative to ||rhs||."""
+    n = C.c_int32()
+    check(lib.wdg_solve_resvec(ctx, None, 0, C.byref(n)), ctx)
+    out = np.empty(n.value, np.float64)
+    if n.value:
+        check(lib.wdg_solve_resvec(ctx, dptr(out), n.value, C.byref(n)), ctx)
+    return out
 
 
 def read_profile(lib, ctx) -> dict:

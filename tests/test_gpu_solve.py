@@ -117,3 +117,38 @@ def test_graph_replay_matches_eager(solved):
         for _, g, x, its, rr in rs[1:]:
             assert its == rs[0][3]
             assert np.array_equal(x, x0), (prec, g, np.max(np.abs(x - x0)))
+
+
+def test_residual_history_matches_the_solve_info():
+    """wdg_solve_resvec: one value per Arnoldi step plus the initial
+    residual; GMRES estimates never grow inside a restart cycle; the last
+    value is the final estimate, below the tolerance, and each restart
+    starts from the true residual the solver recomputes."""
+    from tests._cases import build_case, gpu_operator
+    c = build_case(8, 2, 64, 0.1)
+    op = gpu_operator(c)
+    try:
+        for prec, restart in (("fp64", 12), ("mixed", 12)):
+            x, info = op.solve(SolveOptions(tol=1e-10, restart=restart, precision=prec))
+            r = info["resvec"]
+            print(prec, info["iterations"], r[:3], r[-3:], flush=True)
+            assert info["flag"] == 0 and r.size == info["iterations"] + 1
+            assert r[0] == info["initial_true_relres"]
+            assert np.isclose(r[-1], info["reported_relres"], rtol=1e-6, atol=0)
+            assert np.all(np.isfinite(r)) and np.all(r >= 0)
+            if prec == "fp64":
+                # FP64 cycles run the full restart length: inside each, the
+                # Givens estimates are non-increasing; the last one is below tol
+                assert r[-1] <= 1e-10
+                steps = np.diff(r[1:])            # steps[k]: step k+1 -> k+2
+                inside = [k for k in range(steps.size) if (k + 1) % restart != 0]
+                assert np.all(steps[inside] <= 1e-13 * r[0]), steps[inside].max()
+            else:
+                # inner FP32 cycles stop at a relative target; the estimates
+                # still fall by the tolerance overall
+                assert r[-1] <= 1e-8
+        # a solve that converges before the first step records only the initial value
+        x, info = op.solve(SolveOptions(tol=1e-10, precision="fp64"), x0=x)
+        assert info["iterations"] == 0 and info["resvec"].size == 1
+    finally:
+        op.close()
