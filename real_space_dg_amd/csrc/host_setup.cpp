@@ -177,6 +177,24 @@ void lu_factor(std::vector<cplx>& a, std::vector<int>& piv, int n) {
   }
 }
 
+// One right-hand side in place.
+void lu_solve1(const std::vector<cplx>& lu, const std::vector<int>& piv, int n, cplx* b) {
+  for (int c = 0; c < n; ++c)
+    if (piv[c] != c) std::swap(b[c], b[piv[c]]);
+  for (int r = 1; r < n; ++r) {
+    cplx s = b[r];
+    const cplx* row = lu.data() + (size_t)r * n;
+    for (int k = 0; k < r; ++k) s -= row[k] * b[k];
+    b[r] = s;
+  }
+  for (int r = n - 1; r >= 0; --r) {
+    cplx s = b[r];
+    const cplx* row = lu.data() + (size_t)r * n;
+    for (int k = r + 1; k < n; ++k) s -= row[k] * b[k];
+    b[r] = s / row[r];
+  }
+}
+
 // Two right-hand sides at once (one pass over the factors).
 void lu_solve2(const std::vector<cplx>& lu, const std::vector<int>& piv, int n, std::vector<cplx>& b1,
                std::vector<cplx>& b2) {
@@ -309,6 +327,18 @@ void build_geometry(const wdg_desc& d, Geometry& g) {
         }
       bd->lu = bd->phi;
       lu_factor(bd->lu, bd->piv, g.nk);
+      if (g.nk <= kInflowMapMax) {
+        // the inflow map phi^-1 cosm, one LU solve per column (once per basis)
+        const int nk = g.nk;
+        bd->gm.assign((size_t)nk * nk, cplx(0.0));
+#pragma omp parallel for schedule(dynamic, 4)
+        for (int j = 0; j < nk; ++j) {
+          std::vector<cplx> col(nk);
+          for (int l = 0; l < nk; ++l) col[l] = bd->cosm[(size_t)l * nk + j];
+          lu_solve1(bd->lu, bd->piv, nk, col.data());
+          for (int r = 0; r < nk; ++r) bd->gm[(size_t)r * nk + j] = col[r];
+        }
+      }
       if (cache.size() > 64) cache.clear();
       it = cache.emplace(key, std::move(bd)).first;
     }
@@ -490,7 +520,29 @@ void inflow(const Geometry& g, const wdg_potential& pot, std::vector<cplx>& fnl,
       fR[j] = 2.0 / M_PI * dkz * sr;
     }
   }
-  // Cosine transform to xi, then phi \ f (verteilung.m:109-114).
+  // Cosine transform to xi, then phi \ f (verteilung.m:109-114): with the
+  // precomputed map phi^-1 cosm one row-parallel product per contact
+  if (!g.basis->gm.empty()) {
+    const cplx* G = g.basis->gm.data();
+    const double sc = dk / 2.0 / M_PI;
+    std::vector<cplx> xl(nk), xr(nk);
+#pragma omp parallel for schedule(static) if (nk >= 256)
+    for (int r = 0; r < nk; ++r) {
+      const cplx* row = G + (size_t)r * nk;
+      double lr = 0.0, li = 0.0, rr = 0.0, ri = 0.0;
+      for (int j = 0; j < nk; ++j) {
+        lr += row[j].real() * fL[j];
+        li += row[j].imag() * fL[j];
+        rr += row[j].real() * fR[j];
+        ri += row[j].imag() * fR[j];
+      }
+      xl[r] = cplx(lr * sc, li * sc);
+      xr[r] = cplx(rr * sc, ri * sc);
+    }
+    fnl = std::move(xl);
+    fnr = std::move(xr);
+    return;
+  }
   std::vector<cplx> bl(g.n_xi), br(g.n_xi);
   const double* cm = g.basis->cosm.data();
   for (int l = 0; l < g.n_xi; ++l) {

@@ -55,7 +55,11 @@ struct BasisData {
   std::vector<cplx> lu;      // LU factors of phi (row-major, partial pivoting) for phi\f
   std::vector<int> piv;
   std::vector<double> cosm;  // n_xi x nk: cos(y_l k_j), the cosine transform of verteilung.m:109-110
+  // phi^-1 cosm (nk x nk, row-major) for nk <= kInflowMapMax: the whole
+  // f(k) -> phi \ (cosm f) inflow map as one matrix, empty above that size
+  std::vector<cplx> gm;
 };
+constexpr int kInflowMapMax = 1024;
 
 inline const cplx* Geometry::phi() const { return basis->phi.data(); }
 
