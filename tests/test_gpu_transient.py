@@ -77,3 +77,38 @@ def test_crank_nicolson_1d_matches_oracle(case, precision):
     n_last, _, _ = case["op"].observables(case["m_nodes"])
     n_ref, _, _ = TR.W.observables(case["p"], ref, case["m_nodes"])
     assert np.max(np.abs(n_last - n_ref)) <= 1e-8 * np.max(np.abs(n_ref))
+
+
+def test_crank_nicolson_batch_mixes_1d_and_full2d_instances(case):
+    """wdg_transient_cn_batch over a 1D and a full-2D operator (one option
+    set for both): each instance ends where its own wdg_transient_cn run
+    ends, and on the oracle's directly solved steps.  The shared step is the
+    1D one; the full-2D operator's scale is ~1e21 smaller in these units,
+    so that instance only accumulates its source (the batching, not the
+    2D dynamics, is what this checks; those have their own tests)."""
+    from oracle import full2d as F
+    from real_space_dg_amd import full2d as P
+    from real_space_dg_amd.solver import transient_cn_batch
+    from test_full2d_setup import CONFIGS
+    a = 2e-10
+    x_stat = spla.spsolve(case["K"], case["rhs"])
+    rho0_1d = x_stat + 0.05 * np.linalg.norm(x_stat) / np.sqrt(x_stat.size) * rand_c(x_stat.size, 13)
+    dt = 4.0 / (a * case["lam"])
+    m, EfL, EfR = CONFIGS["upwind-zero-inflow"]
+    p, A2, rhs2, _ = F.solve_direct_full2d(m, m["V"], EfL, EfR)
+    rho0_2d = np.zeros(p["nTotal"], complex)
+    kw = dict(a_scale=a, b_scale=a, tol=1e-12, restart=40)
+    with P.Full2DOperator(P.Full2DSetup(m, m["V"], EfL, EfR)) as op2:
+        op2.upload_rhs()
+        assert op2.set_preconditioner("rowabs") == "rowabs"
+        r1, _ = case["op"].transient_cn(dt, 4, rho0=rho0_1d, **kw)
+        r2, _ = op2.transient_cn(dt, 4, rho0=rho0_2d, **kw)
+        outs, infos = transient_cn_batch([case["op"], op2], dt, 4, rho0=[rho0_1d, rho0_2d], **kw)
+        print("CN batch 1D+2D:", infos, flush=True)
+        assert all(i["flag"] == 0 and i["steps"] == 4 for i in infos)
+        assert np.linalg.norm(outs[0] - r1) <= 1e-12 * np.linalg.norm(r1)
+        assert np.linalg.norm(outs[1] - r2) <= 1e-12 * np.linalg.norm(r2)
+        ref1 = TR.crank_nicolson(case["K"], case["rhs"], rho0_1d, dt, 4, a_scale=a, b_scale=a)
+        assert np.linalg.norm(outs[0] - ref1) <= 1e-9 * np.linalg.norm(ref1)
+        ref2 = TR.crank_nicolson(A2, rhs2, rho0_2d, dt, 4, a_scale=a, b_scale=a)
+        assert np.linalg.norm(outs[1] - ref2) <= 1e-8 * np.linalg.norm(ref2)
