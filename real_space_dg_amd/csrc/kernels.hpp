@@ -771,16 +771,29 @@ __global__ __launch_bounds__(256) void line_apply_kernel(const OpConst* __restri
 // Line preconditioner of the FP32 solver from stored block inverses.  Each
 // (element, k) block D and the coupling column c are fixed for a potential,
 // so line_inverse_kernel factors every block once (FP64 Gaussian elimination
-// with partial pivoting, as in the sweeps above) and stores D^-1 and D^-1 c
-// in FP32 as rows of nk values: linv[(e * (n*n + n) + q) * nk + j] with
-// q = a*n + b for (D^-1)_ab and q = n*n + a for (D^-1 c)_a.  The sweeps then
-// compose s_out = (D^-1 r)_out - (D^-1 c)_out s_in and z = D^-1 r - (D^-1 c) s_in.
+// with partial pivoting, as in the sweeps above) and stores, as rows of nk
+// values, D^-1 as BF16 complex pairs, dinv[(e * n*n + a*n + b) * nk + j], and
+// D^-1 c in FP32, dcpl[(e * n + a) * nk + j].  The sweeps then compose
+// s_out = (D^-1 r)_out - (D^-1 c)_out s_in and z = D^-1 r - (D^-1 c) s_in: the
+// coupling factors multiply along the whole line and stay FP32, while the
+// rounding of D^-1 (2^-9 relative) only perturbs each element's own term, so
+// the preconditioner stays a fixed linear operator close to the FP32 one.
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t lv_pack_bf16x2(float re, float im) {
+  uint32_t ur = __float_as_uint(re), ui = __float_as_uint(im);
+  ur += 0x7FFFu + ((ur >> 16) & 1u);   // round to nearest even
+  ui += 0x7FFFu + ((ui >> 16) & 1u);
+  return (ur >> 16) | (ui & 0xFFFF0000u);
+}
+__device__ __forceinline__ float2 lv_unpack_bf16x2(uint32_t p) {
+  return make_float2(__uint_as_float(p << 16), __uint_as_float(p & 0xFFFF0000u));
+}
+
 template <int NN>
 __global__ __launch_bounds__(256) void line_inverse_kernel(const OpConst* __restrict__ oc,
                                                            const double* __restrict__ kvec,
                                                            const double2* __restrict__ s0s1,
-                                                           float2* __restrict__ linv) {
+                                                           uint32_t* __restrict__ dinv, float2* __restrict__ dcpl) {
   const int nk = oc->nk, ne = oc->n_chi;
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long)ne * nk) return;
@@ -799,13 +812,14 @@ __global__ __launch_bounds__(256) void line_inverse_kernel(const OpConst* __rest
     B.i[a][2 * NN] = 0.0;
   }
   gauss_solve<NN, NN + 1>(B);
-  float2* out = linv + (long)e * (NN * NN + NN) * nk + j;
+  uint32_t* di = dinv + (long)e * NN * NN * nk + j;
+  float2* dc = dcpl + (long)e * NN * nk + j;
 #pragma unroll
   for (int a = 0; a < NN; ++a) {
 #pragma unroll
     for (int b = 0; b < NN; ++b)
-      out[(long)(a * NN + b) * nk] = make_float2((float)B.r[a][NN + b], (float)B.i[a][NN + b]);
-    out[(long)(NN * NN + a) * nk] = make_float2((float)B.r[a][2 * NN], (float)B.i[a][2 * NN]);
+      di[(long)(a * NN + b) * nk] = lv_pack_bf16x2((float)B.r[a][NN + b], (float)B.i[a][NN + b]);
+    dc[(long)a * nk] = make_float2((float)B.r[a][2 * NN], (float)B.i[a][2 * NN]);
   }
 }
 
@@ -814,7 +828,8 @@ __global__ __launch_bounds__(256) void line_inverse_kernel(const OpConst* __rest
 template <int NN>
 __global__ __launch_bounds__(256) void line_aggregate_inv_kernel(const OpConst* __restrict__ oc,
                                                                  const double* __restrict__ kvec,
-                                                                 const float2* __restrict__ linv,
+                                                                 const uint32_t* __restrict__ dinv,
+                                                                 const float2* __restrict__ dcpl,
                                                                  const float2* __restrict__ r,
                                                                  double2* __restrict__ agg, const GmresState* st) {
   if (skip(st)) return;
@@ -828,16 +843,16 @@ __global__ __launch_bounds__(256) void line_aggregate_inv_kernel(const OpConst* 
   double Ar = 0, Ai = 0, Br = 1, Bi = 0;
   for (int step = 0; step < e_hi - e_lo; ++step) {
     const int e = fwd ? e_lo + step : e_hi - 1 - step;
-    const float2* li = linv + (long)e * (NN * NN + NN) * nk + j;
+    const uint32_t* di = dinv + (long)e * NN * NN * nk + j;
     const float2* re = r + (long)e * NN * nk + j;
     float al_r = 0, al_i = 0;
 #pragma unroll
     for (int b = 0; b < NN; ++b) {
-      const float2 m = li[(long)(out * NN + b) * nk], v = re[(long)b * nk];
+      const float2 m = lv_unpack_bf16x2(di[(long)(out * NN + b) * nk]), v = re[(long)b * nk];
       al_r += m.x * v.x - m.y * v.y;
       al_i += m.x * v.y + m.y * v.x;
     }
-    const float2 be = li[(long)(NN * NN + out) * nk];
+    const float2 be = dcpl[((long)e * NN + out) * nk + j];
     // s_out = alpha - beta * s_in, composed with the running map s_in = A + B s0
     const double nAr = al_r - (be.x * Ar - be.y * Ai), nAi = al_i - (be.x * Ai + be.y * Ar);
     const double nBr = -(be.x * Br - be.y * Bi), nBi = -(be.x * Bi + be.y * Br);
@@ -851,7 +866,8 @@ __global__ __launch_bounds__(256) void line_aggregate_inv_kernel(const OpConst* 
 template <int NN>
 __global__ __launch_bounds__(256) void line_apply_inv_kernel(const OpConst* __restrict__ oc,
                                                              const double* __restrict__ kvec,
-                                                             const float2* __restrict__ linv,
+                                                             const uint32_t* __restrict__ dinv,
+                                                             const float2* __restrict__ dcpl,
                                                              const double2* __restrict__ agg,
                                                              const float2* __restrict__ r, float2* __restrict__ z,
                                                              const GmresState* st) {
@@ -867,7 +883,8 @@ __global__ __launch_bounds__(256) void line_apply_inv_kernel(const OpConst* __re
   float sr = (float)s_in.x, si = (float)s_in.y;
   for (int step = 0; step < e_hi - e_lo; ++step) {
     const int e = fwd ? e_lo + step : e_hi - 1 - step;
-    const float2* li = linv + (long)e * (NN * NN + NN) * nk + j;
+    const uint32_t* di = dinv + (long)e * NN * NN * nk + j;
+    const float2* dc = dcpl + (long)e * NN * nk + j;
     const float2* re = r + (long)e * NN * nk + j;
     float2 v[NN];
 #pragma unroll
@@ -875,11 +892,11 @@ __global__ __launch_bounds__(256) void line_apply_inv_kernel(const OpConst* __re
     float2 zo[NN];
 #pragma unroll
     for (int a = 0; a < NN; ++a) {
-      const float2 bc = li[(long)(NN * NN + a) * nk];
+      const float2 bc = dc[(long)a * nk];
       float zr = -(bc.x * sr - bc.y * si), zi = -(bc.x * si + bc.y * sr);
 #pragma unroll
       for (int b = 0; b < NN; ++b) {
-        const float2 m = li[(long)(a * NN + b) * nk];
+        const float2 m = lv_unpack_bf16x2(di[(long)(a * NN + b) * nk]);
         zr += m.x * v[b].x - m.y * v[b].y;
         zi += m.x * v[b].y + m.y * v[b].x;
       }

@@ -97,7 +97,8 @@ struct wdg_ctx {
   DevBuf<double> d_sbar64;
   DevBuf<float> d_sbar32;
   DevBuf<double2> d_s0s1, d_agg;
-  DevBuf<float2> d_linv32;            // FP32 line-preconditioner block inverses (line_inverse_kernel)
+  DevBuf<uint32_t> d_linv16;          // line-preconditioner block inverses D^-1, BF16 complex (line_inverse_kernel)
+  DevBuf<float2> d_lcpl32;            // and the FP32 coupling columns D^-1 c
   DevBuf<double> d_A1, d_A2;          // realified phi / i*phi^H, FP64 (GEMM formulation)
   DevBuf<uint16_t> d_A1b, d_A2b;      // same in BF16
   DevBuf<double2> d_T64, d_Z64;       // Theta intermediates (GEMM formulation)
@@ -541,9 +542,9 @@ void op_precond_n(wdg_ctx* c, const typename cvec<T>::type* r, typename cvec<T>:
     // FP32: stream the stored block inverses
     const double rows = (double)c->N / NN;   // (element, k) blocks
     {
-      ProfScope ps(c, P_LINE_AGG, 8.0 * (c->N + rows * (NN + 1)));
+      ProfScope ps(c, P_LINE_AGG, 8.0 * c->N + rows * (4.0 * NN + 8.0));
       hipLaunchKernelGGL((line_aggregate_inv_kernel<NN>), dim3(grid1(work, 256)), dim3(256), 0, c->stream,
-                         c->d_oc.p, c->d_k.p, c->d_linv32.p, r, c->d_agg.p, st);
+                         c->d_oc.p, c->d_k.p, c->d_linv16.p, c->d_lcpl32.p, r, c->d_agg.p, st);
     }
     {
       ProfScope ps(c, P_LINE_CARRY, 3.0 * c->oc.n_chunks * c->g.nk * 2 * sizeof(double2));
@@ -551,9 +552,9 @@ void op_precond_n(wdg_ctx* c, const typename cvec<T>::type* r, typename cvec<T>:
                          c->d_k.p, c->d_agg.p, st);
     }
     {
-      ProfScope ps(c, P_LINE_APPLY, 8.0 * (2.0 * c->N + rows * (NN * NN + NN)));
+      ProfScope ps(c, P_LINE_APPLY, 16.0 * c->N + rows * (4.0 * NN * NN + 8.0 * NN));
       hipLaunchKernelGGL((line_apply_inv_kernel<NN>), dim3(grid1(work, 256)), dim3(256), 0, c->stream, c->d_oc.p,
-                         c->d_k.p, c->d_linv32.p, c->d_agg.p, r, z, st);
+                         c->d_k.p, c->d_linv16.p, c->d_lcpl32.p, c->d_agg.p, r, z, st);
     }
     return;
   }
@@ -1008,7 +1009,8 @@ void create_impl(wdg_ctx* c, const wdg_desc& d) {
   HIPCHK(hipMemcpyAsync(c->d_phic.p, ph.data(), ph.size() * sizeof(double2), hipMemcpyHostToDevice, c->stream));
   c->d_s0s1.alloc((size_t)g.n_chi * n * n * 2);
   c->d_agg.alloc((size_t)oc.n_chunks * nk * 2);
-  c->d_linv32.alloc((size_t)g.n_chi * nk * (n * n + n));
+  c->d_linv16.alloc((size_t)g.n_chi * nk * n * n);
+  c->d_lcpl32.alloc((size_t)g.n_chi * nk * n);
   c->d_sbar64.alloc((size_t)g.n_chi * oc.n_pairs * (nx + 1));
   c->d_sbar32.alloc((size_t)g.n_chi * oc.n_pairs * (nx + 1));
   c->d_b.alloc(c->N);
@@ -1081,7 +1083,7 @@ void set_potential_impl(wdg_ctx* c, const wdg_potential& pot) {
 #define WDG_LINV(NN)                                                                                   \
   case NN:                                                                                             \
     hipLaunchKernelGGL((line_inverse_kernel<NN>), gi, dim3(256), 0, c->stream, c->d_oc.p, c->d_k.p,    \
-                       c->d_s0s1.p, c->d_linv32.p);                                                    \
+                       c->d_s0s1.p, c->d_linv16.p, c->d_lcpl32.p);                                     \
     break;
       WDG_LINV(2) WDG_LINV(3) WDG_LINV(4) WDG_LINV(5) WDG_LINV(6) WDG_LINV(7) WDG_LINV(8)
 #undef WDG_LINV

@@ -106,8 +106,11 @@ def test_precond_inverts_line_operator(case):
 
 
 def test_precond_fp32_matches_fp64(case):
-    """The FP32 line preconditioner (stored FP32 block inverses) against the
-    FP64 sweeps, both through wdg_precond_device in device units."""
+    """The FP32 line preconditioner (stored block inverses: D^-1 in BF16,
+    the coupling columns in FP32) against the FP64 sweeps, both through
+    wdg_precond_device in device units.  BF16 rounds each stored entry to
+    2^-9 relative, which bounds the difference (the mixed solver's FP64
+    refinement, tested elsewhere, keeps the solution at FP64 accuracy)."""
     import torch
     from real_space_dg_amd import _lib as L
     from real_space_dg_amd.solver import check
@@ -123,4 +126,4 @@ def test_precond_fp32_matches_fp64(case):
     a, b = z32.cpu().numpy().astype(np.complex128), z64.cpu().numpy()
     err = np.linalg.norm(a - b) / np.linalg.norm(b)
     print("precond fp32 vs fp64", err)
-    assert err < 1e-5
+    assert err < 4e-3
