@@ -789,6 +789,22 @@ __device__ __forceinline__ float2 lv_unpack_bf16x2(uint32_t p) {
   return make_float2(__uint_as_float(p << 16), __uint_as_float(p & 0xFFFF0000u));
 }
 
+// One complex value of the preconditioner input r: FP32, or read in place
+// from a 16-bit Krylov basis vector (FP16 pairs stored times 256, or BF16
+// pairs), whose entry i sits in 32-bit word i.
+template <typename RIN>
+__device__ __forceinline__ float2 line_in(const RIN* r, long i);
+template <>
+__device__ __forceinline__ float2 line_in<float2>(const float2* r, long i) { return r[i]; }
+template <>
+__device__ __forceinline__ float2 line_in<ushort2>(const ushort2* r, long i) {
+  const ushort2 p = r[i];
+  return make_float2((float)__builtin_bit_cast(_Float16, p.x) * (1.0f / 256.0f),
+                     (float)__builtin_bit_cast(_Float16, p.y) * (1.0f / 256.0f));
+}
+template <>
+__device__ __forceinline__ float2 line_in<uint32_t>(const uint32_t* r, long i) { return lv_unpack_bf16x2(r[i]); }
+
 template <int NN>
 __global__ __launch_bounds__(256) void line_inverse_kernel(const OpConst* __restrict__ oc,
                                                            const double* __restrict__ kvec,
@@ -825,12 +841,12 @@ __global__ __launch_bounds__(256) void line_inverse_kernel(const OpConst* __rest
 
 // line_aggregate_kernel from the stored inverses (FP32 products, the chunk
 // map composed in FP64).
-template <int NN>
+template <int NN, typename RIN = float2>
 __global__ __launch_bounds__(256) void line_aggregate_inv_kernel(const OpConst* __restrict__ oc,
                                                                  const double* __restrict__ kvec,
                                                                  const uint32_t* __restrict__ dinv,
                                                                  const float2* __restrict__ dcpl,
-                                                                 const float2* __restrict__ r,
+                                                                 const RIN* __restrict__ r,
                                                                  double2* __restrict__ agg, const GmresState* st) {
   if (skip(st)) return;
   const int nk = oc->nk, ne = oc->n_chi, S = oc->chunk, nc = oc->n_chunks;
@@ -844,11 +860,11 @@ __global__ __launch_bounds__(256) void line_aggregate_inv_kernel(const OpConst* 
   for (int step = 0; step < e_hi - e_lo; ++step) {
     const int e = fwd ? e_lo + step : e_hi - 1 - step;
     const uint32_t* di = dinv + (long)e * NN * NN * nk + j;
-    const float2* re = r + (long)e * NN * nk + j;
+    const long re = (long)e * NN * nk + j;
     float al_r = 0, al_i = 0;
 #pragma unroll
     for (int b = 0; b < NN; ++b) {
-      const float2 m = lv_unpack_bf16x2(di[(long)(out * NN + b) * nk]), v = re[(long)b * nk];
+      const float2 m = lv_unpack_bf16x2(di[(long)(out * NN + b) * nk]), v = line_in(r, re + (long)b * nk);
       al_r += m.x * v.x - m.y * v.y;
       al_i += m.x * v.y + m.y * v.x;
     }
@@ -863,13 +879,13 @@ __global__ __launch_bounds__(256) void line_aggregate_inv_kernel(const OpConst* 
 }
 
 // line_apply_kernel from the stored inverses: z_e = D^-1 r_e - (D^-1 c) s_in.
-template <int NN>
+template <int NN, typename RIN = float2>
 __global__ __launch_bounds__(256) void line_apply_inv_kernel(const OpConst* __restrict__ oc,
                                                              const double* __restrict__ kvec,
                                                              const uint32_t* __restrict__ dinv,
                                                              const float2* __restrict__ dcpl,
                                                              const double2* __restrict__ agg,
-                                                             const float2* __restrict__ r, float2* __restrict__ z,
+                                                             const RIN* __restrict__ r, float2* __restrict__ z,
                                                              const GmresState* st) {
   if (skip(st)) return;
   const int nk = oc->nk, ne = oc->n_chi, S = oc->chunk, nc = oc->n_chunks;
@@ -885,10 +901,10 @@ __global__ __launch_bounds__(256) void line_apply_inv_kernel(const OpConst* __re
     const int e = fwd ? e_lo + step : e_hi - 1 - step;
     const uint32_t* di = dinv + (long)e * NN * NN * nk + j;
     const float2* dc = dcpl + (long)e * NN * nk + j;
-    const float2* re = r + (long)e * NN * nk + j;
+    const long re = (long)e * NN * nk + j;
     float2 v[NN];
 #pragma unroll
-    for (int b = 0; b < NN; ++b) v[b] = re[(long)b * nk];
+    for (int b = 0; b < NN; ++b) v[b] = line_in(r, re + (long)b * nk);
     float2 zo[NN];
 #pragma unroll
     for (int a = 0; a < NN; ++a) {

@@ -594,6 +594,46 @@ void op_precond(wdg_ctx* c, const typename cvec<T>::type* r, typename cvec<T>::t
   }
 }
 
+// The FP32 line preconditioner applied to a 16-bit Krylov basis vector read
+// in place (RIN = ushort2 for the FP16 basis, uint32_t for BF16): the same
+// stored-inverse sweeps as op_precond_n<NN, float>, without an FP32 copy.
+template <int NN, typename RIN>
+void op_precond_basis_n(wdg_ctx* c, const RIN* r, float2* z, const GmresState* st) {
+  const long work = (long)c->oc.n_chunks * c->g.nk;
+  const double rows = (double)c->N / NN;
+  {
+    ProfScope ps(c, P_LINE_AGG, 4.0 * c->N + rows * (4.0 * NN + 8.0));
+    hipLaunchKernelGGL((line_aggregate_inv_kernel<NN, RIN>), dim3(grid1(work, 256)), dim3(256), 0, c->stream,
+                       c->d_oc.p, c->d_k.p, c->d_linv16.p, c->d_lcpl32.p, r, c->d_agg.p, st);
+  }
+  {
+    ProfScope ps(c, P_LINE_CARRY, 3.0 * c->oc.n_chunks * c->g.nk * 2 * sizeof(double2));
+    hipLaunchKernelGGL(line_carry_kernel, dim3(grid1(64L * c->g.nk, 256)), dim3(256), 0, c->stream, c->d_oc.p,
+                       c->d_k.p, c->d_agg.p, st);
+  }
+  {
+    ProfScope ps(c, P_LINE_APPLY, 12.0 * c->N + rows * (4.0 * NN * NN + 8.0 * NN));
+    hipLaunchKernelGGL((line_apply_inv_kernel<NN, RIN>), dim3(grid1(work, 256)), dim3(256), 0, c->stream,
+                       c->d_oc.p, c->d_k.p, c->d_linv16.p, c->d_lcpl32.p, c->d_agg.p, r, z, st);
+  }
+}
+
+template <typename VP>
+void op_precond_basis(wdg_ctx* c, const VP* v, float2* z, const GmresState* st) {
+  using RIN = std::conditional_t<std::is_same<VP, ushort4>::value, ushort2, uint32_t>;
+  const RIN* r = reinterpret_cast<const RIN*>(v);
+  switch (c->g.n) {
+    case 2: op_precond_basis_n<2, RIN>(c, r, z, st); break;
+    case 3: op_precond_basis_n<3, RIN>(c, r, z, st); break;
+    case 4: op_precond_basis_n<4, RIN>(c, r, z, st); break;
+    case 5: op_precond_basis_n<5, RIN>(c, r, z, st); break;
+    case 6: op_precond_basis_n<6, RIN>(c, r, z, st); break;
+    case 7: op_precond_basis_n<7, RIN>(c, r, z, st); break;
+    case 8: op_precond_basis_n<8, RIN>(c, r, z, st); break;
+    default: throw std::invalid_argument("n_k_chi outside 2..8");
+  }
+}
+
 void clear_graphs(wdg_ctx* c);
 
 void ensure_krylov(wdg_ctx* c, int m, bool fp64, bool b16) {
@@ -677,8 +717,11 @@ void gmres_cycle(wdg_ctx* c, int m, bool precond, bool single_pass) {
   const long NP = N * (long)sizeof(T2) / 16;        // working packets per vector
   const double vbytes = (double)NP * sizeof(VP);    // bytes of one stored basis vector
   const bool wide = sizeof(VP) == 8 && NP % 2 == 0;
+  // 1D contexts with the line preconditioner read a 16-bit basis vector in
+  // place, so the newest vector needs no FP32 copy
+  const bool direct16 = narrow && precond && !c->f2;
   PT* wp = reinterpret_cast<PT*>(w);
-  PT* vcp = reinterpret_cast<PT*>(vcur);
+  PT* vcp = direct16 ? nullptr : reinterpret_cast<PT*>(vcur);
   GmresBufs gb = c->gb();
   const GmresState* st = c->d_st.p;
   hipLaunchKernelGGL(cycle_init_kernel, dim3(1), dim3(1), 0, c->stream, gb);
@@ -688,7 +731,14 @@ void gmres_cycle(wdg_ctx* c, int m, bool precond, bool single_pass) {
     const T2* vj = narrow ? vcur : reinterpret_cast<const T2*>(V + (size_t)j * NP);
     const T2* opin = vj;
     if (precond) {
-      op_precond<T>(c, vj, z, st);
+      bool done = false;
+      if constexpr (narrow && std::is_same<T, float>::value) {
+        if (direct16) {
+          op_precond_basis<VP>(c, V + (size_t)j * NP, z, st);
+          done = true;
+        }
+      }
+      if (!done) op_precond<T>(c, vj, z, st);
       opin = z;
     }
     op_apply<T>(c, opin, w, st);
