@@ -106,6 +106,18 @@ def load_sim_res(path) -> dict:
     for k in ("sweep", "selfConsistency", "mat", "boundary"):
         if k in d:
             out[k] = _unwrap(d[k])
+    sc = out.get("selfConsistency")
+    if isinstance(sc, dict) and "history" in sc:
+        # one list of iteration records per bias, also for a single bias or iteration
+        hist = sc["history"]
+        hist = [hist] if isinstance(hist, dict) else list(hist)
+        if hist and isinstance(hist[0], dict) and all(isinstance(h, dict) for h in hist) and \
+                len(out.get("sweep", {}).get("status", [None])) == 1:
+            hist = [hist]
+        sc["history"] = [[h] if isinstance(h, dict) else list(h) for h in hist]
+    sw = out.get("sweep")
+    if isinstance(sw, dict) and isinstance(sw.get("status"), str):
+        sw["status"] = [sw["status"]]
     return out
 
 

@@ -55,10 +55,12 @@ def test_stored_sweep_density_and_current(case):
     assert abs(pinfo["physicalRelresBefore"] - case["last_poisson_relres"]) <= 0.05 * case["last_poisson_relres"]
 
 
-def test_self_consistent_loop_converges_at_the_stored_state():
+def test_self_consistent_loop_converges_at_the_stored_state(tmp_path):
     """The warm-started loop, started from the stored 0.2 V potential,
     stops by the run's criteria within three iterations and stays at the
-    stored state."""
+    stored state; its result written as a Sim_Res.mat-layout file reads
+    back with the stored file's sweep shapes and the loop's values."""
+    from real_space_dg_amd.sim_res import load_sim_res, save_sim_res
     case = CASES[1]
     res = PS.solve2d_rtd_self_consistent(MAT, initial_phi=case["phi"], biases=[case["bias"]],
                                          poisson_method="newton-raphson", max_iterations=6)
@@ -68,6 +70,14 @@ def test_self_consistent_loop_converges_at_the_stored_state():
     assert res["converged"] and sw["status"][0] == "fixed-point-converged" and len(hist) <= 3
     assert np.max(np.abs(sw["phi"][0] - case["phi"])) < 1e-4
     assert abs(np.mean(sw["current"][0]) - np.mean(case["j"])) <= 1e-3 * abs(np.mean(case["j"]))
+    f = tmp_path / "Sim_Res.mat"
+    save_sim_res(f, res, MAT, poisson_method="newton-raphson")
+    d = load_sim_res(f)
+    assert d["sweep"]["density"].shape == case["n"].shape + (1,)
+    assert d["sweep"]["potential"].shape == case["V"].shape + (1,)
+    assert np.array_equal(d["sweep"]["phi"][:, :, 0], sw["phi"][0])
+    assert d["sweep"]["status"] == ["fixed-point-converged"]
+    assert len(d["selfConsistency"]["history"][0]) == len(hist)
 
 
 def test_self_consistent_transient_stays_at_the_stationary_state():

@@ -111,3 +111,17 @@ def test_unconverged_sweep_is_marked_and_partial(tmp_path):
     assert d["sweep"]["requestedBias"].shape == (1, 3) and d["sweep"]["bias"].shape == (1, 2)
     assert d["selfConsistency"]["converged"] == 0.0 and d["selfConsistency"]["status"] == "iteration-limit"
     assert np.array_equal(d["sweep"]["converged"], [[1.0, 0.0]])
+
+
+def test_single_bias_and_single_iteration_keep_list_shapes(tmp_path):
+    mat, cases = S.load()
+    for n_it in (1, 2):
+        res = sweep_result(cases[:1])
+        res["histories"] = [res["histories"][0][:n_it]]
+        f = tmp_path / f"one_{n_it}.mat"
+        save_sim_res(f, res, mat)
+        d = load_sim_res(f)
+        assert d["sweep"]["status"] == ["fixed-point-converged"]
+        hist = d["selfConsistency"]["history"]
+        assert len(hist) == 1 and len(hist[0]) == n_it and isinstance(hist[0][0], dict)
+        assert hist[0][-1]["iteration"] == n_it
