@@ -125,7 +125,7 @@ struct wdg_ctx {
   DevBuf<double2> d_V64, d_w64, d_z64, d_s64;
   DevBuf<float2> d_V32, d_w32, d_z32, d_s32;
   DevBuf<uint2> d_Vb16;               // 16-bit basis of the FP32 inner solver (2 complex per packet)
-  DevBuf<float2> d_vcur32;            // FP32 copy of the newest BF16 basis vector
+  DevBuf<float2> d_vcur32;            // FP32 copy of the newest 16-bit basis vector (when needed)
   DevBuf<double2> d_H, d_cs, d_sn, d_g, d_h1, d_h2, d_yv, d_part;
   DevBuf<GmresState> d_st;
   DevBuf<double> d_resv;   // residual history of the last solve (relative norms)
@@ -691,9 +691,10 @@ void residual64(wdg_ctx* c) {
 }
 
 // One restart cycle in precision T (no host synchronisation).  VP is the
-// basis packet: the working packet of T, or uint2 for a BF16 basis of the
-// FP32 solver (then the newest vector is also kept in FP32 in vcur, which
-// is what the preconditioner and the operator read).
+// basis packet: the working packet of T, or uint2 / ushort4 for a BF16 /
+// FP16 basis of the FP32 solver.  With a 16-bit basis the 1D line
+// preconditioner reads the stored vector in place; otherwise the newest
+// vector is also kept in FP32 in vcur for the preconditioner and operator.
 template <typename T, typename VP>
 void gmres_cycle(wdg_ctx* c, int m, bool precond, bool single_pass) {
   using T2 = typename cvec<T>::type;
