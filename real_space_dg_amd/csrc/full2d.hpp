@@ -767,6 +767,22 @@ __global__ void f2_scale_kernel(const typename F2Vec<T>::T2* __restrict__ p, con
   }
 }
 
+// z = p r with a real scale p (row-sum scaling): 12 (FP32) or 24 (FP64)
+// bytes per entry instead of 16 or 32.
+template <typename T>
+__global__ void f2_scale_real_kernel(const T* __restrict__ p, const typename F2Vec<T>::T2* __restrict__ r,
+                                     typename F2Vec<T>::T2* __restrict__ z, long n, const GmresState* st) {
+  if (skip(st)) return;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const T a = p[i];
+    const auto b = r[i];
+    typename F2Vec<T>::T2 o;
+    o.x = a * b.x;
+    o.y = a * b.y;
+    z[i] = o;
+  }
+}
+
 // max |d| and the number of entries with |d| < floor (preconditioner floors).
 __global__ void f2_absmax_kernel(const double2* __restrict__ d, long n, double* __restrict__ part) {
   __shared__ double sm[256];
@@ -784,8 +800,8 @@ __global__ void f2_absmax_kernel(const double2* __restrict__ d, long n, double* 
 
 // Builds the preconditioner vector from the (scaled) diagonal or row sums:
 // jacobi (kind 1): p = 1/d, identity action (p = s_A) where |d| < floor;
-// rowabs (kind 2): p = 1/s, identity action where s < floor.  Counts the
-// floored entries.
+// rowabs (kind 2): p = 1/s, identity action where s < floor, stored real
+// (the first n doubles / floats of p64 / p32).  Counts the floored entries.
 __global__ void f2_precond_build_kernel(const double2* __restrict__ d, long n, int kind, double floor_dev,
                                         double ident, double2* __restrict__ p64, float2* __restrict__ p32,
                                         unsigned long long* __restrict__ nsmall) {
@@ -808,8 +824,13 @@ __global__ void f2_precond_build_kernel(const double2* __restrict__ d, long n, i
         o = make_double2(1.0 / v.x, 0.0);
       }
     }
-    p64[i] = o;
-    p32[i] = make_float2((float)o.x, (float)o.y);
+    if (kind == 1) {
+      p64[i] = o;
+      p32[i] = make_float2((float)o.x, (float)o.y);
+    } else {
+      reinterpret_cast<double*>(p64)[i] = o.x;
+      reinterpret_cast<float*>(p32)[i] = (float)o.x;
+    }
   }
 }
 

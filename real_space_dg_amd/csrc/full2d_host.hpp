@@ -234,6 +234,11 @@ void f2_precond(wdg_ctx* c, const typename F2Vec<T>::T2* r, typename F2Vec<T>::T
   }
   if (std::is_same<T, double>::value) p = f.d_p64.p;
   else p = f.d_p32.p;
+  if (f.precond == WDG2_PRECOND_ROWABS) {   // real scale
+    hipLaunchKernelGGL(f2_scale_real_kernel<T>, dim3(c->red_blocks), dim3(256), 0, c->stream, (const T*)p, r, z, c->N,
+                       st);
+    return;
+  }
   hipLaunchKernelGGL(f2_scale_kernel<T>, dim3(c->red_blocks), dim3(256), 0, c->stream,
                      (const typename F2Vec<T>::T2*)p, r, z, c->N, st);
 }
