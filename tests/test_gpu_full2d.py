@@ -123,6 +123,13 @@ def test_reference_signature_entry_point():
     assert res.n.shape == (nxd, m["Ny"]) and res.j.shape == (nxd,)
     n_or, _, _ = F.observables_full2d(rho_or, p, m)
     assert rel(res.nDG, n_or) < 1e-5
+    # info.solve.resvec as the reference reports it (times ||rhs||,
+    # solve_transport_DG_full2D.m:324): ||rhs|| first (zero start), one value
+    # per iteration, the last one the final estimate below tol * ||rhs||
+    rv, rn = res.info["solve"]["resvec"], np.linalg.norm(rhs)
+    assert rv.size == res.info["solve"]["iterations"] + 1
+    assert abs(rv[0] - rn) <= 1e-10 * rn
+    assert rv[-1] <= 1e-9 * rn and np.all(np.isfinite(rv))
 
 
 @pytest.mark.parametrize("name", ["rusanov-specular-fv", "upwind-zero-inflow", "central-closed-faces-collocated",
