@@ -28,15 +28,23 @@
 extern "C" {
 #endif
 
-/* Status codes.  The MATLAB shim maps them onto the reference identifiers. */
+/* Status codes.  The three the reference raises itself carry its error
+ * identifiers; the others are this library's own (the MATLAB shim raises
+ * them as wdg:<Name>). */
 enum {
   WDG_OK = 0,
-  WDG_ERR_ARG = -1,          /* invalid argument (DG:Full2D:InvalidInitialRho and friends) */
-  WDG_ERR_HIP = -2,          /* HIP runtime failure */
-  WDG_ERR_MEMORY = -3,       /* DG:Full2D:GmresMemoryLimit / LineMemoryLimit */
-  WDG_ERR_STATE = -4,        /* call order: no potential set yet */
-  WDG_ERR_NUMERIC = -5,      /* singular factor / non-finite value */
-  WDG_ERR_UNSUPPORTED = -6   /* option outside the restated method */
+  WDG_ERR_ARG = -1,            /* wdg:InvalidArgument                              */
+  WDG_ERR_HIP = -2,            /* wdg:DeviceError, a HIP runtime failure           */
+  WDG_ERR_MEMORY = -3,         /* wdg:OutOfMemory, device (or host) memory exhausted */
+  WDG_ERR_STATE = -4,          /* wdg:CallOrder, e.g. no potential set yet         */
+  WDG_ERR_NUMERIC = -5,        /* wdg:NumericalFailure, singular factor / non-finite value */
+  WDG_ERR_UNSUPPORTED = -6,    /* wdg:Unsupported, option outside the restated method */
+  WDG_ERR_INITIAL_RHO = -7,    /* DG:Full2D:InvalidInitialRho, a non-finite warm start
+                                  (DG_code/solve_transport_DG_full2D.m:508-525; the size
+                                  check is the caller's: numel(rho0) == wdg_size)    */
+  WDG_ERR_GMRES_MEMORY = -8,   /* DG:Full2D:GmresMemoryLimit (solve_transport_DG_full2D.m:183-203) */
+  WDG_ERR_LINE_MEMORY = -9     /* DG:Full2D:LineMemoryLimit
+                                  (DG_code/core/get_LineTransversePreconditioner_full2D.m:33-43) */
 };
 
 enum { WDG_FERMI_MSA_3D = 0,      /* pure 1D Fermi-Dirac, DG_code/core/verteilung.m:77-114        */
@@ -133,6 +141,15 @@ int wdg_rhs(wdg_ctx* ctx, double* rhs);
  * solver (FP32 transport, split-BF16 MFMA Theta with FP32 accumulation). */
 int wdg_apply(wdg_ctx* ctx, const double* u, double* y, int32_t precision);
 
+/* One part of the operator: WDG_PART_TRANSPORT gives y = A_diff u
+ * (diff.apply), WDG_PART_POTENTIAL gives y = G u, Theta[V] with the CAP
+ * (drift.apply), WDG_PART_ALL their sum as wdg_apply: the two handles that
+ * sysInfo.apply adds (DG_code/core/get_SysM_full2D.m:124-127; for the 1D
+ * operator A = A_diff + G of DG_code/core/get_SysM.m:3-8).  1D contexts
+ * only (full-2D: WDG_PART_ALL). */
+enum { WDG_PART_TRANSPORT = 1, WDG_PART_POTENTIAL = 2, WDG_PART_ALL = 3 };
+int wdg_apply_parts(wdg_ctx* ctx, const double* u, double* y, int32_t precision, int32_t parts);
+
 /* z = M^-1 r for the line preconditioner (pre.apply,
  * DG_code/core/get_LineTransversePreconditioner_full2D.m:199-239, restated
  * for the k basis: exact per-k block-bidiagonal upwind solve along x of
@@ -150,6 +167,13 @@ int wdg_diagonal(wdg_ctx* ctx, double* d);
  * solution stays on the device for wdg_observables). */
 int wdg_solve(wdg_ctx* ctx, const wdg_solve_opts* opts, const double* x0, double* x,
               wdg_solve_info* info);
+
+/* The context's last solution (of wdg_solve, wdg_solve_batch or a transient
+ * run), N interleaved complex doubles in physical units: the rho that
+ * solve_transport_DG_full2D returns (DG_code/solve_transport_DG_full2D.m:67-129)
+ * and solve_transport_DG_2D with st_init (DG_code/solve_transport_DG_2D.m:211-225),
+ * for solutions left on the device.  WDG_ERR_STATE before any solve. */
+int wdg_solution(wdg_ctx* ctx, double* x);
 
 /* Residual history of the context's last solve (wdg_solve, wdg_solve_batch,
  * or the last step of a Crank-Nicolson run): the resvec of the reference's
@@ -257,6 +281,9 @@ int wdg_transient_rk4(wdg_ctx* ctx, const wdg_transient_opts* opts, const double
  * full-2D contexts; full-2D Jacobi / row-sum preconditioners are rebuilt for
  * I + dt/2 a_scale A (use_precond), the 1D line and the line-transverse
  * preconditioners are not used for the shifted system. */
+/* use_graph is ignored (cycles are launched kernel by kernel) for FP64 runs
+ * on full-2D contexts: at 33.5M unknowns such a run with graph replay ended
+ * in a host-side fault inside the HIP runtime whose cause is not identified. */
 typedef struct wdg_cn_opts {
   double  dt;            /* time step [s]                                   */
   int32_t nsteps;
@@ -300,6 +327,32 @@ enum { WDG_THETA_AUTO = 0, WDG_THETA_FFT = 1, WDG_THETA_GEMM = 2 };
 int wdg_set_theta_method(wdg_ctx* ctx, int32_t method);
 /* The formulation in use (WDG_THETA_FFT or WDG_THETA_GEMM), -1 for NULL. */
 int wdg_theta_method(const wdg_ctx* ctx);
+
+/* Memory guards of the reference.  gmres_max_basis_bytes bounds the
+ * estimated Krylov basis 16 * N * (restart + 1) bytes of every solve
+ * (full2D_gmresMaxBasisBytes, 9 GiB on the GPU for the line-transverse
+ * preconditioner, solve_transport_DG_full2D.m:183-203; 0 or +inf: no limit,
+ * the default); a solve above it returns WDG_ERR_GMRES_MEMORY.
+ * line_max_factor_bytes bounds the line-transverse factors of
+ * wdg2_set_line_precond, 32 nT^2 nModes + 32 nLocal^2 nModes nElem +
+ * 16 nModes^2 + 8 nT nModes bytes (full2D_lineMaxFactorBytes, 4 GiB by
+ * default, get_LineTransversePreconditioner_full2D.m:33-43); above it, or
+ * when it is not positive and finite, the call returns WDG_ERR_LINE_MEMORY. */
+int wdg_set_memory_limits(wdg_ctx* ctx, double gmres_max_basis_bytes, double line_max_factor_bytes);
+
+/* Workgroup shape of the FP32 (mixed inner) chirp-z Theta kernel:
+ * WDG_FFT_BLOCK_256 runs 256 threads that loop over the element's nodes;
+ * WDG_FFT_BLOCK_WIDE runs ~1024 threads, four or more transforms at once,
+ * for n_k_chi <= 5; AUTO (the default) picks the one that keeps more waves
+ * resident per CU.  Both compute the same operator (the choice changes
+ * FP32 rounding order only).  No counterpart in the reference: an
+ * implementation choice, exposed so that every variant can be tested.
+ * Returns WDG_ERR_ARG when the grid has no FFT plan or the wide block does
+ * not fit it.  wdg_fft_block gives the threads per workgroup in use (-1: no
+ * FFT plan). */
+enum { WDG_FFT_BLOCK_AUTO = 0, WDG_FFT_BLOCK_256 = 1, WDG_FFT_BLOCK_WIDE = 2 };
+int wdg_set_fft_block(wdg_ctx* ctx, int32_t mode);
+int wdg_fft_block(const wdg_ctx* ctx);
 
 /* Enable per-kernel timing with HIP events on the context stream for the
  * kernels selected by mask (bit i = entry i of wdg_profile's list); 0 turns
@@ -408,6 +461,17 @@ typedef struct wdg2_potential {
   double cap_coef;             /* drift_scale*cap_scale*Q_drift (:162)               */
 } wdg2_potential;
 int wdg2_set_potential(wdg_ctx* ctx, const wdg2_potential* pot);
+
+/* Source of Delta V in the FV-consistent apply.  WDG2_DV_INTERP (the
+ * default) interpolates the material grid inside the apply kernel (the
+ * grid and the per-sample cell tables sit in LDS), so an apply streams only
+ * u and y; WDG2_DV_TABLE stores Delta V per (element, quadrature point,
+ * relative vertex) in HBM (12 bytes per entry for both precisions) and
+ * streams it.  Both compute the same values (the table is filled by the
+ * same interpolation).  Takes effect at once (or at wdg2_set_potential).
+ * No counterpart in the reference: an implementation choice. */
+enum { WDG2_DV_INTERP = 0, WDG2_DV_TABLE = 1 };
+int wdg2_set_dv_mode(wdg_ctx* ctx, int32_t mode);
 
 /* The inflow right-hand side assembled by the caller (sysInfo.assembleRhs(),
  * get_Diff_full2D.m:609-616), N interleaved complex doubles; uploaded once

@@ -15,13 +15,16 @@ solve_transport_DG_2D_transient_SC_new.m:172-178 (n_xi even branch).
 Also the Crank-Nicolson (trapezoidal) step of the same ODE,
 (I + dt/2 a A) rho+ = (I - dt/2 a A) rho + dt b rhs            -> crank_nicolson
 which has no counterpart in the reference (its full-2D transient entry points
-are stubs, DG_code/solve_transport_DG_full2D_transient_flatband.m:8-11).
+are stubs, DG_code/solve_transport_DG_full2D_transient_flatband.m:8-11), and
+the same step for operators too large to assemble, each step solved by the
+oracle's GMRES to a tight tolerance                     -> crank_nicolson_mf
 
 Pinning (tests/test_oracle_transient.py): against the exact solution of the
 linear ODE, rho(t) = rho_inf + expm(-a A t)(rho0 - rho_inf), with the global
 error falling as dt^4 (RK4) and dt^2 (Crank-Nicolson); the stationary state
 is a fixed point of both; on a diagonal A each Crank-Nicolson step multiplies
-a mode by (1 - z/2)/(1 + z/2), z = dt a lambda.
+a mode by (1 - z/2)/(1 + z/2), z = dt a lambda; crank_nicolson_mf equals
+crank_nicolson on an assembled operator.
 """
 from __future__ import annotations
 
@@ -68,3 +71,27 @@ def crank_nicolson(A, rhs, rho0, dt, nsteps, a_scale=1.0, b_scale=1.0):
         rho = lu.solve(minus @ rho + g)
     return rho
 
+
+
+def crank_nicolson_mf(apply_a, rhs, rho0, dt, nsteps, a_scale=1.0, b_scale=1.0, tol=1e-13, restart=60,
+                      row_scale=None):
+    """crank_nicolson with A given by its apply: each step's system
+    (I + dt/2 a A) x = (I - dt/2 a A) rho + dt b rhs solved by the oracle's
+    right-preconditioned GMRES to true relative residual tol, with the row
+    scaling 1 / (1 + dt/2 a row_scale) when row_scale (|A| row sums) is given."""
+    from .krylov import gmres
+    h = 0.5 * dt * a_scale
+    g = dt * b_scale * np.asarray(rhs, complex)
+    rho = np.asarray(rho0, complex).copy()
+    if row_scale is None:
+        apply_m = lambda v: v  # noqa: E731
+    else:
+        inv = 1.0 / (1.0 + h * np.asarray(row_scale, float))
+        apply_m = lambda v: inv * v  # noqa: E731
+    shifted = lambda v: v + h * apply_a(v)  # noqa: E731
+    for _ in range(nsteps):
+        b = rho - h * apply_a(rho) + g
+        rho, info = gmres(shifted, apply_m, b, restart=restart, tol=tol, maxit=200000, x0=rho)
+        if info["flag"] != 0:
+            raise RuntimeError(f"crank_nicolson_mf: step solve stopped at relres {info['true_relres']:.2e}")
+    return rho

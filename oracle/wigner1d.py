@@ -249,10 +249,24 @@ def init_params(mat: dict, EfL: float, EfR: float, V, m, deg_factor: float) -> d
 
 def verteilung(mat, p, N_xi, phi, EfL, EfR, V, m):
     """Inflow distributions in the k basis.  DG_code/core/verteilung.m:77-170."""
+    fL, fR, k_fermi, dk = inflow_k(mat, p, EfL, EfR, V, m)
+    y = p["xi_Rechengebiet_Elem"]
+    # cosine transform to xi (:109-112 / :158-161), then phi \ f (:113-114 / :167-168)
+    Cm = np.cos(np.outer(y[:N_xi], k_fermi))
+    fLn = Cm @ fL * dk / 2 / np.pi
+    fRn = Cm @ fR * dk / 2 / np.pi
+    fnl = np.linalg.solve(phi, fLn.astype(complex))
+    fnr = np.linalg.solve(phi, fRn.astype(complex))
+    return fnl, fnr
+
+
+def inflow_k(mat, p, EfL, EfR, V, m):
+    """Contact occupations f_L(k), f_R(k) on the k grid before the transform
+    to xi (DG_code/core/verteilung.m:77-99 pure 1D Fermi-Dirac, :118-153
+    k_z-integrated).  Returns (fL, fR, k, dk)."""
     hb = H_VT / 2 / np.pi
     q, kB, Temp = Q_VT, KB_VT, float(mat["Temp"])
     Nx = int(mat["Nx"])
-    y = p["xi_Rechengebiet_Elem"]
     Nk = p["Nk"]
     k_fermi = -2 * np.pi / p["L_xi"] * (np.arange(1, Nk + 1) - 0.5 * (Nk + 1))
     dk = abs(k_fermi[1] - k_fermi[0])
@@ -280,13 +294,7 @@ def verteilung(mat, p, N_xi, phi, EfL, EfR, V, m):
         fR = 2 / np.pi * dkz * np.sum(1.0 / (1.0 + np.exp(q * (ER[:, None] + Ekz[None, :] - EfR) / (kB * Temp))), axis=1)
     else:
         raise NotImplementedError(p["choose_fermi"])
-    # cosine transform to xi (:109-112 / :158-161), then phi \ f (:113-114 / :167-168)
-    Cm = np.cos(np.outer(y[:N_xi], k_fermi))
-    fLn = Cm @ fL * dk / 2 / np.pi
-    fRn = Cm @ fR * dk / 2 / np.pi
-    fnl = np.linalg.solve(phi, fLn.astype(complex))
-    fnr = np.linalg.solve(phi, fRn.astype(complex))
-    return fnl, fnr
+    return fL, fR, k_fermi, dk
 
 
 # ---------------------------------------------------------------------------

@@ -21,6 +21,12 @@ WDG_FERMI_TSU_ESAKI = 2
 WDG_THETA_AUTO = 0
 WDG_THETA_FFT = 1
 WDG_THETA_GEMM = 2
+WDG_FFT_BLOCK_AUTO = 0
+WDG_FFT_BLOCK_256 = 1
+WDG_FFT_BLOCK_WIDE = 2
+WDG_PART_TRANSPORT = 1
+WDG_PART_POTENTIAL = 2
+WDG_PART_ALL = 3
 WDG_BASIS_WORKING = 0
 WDG_BASIS_BF16 = 1
 WDG_BASIS_FP16 = 2
@@ -35,26 +41,43 @@ WDG2_REL_IDENTITY = 1
 WDG2_REL_DENSE = 2
 WDG2_POT_FV_CONSISTENT = 0
 WDG2_POT_COLLOCATED = 1
+WDG2_DV_INTERP = 0
+WDG2_DV_TABLE = 1
 WDG2_PRECOND_NONE = 0
 WDG2_PRECOND_JACOBI = 1
 WDG2_PRECOND_ROWABS = 2
 WDG2_PRECOND_LINE = 3
 
-# Status code -> reference error identifier (DG:Full2D:* in the MATLAB code).
+WDG_ERR_ARG = -1
+WDG_ERR_HIP = -2
+WDG_ERR_MEMORY = -3
+WDG_ERR_STATE = -4
+WDG_ERR_NUMERIC = -5
+WDG_ERR_UNSUPPORTED = -6
+WDG_ERR_INITIAL_RHO = -7
+WDG_ERR_GMRES_MEMORY = -8
+WDG_ERR_LINE_MEMORY = -9
+
+# Status code -> error identifier: the reference's own DG:Full2D:* identifiers
+# for the three errors it raises (solve_transport_DG_full2D.m:197-202,519-525,
+# get_LineTransversePreconditioner_full2D.m:36-43), wdg:* for this library's.
 ERROR_IDS = {
-    -1: "DG:Full2D:InvalidArgument",
-    -2: "DG:Full2D:DeviceError",
-    -3: "DG:Full2D:GmresMemoryLimit",
-    -4: "DG:Full2D:OperatorNotReady",
-    -5: "DG:Full2D:NumericalFailure",
-    -6: "DG:Full2D:Unsupported",
+    WDG_ERR_ARG: "wdg:InvalidArgument",
+    WDG_ERR_HIP: "wdg:DeviceError",
+    WDG_ERR_MEMORY: "wdg:OutOfMemory",
+    WDG_ERR_STATE: "wdg:CallOrder",
+    WDG_ERR_NUMERIC: "wdg:NumericalFailure",
+    WDG_ERR_UNSUPPORTED: "wdg:Unsupported",
+    WDG_ERR_INITIAL_RHO: "DG:Full2D:InvalidInitialRho",
+    WDG_ERR_GMRES_MEMORY: "DG:Full2D:GmresMemoryLimit",
+    WDG_ERR_LINE_MEMORY: "DG:Full2D:LineMemoryLimit",
 }
 
 
 class WdgError(RuntimeError):
     def __init__(self, code: int, msg: str):
         self.code = code
-        self.identifier = ERROR_IDS.get(code, "DG:Full2D:Error")
+        self.identifier = ERROR_IDS.get(code, "wdg:Error")
         super().__init__(f"{self.identifier} ({code}): {msg}")
 
 
@@ -151,6 +174,7 @@ SIGNATURES = {
     "wdg_set_potential": (C.c_int, [_P, C.POINTER(wdg_potential)]),
     "wdg_rhs": (C.c_int, [_P, _D]),
     "wdg_apply": (C.c_int, [_P, _D, _D, C.c_int32]),
+    "wdg_apply_parts": (C.c_int, [_P, _D, _D, C.c_int32, C.c_int32]),
     "wdg_precond": (C.c_int, [_P, _D, _D]),
     "wdg_diagonal": (C.c_int, [_P, _D]),
     "wdg_solve": (C.c_int, [_P, C.POINTER(wdg_solve_opts), _D, _D, C.POINTER(wdg_solve_info)]),
@@ -163,14 +187,19 @@ SIGNATURES = {
     "wdg_transient_rk4": (C.c_int, [_P, C.POINTER(wdg_transient_opts), _D, _D, _D, _D, _D, C.c_int32]),
     "wdg_transient_cn": (C.c_int, [_P, C.POINTER(wdg_cn_opts), _D, _D, C.POINTER(wdg_cn_info)]),
     "wdg_solve_resvec": (C.c_int, [_P, _D, C.c_int32, C.POINTER(C.c_int32)]),
+    "wdg_solution": (C.c_int, [_P, _D]),
     "wdg_solve_batch": (C.c_int, [C.POINTER(_P), C.c_int32, C.POINTER(wdg_solve_opts), C.POINTER(wdg_solve_info)]),
     "wdg_transient_cn_batch": (C.c_int, [C.POINTER(_P), C.c_int32, C.POINTER(wdg_cn_opts), C.POINTER(_D),
                                          C.POINTER(_D), C.POINTER(wdg_cn_info)]),
     "wdg_set_theta_method": (C.c_int, [_P, C.c_int32]),
     "wdg_theta_method": (C.c_int, [_P]),
+    "wdg_set_fft_block": (C.c_int, [_P, C.c_int32]),
+    "wdg_set_memory_limits": (C.c_int, [_P, C.c_double, C.c_double]),
+    "wdg_fft_block": (C.c_int, [_P]),
     "wdg2_create": (C.c_int, [C.POINTER(wdg2_desc), C.POINTER(wdg2_part), C.c_int32, C.POINTER(_P)]),
     "wdg2_set_potential": (C.c_int, [_P, C.POINTER(wdg2_potential)]),
     "wdg2_set_rhs": (C.c_int, [_P, _D]),
+    "wdg2_set_dv_mode": (C.c_int, [_P, C.c_int32]),
     "wdg2_rowabs": (C.c_int, [_P, _D]),
     "wdg2_set_precond": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_int32)]),
     "wdg2_set_line_precond": (C.c_int, [_P, C.POINTER(wdg2_line_factors)]),

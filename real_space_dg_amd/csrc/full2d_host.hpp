@@ -32,7 +32,9 @@ struct F2Data {
   DevBuf<F2Const> d_c;
   DevBuf<int> d_tix, d_tiy;
   DevBuf<double> d_ttx, d_tty, d_v, d_capx, d_capy;
-  DevBuf<double> d_dv64;                // stored potential differences (FV-consistent)
+  DevBuf<double> d_dv64;                // stored potential differences (FV-consistent, WDG2_DV_TABLE)
+  int dv_mode = WDG2_DV_TABLE;          // wdg2_set_dv_mode
+  long nv = 0;                          // material grid points
   DevBuf<float> d_dv32;
   bool fv = true;
   size_t lds_apply64 = 0, lds_apply32 = 0, lds_dense64 = 0, lds_dense32 = 0;
@@ -513,6 +515,60 @@ void f2_sample_table(const double* centre, int nc, const double* rel, int nr, co
       }
 }
 
+// Where the FV-consistent apply gets Delta V from (c->f2->dv_mode): a table
+// per (element, quadrature point, relative vertex) in HBM that the apply
+// streams (TABLE), or the material grid, interpolated twice per sample
+// inside the apply from LDS (INTERP, the default: nothing streamed beyond u
+// and y).  The choice depends on the mode alone, never on free memory.
+// Sets the apply tile (rows of rho_y per workgroup, LDS) for that choice and
+// uploads the constants.
+void f2_configure_dv(wdg_ctx* c) {
+  F2Data& f = *c->f2;
+  const wdg2_desc& d = f.d;
+  F2Const& h = f.hc;
+  const bool fv = f.fv;
+  const int nrx = d.n_rho_x, nry = d.n_rho_y;
+  const int npx = h.npx;
+  const bool tab = fv && f.dv_mode == WDG2_DV_TABLE;
+  if (tab) {
+    const size_t cnt = (size_t)d.ne_x * d.ne_y * h.nqx * h.nqy * (nrx + 1) * (nry + 1);
+    f.d_dv64.alloc(cnt);
+    f.d_dv32.alloc(cnt);
+  } else {   // the table's memory goes back
+    f.d_dv64.release();
+    f.d_dv32.release();
+  }
+  // rho_y rows per apply tile: as many as fit 256 threads and the LDS budget;
+  // the material grid joins the tile in LDS when it is small
+  const int nl = d.nl_x * d.nl_y;
+  const int nvl = (size_t)f.nv * sizeof(double) <= 48 * 1024 ? (int)f.nv : 0;
+  h.v_lds = (!tab && nvl > 0) ? 1 : 0;
+  auto lds = [&](int rows, size_t t2) {
+    return f2_lds_layout(nl, rows, nrx, t2, fv, fv && !tab, h.nqx, h.nqy, npx, tab ? 0 : nvl).total;
+  };
+  int rows = std::max(1, std::min(nry, kF2Block / nrx));
+  while (rows > 1 && lds(rows, sizeof(double2)) > 96 * 1024) --rows;
+  if (lds(rows, sizeof(double2)) > kF2MaxLds) throw std::invalid_argument("n_rho_x too large for one apply tile");
+  h.rows = rows;
+  h.tiles = (nry + rows - 1) / rows;
+  f.lds_apply64 = lds(rows, sizeof(double2));
+  f.lds_apply32 = lds(rows, sizeof(float2));
+  h.dv64 = nullptr;
+  h.dv32 = nullptr;
+  HIPCHK(hipMemcpyAsync(f.d_c.p, &h, sizeof(F2Const), hipMemcpyHostToDevice, c->stream));
+  if (tab) {
+    const size_t cnt = f.d_dv64.n;
+    const unsigned grid = (unsigned)std::min<size_t>(65536, (cnt + 255) / 256);
+    hipLaunchKernelGGL(f2_dv_table_kernel, dim3(grid), dim3(256), 0, c->stream, f.d_c.p, f.d_dv64.p, f.d_dv32.p);
+    HIPCHK(hipGetLastError());
+    h.dv64 = f.d_dv64.p;
+    h.dv32 = f.d_dv32.p;
+    HIPCHK(hipMemcpyAsync(f.d_c.p, &h, sizeof(F2Const), hipMemcpyHostToDevice, c->stream));
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  clear_graphs(c);
+}
+
 void f2_set_potential_impl(wdg_ctx* c, const wdg2_potential& p) {
   F2Data& f = *c->f2;
   const wdg2_desc& d = f.d;
@@ -584,48 +640,9 @@ void f2_set_potential_impl(wdg_ctx* c, const wdg2_potential& p) {
         h.ipy[k] = (q < p.nq_y && a < d.nl_y) ? p.interp_y[q * d.nl_y + a] : 0.0;
         h.pry[k] = (q < p.nq_y && a < d.nl_y) ? p.proj_y[q * d.nl_y + a] : 0.0;
       }
-  // Store DeltaV per (element, quadrature point, relative vertex) when it fits
-  // a quarter of free device memory: the apply then streams it instead of
-  // interpolating the material grid twice per sample.
-  bool tab = false;
-  if (fv) {
-    const size_t cnt = (size_t)d.ne_x * d.ne_y * p.nq_x * p.nq_y * (nrx + 1) * (nry + 1);
-    size_t free_b = 0, total_b = 0;
-    HIPCHK(hipMemGetInfo(&free_b, &total_b));
-    if (cnt * 12 <= free_b / 4 + f.d_dv64.n * 8 + f.d_dv32.n * 4) {
-      f.d_dv64.alloc(cnt);
-      f.d_dv32.alloc(cnt);
-      tab = true;
-    }
-  }
-  // rho_y rows per apply tile: as many as fit 256 threads and the LDS budget;
-  // the material grid joins the tile in LDS when it is small
-  const int nl = d.nl_x * d.nl_y;
-  const int nvl = (size_t)nv * sizeof(double) <= 48 * 1024 ? (int)nv : 0;
-  h.v_lds = nvl > 0 ? 1 : 0;
-  auto lds = [&](int rows, size_t t2) {
-    return f2_lds_layout(nl, rows, nrx, t2, fv, fv && !tab, h.nqx, h.nqy, npx, tab ? 0 : nvl).total;
-  };
-  int rows = std::max(1, std::min(nry, kF2Block / nrx));
-  while (rows > 1 && lds(rows, sizeof(double2)) > 96 * 1024) --rows;
-  if (lds(rows, sizeof(double2)) > kF2MaxLds) throw std::invalid_argument("n_rho_x too large for one apply tile");
-  h.rows = rows;
-  h.tiles = (nry + rows - 1) / rows;
   f.fv = fv;
-  f.lds_apply64 = lds(rows, sizeof(double2));
-  f.lds_apply32 = lds(rows, sizeof(float2));
-  h.dv64 = nullptr;
-  h.dv32 = nullptr;
-  HIPCHK(hipMemcpyAsync(f.d_c.p, &h, sizeof(F2Const), hipMemcpyHostToDevice, c->stream));
-  if (tab) {
-    const size_t cnt = f.d_dv64.n;
-    const unsigned grid = (unsigned)std::min<size_t>(65536, (cnt + 255) / 256);
-    hipLaunchKernelGGL(f2_dv_table_kernel, dim3(grid), dim3(256), 0, c->stream, f.d_c.p, f.d_dv64.p, f.d_dv32.p);
-    HIPCHK(hipGetLastError());
-    h.dv64 = f.d_dv64.p;
-    h.dv32 = f.d_dv32.p;
-    HIPCHK(hipMemcpyAsync(f.d_c.p, &h, sizeof(F2Const), hipMemcpyHostToDevice, c->stream));
-  }
+  f.nv = nv;
+  f2_configure_dv(c);
   HIPCHK(hipStreamSynchronize(c->stream));
   clear_graphs(c);
   // the preconditioner depends on the potential: rebuild on the next request
@@ -702,6 +719,14 @@ void f2_set_line_impl(wdg_ctx* c, const wdg2_line_factors& lf) {
   F2Data& f = *c->f2;
   const int nrx = f.d.n_rho_x, nl = f.d.nl_x, ne = f.d.ne_x;
   const int nT = f.d.n_rho_y * f.nyd;
+  // the reference's factor-memory guard (get_LineTransversePreconditioner_full2D.m:33-43)
+  const double factor_bytes = 32.0 * nT * nT * nrx + 32.0 * nl * nl * nrx * ne + 16.0 * nrx * nrx + 8.0 * nT * nrx;
+  if (factor_bytes > c->line_max_factor_bytes) {
+    char msg[256];
+    std::snprintf(msg, sizeof msg, "Estimated factors %.3g GiB exceed full2D_lineMaxFactorBytes %.3g GiB.",
+                  factor_bytes / 1073741824.0, c->line_max_factor_bytes / 1073741824.0);
+    throw StatusError(WDG_ERR_LINE_MEMORY, msg);
+  }
   if (!(lf.sigma > 0) || !std::isfinite(lf.sigma)) throw std::invalid_argument("sigma must be positive and finite");
   if (!lf.modes || !lf.forward || !lf.x_inverse || !lf.x_coupling || !lf.t_inverse)
     throw std::invalid_argument("line factor arrays missing");

@@ -17,8 +17,9 @@
 //   T_b = phi U_{e,b}            for the n nodes b    (n forward chirp-z)
 //   Z_a = sum_b Cbar_ab (*) T_b  tridiagonal in l     (the Theta combine)
 //   Y_{e,a} = (transport u)_{e,a} + i phi^H Z_a       (n backward chirp-z)
-// with T held in LDS, so HBM sees one read of u (the transport rows re-read
-// it from L2), one write of y and the edge table: the whole operator apply.  FFTs are Stockham autosort passes whose first and
+// with T held in LDS, so HBM sees one read of u, one write of y (256-thread
+// blocks also write the transport rows early and re-read them) and half of
+// the edge table (it is odd in xi): the whole operator apply.  FFTs are Stockham autosort passes whose first and
 // last passes keep the data in registers (see fft_regs), so the chirp and
 // spectrum multiplies and the HBM traffic need no extra LDS round trips.
 #pragma once
@@ -329,10 +330,41 @@ __device__ __forceinline__ T2 transport_row(const OpConst* __restrict__ oc, cons
   return o;
 }
 
-// An opaque copy of a pointer (uniform: SGPRs, per lane: VGPRs).  Loads
-// through the copy stay below this point: the scheduler would otherwise
-// hoist the independent table loads of a straight-line wide block above the
-// transforms and keep them live through them, past a 128-VGPR budget.
+// The same row with every value read from u (the element's own nodes were
+// staged earlier in the kernel, so these reads mostly hit L2).
+template <int NN, typename T2>
+__device__ __forceinline__ T2 transport_row_g(const OpConst* __restrict__ oc, const double* __restrict__ kvec,
+                                              const T2* __restrict__ u, int e, int a, int j) {
+  using S = decltype(T2{}.x);
+  const int nk = oc->nk, ne = oc->n_chi;
+  const S k = (S)kvec[j], ak = k < 0 ? -k : k;
+  const T2* __restrict__ ue = u + (long)e * NN * nk + j;
+  S sr = 0, si = 0;
+#pragma unroll
+  for (int b = 0; b < NN; ++b) {
+    const T2 v = ue[(long)b * nk];
+    const S c = k * (S)oc->K1[a * kMaxN + b] + ak * (S)oc->K2[a * kMaxN + b];
+    sr += c * v.x;
+    si += c * v.y;
+  }
+  if (k > 0 && e > 0) {               // -(k+|k|)/2 invM(:,0) u_{e-1,n-1}
+    const T2 v = ue[-(long)nk];
+    const S cf = (S)-0.5 * (k + ak) * (S)oc->invM0[a];
+    sr += cf * v.x;
+    si += cf * v.y;
+  } else if (k < 0 && e < ne - 1) {   // (k-|k|)/2 invM(:,n-1) u_{e+1,0}
+    const T2 v = ue[(long)NN * nk];
+    const S cf = (S)0.5 * (k - ak) * (S)oc->invMn[a];
+    sr += cf * v.x;
+    si += cf * v.y;
+  }
+  const S tq = (S)oc->tq;
+  T2 o;
+  o.x = tq * sr;
+  o.y = tq * si;
+  return o;
+}
+
 template <typename Q>
 __device__ __forceinline__ Q* pin_uniform(Q* p) {
   asm volatile("" : "+s"(p));
@@ -480,7 +512,8 @@ __global__ __launch_bounds__(NT) void theta_fft_kernel(const OpConst* __restrict
                                                        const TS* __restrict__ sbar, const double* __restrict__ cap,
                                                        const double* __restrict__ kvec,
                                                        const typename cvec<T>::type* __restrict__ u,
-                                                       typename cvec<T>::type* __restrict__ y, const GmresState* st) {
+                                                       typename cvec<T>::type* __restrict__ y, const GmresState* st,
+                                                       int with_transport) {
   using T2 = typename cvec<T>::type;
   if (skip(st)) return;
   extern __shared__ __align__(16) unsigned char theta_smem[];
@@ -525,16 +558,21 @@ __global__ __launch_bounds__(NT) void theta_fft_kernel(const OpConst* __restrict
   }
   __syncthreads();
 
-  // transport rows first: Y_a = (transport u)_a, written by the thread that
-  // adds i phi^H Z_a to the same values below (no other thread reads them)
-  for (int a0 = 0; a0 < NN; a0 += G) {
-    const int a = a0 + g;
-    if (a < NN) {
-      T2* __restrict__ yb = y + ((long)e * NN + a) * nk;
+  // 256-thread blocks loop over the nodes, so the transport rows of all
+  // nodes cannot wait in registers for the backward transforms: they are
+  // written to y now, from the staged u, and read back by the same thread
+  // when it adds i phi^H Z_a (no other thread touches those values).  Wide
+  // blocks compute them where y is stored instead.
+  if (kPre && with_transport) {
+    for (int a0 = 0; a0 < NN; a0 += G) {
+      const int a = a0 + g;
+      if (a < NN) {
+        T2* __restrict__ yb = y + ((long)e * NN + a) * nk;
 #pragma unroll
-      for (int r = 0; r < 8; ++r) {
-        const int j = t + TT * r;
-        if (j < nk) yb[j] = transport_row<NN>(oc, kvec, Tst, u, e, a, j);
+        for (int r = 0; r < 8; ++r) {
+          const int j = t + TT * r;
+          if (j < nk) yb[j] = transport_row<NN>(oc, kvec, Tst, u, e, a, j);
+        }
       }
     }
   }
@@ -581,10 +619,16 @@ __global__ __launch_bounds__(NT) void theta_fft_kernel(const OpConst* __restrict
     const int l = threadIdx.x + k * blockDim.x;
     if (l >= nx) continue;
     T sv0[np], sv1[np];
+    // The edges are symmetric about xi = 0 and Delta V(-xi) = -Delta V(xi),
+    // so Sbar(n_xi - l) = -Sbar(l): only the half l <= n_xi/2 is read from
+    // HBM (the mirrored reads of the other half hit the same lines in L2).
+    const int hx = nx >> 1;
+    const int l0 = l <= hx ? l : nx - l, l1 = l + 1 <= hx ? l + 1 : nx - l - 1;
+    const T s0 = l <= hx ? (T)1 : (T)-1, s1 = l + 1 <= hx ? (T)1 : (T)-1;
 #pragma unroll
     for (int p = 0; p < np; ++p) {
-      sv0[p] = (T)sb[(long)p * nl + l];
-      sv1[p] = (T)sb[(long)p * nl + l + 1];
+      sv0[p] = s0 * (T)sb[(long)p * nl + l0];
+      sv1[p] = s1 * (T)sb[(long)p * nl + l1];
     }
     const T c0 = (T)cap[l], c1 = (T)cap[l + 1];
     T2 A[NN], Cs[NN];
@@ -648,7 +692,8 @@ __global__ __launch_bounds__(NT) void theta_fft_kernel(const OpConst* __restrict
     for (int r = 0; r < 8; ++r) {
       const int i = t + TT * r;
       v[r] = (act && i < nx) ? c2mul(kPre ? b_pre[r] : tb.b_pre[i], lds_ld<T2>(Tst, a * nx + i)) : zero;
-      if constexpr (kPre) yo[r] = (act && i < nk) ? yb[i] : zero;   // early, to hide its latency
+      // the transport row written above, early to hide its latency
+      if constexpr (kPre) yo[r] = (with_transport && act && i < nk) ? yb[i] : zero;
     }
     if (alias) __syncthreads();   // Z is read before the work buffers overwrite it
     fft_any<PC, false, T2>(v, buf0, buf1, cur, twl, pl, t, TT);
@@ -656,12 +701,15 @@ __global__ __launch_bounds__(NT) void theta_fft_kernel(const OpConst* __restrict
     for (int r = 0; r < 8; ++r) v[r] = c2mul(v[r], kPre ? g2[r] : pin_uniform(tb.g2)[t + TT * r]);
     fft_any<PC, true, T2>(v, buf0, buf1, cur, twl, pl, t, TT);
     const T2* app = kPre ? tb.a_post : pin_uniform(tb.a_post);
-    if constexpr (!kPre) yb = pin_lane(yb);
+    const T2* up = kPre ? u : pin_lane(u);
     if (act) {
 #pragma unroll
       for (int r = 0; r < 8; ++r) {
         const int j = t + TT * r;
-        if (j < nk) yb[j] = c2add(kPre ? yo[r] : yb[j], c2mul(kPre ? a_post[r] : app[j], v[r]));
+        if (j < nk) {
+          const T2 tr = kPre ? yo[r] : (with_transport ? transport_row_g<NN>(oc, kvec, up, e, a, j) : zero);
+          yb[j] = c2add(tr, c2mul(kPre ? a_post[r] : app[j], v[r]));
+        }
       }
     }
   }

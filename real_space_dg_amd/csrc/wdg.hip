@@ -10,9 +10,11 @@
 // Precision modes share that loop:
 //   FP64  - every cycle in FP64 (FP64 MFMA GEMMs for Theta).
 //   mixed - the residual r = b - A x and the update x += M^-1 V y are FP64;
-//           each cycle's Krylov basis, preconditioner I/O and operator are
-//           FP32 with Theta on split-BF16 MFMA (FP32 accumulation).  This is
-//           GMRES-based iterative refinement with one cycle per refinement.
+//           each cycle's Krylov basis (stored in 16 bits by default),
+//           preconditioner I/O and operator are FP32 (Theta by FP32 chirp-z
+//           transforms, or split-BF16 MFMA GEMMs in the GEMM formulation).
+//           This is GMRES-based iterative refinement with one cycle per
+//           refinement.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -42,6 +44,12 @@ thread_local std::string g_create_error;
 struct HipError : std::runtime_error {
   using std::runtime_error::runtime_error;
 };
+// A failure with its own status code (device memory exhausted, the
+// reference's InvalidInitialRho / GmresMemoryLimit / LineMemoryLimit).
+struct StatusError : std::runtime_error {
+  int code;
+  StatusError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
 
 #define HIPCHK(x)                                                                          \
   do {                                                                                     \
@@ -57,7 +65,14 @@ struct DevBuf {
   void alloc(size_t count) {
     if (count <= n && p) return;
     release();
-    HIPCHK(hipMalloc((void**)&p, std::max<size_t>(count, 1) * sizeof(T)));
+    const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+    const hipError_t e = hipMalloc((void**)&p, bytes);
+    if (e == hipErrorOutOfMemory) {
+      (void)hipGetLastError();
+      p = nullptr;
+      throw StatusError(WDG_ERR_MEMORY, "device memory exhausted (allocating " + std::to_string(bytes >> 20) + " MiB)");
+    }
+    if (e != hipSuccess) throw HipError(std::string("hipMalloc: ") + hipGetErrorString(e));
     n = count;
   }
   void release() {
@@ -116,6 +131,14 @@ struct wdg_ctx {
   size_t lds64 = 0, lds32 = 0, lds_phi = 0;
   int fft_block = 0, fft_groups = 0;  // phi_fft_kernel: threads per workgroup, transforms per workgroup
   int theta_block64 = 0, theta_block32 = 0;   // theta_fft_kernel threads per workgroup (256 or 1024)
+  int theta32_base = 0, theta32_wide = 0;     // FP32 candidates: 256-thread block, wide block (0: unusable)
+  size_t lds32_base = 0, lds32_wide = 0;
+  bool theta32_wide_auto = false;             // the occupancy calculator prefers the wide block
+  int apply_parts = WDG_PART_ALL;             // operator parts op_apply computes (wdg_apply_parts)
+  // wdg_set_memory_limits: GMRES basis estimate (16 N (m+1) bytes, 0: no
+  // limit) and line-transverse factor bytes (the reference's 4 GiB default)
+  double gmres_max_basis_bytes = 0.0;
+  double line_max_factor_bytes = 4.0 * 1024 * 1024 * 1024;
   DevBuf<double2> d_b, d_x, d_r, d_y; // FP64 solve vectors (normalised units)
   DevBuf<double2> d_u64, d_v64;       // scratch for host-buffer entry points
   DevBuf<float2> d_u32, d_v32;
@@ -189,6 +212,8 @@ int guarded(wdg_ctx* c, F&& f) {
     return f();
   } catch (const HipError& e) {
     return fail(c, WDG_ERR_HIP, e.what());
+  } catch (const StatusError& e) {
+    return fail(c, e.code, e.what());
   } catch (const std::bad_alloc&) {
     return fail(c, WDG_ERR_MEMORY, "host allocation failed");
   } catch (const std::invalid_argument& e) {
@@ -407,7 +432,8 @@ double combine_bytes(wdg_ctx* c, int s, int ts) {
 
 // y += Theta u by the fused chirp-z kernel.
 template <typename T>
-void op_theta_fft(wdg_ctx* c, const typename cvec<T>::type* u, typename cvec<T>::type* y, const GmresState* st) {
+void op_theta_fft(wdg_ctx* c, const typename cvec<T>::type* u, typename cvec<T>::type* y, const GmresState* st,
+                  int with_transport) {
   constexpr bool f64 = std::is_same<T, double>::value;
   const size_t s = sizeof(typename cvec<T>::type), ts = f64 ? 8 : 4;
   // algorithmic HBM bytes: read u, write y, read the edge table
@@ -417,13 +443,13 @@ void op_theta_fft(wdg_ctx* c, const typename cvec<T>::type* u, typename cvec<T>:
 #define WDG_THETA_P(NN, PC)                                                                                    \
   if constexpr (f64)                                                                                           \
     hipLaunchKernelGGL((theta_fft_kernel<NN, double, double, 256, PC>), grid, block, lds, c->stream, c->d_oc.p, \
-                       c->fplan, c->tt64, c->d_sbar64.p, c->d_cap.p, c->d_k.p, u, y, st);                      \
+                       c->fplan, c->tt64, c->d_sbar64.p, c->d_cap.p, c->d_k.p, u, y, st, with_transport);                      \
   else if (c->theta_block32 > kFftThreads)                                                                     \
     hipLaunchKernelGGL((theta_fft_kernel<NN, float, float, 1024, PC>), grid, block, lds, c->stream, c->d_oc.p,  \
-                       c->fplan, c->tt32, c->d_sbar32.p, c->d_cap.p, c->d_k.p, u, y, st);                      \
+                       c->fplan, c->tt32, c->d_sbar32.p, c->d_cap.p, c->d_k.p, u, y, st, with_transport);                      \
   else                                                                                                         \
     hipLaunchKernelGGL((theta_fft_kernel<NN, float, float, 256, PC>), grid, block, lds, c->stream, c->d_oc.p,   \
-                       c->fplan, c->tt32, c->d_sbar32.p, c->d_cap.p, c->d_k.p, u, y, st);
+                       c->fplan, c->tt32, c->d_sbar32.p, c->d_cap.p, c->d_k.p, u, y, st, with_transport);
 // transform lengths with a compile-time plan (n <= 5), else the run-time plan
 #define WDG_THETA(NN)                              \
   if (NN <= 5 && c->fplan.P == 2048) {             \
@@ -511,11 +537,19 @@ void op_apply_base(wdg_ctx* c, const typename cvec<T>::type* u, typename cvec<T>
     f2_apply<T>(c, u, y, st);
     return;
   }
-  if (c->use_fft) {   // the chirp-z kernel writes transport + Theta
-    op_theta_fft<T>(c, u, y, st);
+  const int parts = c->apply_parts;
+  if (parts == WDG_PART_TRANSPORT) {
+    op_transport<T>(c, u, y, st);
     return;
   }
-  op_transport<T>(c, u, y, st);
+  if (c->use_fft) {   // the chirp-z kernel writes transport + Theta
+    op_theta_fft<T>(c, u, y, st, (parts & WDG_PART_TRANSPORT) ? 1 : 0);
+    return;
+  }
+  if (parts & WDG_PART_TRANSPORT)
+    op_transport<T>(c, u, y, st);
+  else   // the potential part alone (never inside a captured cycle)
+    HIPCHK(hipMemsetAsync(y, 0, c->N * sizeof(typename cvec<T>::type), c->stream));
   const int M1 = 2 * c->g.n_xi, K1 = 2 * c->g.nk;
   const long work = (long)c->g.n_chi * c->g.n_xi;
   if constexpr (std::is_same<T, double>::value) {
@@ -536,7 +570,8 @@ void op_apply_base(wdg_ctx* c, const typename cvec<T>::type* u, typename cvec<T>
 }
 
 template <int NN, typename T>
-void op_precond_n(wdg_ctx* c, const typename cvec<T>::type* r, typename cvec<T>::type* z, const GmresState* st) {
+void op_precond_n(wdg_ctx* c, const typename cvec<T>::type* r, typename cvec<T>::type* z, const GmresState* st,
+                  double2* agg) {
   const long work = (long)c->oc.n_chunks * c->g.nk;
   if constexpr (std::is_same<T, float>::value) {
     // FP32: stream the stored block inverses
@@ -544,52 +579,56 @@ void op_precond_n(wdg_ctx* c, const typename cvec<T>::type* r, typename cvec<T>:
     {
       ProfScope ps(c, P_LINE_AGG, 8.0 * c->N + rows * (4.0 * NN + 8.0));
       hipLaunchKernelGGL((line_aggregate_inv_kernel<NN>), dim3(grid1(work, 256)), dim3(256), 0, c->stream,
-                         c->d_oc.p, c->d_k.p, c->d_linv16.p, c->d_lcpl32.p, r, c->d_agg.p, st);
+                         c->d_oc.p, c->d_k.p, c->d_linv16.p, c->d_lcpl32.p, r, agg, st);
     }
     {
       ProfScope ps(c, P_LINE_CARRY, 3.0 * c->oc.n_chunks * c->g.nk * 2 * sizeof(double2));
       hipLaunchKernelGGL(line_carry_kernel, dim3(grid1(64L * c->g.nk, 256)), dim3(256), 0, c->stream, c->d_oc.p,
-                         c->d_k.p, c->d_agg.p, st);
+                         c->d_k.p, agg, st);
     }
     {
       ProfScope ps(c, P_LINE_APPLY, 16.0 * c->N + rows * (4.0 * NN * NN + 8.0 * NN));
       hipLaunchKernelGGL((line_apply_inv_kernel<NN>), dim3(grid1(work, 256)), dim3(256), 0, c->stream, c->d_oc.p,
-                         c->d_k.p, c->d_linv16.p, c->d_lcpl32.p, c->d_agg.p, r, z, st);
+                         c->d_k.p, c->d_linv16.p, c->d_lcpl32.p, agg, r, z, st);
     }
     return;
   }
   {
     ProfScope ps(c, P_LINE_AGG, 1.0 * c->N * sizeof(typename cvec<T>::type));
     hipLaunchKernelGGL((line_aggregate_kernel<NN, T>), dim3(grid1(work, 256)), dim3(256), 0, c->stream, c->d_oc.p,
-                       c->d_k.p, c->d_s0s1.p, r, c->d_agg.p, st);
+                       c->d_k.p, c->d_s0s1.p, r, agg, st);
   }
   {
     // chunk maps read twice, carries written once
     ProfScope ps(c, P_LINE_CARRY, 3.0 * c->oc.n_chunks * c->g.nk * 2 * sizeof(double2));
     hipLaunchKernelGGL(line_carry_kernel, dim3(grid1(64L * c->g.nk, 256)), dim3(256), 0, c->stream, c->d_oc.p,
-                       c->d_k.p, c->d_agg.p, st);
+                       c->d_k.p, agg, st);
   }
   {
     ProfScope ps(c, P_LINE_APPLY, 2.0 * c->N * sizeof(typename cvec<T>::type));
     hipLaunchKernelGGL((line_apply_kernel<NN, T>), dim3(grid1(work, 256)), dim3(256), 0, c->stream, c->d_oc.p,
-                       c->d_k.p, c->d_s0s1.p, c->d_agg.p, r, z, st);
+                       c->d_k.p, c->d_s0s1.p, agg, r, z, st);
   }
 }
 
+// z = M^-1 r; agg is the chunk-map workspace of the 1D line sweeps
+// (n_chunks * nk * 2 complex; the context's own unless given).
 template <typename T>
-void op_precond(wdg_ctx* c, const typename cvec<T>::type* r, typename cvec<T>::type* z, const GmresState* st) {
+void op_precond(wdg_ctx* c, const typename cvec<T>::type* r, typename cvec<T>::type* z, const GmresState* st,
+                double2* agg = nullptr) {
   if (c->f2) {
     f2_precond<T>(c, r, z, st);
     return;
   }
+  if (!agg) agg = c->d_agg.p;
   switch (c->g.n) {
-    case 2: op_precond_n<2, T>(c, r, z, st); break;
-    case 3: op_precond_n<3, T>(c, r, z, st); break;
-    case 4: op_precond_n<4, T>(c, r, z, st); break;
-    case 5: op_precond_n<5, T>(c, r, z, st); break;
-    case 6: op_precond_n<6, T>(c, r, z, st); break;
-    case 7: op_precond_n<7, T>(c, r, z, st); break;
-    case 8: op_precond_n<8, T>(c, r, z, st); break;
+    case 2: op_precond_n<2, T>(c, r, z, st, agg); break;
+    case 3: op_precond_n<3, T>(c, r, z, st, agg); break;
+    case 4: op_precond_n<4, T>(c, r, z, st, agg); break;
+    case 5: op_precond_n<5, T>(c, r, z, st, agg); break;
+    case 6: op_precond_n<6, T>(c, r, z, st, agg); break;
+    case 7: op_precond_n<7, T>(c, r, z, st, agg); break;
+    case 8: op_precond_n<8, T>(c, r, z, st, agg); break;
     default: throw std::invalid_argument("n_k_chi outside 2..8");
   }
 }
@@ -975,7 +1014,15 @@ void setup_theta(wdg_ctx* c) {
         case 5: theta32_occupancy<5>(c->fft_block, c->lds32, wide, lw, nb_base, nb_wide); break;
         default: break;
       }
-      if (lw <= (size_t)kMaxLdsBytes && nb_wide * ((wide + 63) / 64) > nb_base * ((c->fft_block + 63) / 64)) {
+      c->theta32_base = c->fft_block;
+      c->lds32_base = c->lds32;
+      if (lw <= (size_t)kMaxLdsBytes && g.n <= 5 && wide / tt >= 4) {
+        c->theta32_wide = wide;
+        c->lds32_wide = lw;
+      }
+      c->theta32_wide_auto =
+          c->theta32_wide > 0 && nb_wide * ((wide + 63) / 64) > nb_base * ((c->fft_block + 63) / 64);
+      if (c->theta32_wide_auto) {
         c->theta_block32 = wide;
         c->lds32 = lw;
       }
@@ -1194,6 +1241,16 @@ struct SolveRun {
       throw std::invalid_argument("unknown krylov_basis");
     basis = fp64 ? WDG_BASIS_WORKING : o.krylov_basis;
     b16 = basis != WDG_BASIS_WORKING;
+    // the reference's estimate of the basis (complex double, restart + 1
+    // vectors; solve_transport_DG_full2D.m:183-203)
+    const double basis_bytes = 16.0 * (double)N * (double)(std::min<long>(m_max, N) + 1);
+    if (c->gmres_max_basis_bytes > 0 && basis_bytes > c->gmres_max_basis_bytes) {
+      char msg[256];
+      std::snprintf(msg, sizeof msg,
+                    "Estimated GMRES basis %.3g GiB exceeds full2D_gmresMaxBasisBytes. Reduce restart or explicitly "
+                    "increase the memory budget.", basis_bytes / 1073741824.0);
+      throw StatusError(WDG_ERR_GMRES_MEMORY, msg);
+    }
     ensure_krylov(c, m, fp64, b16);
     // residual history, one value per Arnoldi step (captured cycles hold its
     // address: a larger buffer drops them)
@@ -1212,7 +1269,7 @@ struct SolveRun {
     if (x0) {
       c->stage.resize(2 * N);
       for (long i = 0; i < 2 * N; ++i) {
-        if (!std::isfinite(x0[i])) throw std::invalid_argument("x0 must contain only finite values");
+        if (!std::isfinite(x0[i])) throw StatusError(WDG_ERR_INITIAL_RHO, "rho0 must contain only finite values.");
         c->stage[i] = x0[i] / xs;
       }
       HIPCHK(hipMemcpy(c->d_x.p, c->stage.data(), 2 * N * 8, hipMemcpyHostToDevice));
@@ -1503,6 +1560,19 @@ int wdg_apply(wdg_ctx* ctx, const double* u, double* y, int32_t precision) {
   });
 }
 
+int wdg_apply_parts(wdg_ctx* ctx, const double* u, double* y, int32_t precision, int32_t parts) {
+  if (!ctx) return fail(nullptr, WDG_ERR_ARG, "null argument");
+  if (parts != WDG_PART_TRANSPORT && parts != WDG_PART_POTENTIAL && parts != WDG_PART_ALL)
+    return fail(ctx, WDG_ERR_ARG, "parts must be WDG_PART_TRANSPORT, WDG_PART_POTENTIAL or WDG_PART_ALL");
+  if (ctx->f2 && parts != WDG_PART_ALL)
+    return fail(ctx, WDG_ERR_UNSUPPORTED, "separate operator parts are not available for a full-2D context");
+  if (ctx->shift_on) return fail(ctx, WDG_ERR_STATE, "a Crank-Nicolson shift is active");
+  ctx->apply_parts = parts;
+  const int rc = wdg_apply(ctx, u, y, precision);
+  ctx->apply_parts = WDG_PART_ALL;
+  return rc;
+}
+
 int wdg_apply_device(wdg_ctx* ctx, const void* d_u, void* d_y, int32_t precision) {
   if (!ctx || !d_u || !d_y) return fail(ctx, WDG_ERR_ARG, "null argument");
   if (!ctx->have_potential) return fail(ctx, WDG_ERR_STATE, "wdg_set_potential has not been called");
@@ -1610,6 +1680,17 @@ int wdg_solve(wdg_ctx* ctx, const wdg_solve_opts* opts, const double* x0, double
   });
 }
 
+int wdg_solution(wdg_ctx* ctx, double* x) {
+  if (!ctx || !x) return fail(ctx, WDG_ERR_ARG, "null argument");
+  return guarded(ctx, [&]() {
+    if (!ctx->have_solution) throw std::logic_error("no solution available: call wdg_solve first");
+    const long N = ctx->N;
+    HIPCHK(hipMemcpy(x, ctx->d_x.p, (size_t)N * 16, hipMemcpyDeviceToHost));
+    for (long i = 0; i < 2 * N; ++i) x[i] *= ctx->sol_scale;
+    return WDG_OK;
+  });
+}
+
 int wdg_solve_resvec(wdg_ctx* ctx, double* out, int32_t cap, int32_t* n_out) {
   if (!ctx || !n_out || cap < 0 || (cap > 0 && !out)) return fail(ctx, WDG_ERR_ARG, "null argument or negative cap");
   return guarded(ctx, [&]() {
@@ -1670,7 +1751,13 @@ int wdg_solve_batch(wdg_ctx* const* ctxs, int32_t n, const wdg_solve_opts* opts,
     });
     if (rc != WDG_OK) failed = i;
   }
-  if (rc != WDG_OK && failed >= 0) batch_fail(ctxs[failed], failed, rc, ctxs[failed]->err);
+  if (rc != WDG_OK) {
+    // other instances may still have restart cycles queued: let them drain
+    // before their runs (events) are released
+    for (size_t i = 0; i < runs.size(); ++i) (void)hipStreamSynchronize(ctxs[i]->stream);
+    (void)hipGetLastError();
+    if (failed >= 0) batch_fail(ctxs[failed], failed, rc, ctxs[failed]->err);
+  }
   return rc;
 }
 
@@ -1834,7 +1921,7 @@ int wdg_transient_rk4(wdg_ctx* ctx, const wdg_transient_opts* o, const double* r
     if (rho0) {
       ctx->stage.resize(2 * N);
       for (long i = 0; i < 2 * N; ++i) {
-        if (!std::isfinite(rho0[i])) throw std::invalid_argument("rho0 must contain only finite values");
+        if (!std::isfinite(rho0[i])) throw StatusError(WDG_ERR_INITIAL_RHO, "rho0 must contain only finite values.");
         ctx->stage[i] = rho0[i] / sigma;
       }
       HIPCHK(hipMemcpy(ctx->d_x.p, ctx->stage.data(), 2 * N * 8, hipMemcpyHostToDevice));
@@ -1919,7 +2006,7 @@ struct CnRun {
     if (rho0) {
       ctx->stage.resize(2 * N);
       for (long i = 0; i < 2 * N; ++i) {
-        if (!std::isfinite(rho0[i])) throw std::invalid_argument("rho0 must contain only finite values");
+        if (!std::isfinite(rho0[i])) throw StatusError(WDG_ERR_INITIAL_RHO, "rho0 must contain only finite values.");
         ctx->stage[i] = rho0[i] / sigma;
       }
       HIPCHK(hipMemcpy(ctx->d_x.p, ctx->stage.data(), 2 * N * 8, hipMemcpyHostToDevice));
@@ -1946,7 +2033,11 @@ struct CnRun {
     so.max_iter = o->max_iter > 0 ? o->max_iter : 20000;
     so.precision = o->precision;
     so.inner_tol = 1e-4;
-    so.use_graph = o->use_graph ? 1 : 0;
+    // Graph replay of FP64 restart cycles on full-2D Crank-Nicolson contexts
+    // stays off: at 33.5M unknowns such a run ended in a host-side fault
+    // inside the runtime whose cause is not identified; the cycles are
+    // launched kernel by kernel instead (same kernels, same results).
+    so.use_graph = (o->use_graph && !(ctx->f2 && o->precision == WDG_PREC_FP64)) ? 1 : 0;
     so.orthogonalization = 1;
     so.warm_device = 1;
     so.krylov_basis = o->krylov_basis;
@@ -2073,6 +2164,12 @@ int wdg_transient_cn_batch(wdg_ctx* const* ctxs, int32_t n, const wdg_cn_opts* o
     each([&](CnRun& r, int) { r.step_end(step); });
   }
   each([&](CnRun& r, int i) { r.finish(rho_out ? rho_out[i] : nullptr); });
+  if (rc != WDG_OK) {
+    // instances after a failure may still have work queued: drain every
+    // stream before the captured graphs and events are released
+    for (int i = 0; i < n; ++i) (void)hipStreamSynchronize(ctxs[i]->stream);
+    (void)hipGetLastError();
+  }
   for (auto& r : runs) {
     try {
       r->restore();
@@ -2102,6 +2199,41 @@ int wdg_set_theta_method(wdg_ctx* ctx, int32_t method) {
     ctx->use_fft = fft;
     return WDG_OK;
   });
+}
+
+int wdg_set_memory_limits(wdg_ctx* ctx, double gmres_max_basis_bytes, double line_max_factor_bytes) {
+  if (!ctx) return fail(nullptr, WDG_ERR_ARG, "null argument");
+  if (std::isnan(gmres_max_basis_bytes) || gmres_max_basis_bytes < 0)
+    return fail(ctx, WDG_ERR_ARG, "gmres_max_basis_bytes must be >= 0 (0: no limit)");
+  if (!(line_max_factor_bytes > 0) || !std::isfinite(line_max_factor_bytes))
+    return fail(ctx, WDG_ERR_LINE_MEMORY, "lineMaxFactorBytes must be positive and finite.");
+  ctx->gmres_max_basis_bytes = std::isinf(gmres_max_basis_bytes) ? 0.0 : gmres_max_basis_bytes;
+  ctx->line_max_factor_bytes = line_max_factor_bytes;
+  return WDG_OK;
+}
+
+int wdg_set_fft_block(wdg_ctx* ctx, int32_t mode) {
+  if (!ctx) return fail(nullptr, WDG_ERR_ARG, "null argument");
+  if (ctx->f2) return fail(ctx, WDG_ERR_UNSUPPORTED, "not available for a full-2D context");
+  return guarded(ctx, [&]() {
+    if (mode != WDG_FFT_BLOCK_AUTO && mode != WDG_FFT_BLOCK_256 && mode != WDG_FFT_BLOCK_WIDE)
+      throw std::invalid_argument("unknown FFT block mode");
+    if (!ctx->fft_ok) throw std::invalid_argument("the chirp-z FFT formulation does not apply to this grid");
+    if (mode == WDG_FFT_BLOCK_WIDE && ctx->theta32_wide == 0)
+      throw std::invalid_argument("the wide FP32 Theta block does not fit this grid (n_k_chi > 5, fewer than "
+                                  "four transforms per round, or LDS)");
+    const bool wide = mode == WDG_FFT_BLOCK_WIDE || (mode == WDG_FFT_BLOCK_AUTO && ctx->theta32_wide_auto);
+    const int blk = wide ? ctx->theta32_wide : ctx->theta32_base;
+    if (blk != ctx->theta_block32) clear_graphs(ctx);   // captured cycles hold the launch shape
+    ctx->theta_block32 = blk;
+    ctx->lds32 = wide ? ctx->lds32_wide : ctx->lds32_base;
+    return WDG_OK;
+  });
+}
+
+int wdg_fft_block(const wdg_ctx* ctx) {
+  if (!ctx || !ctx->fft_ok) return -1;
+  return ctx->theta_block32;
 }
 
 int wdg_theta_method(const wdg_ctx* ctx) {
@@ -2229,6 +2361,18 @@ int wdg2_set_line_precond(wdg_ctx* ctx, const wdg2_line_factors* lf) {
   if (!ctx->have_potential) return fail(ctx, WDG_ERR_STATE, "wdg2_set_potential has not been called");
   return guarded(ctx, [&]() {
     f2_set_line_impl(ctx, *lf);
+    return WDG_OK;
+  });
+}
+
+int wdg2_set_dv_mode(wdg_ctx* ctx, int32_t mode) {
+  if (!ctx) return fail(nullptr, WDG_ERR_ARG, "null argument");
+  if (!ctx->f2) return fail(ctx, WDG_ERR_UNSUPPORTED, "not a full-2D context");
+  if (mode != WDG2_DV_INTERP && mode != WDG2_DV_TABLE) return fail(ctx, WDG_ERR_ARG, "unknown Delta V mode");
+  return guarded(ctx, [&]() {
+    if (ctx->f2->dv_mode == mode) return WDG_OK;
+    ctx->f2->dv_mode = mode;
+    if (ctx->have_potential) f2_configure_dv(ctx);   // else at wdg2_set_potential
     return WDG_OK;
   });
 }

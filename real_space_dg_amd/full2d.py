@@ -32,7 +32,7 @@ from numpy.polynomial import legendre as npleg
 from . import _lib
 from .contact_modes import contact_mode_reservoir
 from ._lib import check, dptr
-from .solver import SolveOptions
+from .solver import SolveOptions, _as_c128
 
 HBAR = 1.054571817e-34      # get_Diff_full2D.m:810-813 physicalConstants
 M0 = 9.1093837015e-31
@@ -790,7 +790,7 @@ class Full2DOperator:
         self.rhs_norm = float(np.linalg.norm(r))
 
     def apply(self, u, precision: str = "fp64") -> np.ndarray:
-        u = _c128(u).reshape(-1)
+        u = _as_c128(u, self.n)
         y = np.empty_like(u)
         prec = {"fp64": _lib.WDG_PREC_FP64, "mixed": _lib.WDG_PREC_MIXED}[precision]
         check(self._lib.wdg_apply(self.ctx, dptr(u), dptr(y), prec), self.ctx)
@@ -800,6 +800,14 @@ class Full2DOperator:
         d = np.empty(self.n, np.complex128)
         check(self._lib.wdg_diagonal(self.ctx, dptr(d)), self.ctx)
         return d
+
+    def set_dv_mode(self, mode: str):
+        """'interp' (Delta V interpolated in the apply, the default) or 'table'
+        (Delta V streamed from a stored table): wdg2_set_dv_mode."""
+        modes = {"interp": _lib.WDG2_DV_INTERP, "table": _lib.WDG2_DV_TABLE}
+        if mode not in modes:
+            raise ValueError(f"mode must be one of {sorted(modes)}")
+        check(self._lib.wdg2_set_dv_mode(self.ctx, modes[mode]), self.ctx)
 
     def row_abs_sum(self) -> np.ndarray:
         d = np.empty(self.n, np.float64)
@@ -851,7 +859,7 @@ class Full2DOperator:
         return read_profile(self._lib, self.ctx)
 
     def precond(self, r) -> np.ndarray:
-        r = _c128(r).reshape(-1)
+        r = _as_c128(r, self.n)
         z = np.empty_like(r)
         check(self._lib.wdg_precond(self.ctx, dptr(r), dptr(z)), self.ctx)
         return z
@@ -862,7 +870,7 @@ class Full2DOperator:
         x = np.empty(self.n, np.complex128) if return_x else None
         x0p = None
         if x0 is not None:
-            x0 = _c128(x0).reshape(-1)
+            x0 = validate_initial_rho(x0, self.n)
             x0p = dptr(x0)
         check(self._lib.wdg_solve(self.ctx, C.byref(opts.to_c()), x0p, dptr(x) if return_x else None,
                                   C.byref(info)), self.ctx)
@@ -877,6 +885,7 @@ class Full2DOperator:
     def observables(self, rho=None):
         """nDG, jxDG, jyDG on the (X dof, Y dof) grid (get_Observables_full2D.m:24-60);
         rho=None uses the device solution of the last solve."""
+        rho = None if rho is None else _as_c128(rho, self.n)
         s = self.setup
         wn, wx, wy = s.observable_weights()
         idx = np.zeros(3 * 8, np.int32)
@@ -892,7 +901,6 @@ class Full2DOperator:
         out = np.empty(3 * s.ncen, np.complex128)
         rp = None
         if rho is not None:
-            rho = _c128(rho).reshape(-1)
             rp = dptr(rho)
         check(self._lib.wdg2_observables(self.ctx, rp, idx.ctypes.data_as(C.POINTER(C.c_int32)), dptr(w),
                                          cnt.ctypes.data_as(C.POINTER(C.c_int32)), dptr(out)), self.ctx)
@@ -958,9 +966,9 @@ def line_transverse_factors(op: "Full2DOperator") -> dict:
     factor_bytes = 32 * nT ** 2 * nrx + 32 * nl ** 2 * nrx * ne + 16 * nrx ** 2 + 8 * nT * nrx
     max_bytes = read_param(params, "full2D_lineMaxFactorBytes", 4 * 1024 ** 3)
     if not (np.isscalar(max_bytes) and np.isfinite(max_bytes) and max_bytes > 0):
-        raise ValueError("DG:Full2D:LineMemoryLimit: lineMaxFactorBytes must be positive and finite")
+        raise _lib.WdgError(_lib.WDG_ERR_LINE_MEMORY, "lineMaxFactorBytes must be positive and finite.")
     if factor_bytes > max_bytes:
-        raise MemoryError(f"DG:Full2D:LineMemoryLimit: estimated factors {factor_bytes / 2**30:.3g} GiB exceed "
+        raise _lib.WdgError(_lib.WDG_ERR_LINE_MEMORY, f"Estimated factors {factor_bytes / 2**30:.3g} GiB exceed "
                           f"full2D_lineMaxFactorBytes {max_bytes / 2**30:.3g} GiB")
     ix_ref = read_param(params, "full2D_lineReferenceXIndex", 1)
     if not (float(ix_ref).is_integer() and 1 <= ix_ref <= int(s.mat["Nx"])):
@@ -1042,6 +1050,38 @@ class Full2DResult:
     setup: "Full2DSetup | None" = None    # the host setup of the solve (grids, DG axes, reservoirs)
 
 
+def validate_initial_rho(rho0, n_total: int) -> np.ndarray:
+    """rho0 as a flat complex vector of n_total finite entries, or
+    DG:Full2D:InvalidInitialRho (validateInitialRho,
+    DG_code/solve_transport_DG_full2D.m:508-525: a full vector with nTotal
+    entries in rho_x, rho_y, X, Y order, never a reshaped 4-D array; finite)."""
+    a = np.asarray(rho0)
+    if a.dtype.kind not in "fc" or (a.ndim > 1 and sum(d > 1 for d in a.shape) > 1) or a.ndim > 2 \
+            or a.size != n_total:
+        raise _lib.WdgError(_lib.WDG_ERR_INITIAL_RHO, f"rho0 must be a full single/double vector with {n_total} "
+                                                      "entries in rho_x,rho_y,X,Y order.")
+    a = np.ascontiguousarray(a.reshape(-1), dtype=np.complex128)
+    if not np.all(np.isfinite(a)):
+        raise _lib.WdgError(_lib.WDG_ERR_INITIAL_RHO, "rho0 must contain only finite values.")
+    return a
+
+
+def gmres_basis_guard(params: dict, n_total: int, restart: int, pmode: str):
+    """DG:Full2D:GmresMemoryLimit of solve_transport_DG_full2D.m:183-203: with
+    the line-transverse preconditioner the estimated basis 16 nTotal
+    (min(restart, nTotal) + 1) bytes may not exceed full2D_gmresMaxBasisBytes
+    (9 GiB on the GPU).  Returns (estimate, limit)."""
+    est = 16 * n_total * (min(restart, n_total) + 1)
+    limit = float("inf")
+    if pmode == "line-transverse":
+        limit = float(read_param(params, "full2D_gmresMaxBasisBytes", 9 * 1024 ** 3))
+        if est > limit:
+            raise _lib.WdgError(_lib.WDG_ERR_GMRES_MEMORY,
+                                f"Estimated GMRES basis {est / 1024 ** 3:.3g} GiB exceeds full2D_gmresMaxBasisBytes. "
+                                "Reduce restart or explicitly increase the memory budget.")
+    return est, limit
+
+
 def solve_transport_DG_full2D(mat: dict, Vxy=None, EfL=None, EfR=None, rho0=None,
                               solver: SolveOptions | None = None) -> Full2DResult:
     """Stationary full-2D solve with the reference's signature and outputs
@@ -1062,7 +1102,15 @@ def solve_transport_DG_full2D(mat: dict, Vxy=None, EfL=None, EfR=None, rho0=None
         restart = int(read_param(params, "full2D_gmresRestart", read_param(params, "full2D_gpuGmresRestart", 20)))
         solver = SolveOptions(tol=float(read_param(params, "full2D_solverTol", 1e-4)), restart=restart,
                               max_iter=int(read_param(params, "full2D_solverMaxIt", 1400)) * restart)
+    # both checks come before the device operator is built, as in the reference
+    if rho0 is not None:
+        rho0 = validate_initial_rho(rho0, setup.n_total)
+    _, basis_limit = gmres_basis_guard(params, setup.n_total, max(solver.restart, solver.restart_max), pmode)
     with Full2DOperator(setup) as op:
+        if np.isfinite(basis_limit):
+            check(op._lib.wdg_set_memory_limits(op.ctx, basis_limit,
+                                                float(read_param(params, "full2D_lineMaxFactorBytes", 4 * 1024 ** 3))),
+                  op.ctx)
         if pmode == "line-transverse":
             op.set_line_transverse()
             used = pmode

@@ -30,6 +30,13 @@ FERMI_MODELS = {"DG_MSA_3D": _lib.WDG_FERMI_MSA_3D, "DG_MSA_2D": _lib.WDG_FERMI_
                 "TSU_ESAKI": _lib.WDG_FERMI_TSU_ESAKI}
 PRECISIONS = {"fp64": _lib.WDG_PREC_FP64, "mixed": _lib.WDG_PREC_MIXED}
 
+# Stopping tolerance (true relative residual) of the RTD benchmark solves.
+# It is below the 1e-10 residual target so that density and current land
+# within 1e-8 of the FP64 oracle's converged solution: on the 512 x 4 x 1024
+# RTD at 0.2 V the observables move 250-500x the residual (a mixed solve
+# stopped at 1.9e-11 is 4.5e-9 / 9.2e-9 off in n / j).
+RTD_BENCH_TOL = 1e-11
+
 
 @dataclass
 class SolveOptions:
@@ -140,10 +147,18 @@ class WignerOperator:
 
     assembleRhs = rhs
 
-    def apply(self, u, precision: str = "fp64") -> np.ndarray:
+    PARTS = {"all": _lib.WDG_PART_ALL, "transport": _lib.WDG_PART_TRANSPORT, "potential": _lib.WDG_PART_POTENTIAL}
+
+    def apply(self, u, precision: str = "fp64", parts: str = "all") -> np.ndarray:
+        """y = A u; parts 'transport' (diff.apply) or 'potential' (drift.apply:
+        Theta[V] with the CAP) gives one of the two handles sysInfo.apply adds."""
         u = _as_c128(u, self.N)
         y = np.empty(self.N, np.complex128)
-        check(self._lib.wdg_apply(self._h, dptr(u), dptr(y), PRECISIONS[precision]), self._h)
+        if parts == "all":
+            check(self._lib.wdg_apply(self._h, dptr(u), dptr(y), PRECISIONS[precision]), self._h)
+        else:
+            check(self._lib.wdg_apply_parts(self._h, dptr(u), dptr(y), PRECISIONS[precision], self.PARTS[parts]),
+                  self._h)
         return y
 
     def precond(self, r) -> np.ndarray:
@@ -171,6 +186,12 @@ class WignerOperator:
         d["precision"] = opts.precision
         d["resvec"] = read_resvec(self._lib, self._h)   # relative to ||rhs||
         return x, d
+
+    def solution(self) -> np.ndarray:
+        """The last solution left on the device (wdg_solution), physical units."""
+        x = np.empty(self.N, np.complex128)
+        check(self._lib.wdg_solution(self._h, dptr(x)), self._h)
+        return x
 
     def observables(self, m_nodes, rho=None, want_rho_u: bool = False):
         """n, j per node (and rho_u, n_xi x nodes) of rho or of the last solution."""
@@ -245,6 +266,19 @@ class WignerOperator:
     @property
     def theta_method(self) -> str:
         return {_lib.WDG_THETA_FFT: "fft", _lib.WDG_THETA_GEMM: "gemm"}[self._lib.wdg_theta_method(self._h)]
+
+    FFT_BLOCKS = {"auto": _lib.WDG_FFT_BLOCK_AUTO, "256": _lib.WDG_FFT_BLOCK_256, "wide": _lib.WDG_FFT_BLOCK_WIDE}
+
+    def set_fft_block(self, mode: str):
+        """Workgroup shape of the FP32 chirp-z kernel: 'auto', '256' or 'wide'."""
+        if mode not in self.FFT_BLOCKS:
+            raise ValueError(f"mode must be one of {sorted(self.FFT_BLOCKS)}")
+        check(self._lib.wdg_set_fft_block(self._h, self.FFT_BLOCKS[mode]), self._h)
+
+    @property
+    def fft_block(self) -> int:
+        """Threads per workgroup of the FP32 chirp-z kernel (-1: no FFT plan)."""
+        return self._lib.wdg_fft_block(self._h)
 
     # ---- profiling ----
     def set_profiling(self, mask: int):

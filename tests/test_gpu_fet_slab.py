@@ -1,0 +1,100 @@
+"""The full-2D FET-slab transient (Crank-Nicolson, dt = 1 fs) on a reduced
+grid against the oracle: 8 x 4 DG elements p = 1, 8 x 8 relative cells
+(N = 8,192; the benchmark runs 64 x 32 x 64 x 64 = 33.5 M), Gaussian gate
+barrier 0.2 eV with sigma 5 nm, rusanov fluxes, row-sum preconditioner.
+
+The oracle applies the full-2D operator (oracle/full2d.py: transport parts
+of DG_code/core/get_Diff_full2D.m and the FV-consistent drift of
+get_Drift_full2D.m) and steps it with a direct sparse LU per step
+(oracle.transient.crank_nicolson).  Single instance in both precisions, and
+the eight transverse-mode instances (gate potential + 5 meV per mode) of one
+batched launch (wdg_transient_cn_batch).
+"""
+import numpy as np
+import pytest
+
+from oracle import full2d as F
+from oracle import transient as TR
+from real_space_dg_amd import full2d as P
+from real_space_dg_amd.devices import fet_slab
+
+pytestmark = pytest.mark.gpu
+
+DT = 1e-15
+EFL, EFR = 0.1, 0.05
+
+
+def _system(m, V):
+    """The oracle's matrix-free full-2D operator: apply, row-abs sums, rhs."""
+    p = F.init_params_full2d(m, V, EFL, EFR)
+    bnd = F.boundary_full2d(m, p, EFL, EFR, V)
+    parts, rparts, _ = F.get_diff_full2d(m, p, bnd)
+    drift = F.DriftFull2D(m, p, V)
+    apply_a = lambda u: F.apply_parts(parts, u, p) + drift.apply(u)  # noqa: E731
+    rowabs = F.kron_row_abs_sum(parts, p) + drift.row_abs()
+    return p, apply_a, rowabs, F.assemble_rhs(rparts, p)
+
+
+def _cn(system, nsteps):
+    p, apply_a, rowabs, rhs = system
+    return TR.crank_nicolson_mf(apply_a, rhs, np.zeros(p["nTotal"], complex), DT, nsteps, tol=1e-13,
+                                row_scale=rowabs)
+
+
+def rel(a, b):
+    return np.max(np.abs(a - b)) / np.max(np.abs(b))
+
+
+@pytest.fixture(scope="module")
+def slab():
+    m, V = fet_slab(n_el_x=8, n_el_y=4, n_rho=8)
+    print(f"\n[case] FET slab {m['Nx']}x{m['Ny']} material points, N_rho 8x8", flush=True)
+    return m, V, _system(m, V)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_fet_slab_crank_nicolson_matches_oracle(slab, precision):
+    m, V, system = slab
+    p = system[0]
+    n = p["nTotal"]
+    ref = _cn(system, 3)
+    with P.Full2DOperator(P.Full2DSetup(m, V, EFL, EFR)) as op:
+        assert op.n == n
+        op.upload_rhs()
+        assert op.set_preconditioner("rowabs") == "rowabs"
+        tol = 1e-12 if precision == "fp64" else 1e-10
+        rho, info = op.transient_cn(DT, 3, tol=tol, restart=30, precision=precision, rho0=np.zeros(n, complex))
+        print(f"FET slab CN {precision}: {info}", flush=True)
+        assert info["flag"] == 0 and info["steps"] == 3
+        assert rel(rho, ref) < 1e-8, rel(rho, ref)
+        nDG, jx, _ = op.observables()
+        n_ref, jx_ref, _ = F.observables_full2d(ref, p, m)
+        assert rel(nDG, n_ref) < 1e-8
+        assert rel(jx, jx_ref) < 1e-7
+
+
+def test_fet_slab_eight_mode_batch_matches_oracle(slab):
+    from real_space_dg_amd.solver import transient_cn_batch
+    m, V, system0 = slab
+    p = system0[0]
+    n = p["nTotal"]
+    ops, refs = [], []
+    try:
+        for k in range(8):
+            Vk = V + 0.005 * k
+            refs.append(_cn(system0 if k == 0 else _system(m, Vk), 2))
+            op = P.Full2DOperator(P.Full2DSetup(m, Vk, EFL, EFR))
+            op.upload_rhs()
+            op.set_preconditioner("rowabs")
+            ops.append(op)
+        outs, infos = transient_cn_batch(ops, DT, 2, tol=1e-10, restart=30, precision="mixed",
+                                         rho0=[np.zeros(n, complex)] * 8)
+        print(f"FET slab 8-mode batch: {[i['iterations'] for i in infos]}", flush=True)
+        for rho, info, ref in zip(outs, infos, refs):
+            assert info["flag"] == 0 and info["steps"] == 2
+            assert rel(rho, ref) < 1e-8, rel(rho, ref)
+        # the modes differ (the potential offset moves the state)
+        assert rel(outs[7], outs[0]) > 1e-6
+    finally:
+        for op in ops:
+            op.close()

@@ -48,16 +48,20 @@ def rel(a, b):
     return np.max(np.abs(a - b)) / np.max(np.abs(b))
 
 
+@pytest.mark.parametrize("dv", ["interp", "table"])
 @pytest.mark.parametrize("name", list(ALL))
-def test_apply_diagonal_rowabs(name):
+def test_apply_diagonal_rowabs(name, dv):
+    """Both sources of Delta V (interpolated in the apply, or streamed from
+    the stored table; wdg2_set_dv_mode) against the oracle."""
     m, EfL, EfR = ALL[name]
-    print(f"full2d apply {name}", flush=True)
+    print(f"full2d apply {name} dv={dv}", flush=True)
     p, parts, rparts, drift = oracle_parts(m, EfL, EfR)
     n = p["nTotal"]
     rng = np.random.default_rng(0)
     u = rng.standard_normal(n) + 1j * rng.standard_normal(n)
     y_or = F.apply_parts(parts, u, p) + drift.apply(u)
     with P.Full2DOperator(P.Full2DSetup(m, m["V"], EfL, EfR)) as op:
+        op.set_dv_mode(dv)
         y = op.apply(u)
         assert rel(y, y_or) < 1e-12, rel(y, y_or)
         ym = op.apply(u, precision="mixed")

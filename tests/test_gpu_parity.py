@@ -6,14 +6,18 @@ from tests._cases import build_case, gpu_operator, line_operator, oracle_system,
 
 pytestmark = pytest.mark.gpu
 
-CASES = [(16, 2, 64, 0.0), (8, 3, 64, 0.2), (12, 1, 96, 0.1), (6, 5, 64, 0.1)]
+# (elements, degree, nk, bias, inflow model).  The negative bias makes the
+# potential rise towards x = L, so min(EM) != EM(end) and the x >= L clamp of
+# V_transient (to min(EM)) is exercised; DG_MSA_2D is the k_z-integrated inflow.
+CASES = [(16, 2, 64, 0.0, "DG_MSA_3D"), (8, 3, 64, 0.2, "DG_MSA_3D"), (12, 1, 96, 0.1, "DG_MSA_3D"),
+         (6, 5, 64, 0.1, "DG_MSA_3D"), (8, 2, 64, -0.15, "DG_MSA_3D"), (10, 3, 64, 0.1, "DG_MSA_2D")]
 
 
-@pytest.fixture(scope="module", params=CASES, ids=lambda c: "ne%d_p%d_nk%d_b%.1f" % c)
+@pytest.fixture(scope="module", params=CASES, ids=lambda c: "ne%d_p%d_nk%d_b%.2f_%s" % c)
 def case(request):
-    ne, deg, nk, bias = request.param
-    print(f"\n[case] ne={ne} degree={deg} nk={nk} bias={bias}", flush=True)
-    c = build_case(ne, deg, nk, bias)
+    ne, deg, nk, bias, fermi = request.param
+    print(f"\n[case] ne={ne} degree={deg} nk={nk} bias={bias} inflow={fermi}", flush=True)
+    c = build_case(ne, deg, nk, bias, fermi_model=fermi)
     c["A"], c["G"], c["rhs"] = oracle_system(c)
     c["op"] = gpu_operator(c)
     yield c
@@ -49,10 +53,13 @@ def test_apply_theta_part(case, method):
     case["op"].set_theta_method(method)
     u = rand_c(case["A"].shape[0], 2)
     y_ref = case["G"] @ u
-    y = case["op"].apply(u, "fp64") - case["A"] @ u
+    y = case["op"].apply(u, "fp64", parts="potential")
     err = np.linalg.norm(y - y_ref) / np.linalg.norm(y_ref)
     print(method, "theta fp64 rel err", err)
-    assert err < 1e-12
+    assert err < 1e-13
+    yt = case["op"].apply(u, "fp64", parts="transport")
+    y_ref = case["A"] @ u
+    assert np.linalg.norm(yt - y_ref) <= 1e-13 * np.linalg.norm(y_ref)
 
 
 @pytest.mark.parametrize("method", METHODS)
@@ -62,7 +69,8 @@ def test_apply_mixed(case, method):
     y_ref = (case["A"] + case["G"]) @ u
     y = case["op"].apply(u, "mixed")
     err = np.linalg.norm(y - y_ref) / np.linalg.norm(y_ref)
-    th = np.linalg.norm((y - case["A"] @ u) - case["G"] @ u) / np.linalg.norm(case["G"] @ u)
+    th = np.linalg.norm(case["op"].apply(u, "mixed", parts="potential") - case["G"] @ u) / \
+        np.linalg.norm(case["G"] @ u)
     print(method, "apply mixed rel err", err, "theta part", th)
     if method == "gemm":
         # BF16 rounding of phi (unit roundoff 2^-9) in two GEMMs bounds the

@@ -73,3 +73,16 @@ def test_crank_nicolson_second_order_and_amplification_factor():
     z = 0.7 * d
     r = TR.crank_nicolson(sp.diags(d), np.zeros(4), np.ones(4), 0.7, 3)
     assert np.allclose(r, ((1 - z / 2) / (1 + z / 2)) ** 3, rtol=1e-14, atol=0)
+
+
+def test_matrix_free_crank_nicolson_equals_direct():
+    c, K, rhs = _small()
+    a = 2e-10
+    Kd = K * a
+    lam = abs(Kd).sum(axis=1).max()
+    rng = np.random.default_rng(2)
+    rho0 = rng.standard_normal(K.shape[0]) + 1j * rng.standard_normal(K.shape[0])
+    dt = 5.0 / lam
+    ref = TR.crank_nicolson(K, rhs, rho0, dt, 3, a_scale=a)
+    mf = TR.crank_nicolson_mf(lambda v: K @ v, rhs, rho0, dt, 3, a_scale=a, row_scale=abs(K).sum(axis=1).A1)
+    assert np.linalg.norm(mf - ref) <= 1e-11 * np.linalg.norm(ref)
