@@ -242,6 +242,17 @@ int wdg_poisson_step(wdg_ctx* ctx, const double* phi_in, double bias, double* ph
 int wdg_apply_device(wdg_ctx* ctx, const void* d_u, void* d_y, int32_t precision);
 int wdg_precond_device(wdg_ctx* ctx, const void* d_r, void* d_z, int32_t precision);
 
+/* wdg_precond_device that allocates nothing: for callers that cannot call
+ * hipMalloc (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid allocation).  No counterpart in the reference.  1D contexts
+ * only.  The chunk-map workspace of the line sweeps (n_chunks * nk * 2
+ * complex doubles) lives in a file-scope device array of 8 MiB that each
+ * GPU the library runs on holds until the process ends; grids that need
+ * more return WDG_ERR_ARG.  All calls on one GPU share that array: calls
+ * that overlap on one GPU (several streams or threads) race.  Results equal
+ * wdg_precond_device's bit for bit. */
+int wdg_precond_device_noalloc(wdg_ctx* ctx, const void* d_r, void* d_z, int32_t precision);
+
 /* ---- host-only helpers (no GPU needed) ---- */
 /* The host-side setup of wdg_create + wdg_set_potential without a device:
  * rhs (2N doubles) and the per-element transport matrices K1,K2L,K2R,K3,K5

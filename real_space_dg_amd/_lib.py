@@ -183,6 +183,7 @@ SIGNATURES = {
     "wdg_poisson_step": (C.c_int, [_P, _D, C.c_double, _D, _D, C.POINTER(wdg_poisson_info)]),
     "wdg_apply_device": (C.c_int, [_P, _P, _P, C.c_int32]),
     "wdg_precond_device": (C.c_int, [_P, _P, _P, C.c_int32]),
+    "wdg_precond_device_noalloc": (C.c_int, [_P, _P, _P, C.c_int32]),
     "wdg_host_setup": (C.c_int, [C.POINTER(wdg_desc), C.POINTER(wdg_potential), _D, _D]),
     "wdg_transient_rk4": (C.c_int, [_P, C.POINTER(wdg_transient_opts), _D, _D, _D, _D, _D, C.c_int32]),
     "wdg_transient_cn": (C.c_int, [_P, C.POINTER(wdg_cn_opts), _D, _D, C.POINTER(wdg_cn_info)]),

@@ -58,6 +58,32 @@ struct StatusError : std::runtime_error {
       throw HipError(std::string(#x) + ": " + hipGetErrorString(e_));                      \
   } while (0)
 
+// Workspace of wdg_precond_device_noalloc: the chunk maps of the 1D line
+// sweeps (n_chunks * nk * 2 complex doubles), in a file-scope device array so
+// that the entry point allocates nothing.  8 MiB, held by every GPU the
+// library runs on until the process ends.  Host code never names the array
+// itself (its host-side symbol is not a device address): it uses the address
+// hipGetSymbolAddress returns for the current device, kept per device.
+constexpr size_t kNoAllocScratch = (size_t)8 << 20;   // bytes
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) double2 scratch[kNoAllocScratch / sizeof(double2)];
+std::mutex g_scratch_mu;
+double2* g_scratch_ptr[kMaxDevices] = {};
+
+// The current device's address of scratch, resolved once per device.
+double2* scratch_for_current_device() {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) throw HipError("device index beyond the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[dev] = static_cast<double2*>(p);
+  }
+  return g_scratch_ptr[dev];
+}
+
 template <typename T>
 struct DevBuf {
   T* p = nullptr;
@@ -135,6 +161,7 @@ struct wdg_ctx {
   size_t lds32_base = 0, lds32_wide = 0;
   bool theta32_wide_auto = false;             // the occupancy calculator prefers the wide block
   int apply_parts = WDG_PART_ALL;             // operator parts op_apply computes (wdg_apply_parts)
+  double2* scratch_dev = nullptr;             // this device's scratch array (wdg_precond_device_noalloc)
   // wdg_set_memory_limits: GMRES basis estimate (16 N (m+1) bytes, 0: no
   // limit) and line-transverse factor bytes (the reference's 4 GiB default)
   double gmres_max_basis_bytes = 0.0;
@@ -1477,6 +1504,16 @@ int wdg_create(const wdg_desc* desc, wdg_ctx** out) {
   auto c = std::make_unique<wdg_ctx>();
   int rc = guarded(nullptr, [&]() {
     create_impl(c.get(), *desc);
+    // set-up of the allocation-free preconditioner entry point: this
+    // device's address of the scratch array (resolved once per device).  A
+    // failure here disables only that entry point (it then reports it).
+    try {
+      c->scratch_dev = scratch_for_current_device();
+    } catch (const HipError& e) {
+      c->scratch_dev = nullptr;
+      c->err = e.what();
+      (void)hipGetLastError();
+    }
     return WDG_OK;
   });
   if (rc != WDG_OK) {
@@ -1624,6 +1661,42 @@ int wdg_precond_device(wdg_ctx* ctx, const void* d_r, void* d_z, int32_t precisi
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(ctx->stream));
     prof_collect(ctx);
+    return WDG_OK;
+  });
+}
+
+int wdg_precond_device_noalloc(wdg_ctx* ctx, const void* d_r, void* d_z, int32_t precision) {
+  if (!ctx || !d_r || !d_z) return fail(ctx, WDG_ERR_ARG, "null argument");
+  if (ctx->f2) return fail(ctx, WDG_ERR_UNSUPPORTED, "not available for a full-2D context (1D line preconditioner only)");
+  if (!ctx->have_potential) return fail(ctx, WDG_ERR_STATE, "wdg_set_potential has not been called");
+  if (precision != WDG_PREC_FP64 && precision != WDG_PREC_MIXED) return fail(ctx, WDG_ERR_ARG, "unknown precision");
+  const size_t need = (size_t)ctx->oc.n_chunks * ctx->g.nk * 2 * sizeof(double2);
+  if (need > kNoAllocScratch) {
+    char msg[200];
+    std::snprintf(msg, sizeof msg, "the line sweeps of this grid need %zu bytes of workspace, above the %zu bytes of "
+                  "the allocation-free entry point (use wdg_precond_device)", need, kNoAllocScratch);
+    return fail(ctx, WDG_ERR_ARG, msg);
+  }
+  if (!ctx->scratch_dev)
+    return fail(ctx, WDG_ERR_HIP, "hipGetSymbolAddress of the scratch array failed on this device: " + ctx->err);
+  return guarded(ctx, [&]() {
+    // no per-kernel event timing here (it would create events)
+    const unsigned mask = ctx->prof_mask;
+    ctx->prof_mask = 0;
+    try {
+      // every chunk map the sweeps read is written earlier in the same call,
+      // so the array needs no clearing
+      if (precision == WDG_PREC_FP64)
+        op_precond<double>(ctx, (const double2*)d_r, (double2*)d_z, nullptr, ctx->scratch_dev);
+      else
+        op_precond<float>(ctx, (const float2*)d_r, (float2*)d_z, nullptr, ctx->scratch_dev);
+    } catch (...) {
+      ctx->prof_mask = mask;
+      throw;
+    }
+    ctx->prof_mask = mask;
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(ctx->stream));
     return WDG_OK;
   });
 }
