@@ -489,6 +489,56 @@ int wdg2_set_dv_mode(wdg_ctx* ctx, int32_t mode);
  * (solve_transport_DG_full2D.m:372). */
 int wdg2_set_rhs(wdg_ctx* ctx, const double* rhs);
 
+/* Contact-mode reservoirs and the inflow right-hand side built on the
+ * device from host-solved transverse contact modes (the per-bias rebuild of
+ * get_ContactModeReservoirRho_full2D.m:35-137 with the longitudinal
+ * occupation of :406-424, and the X-face inflow parts of
+ * get_Diff_full2D.m:393-452 / assembleRhs :609-616).  Per contact and mode m
+ *   f_m(kx) = 2 deg/pi trapz_kz 1/(1 + e^{q(E_m + hb^2 kx^2/2m_x + hb^2 kz^2/2m_z - E_F)/kT})
+ *             (n_kz = 0: 2 deg/(1 + e^{...}); exponent clipped to +-80),
+ *   F_m(rho_x) = sum_kx cos(rho_x kx) f_m(kx) dkx/(2 pi),
+ *   rho(rho_y, rho_x; Y) = sum_m phi_m(Y + rho_y/2) conj(phi_m(Y - rho_y/2)) F_m(rho_x)
+ * on the contact rows (mask), and the rhs of the first (source: +A+) and last
+ * (drain: -A-) X element, scale_x * lift_a * (A rho), replacing the one
+ * wdg2_set_rhs would upload.  The modes, their masses and the Fermi levels
+ * (neutrality) stay host work: small dense eigenproblems and a scalar
+ * bisection. */
+enum { WDG2_MODES_BOX = 0, WDG2_MODES_FOURIER = 1 };
+typedef struct wdg2_contact_modes {
+  int32_t n_modes;            /* 0..128; 0: no reservoir on this contact          */
+  int32_t kind;               /* BOX: phi by linear interpolation on grid, zero
+                                 outside; FOURIER: sum_i c_i e^{i k_i (y-y0)}/sqrt(width) */
+  int32_t n_grid;             /* grid points (BOX) or Fourier terms (FOURIER)     */
+  const double* grid;         /* BOX: ascending Y grid [m] including one zero ghost
+                                 node beyond each edge; FOURIER: k_i [1/m]          */
+  const double* values;       /* BOX: n_modes x n_grid reals, mode-major;
+                                 FOURIER: n_modes x n_grid complex (interleaved)    */
+  double y0, width;           /* FOURIER: phase origin and periodic cell width [m] */
+  const double* energy;       /* n_modes mode energies [eV]                       */
+  const double* mass_x;       /* n_modes longitudinal masses [kg]                 */
+  const double* mass_z;       /* n_modes k_z masses [kg]                          */
+  double fermi_level;         /* E_F [eV]                                          */
+  const int32_t* mask;        /* n_Y_dof: nonzero on contact rows                  */
+  const double* char_matrix;  /* n_rho_x^2 complex, row-major: A+ (source), A- (drain) */
+  const double* lift;         /* n_l_x: face lift into the first / last X element   */
+  double* reservoir_out;      /* NULL, or rho: (n_rho_x n_rho_y) x n_Y_dof complex,
+                                 rho_x fastest, Y dof slowest                      */
+} wdg2_contact_modes;
+typedef struct wdg2_reservoir_grid {
+  double temp, deg_factor;    /* [K], valley degeneracy                            */
+  int32_t n_kx;               /* longitudinal k grid (makeKGrid) and its spacing   */
+  const double* kx;
+  double dkx;
+  int32_t n_kz;               /* 0: no k_z integration                             */
+  const double* kz;           /* ascending, from 0                                 */
+  const double* rho_x;        /* n_rho_x relative cell centres [m]                 */
+  const double* rho_y;        /* n_rho_y relative cell centres [m]                 */
+  const double* y_nodes;      /* n_Y_dof DG node coordinates [m]                   */
+  double scale_x;             /* transport scale of the X inflow parts             */
+} wdg2_reservoir_grid;
+int wdg2_set_contact_rhs(wdg_ctx* ctx, const wdg2_reservoir_grid* grid, const wdg2_contact_modes* source,
+                         const wdg2_contact_modes* drain);
+
 /* sysInfo.getRowAbsSum() (get_SysM_full2D.m:165-168): transport row sums of
  * the Kronecker parts plus the drift triangle-inequality bound
  * (get_Drift_full2D.m:490-523); N doubles. */

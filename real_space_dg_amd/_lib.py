@@ -43,6 +43,8 @@ WDG2_POT_FV_CONSISTENT = 0
 WDG2_POT_COLLOCATED = 1
 WDG2_DV_INTERP = 0
 WDG2_DV_TABLE = 1
+WDG2_MODES_BOX = 0
+WDG2_MODES_FOURIER = 1
 WDG2_PRECOND_NONE = 0
 WDG2_PRECOND_JACOBI = 1
 WDG2_PRECOND_ROWABS = 2
@@ -157,6 +159,22 @@ class wdg2_potential(C.Structure):
                [("pot_coef", C.c_double), ("cap_coef", C.c_double)]
 
 
+class wdg2_contact_modes(C.Structure):
+    _fields_ = [("n_modes", C.c_int32), ("kind", C.c_int32), ("n_grid", C.c_int32),
+                ("grid", C.POINTER(C.c_double)), ("values", C.POINTER(C.c_double)), ("y0", C.c_double),
+                ("width", C.c_double), ("energy", C.POINTER(C.c_double)), ("mass_x", C.POINTER(C.c_double)),
+                ("mass_z", C.POINTER(C.c_double)), ("fermi_level", C.c_double), ("mask", C.POINTER(C.c_int32)),
+                ("char_matrix", C.POINTER(C.c_double)), ("lift", C.POINTER(C.c_double)),
+                ("reservoir_out", C.POINTER(C.c_double))]
+
+
+class wdg2_reservoir_grid(C.Structure):
+    _fields_ = [("temp", C.c_double), ("deg_factor", C.c_double), ("n_kx", C.c_int32),
+                ("kx", C.POINTER(C.c_double)), ("dkx", C.c_double), ("n_kz", C.c_int32),
+                ("kz", C.POINTER(C.c_double)), ("rho_x", C.POINTER(C.c_double)), ("rho_y", C.POINTER(C.c_double)),
+                ("y_nodes", C.POINTER(C.c_double)), ("scale_x", C.c_double)]
+
+
 class wdg2_line_factors(C.Structure):
     _fields_ = [("sigma", C.c_double), ("modes", C.POINTER(C.c_double)), ("forward", C.POINTER(C.c_int32)),
                 ("x_inverse", C.POINTER(C.c_double)), ("x_coupling", C.POINTER(C.c_double)),
@@ -201,6 +219,8 @@ SIGNATURES = {
     "wdg2_set_potential": (C.c_int, [_P, C.POINTER(wdg2_potential)]),
     "wdg2_set_rhs": (C.c_int, [_P, _D]),
     "wdg2_set_dv_mode": (C.c_int, [_P, C.c_int32]),
+    "wdg2_set_contact_rhs": (C.c_int, [_P, C.POINTER(wdg2_reservoir_grid), C.POINTER(wdg2_contact_modes),
+                                       C.POINTER(wdg2_contact_modes)]),
     "wdg2_rowabs": (C.c_int, [_P, _D]),
     "wdg2_set_precond": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_int32)]),
     "wdg2_set_line_precond": (C.c_int, [_P, C.POINTER(wdg2_line_factors)]),

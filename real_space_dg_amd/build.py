@@ -11,7 +11,7 @@ CSRC = PKG / "csrc"
 LIB = PKG / "libwdg.so"
 SOURCES = [CSRC / "wdg.hip", CSRC / "host_setup.cpp"]
 HEADERS = [CSRC / "kernels.hpp", CSRC / "theta_fft.hpp", CSRC / "full2d.hpp", CSRC / "full2d_host.hpp",
-           CSRC / "host_setup.hpp", PKG.parent / "include" / "wigner_dg.h"]
+           CSRC / "host_setup.hpp", CSRC / "reservoir.hpp", PKG.parent / "include" / "wigner_dg.h"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

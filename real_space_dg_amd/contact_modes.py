@@ -351,46 +351,20 @@ def _interp_zero(xg, fg, x):
     return np.where((x < xg[0]) | (x > xg[-1]), 0.0, out)
 
 
-def contact_mode_reservoir(setup, side: str, Ef, Vxy):
+def contact_mode_reservoir(setup, side: str, Ef, Vxy, data=None):
     """get_ContactModeReservoirRho_full2D.m:35-137 for one contact: returns
-    (rho [N_rho_x*N_rho_y, N_Y_dof], rho_x fastest, and an info dict)."""
-    mat, params = setup.mat, setup.params
-    temp = float(np.asarray(mat.get("Temp", 300), float).ravel()[0])
-    deg = float(np.asarray(mat.get("deg_factor", 1), float).ravel()[0]) if mat.get("deg_factor") is not None else 1.0
-    if setup.periodic:
-        contact, energy, modes, norm = periodic_modes(setup, side, Vxy)
-        mx_def = float(contact["mXRelative"][0])
-        mz_def = float(contact["mZRelative"][0])
-    else:
-        contact, energy, modes, norm = box_modes(setup, side, Vxy)
-        mx_def, mz_def = contact["defaults"]
-    gy = contact["gridY"]
-    mass_x = mode_averaged_mass(modes, contact["mXRelative"], mx_def, gy)
-    mass_z = mode_averaged_mass(modes, contact["mZRelative"], mz_def, gy)
-    nkx = _param(params, "full2D_contactMode_Nkx", _param(params, "full2D_reservoir_Nkx", setup.RX.n))
-    kx, dkx = k_grid(float(np.asarray(nkx).ravel()[0]), setup.RX.length)
-    cosx = np.cos(np.outer(setup.RX.cells, kx))
-    integrate = _logical(params, "full2D_contactModeIntegrateKz", ("me_z" in mat) or ("me_z_ch" in mat))
-    model = fermi_model(_param(params, "full2D_contactModeFermiModel", "contact-neutrality"))
-    target, origin = target_sheet_density(mat, params, side, setup.coord_scale)
-    Ef_in = _scalar(Ef)
-    fermi = {"requestedModel": model, "model": model, "inputEV": Ef_in, "targetSheetDensityM2": target,
-             "targetOrigin": origin, "relativeResidual": np.nan}
-    Ef_used = Ef_in
-    if model == "contact-neutrality" and integrate and np.isfinite(target) and target > 0:
-        Ef_used, sol = solve_neutrality(energy, mass_x, mass_z, target, temp, deg, params, Ef_in)
-        fermi.update({"achievedSheetDensityM2": sol["achieved"], "relativeResidual": sol["relative_residual"],
-                      "iterations": sol["iterations"], "converged": sol["converged"]})
-    elif model == "contact-neutrality":
-        fermi["model"] = "external-fallback"
-    info = {"origin": "contact-modes", "modeCount": int(energy.size), "modeEnergy": energy, "modeMassXKg": mass_x,
-            "modeMassZKg": mass_z, "modeNormalization": norm, "contact": contact, "fermiLevelInput": Ef_in,
-            "fermiLevel": fermi}
-    if not np.isfinite(Ef_used):
-        info.update({"origin": "zero-placeholder", "fermiLevelUsed": np.nan})
+    (rho [N_rho_x*N_rho_y, N_Y_dof], rho_x fastest, and an info dict).  The
+    host-solved part (modes, masses, Fermi level, k grids) comes from
+    contact_mode_data; the device builds the same reservoir from it
+    (wdg2_set_contact_rhs)."""
+    d = contact_mode_data(setup, side, Ef, Vxy) if data is None else data
+    info = d["info"]
+    if not np.isfinite(d["Ef"]):
         return np.zeros((setup.nrel, setup.Y.ndof), complex), info
-    fermi["usedEV"] = Ef_used
-    kz = kz_info(params, mass_z, energy, Ef_used, temp, integrate)
+    energy, mass_x, mass_z, modes, contact = d["energy"], d["mass_x"], d["mass_z"], d["modes"], d["contact"]
+    kx, dkx, kz, Ef_used = d["kx"], d["dkx"], d["kz"], d["Ef"]
+    temp, deg, gy = d["temp"], d["deg"], contact["gridY"]
+    cosx = np.cos(np.outer(setup.RX.cells, kx))
     yface = np.asarray(setup.Y.nodes, float)
     rhoy_cells = np.asarray(setup.RY.cells, float)
     out = np.zeros((setup.nrel, yface.size), complex)
@@ -414,6 +388,54 @@ def contact_mode_reservoir(setup, side: str, Ef, Vxy):
         rhoy = php * np.conj(phm)                            # (rho_y, Y node)
         for iy in range(yface.size):
             out[:, iy] += np.outer(rhoy[:, iy], rx).ravel()  # rho_x fastest
-    info.update({"fermiLevelUsed": Ef_used, "kz": kz, "rhoXModeProfiles": profiles, "kxRange": (kx.min(), kx.max()),
-                 "dkx": dkx})
+    info["rhoXModeProfiles"] = profiles
     return out, info
+
+
+def contact_mode_data(setup, side: str, Ef, Vxy):
+    """The host-solved part of get_ContactModeReservoirRho_full2D.m:35-137
+    for one contact: transverse modes (box or periodic Fourier), mode-averaged
+    masses, the Fermi level (contact neutrality when requested) and the k_x /
+    k_z grids.  Ef = NaN in the result means a zero reservoir."""
+    mat, params = setup.mat, setup.params
+    temp = float(np.asarray(mat.get("Temp", 300), float).ravel()[0])
+    deg = float(np.asarray(mat.get("deg_factor", 1), float).ravel()[0]) if mat.get("deg_factor") is not None else 1.0
+    if setup.periodic:
+        contact, energy, modes, norm = periodic_modes(setup, side, Vxy)
+        mx_def = float(contact["mXRelative"][0])
+        mz_def = float(contact["mZRelative"][0])
+    else:
+        contact, energy, modes, norm = box_modes(setup, side, Vxy)
+        mx_def, mz_def = contact["defaults"]
+    gy = contact["gridY"]
+    mass_x = mode_averaged_mass(modes, contact["mXRelative"], mx_def, gy)
+    mass_z = mode_averaged_mass(modes, contact["mZRelative"], mz_def, gy)
+    nkx = _param(params, "full2D_contactMode_Nkx", _param(params, "full2D_reservoir_Nkx", setup.RX.n))
+    kx, dkx = k_grid(float(np.asarray(nkx).ravel()[0]), setup.RX.length)
+    integrate = _logical(params, "full2D_contactModeIntegrateKz", ("me_z" in mat) or ("me_z_ch" in mat))
+    model = fermi_model(_param(params, "full2D_contactModeFermiModel", "contact-neutrality"))
+    target, origin = target_sheet_density(mat, params, side, setup.coord_scale)
+    Ef_in = _scalar(Ef)
+    fermi = {"requestedModel": model, "model": model, "inputEV": Ef_in, "targetSheetDensityM2": target,
+             "targetOrigin": origin, "relativeResidual": np.nan}
+    Ef_used = Ef_in
+    if model == "contact-neutrality" and integrate and np.isfinite(target) and target > 0:
+        Ef_used, sol = solve_neutrality(energy, mass_x, mass_z, target, temp, deg, params, Ef_in)
+        fermi.update({"achievedSheetDensityM2": sol["achieved"], "relativeResidual": sol["relative_residual"],
+                      "iterations": sol["iterations"], "converged": sol["converged"]})
+    elif model == "contact-neutrality":
+        fermi["model"] = "external-fallback"
+    info = {"origin": "contact-modes", "modeCount": int(energy.size), "modeEnergy": energy, "modeMassXKg": mass_x,
+            "modeMassZKg": mass_z, "modeNormalization": norm, "contact": contact, "fermiLevelInput": Ef_in,
+            "fermiLevel": fermi}
+    out = {"energy": energy, "mass_x": mass_x, "mass_z": mass_z, "modes": modes, "contact": contact,
+           "kx": kx, "dkx": dkx, "temp": temp, "deg": deg, "info": info, "Ef": Ef_used}
+    if not np.isfinite(Ef_used):
+        info.update({"origin": "zero-placeholder", "fermiLevelUsed": np.nan})
+        out["Ef"] = np.nan
+        return out
+    fermi["usedEV"] = Ef_used
+    kz = kz_info(params, mass_z, energy, Ef_used, temp, integrate)
+    out["kz"] = kz
+    info.update({"fermiLevelUsed": Ef_used, "kz": kz, "kxRange": (kx.min(), kx.max()), "dkx": dkx})
+    return out

@@ -34,6 +34,7 @@
 #include "kernels.hpp"
 #include "theta_fft.hpp"
 #include "full2d.hpp"
+#include "reservoir.hpp"
 
 using namespace wdg;
 
@@ -2398,6 +2399,146 @@ int wdg2_set_rhs(wdg_ctx* ctx, const double* rhs) {
     HIPCHK(hipStreamSynchronize(ctx->stream));
     if (ctx->rhs_norm > 0) ctx->sol_scale = ctx->rhs_norm / ctx->f2->sA;
     ctx->f2->have_rhs = true;
+    return WDG_OK;
+  });
+}
+
+}  // extern "C"
+
+namespace {
+
+// Device copies of one contact's mode data (wdg2_set_contact_rhs).
+struct ResUpload {
+  DevBuf<double> grid, vals, energy, mx, mz, lift;
+  DevBuf<double2> coef, achar, res;
+  DevBuf<int> mask;
+};
+
+template <typename T>
+void upload_vec(wdg_ctx* c, DevBuf<T>& b, const T* v, size_t n) {
+  b.alloc(n);
+  if (n) HIPCHK(hipMemcpyAsync(b.p, v, n * sizeof(T), hipMemcpyHostToDevice, c->stream));
+}
+
+ResContact res_contact(wdg_ctx* c, const wdg2_contact_modes& m, ResUpload& u, int face, int nyd, int nrx, int nl) {
+  if (m.n_modes < 0 || m.n_modes > kResMaxModes)
+    throw std::invalid_argument("n_modes must be in 0.." + std::to_string(kResMaxModes));
+  if (m.kind != WDG2_MODES_BOX && m.kind != WDG2_MODES_FOURIER) throw std::invalid_argument("unknown mode kind");
+  if (!m.mask || !m.char_matrix || !m.lift) throw std::invalid_argument("contact mask, characteristic matrix and lift are required");
+  ResContact rc{};
+  rc.n_modes = m.n_modes;
+  rc.kind = m.kind;
+  rc.n_grid = m.n_grid;
+  rc.face = face;
+  rc.sign = face ? -1.0 : 1.0;
+  rc.ef = m.fermi_level;
+  std::vector<int> mask(m.mask, m.mask + nyd);
+  upload_vec(c, u.mask, mask.data(), mask.size());
+  upload_vec(c, u.achar, (const double2*)m.char_matrix, (size_t)nrx * nrx);
+  upload_vec(c, u.lift, m.lift, (size_t)nl);
+  rc.mask = u.mask.p;
+  rc.achar = u.achar.p;
+  if (m.n_modes > 0) {
+    if (!m.grid || !m.values || !m.energy || !m.mass_x || !m.mass_z || m.n_grid < 2)
+      throw std::invalid_argument("mode grid, values, energies and masses are required");
+    if (!std::isfinite(m.fermi_level)) throw std::invalid_argument("the Fermi level must be finite");
+    for (int i = 0; i < m.n_modes; ++i)
+      if (!(m.mass_x[i] > 0) || !(m.mass_z[i] > 0) || !std::isfinite(m.energy[i]))
+        throw std::invalid_argument("mode masses must be positive and energies finite");
+    upload_vec(c, u.grid, m.grid, (size_t)m.n_grid);
+    if (m.kind == WDG2_MODES_BOX) {
+      for (int i = 1; i < m.n_grid; ++i)
+        if (!(m.grid[i] > m.grid[i - 1])) throw std::invalid_argument("the mode grid must be strictly ascending");
+      upload_vec(c, u.vals, m.values, (size_t)m.n_modes * m.n_grid);
+      rc.vals = u.vals.p;
+    } else {
+      if (!(m.width > 0)) throw std::invalid_argument("the periodic cell width must be positive");
+      upload_vec(c, u.coef, (const double2*)m.values, (size_t)m.n_modes * m.n_grid);
+      rc.coef = u.coef.p;
+      rc.y0 = m.y0;
+      rc.inv_sqrt_w = 1.0 / std::sqrt(m.width);
+    }
+    upload_vec(c, u.energy, m.energy, (size_t)m.n_modes);
+    upload_vec(c, u.mx, m.mass_x, (size_t)m.n_modes);
+    upload_vec(c, u.mz, m.mass_z, (size_t)m.n_modes);
+    rc.grid = u.grid.p;
+    rc.energy = u.energy.p;
+    rc.mass_x = u.mx.p;
+    rc.mass_z = u.mz.p;
+  }
+  return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int wdg2_set_contact_rhs(wdg_ctx* ctx, const wdg2_reservoir_grid* gr, const wdg2_contact_modes* source,
+                         const wdg2_contact_modes* drain) {
+  if (!ctx || !gr || !source || !drain) return fail(ctx, WDG_ERR_ARG, "null argument");
+  if (!ctx->f2) return fail(ctx, WDG_ERR_UNSUPPORTED, "not a full-2D context");
+  return guarded(ctx, [&]() {
+    F2Data& f = *ctx->f2;
+    const wdg2_desc& d = f.d;
+    if (d.ne_x < 2) throw std::invalid_argument("the inflow faces need at least two X elements");
+    if (gr->n_kx < 2 || !gr->kx || !(gr->dkx > 0)) throw std::invalid_argument("k_x grid missing");
+    if (gr->n_kz < 0 || (gr->n_kz > 0 && (!gr->kz || gr->n_kz < 2))) throw std::invalid_argument("k_z grid invalid");
+    if (!gr->rho_x || !gr->rho_y || !gr->y_nodes) throw std::invalid_argument("relative cells and Y nodes required");
+    if (!(gr->temp > 0) || !std::isfinite(gr->deg_factor) || !std::isfinite(gr->scale_x))
+      throw std::invalid_argument("temperature must be positive, degeneracy and scale finite");
+    const int nrx = d.n_rho_x, nry = d.n_rho_y, nyd = f.nyd, nxd = f.nxd, nl = d.nl_x;
+    static thread_local ResUpload up[2];
+    DevBuf<double> kx, kz, rx, ry, yn, prof;
+    upload_vec(ctx, kx, gr->kx, (size_t)gr->n_kx);
+    if (gr->n_kz > 0) upload_vec(ctx, kz, gr->kz, (size_t)gr->n_kz);
+    upload_vec(ctx, rx, gr->rho_x, (size_t)nrx);
+    upload_vec(ctx, ry, gr->rho_y, (size_t)nry);
+    upload_vec(ctx, yn, gr->y_nodes, (size_t)nyd);
+    const ResContact c0 = res_contact(ctx, *source, up[0], 0, nyd, nrx, nl);
+    const ResContact c1 = res_contact(ctx, *drain, up[1], 1, nyd, nrx, nl);
+    ResGrid g{};
+    g.nkx = gr->n_kx; g.nkz = gr->n_kz; g.nrx = nrx; g.nry = nry; g.nyd = nyd; g.nxd = nxd; g.nl = nl;
+    g.kx = kx.p; g.kz = gr->n_kz > 0 ? kz.p : nullptr; g.rx = rx.p; g.ry = ry.p; g.ynodes = yn.p;
+    g.dkx = gr->dkx; g.temp = gr->temp; g.deg = gr->deg_factor; g.sx = gr->scale_x;
+    prof.alloc((size_t)2 * kResMaxModes * nrx);
+    const int mmax = std::max(1, std::max(c0.n_modes, c1.n_modes));
+    hipLaunchKernelGGL(res_profile_kernel, dim3(mmax, 2), dim3(256), (size_t)gr->n_kx * sizeof(double), ctx->stream,
+                       c0, c1, g, prof.p);
+    const long N = ctx->N;
+    hipLaunchKernelGGL(fill_zero_kernel, dim3(ctx->red_blocks), dim3(256), 0, ctx->stream, ctx->d_b.p, N);
+    const bool want0 = source->reservoir_out != nullptr, want1 = drain->reservoir_out != nullptr;
+    const size_t nres = (size_t)nrx * nry * nyd;
+    if (want0) up[0].res.alloc(nres);
+    if (want1) up[1].res.alloc(nres);
+    const int threads = std::min(256, ((nrx + 63) / 64) * 64);
+    hipLaunchKernelGGL(res_rhs_kernel, dim3(nyd, nry, 2), dim3(threads), (size_t)(kResMaxModes + nrx) * sizeof(double2),
+                       ctx->stream, c0, c1, g, prof.p, up[0].lift.p, up[1].lift.p, ctx->d_b.p,
+                       want0 ? up[0].res.p : nullptr, want1 ? up[1].res.p : nullptr);
+    HIPCHK(hipGetLastError());
+    // normalise as wdg2_set_rhs does (b_hat = b / ||b||)
+    const int nb = ctx->red_blocks;
+    hipLaunchKernelGGL(norm2_partials_kernel, dim3(nb), dim3(256), 0, ctx->stream, ctx->d_b.p, N,
+                       (double*)ctx->d_part.p);
+    std::vector<double> part(nb);
+    HIPCHK(hipMemcpyAsync(part.data(), ctx->d_part.p, nb * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    if (want0)
+      HIPCHK(hipMemcpyAsync(source->reservoir_out, up[0].res.p, nres * 16, hipMemcpyDeviceToHost, ctx->stream));
+    if (want1)
+      HIPCHK(hipMemcpyAsync(drain->reservoir_out, up[1].res.p, nres * 16, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    double s2 = 0;
+    for (double v : part) s2 += v;
+    if (!std::isfinite(s2)) throw std::runtime_error("the contact right-hand side is not finite");
+    const double old_norm = ctx->rhs_norm;
+    ctx->rhs_norm = std::sqrt(s2);
+    if (ctx->rhs_norm > 0)
+      hipLaunchKernelGGL(scale_complex_kernel, dim3(nb), dim3(256), 0, ctx->stream, ctx->d_b.p, 1.0 / ctx->rhs_norm, N);
+    if (ctx->have_solution && old_norm > 0 && ctx->rhs_norm > 0 && old_norm != ctx->rhs_norm)
+      hipLaunchKernelGGL(scale_complex_kernel, dim3(nb), dim3(256), 0, ctx->stream, ctx->d_x.p,
+                         old_norm / ctx->rhs_norm, N);
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (ctx->rhs_norm > 0) ctx->sol_scale = ctx->rhs_norm / f.sA;
+    f.have_rhs = true;
     return WDG_OK;
   });
 }

@@ -30,7 +30,7 @@ import numpy as np
 from numpy.polynomial import legendre as npleg
 
 from . import _lib
-from .contact_modes import contact_mode_reservoir
+from .contact_modes import contact_mode_data, contact_mode_reservoir
 from ._lib import check, dptr
 from .solver import SolveOptions, _as_c128
 
@@ -326,9 +326,13 @@ def y_boundary_type(params: dict) -> str:
 class Full2DSetup:
     """Everything the device needs for one (mat, Vxy, EfL, EfR) instance."""
 
-    def __init__(self, mat: dict, Vxy=None, EfL=None, EfR=None):
+    def __init__(self, mat: dict, Vxy=None, EfL=None, EfR=None, host_reservoirs: bool = True):
+        """host_reservoirs=False keeps only the host-solved contact modes of
+        contact-mode reservoirs; the reservoirs and the inflow rhs are then
+        built on the device (Full2DOperator.upload_contact_rhs)."""
         params = dg_params(mat)
         self.mat, self.params = mat, params
+        self.host_reservoirs = host_reservoirs
         self.EfL, self.EfR = EfL, EfR
         self._vxy = Vxy
         ytype = y_boundary_type(params)
@@ -512,6 +516,7 @@ class Full2DSetup:
         nyd = self.Y.ndof
         self.boundary = {}
         self.reservoir_info = {}
+        self.mode_data = {}     # host-solved contact modes, per side (contact-mode reservoirs)
         for side, Ef in (("source", self.EfL), ("drain", self.EfR)):
             direct = read_param(params, f"full2D_{side}Rho")
             if direct is not None:
@@ -524,14 +529,19 @@ class Full2DSetup:
                 if model in ("material", "material-default", "bulk", "bulk-local", "local-bulk"):
                     rho = self._reservoir(side, Ef, Vxy)
                 elif model in ("contact-mode", "contact-modes", "modes", "transverse-modes"):
-                    rho, self.reservoir_info[side] = contact_mode_reservoir(self, side, Ef, Vxy)
+                    self.mode_data[side] = contact_mode_data(self, side, Ef, Vxy)
+                    self.reservoir_info[side] = self.mode_data[side]["info"]
+                    rho = None
+                    if self.host_reservoirs:
+                        rho, _ = contact_mode_reservoir(self, side, Ef, Vxy, data=self.mode_data[side])
                 elif model in ("zero", "none", "off", "false"):
                     rho = np.zeros((self.nrel, nyd))
                 else:
                     raise NotImplementedError(f"DG:Full2D:Unsupported: reservoir model '{model}'")
             mask = self._contact_mask(side)
-            rho = np.array(rho, complex)
-            rho[:, ~mask] = 0
+            if rho is not None:
+                rho = np.array(rho, complex)
+                rho[:, ~mask] = 0
             self.boundary[side] = {"rho": rho, "mask": mask}
         sx, sy = self.scales
         ox = self.axis_ops[0]
@@ -562,6 +572,9 @@ class Full2DSetup:
         for side, A, sign, lift, cols in (("source", ch["Aplus"], 1.0, ox["lift_low"], slice(0, nl)),
                                           ("drain", ch["Aminus"], -1.0, ox["lift_high"], slice(nxd - nl, nxd))):
             b = self.boundary[side]
+            if b["rho"] is None:
+                raise ValueError("the reservoirs of this setup are built on the device (host_reservoirs=False): "
+                                 "use Full2DOperator.upload_contact_rhs")
             for iy in np.nonzero(b["mask"])[0]:
                 rb = b["rho"][:, iy].reshape(self.RY.n, self.RX.n)      # (rho_y, rho_x)
                 g = (rb @ A.T).ravel() * sign
@@ -788,6 +801,83 @@ class Full2DOperator:
             raise ValueError("rhs has the wrong size")
         check(self._lib.wdg2_set_rhs(self.ctx, dptr(r)), self.ctx)
         self.rhs_norm = float(np.linalg.norm(r))
+
+    def device_rhs(self) -> np.ndarray:
+        """The inflow rhs the device holds, in physical units (wdg_rhs)."""
+        r = np.empty(self.n, np.complex128)
+        check(self._lib.wdg_rhs(self.ctx, dptr(r)), self.ctx)
+        return r
+
+    def upload_contact_rhs(self, want_reservoirs: bool = False):
+        """Build the contact-mode reservoirs and the inflow rhs on the device
+        (wdg2_set_contact_rhs) from the setup's host-solved modes; the
+        alternative to upload_rhs for the contact-mode reservoir model.
+        Returns the two reservoirs (rho_x*rho_y x Y dof) when asked."""
+        s = self.setup
+        if set(s.mode_data) != {"source", "drain"}:
+            raise ValueError("upload_contact_rhs needs contact-mode reservoirs on both contacts "
+                             "(full2D_reservoirModel = 'contact-modes', no explicit full2D_<side>Rho)")
+        keep = []
+        nrx, nyd = s.RX.n, s.Y.ndof
+        ch, ox = s.char["X"], s.axis_ops[0]
+        structs = []
+        outs = {}
+        for side, A, lift in (("source", ch["Aplus"], ox["lift_low"]), ("drain", ch["Aminus"], ox["lift_high"])):
+            d = s.mode_data[side]
+            cm = _lib.wdg2_contact_modes()
+            A = _c128(np.asarray(A, complex).reshape(nrx, nrx))
+            lift = _f64(np.asarray(lift, float))
+            mask = np.ascontiguousarray(s.boundary[side]["mask"].astype(np.int32))
+            keep += [A, lift, mask]
+            cm.mask = mask.ctypes.data_as(C.POINTER(C.c_int32))
+            cm.char_matrix = dptr(A)
+            cm.lift = dptr(lift)
+            if np.isfinite(d["Ef"]) and d["energy"].size:
+                contact = d["contact"]
+                nm = d["energy"].size
+                cm.n_modes = nm
+                if contact["periodic"]:
+                    cm.kind = _lib.WDG2_MODES_FOURIER
+                    grid = _f64(contact["k"])
+                    vals = _c128(np.asarray(contact["coef"], complex).T)          # mode-major
+                    cm.y0, cm.width = float(contact["gridY"][0]), float(contact["width"])
+                else:
+                    cm.kind = _lib.WDG2_MODES_BOX
+                    gy = contact["gridY"]
+                    dy = float(np.mean(np.diff(gy)))
+                    grid = _f64(np.concatenate(([gy[0] - dy], gy, [gy[-1] + dy])))
+                    modes = np.real(np.asarray(d["modes"]))
+                    vals = _f64(np.concatenate((np.zeros((1, nm)), modes, np.zeros((1, nm))), axis=0).T)
+                cm.n_grid = grid.size
+                e, mx, mz = _f64(np.real(d["energy"])), _f64(d["mass_x"]), _f64(d["mass_z"])
+                keep += [grid, vals, e, mx, mz]
+                cm.grid, cm.values = dptr(grid), dptr(vals)
+                cm.energy, cm.mass_x, cm.mass_z = dptr(e), dptr(mx), dptr(mz)
+                cm.fermi_level = float(d["Ef"])
+            if want_reservoirs:
+                outs[side] = np.zeros((nyd, s.nrel), np.complex128)
+                cm.reservoir_out = dptr(outs[side])
+            structs.append(cm)
+        d0 = s.mode_data["source"]
+        g = _lib.wdg2_reservoir_grid()
+        kx = _f64(d0["kx"])
+        kz = d0.get("kz", {"integrate": False})
+        kzv = _f64(kz["k"]) if kz.get("integrate") else None
+        rx, ry, yn = _f64(s.RX.cells), _f64(s.RY.cells), _f64(s.Y.nodes)
+        keep += [kx, rx, ry, yn] + ([kzv] if kzv is not None else [])
+        g.temp, g.deg_factor = float(d0["temp"]), float(d0["deg"])
+        g.n_kx, g.kx, g.dkx = kx.size, dptr(kx), float(d0["dkx"])
+        if kzv is not None:
+            g.n_kz, g.kz = kzv.size, dptr(kzv)
+        g.rho_x, g.rho_y, g.y_nodes = dptr(rx), dptr(ry), dptr(yn)
+        g.scale_x = float(s.scales[0])
+        check(self._lib.wdg2_set_contact_rhs(self.ctx, C.byref(g), C.byref(structs[0]), C.byref(structs[1])),
+              self.ctx)
+        self._rhs = None
+        self.rhs_norm = float(np.linalg.norm(self.device_rhs()))
+        if want_reservoirs:
+            return {side: outs[side].T.copy() for side in outs}
+        return None
 
     def apply(self, u, precision: str = "fp64") -> np.ndarray:
         u = _as_c128(u, self.n)
@@ -1088,7 +1178,7 @@ def solve_transport_DG_full2D(mat: dict, Vxy=None, EfL=None, EfR=None, rho0=None
     (solve_transport_DG_full2D.m:1,67-129): GMRES on the device with the
     preconditioner named by full2D_preconditioner (auto/jacobi/rowabs/none),
     tolerance full2D_solverTol, restart full2D_gmresRestart."""
-    setup = Full2DSetup(mat, Vxy, EfL, EfR)
+    setup = Full2DSetup(mat, Vxy, EfL, EfR, host_reservoirs=False)
     params = setup.params
     if str(read_param(params, "full2D_solver", "gmres")).lower() not in ("gmres", "auto"):
         raise NotImplementedError("DG:Full2D:Unsupported: only the GMRES solver runs on the device")
@@ -1117,7 +1207,10 @@ def solve_transport_DG_full2D(mat: dict, Vxy=None, EfL=None, EfR=None, rho0=None
         else:
             used = op.set_preconditioner(pmode)
         solver.precondition = used != "none"
-        op.upload_rhs()
+        if set(setup.mode_data) == {"source", "drain"}:
+            op.upload_contact_rhs()      # contact-mode reservoirs and the inflow rhs built on the device
+        else:
+            op.upload_rhs()
         rho, info = op.solve(solver, x0=rho0)
         # resvec * ||rhs|| as the reference reports it (solve_transport_DG_full2D.m:324,333)
         resvec = op.resvec() * (op.rhs_norm if op.rhs_norm > 0 else 1.0)
