@@ -43,6 +43,7 @@ struct GmresState {
   int breakdown;
   unsigned int ticket;   // unused (kept for layout stability)
   int its_total;     // Arnoldi steps of the whole solve (index into the residual history)
+  int slow_norm;     // 1: this step's norm needs the explicit pass (arnoldi_update_pyth_kernel)
 };
 
 __device__ __forceinline__ bool skip(const GmresState* st) { return st && st->done; }
@@ -1394,9 +1395,10 @@ template <typename T, typename VP, int NT>
 __global__ __launch_bounds__(NT) void arnoldi_finish_kernel(GmresBufs gb, int nblocks, int j,
                                                             const typename Pk<T>::type* __restrict__ w,
                                                             VP* __restrict__ vnext,
-                                                            typename Pk<T>::type* __restrict__ vcur, long NP) {
+                                                            typename Pk<T>::type* __restrict__ vcur, long NP,
+                                                            int only_slow) {
   GmresState* st = gb.st;
-  if (st->done) return;
+  if (st->done || (only_slow && !st->slow_norm)) return;
   const int stride = kDotGroup + 1;
   __shared__ double wsum[NT / 64];
   __shared__ double2 col[kMaxRestart + 2];
@@ -1427,6 +1429,121 @@ __global__ __launch_bounds__(NT) void arnoldi_finish_kernel(GmresBufs gb, int nb
     vnext[x] = BasisIO<VP>::store(v);
     if (vcur) vcur[x] = v;
   }
+}
+
+// Single-pass Gram-Schmidt with the norm by Pythagoras: for an orthonormal
+// V, ||w - V h||^2 = ||w||^2 - ||h||^2, with ||w|| from the multi-dot.  When
+// the difference keeps at least 1 % of ||w||^2 (H[j+1,j] >= ||w||/10, nearly
+// every step: the ratio sits near 0.5 on these operators), every block writes
+// V_{j+1} = (w - V h)/H[j+1,j] at once and block 0 finishes the Hessenberg
+// column: w is neither written back nor read again by a finish pass.  V_{j+1}
+// times H[j+1,j] equals w - V h to rounding whatever the error of the norm,
+// so the Arnoldi relation holds exactly; the norm only scales the new
+// column.  Below the threshold (cancellation) the kernel does the two-pass
+// form instead: w -= V h with the norm partials, and arnoldi_finish_kernel
+// (run with only_slow) normalises.  Every block takes the same branch (the
+// same h and ||w||).
+template <typename T, typename VP, int NT, int W = 1>
+__global__ __launch_bounds__(NT) void arnoldi_update_pyth_kernel(GmresBufs gb, const VP* __restrict__ V,
+                                                                 typename Pk<T>::type* __restrict__ w,
+                                                                 VP* __restrict__ vnext,
+                                                                 typename Pk<T>::type* __restrict__ vcur, long NP,
+                                                                 int j) {
+  using P = Pk<T>;
+  using IO = BasisIO<VP>;
+  GmresState* st = gb.st;
+  if (st->done) return;
+  const int nv = j + 1;
+  __shared__ double2 hs[kMaxRestart + 1];
+  __shared__ double2 col[kMaxRestart + 2];
+  double hh = 0.0;
+  for (int i = 0; i < nv; ++i) {
+    const double2 hi = gb.h1[i];
+    hh += hi.x * hi.x + hi.y * hi.y;
+  }
+  for (int i = threadIdx.x; i < nv; i += NT) hs[i] = mk(IO::raw_scale * gb.h1[i].x, IO::raw_scale * gb.h1[i].y);
+  const double w2 = st->nrm0 * st->nrm0;
+  const double b2 = w2 - hh;
+  const bool fast = w2 > 0 && b2 >= 0.01 * w2;
+  const double beta = fast ? sqrt(b2) : 0.0;
+  __syncthreads();
+  if (fast && blockIdx.x == 0) {
+    if (threadIdx.x == 0) {
+      st->slow_norm = 0;
+      st->reorth = 0;
+      st->hnext = beta;
+      col[nv] = mk(beta, 0);
+    }
+    for (int i = threadIdx.x; i < nv; i += NT) col[i] = gb.h1[i];
+    __syncthreads();
+    givens_column(gb, j, col);
+  } else if (!fast && blockIdx.x == 0 && threadIdx.x == 0) {
+    st->slow_norm = 1;
+  }
+  const double sc = fast ? 1.0 / beta : 0.0;
+  double2 acc[kDotGroup + 1];
+#pragma unroll
+  for (int i = 0; i < kDotGroup + 1; ++i) acc[i] = mk(0, 0);
+  if constexpr (W == 2) {
+    static_assert(sizeof(VP) == 8 && sizeof(VPPair<VP>) == 16, "two 8-byte packets per 16-byte load");
+    const long NQ = NP / 2;
+    const uint4* Vq = reinterpret_cast<const uint4*>(V);
+    for (long x = (long)blockIdx.x * NT + threadIdx.x; x < NQ; x += (long)gridDim.x * NT) {
+      typename P::type w0 = w[2 * x], w1 = w[2 * x + 1];
+      int i = 0;
+      for (; i + 8 <= nv; i += 8) {
+        uint4 raw[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) raw[q] = Vq[(long)(i + q) * NQ + x];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          const VPPair<VP> pr = __builtin_bit_cast(VPPair<VP>, raw[q]);
+          P::axpy(w0, hs[i + q], IO::load_raw(pr.a));
+          P::axpy(w1, hs[i + q], IO::load_raw(pr.b));
+        }
+      }
+      for (; i < nv; ++i) {
+        const VPPair<VP> pr = __builtin_bit_cast(VPPair<VP>, Vq[(long)i * NQ + x]);
+        P::axpy(w0, hs[i], IO::load_raw(pr.a));
+        P::axpy(w1, hs[i], IO::load_raw(pr.b));
+      }
+      if (fast) {
+        const typename P::type v0 = P::scale(w0, sc), v1 = P::scale(w1, sc);
+        vnext[2 * x] = IO::store(v0);
+        vnext[2 * x + 1] = IO::store(v1);
+        if (vcur) {
+          vcur[2 * x] = v0;
+          vcur[2 * x + 1] = v1;
+        }
+      } else {
+        w[2 * x] = w0;
+        w[2 * x + 1] = w1;
+        acc[0].x += P::nrm2(w0) + P::nrm2(w1);
+      }
+    }
+  } else {
+    for (long x = (long)blockIdx.x * NT + threadIdx.x; x < NP; x += (long)gridDim.x * NT) {
+      typename P::type wv = w[x];
+      int i = 0;
+      for (; i + 8 <= nv; i += 8) {   // eight basis loads in flight
+        VP raw[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) raw[q] = V[(long)(i + q) * NP + x];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) P::axpy(wv, hs[i + q], IO::load_raw(raw[q]));
+      }
+      for (; i < nv; ++i) P::axpy(wv, hs[i], IO::load_raw(V[(long)i * NP + x]));
+      if (fast) {
+        const typename P::type v = P::scale(wv, sc);
+        vnext[x] = IO::store(v);
+        if (vcur) vcur[x] = v;
+      } else {
+        w[x] = wv;
+        acc[0].x += P::nrm2(wv);
+      }
+    }
+  }
+  if (!fast) block_reduce_store<NT>(acc, gb.part + (long)blockIdx.x * (kDotGroup + 1));
 }
 
 // V_{j+1} = w / H[j+1,j] (and, for a narrowed basis, the working-precision

@@ -843,14 +843,32 @@ void gmres_cycle(wdg_ctx* c, int m, bool precond, bool single_pass) {
       dot(0);
     }
     hipLaunchKernelGGL((arnoldi_dot_finalize_kernel<256>), dim3(dg.y), dim3(256), 0, c->stream, gb, nbd, nv, 0);
+    if (single_pass) {
+      // Pythagorean norm: the update writes V_{j+1} directly; the finish
+      // pass runs only for a step whose norm cancels (st->slow_norm)
+      VP* vnext = V + (size_t)(j + 1) * NP;
+      {
+        ProfScope ps(c, P_UPDATE, nv * vbytes + N * sizeof(T2) + vbytes);
+        if constexpr (sizeof(VP) == 8) {
+          if (wide) {
+            hipLaunchKernelGGL((arnoldi_update_pyth_kernel<T, VP, 256, 2>), dim3(nb), dim3(256), 0, c->stream, gb,
+                               V, wp, vnext, vcp, NP, j);
+          } else {
+            hipLaunchKernelGGL((arnoldi_update_pyth_kernel<T, VP, 256, 1>), dim3(nb), dim3(256), 0, c->stream, gb,
+                               V, wp, vnext, vcp, NP, j);
+          }
+        } else {
+          hipLaunchKernelGGL((arnoldi_update_pyth_kernel<T, VP, 256, 1>), dim3(nb), dim3(256), 0, c->stream, gb, V,
+                             wp, vnext, vcp, NP, j);
+        }
+      }
+      hipLaunchKernelGGL((arnoldi_finish_kernel<T, VP, 256>), dim3(nb), dim3(256), 0, c->stream, gb, nb, j, wp,
+                         vnext, vcp, NP, 1);
+      continue;
+    }
     {
       ProfScope ps(c, P_UPDATE, nv * vbytes + 2.0 * N * sizeof(T2));
       update(0);
-    }
-    if (single_pass) {
-      hipLaunchKernelGGL((arnoldi_finish_kernel<T, VP, 256>), dim3(nb), dim3(256), 0, c->stream, gb, nb, j, wp,
-                         V + (size_t)(j + 1) * NP, vcp, NP);
-      continue;
     }
     hipLaunchKernelGGL((arnoldi_update_finalize_kernel<256>), dim3(1), dim3(256), 0, c->stream, gb, nb, j, 0);
     // DGKS second pass, taken on the device only when requested (st->reorth)
