@@ -59,7 +59,8 @@ def parse():
     ap.add_argument("--cpu-sample-iters", type=int, default=6)
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the batched-sweep and self-consistent workloads reported beside the metric")
-    ap.add_argument("--only", choices=["batched_sweep", "self_consistent", "transient_full2d", "transient_full2d_batch"],
+    ap.add_argument("--only", choices=["batched_sweep", "self_consistent", "transient_full2d", "transient_full2d_batch",
+                                       "transient_full2d_sc"],
                     default=None,
                     help="run just this side workload and print its JSON (profiling runs)")
     ap.add_argument("--full2d-steps", type=int, default=2, help="Crank-Nicolson steps of the full-2D transient")
@@ -296,6 +297,55 @@ def transient_full2d(steps: int = 2, dt: float = 1e-15, precisions=("mixed", "fp
     return out
 
 
+def transient_full2d_sc(instances: int = 8, steps: int = 20, poisson_every: int = 10, dt: float = 1e-15,
+                        bias: float = 0.05, precisions=("mixed",)) -> dict:
+    """The self-consistent FET-slab transient: the slab with n+ contacts,
+    contact-mode reservoirs (built on the device) and `instances`
+    transverse-mode instances (gate potential + 5 meV per mode) stepped
+    together by Crank-Nicolson (dt 1 fs), a 2D Poisson update of the shared
+    potential every `poisson_every` steps from the summed density
+    (poisson2d.transient_cn_batch_self_consistent).  Reports instance-steps/s
+    per precision, GMRES iterations and the Poisson updates."""
+    from real_space_dg_amd import full2d as F2
+    from real_space_dg_amd import poisson2d as PS
+    from real_space_dg_amd.devices import fet_slab_self_consistent
+    mat = fet_slab_self_consistent()
+    x = np.asarray(mat["x"], float)
+    phi0 = np.repeat((bias * x / x[-1])[:, None], mat["Ny"], axis=1)
+    Vc = np.asarray(mat["Vc"], float)
+    offsets = [0.005 * i for i in range(instances)]
+    out = {"workload": f"FET slab 40x20 nm, 64x32 DG elements p=1 x 64x64 relative cells, n+ 1e24 m^-3 contacts, "
+                       f"contact-mode reservoirs, {bias} V; {instances} transverse-mode instances (+5 meV each) in "
+                       f"one batched Crank-Nicolson call, dt {dt:g} s x {steps} steps from zero, 2D Poisson every "
+                       f"{poisson_every} steps, GMRES(30) + row scaling to 1e-8 per step",
+           "instances": instances, "steps": steps}
+    for prec in precisions:
+        ops = []
+        try:
+            for off in offsets:
+                op = F2.Full2DOperator(F2.Full2DSetup(mat, Vc - phi0 + off, 0.1, 0.1, host_reservoirs=False))
+                op.upload_contact_rhs()
+                op.set_preconditioner("rowabs")
+                ops.append(op)
+            n = ops[0].n
+            t0 = time.perf_counter()
+            phi, hist = PS.transient_cn_batch_self_consistent(
+                ops, offsets, phi0, bias, dt, steps, poisson_every=poisson_every, precision=prec, tol=1e-8,
+                rho0=[np.zeros(n, complex)] * instances, use_graph=prec != "fp64")
+            wall = time.perf_counter() - t0
+        finally:
+            for op in ops:
+                op.close()
+        its = [sum(h["gmresIterations"]) for h in hist]
+        out[prec] = {"instance_steps_per_s": instances * steps / wall, "seconds": wall,
+                     "gmres_iterations_per_instance_step": sum(its) / (instances * steps),
+                     "flag": max(h["flag"] for h in hist), "max_step_relres": max(h["maxStepRelres"] for h in hist),
+                     "poisson_updates_V": [h["potentialUpdate"] for h in hist],
+                     "poisson_relres": [h["poissonPhysicalRelres"] for h in hist]}
+        print(f"[bench] self-consistent FET slab {prec}: {wall:.1f} s", file=sys.stderr, flush=True)
+    return out
+
+
 def transient_full2d_batch(instances: int = 8, steps: int = 1, dt: float = 1e-15, precision: str = "mixed") -> dict:
     """The full-2D FET slab transient with several transverse-mode instances
     (subband offsets of 5 meV each on the gate potential) stepped together by
@@ -360,7 +410,10 @@ def main():
                                                            precisions=tuple(args.full2d_precisions.split(",")),
                                                            dv=args.full2d_dv),
               "transient_full2d_batch": lambda: transient_full2d_batch(instances=args.full2d_instances,
-                                                                       steps=args.full2d_steps)
+                                                                       steps=args.full2d_steps),
+              "transient_full2d_sc": lambda: transient_full2d_sc(instances=args.full2d_instances,
+                                                                 steps=args.full2d_steps,
+                                                                 precisions=tuple(args.full2d_precisions.split(",")))
               }[args.only]
         print(json.dumps({args.only: fn()}))
         return

@@ -135,3 +135,34 @@ def fet_slab(length_nm: float = 40.0, width_nm: float = 20.0, n_el_x: int = 64, 
            "dg": {"params": {"N_K_X": degree + 1, "N_K_Y": degree + 1, "N_rho_x": n_rho, "N_rho_y": n_rho,
                              "full2D_fluxType": flux}}}
     return mat, V
+
+
+def fet_slab_self_consistent(length_nm: float = 40.0, width_nm: float = 20.0, n_el_x: int = 64, n_el_y: int = 32,
+                             degree: int = 1, n_rho: int = 64, barrier_ev: float = 0.2, sigma_nm: float = 5.0,
+                             mass: float = 0.067, temp: float = 300.0, eps_r: float = 12.9,
+                             contact_nm: float = 10.0, nd_contact: float = 1e24, nd_channel: float = 1e21,
+                             n_modes: int = 4, nkz: int = 101):
+    """The FET slab of ``fet_slab`` with the data of a self-consistent run
+    (the RTD conventions of RTD/solve2D_rtd_self_consistent.m): band offset
+    ``Vc`` = the Gaussian gate barrier [eV], relative permittivity ``eps``,
+    donors ``Nd`` [m^-3] (``nd_contact`` within ``contact_nm`` of either
+    contact, ``nd_channel`` between), and contact-mode reservoirs (the
+    ``n_modes`` lowest transverse modes of each contact edge, k_z-integrated,
+    Fermi levels from contact neutrality with the contact donors' sheet
+    density ``nd_contact * width``).  The electron energy is Vc - phi.
+    Returns ``mat``."""
+    mat, V = fet_slab(length_nm, width_nm, n_el_x, n_el_y, degree, n_rho, barrier_ev, sigma_nm, mass, temp)
+    x, y = mat["x"], mat["y"]
+    X = np.repeat(x[:, None], y.size, axis=1)
+    shape = (x.size, y.size)
+    mat.update({"Vc": V, "eps": np.full(shape, eps_r),
+                "Nd": np.where((X <= contact_nm) | (X >= length_nm - contact_nm), nd_contact, nd_channel),
+                "me_z_ch": mass, "me_x": np.full(shape, mass), "me_y": np.full(shape, mass),
+                "me_z": np.full(shape, mass), "n_of_modes": n_modes})
+    sheet = nd_contact * width_nm * 1e-9
+    mat["dg"]["params"].update({"full2D_reservoirModel": "contact-modes",
+                                "full2D_contactModeFermiModel": "contact-neutrality",
+                                "full2D_contactModeIntegrateKz": 1, "full2D_contactModeCount": n_modes,
+                                "full2D_contactMode_Nkz": nkz,
+                                "full2D_sourceContactSheetDensity": sheet, "full2D_drainContactSheetDensity": sheet})
+    return mat

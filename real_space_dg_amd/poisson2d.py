@@ -361,3 +361,47 @@ def transient_cn_self_consistent(op, phi0, bias: float, dt: float, steps: int, p
         phi = new
         op.set_potential(Vc - phi)
     return phi, hist
+
+
+def transient_cn_batch_self_consistent(ops, offsets, phi0, bias: float, dt: float, steps: int,
+                                       poisson_every: int = 10, precision: str = "mixed", tol: float = 1e-8,
+                                       restart: int = 30, poisson_method: str = "newton-raphson", rho0=None,
+                                       use_graph: bool = True):
+    """Crank-Nicolson steps of several full-2D instances of one device (the
+    transverse modes: instance i sees the potential Vc - phi + offsets[i]),
+    all stepped in one batched call (wdg_transient_cn_batch), with a 2D
+    Poisson update every `poisson_every` steps driven by the summed density
+    of the instances: project the summed device density onto the Poisson dual
+    volumes, one Poisson step from the current phi, and every instance's
+    potential becomes Vc - phi_new + offset (the contacts keep phi = 0 and
+    bias, so the inflow data stay valid).  rho0: None (continue from the
+    device states) or one initial state per instance.  Returns (phi, history)
+    with one entry per block, holding the phi the block ran with."""
+    from .solver import transient_cn_batch
+    ops = list(ops)
+    if len(offsets) != len(ops):
+        raise ValueError("one potential offset per instance")
+    mat = ops[0].setup.mat
+    Vc = np.asarray(mat["Vc"], float)
+    M, types, periodic = eval_poisson2d(mat)
+    phi = np.asarray(phi0, float).copy()
+    for op, off in zip(ops, offsets):
+        op.set_potential(Vc - phi + float(off))
+    done, hist = 0, []
+    while done < steps:
+        k = min(int(poisson_every), steps - done)
+        _, infos = transient_cn_batch(ops, dt, k, tol=tol, restart=restart, precision=precision,
+                                      rho0=rho0 if done == 0 else None, return_rho=False, use_graph=use_graph)
+        done += k
+        nDG = sum(np.real(op.observables()[0]) for op in ops)
+        n, tinfo = project_dg_density(nDG, ops[0].setup, periodic)
+        new, pinfo = poisson_step(M, types, n, mat, phi, bias, poisson_method)
+        hist.append({"steps": done, "phi": phi, "gmresIterations": [i["iterations"] for i in infos],
+                     "maxStepRelres": max(i["max_relres"] for i in infos), "flag": max(i["flag"] for i in infos),
+                     "potentialUpdate": float(np.max(np.abs(new - phi))),
+                     "poissonPhysicalRelres": pinfo["physicalRelresBefore"], "poissonConverged": pinfo["converged"],
+                     "chargeTransferError": tinfo["relativeConservationError"]})
+        phi = new
+        for op, off in zip(ops, offsets):
+            op.set_potential(Vc - phi + float(off))
+    return phi, hist

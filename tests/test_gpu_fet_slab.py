@@ -98,3 +98,72 @@ def test_fet_slab_eight_mode_batch_matches_oracle(slab):
     finally:
         for op in ops:
             op.close()
+
+
+def test_self_consistent_batch_matches_oracle_blocks():
+    """The self-consistent FET slab (contact-mode reservoirs built on the
+    device, n+ contacts, 0.05 V) with two transverse-mode instances (offsets
+    0 and 5 meV) stepped together, a Poisson update every 3 steps from their
+    summed density (poisson2d.transient_cn_batch_self_consistent), against
+    the oracle block by block: each instance's Crank-Nicolson steps by the
+    oracle's matrix-free operator at the potential Vc - phi + offset the
+    block ran with (reservoirs as the setup built them), and the next phi
+    from the oracle states through the same projection and Poisson step."""
+    import copy
+
+    from real_space_dg_amd import poisson2d as PS
+    from real_space_dg_amd.devices import fet_slab_self_consistent
+
+    mat = fet_slab_self_consistent(n_el_x=8, n_el_y=4, n_rho=8)
+    bias, offsets = 0.05, [0.0, 0.005]
+    x = np.asarray(mat["x"], float)
+    phi0 = np.repeat((bias * x / x[-1])[:, None], mat["Ny"], axis=1)
+    Vc = np.asarray(mat["Vc"], float)
+    ops = []
+    try:
+        for off in offsets:
+            s = P.Full2DSetup(mat, Vc - phi0 + off, EFL, EFR)
+            op = P.Full2DOperator(s)
+            op.upload_contact_rhs()
+            op.set_preconditioner("rowabs")
+            ops.append(op)
+        n = ops[0].n
+        phi, hist = PS.transient_cn_batch_self_consistent(ops, offsets, phi0, bias, DT, 6, poisson_every=3,
+                                                          precision="fp64", tol=1e-12,
+                                                          rho0=[np.zeros(n, complex)] * 2)
+        print("SC blocks:", [(h["steps"], h["gmresIterations"], round(h["potentialUpdate"], 6)) for h in hist],
+              flush=True)
+        assert len(hist) == 2 and all(h["flag"] == 0 for h in hist)
+        # the oracle, block by block, with the reservoirs of each instance's setup
+        M, types, periodic = PS.eval_poisson2d(mat)
+        states = [np.zeros(n, complex)] * 2
+        phi_ref = phi0
+        for b, h in enumerate(hist):
+            assert np.max(np.abs(h["phi"] - phi_ref)) <= 1e-9 * max(1.0, np.max(np.abs(phi_ref)))
+            nsum = 0
+            for i, (op, off) in enumerate(zip(ops, offsets)):
+                mi = copy.deepcopy(mat)
+                mi["dg"]["params"]["full2D_reservoirModel"] = "zero"
+                host = P.Full2DSetup(mat, Vc - phi0 + off, EFL, EFR)
+                mi["dg"]["params"]["full2D_sourceRho"] = host.boundary["source"]["rho"]
+                mi["dg"]["params"]["full2D_drainRho"] = host.boundary["drain"]["rho"]
+                V = Vc - phi_ref + off
+                p, apply_a, rowabs, rhs = _system(mi, V)
+                states[i] = TR.crank_nicolson_mf(apply_a, rhs, states[i], DT, 3, tol=1e-13, row_scale=rowabs)
+                nsum = nsum + np.real(F.observables_full2d(states[i], p, mi)[0])
+            npois, _ = PS.project_dg_density(nsum, ops[0].setup, periodic)
+            phi_ref, _ = PS.poisson_step(M, types, npois, mat, phi_ref, bias, "newton-raphson")
+        for op, st in zip(ops, states):
+            assert rel(_device_state(op), st) < 1e-8
+        assert np.max(np.abs(phi - phi_ref)) <= 1e-8 * max(1.0, np.max(np.abs(phi_ref)))
+    finally:
+        for op in ops:
+            op.close()
+
+
+def _device_state(op):
+    from real_space_dg_amd._lib import check
+    from real_space_dg_amd._lib import dptr
+    x = np.empty(op.n, np.complex128)
+    check(op._lib.wdg_solution(op.ctx, dptr(x)), op.ctx)
+    return x
