@@ -100,10 +100,19 @@ def kernel_roofline(prof: dict, name: str, peak_kind: str, precision: str = "mix
         peak = MFMA_PEAK_TFS[peak_kind]
         out = {"bound": "mfma", "achieved": ach, "peak": peak, "unit": "TFLOP/s", "frac": ach / peak}
     else:
+        if name == "theta_fft":
+            # the whole operator apply: read u once, write y once (2 s N bytes;
+            # the potential's edge table counts as generated on the fly)
+            s = 16 if precision == "fp64" else 8
+            N = WORKLOAD["n_elem"] * (WORKLOAD["degree"] + 1) * WORKLOAD["nk"]
+            per_launch = 2.0 * s * N
         ach = per_launch / (t_ms * 1e-3) / 1e9
         out = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS}
     out.update({"kernel": name, "avg_launch_ms": t_ms, "launches": st["launches"],
                 "algorithmic_per_launch": per_launch, "traffic": traffic_for(name, precision)})
+    busy = valu_busy_for(name, precision)
+    if busy is not None:
+        out["valu_busy"] = busy
     if name == "theta_fft":
         # VALU-bound: four length-P transforms per DG node at 5 P log2 P flops each
         P = chirp_length(WORKLOAD["nk"])
@@ -112,6 +121,22 @@ def kernel_roofline(prof: dict, name: str, peak_kind: str, precision: str = "mix
         out["fft_flops_per_launch"] = flops
         out["fft_tflops"] = flops / (t_ms * 1e-3) / 1e12
     return out
+
+
+def valu_busy_for(name: str, precision: str = "mixed"):
+    """Fraction of the kernel's duration its SIMDs spend issuing VALU
+    instructions (SQ_INSTS_VALU x 4 cycles / (SIMDs x cycles)), from the
+    rocprofv3 SQ counter pass recorded in profiles/traffic.json; None if not
+    recorded."""
+    f = ROOT / "profiles" / "traffic.json"
+    if not f.exists():
+        return None
+    try:
+        d = json.loads(f.read_text())
+        v = d.get("kernels_fp64" if precision == "fp64" else "kernels", {}).get(name)
+        return None if v is None else v.get("valu_busy")
+    except (OSError, ValueError):
+        return None
 
 
 def traffic_for(name: str, precision: str = "mixed"):
@@ -197,9 +222,15 @@ def batched_sweep(n_instances: int = 64, seed: int = 0) -> dict:
         infos = solve_batch(ops, opts)
         dt = time.perf_counter() - t0
         its = [i["iterations"] for i in infos]
+        # total work in unknown-iterations: the batch time over it is the
+        # batch's cost per unknown and GMRES iteration, comparable with one
+        # large solve's (the batch is work-bound when the two agree)
+        work = float(sum(it * op.N for it, op in zip(its, ops)))
         out[prec] = {"solves_per_s": n_instances / dt, "seconds": dt, "converged": int(sum(i["flag"] == 0 for i in infos)),
                      "iterations_mean": float(np.mean(its)), "iterations_max": int(max(its)),
-                     "true_relres_max": float(max(i["true_relres"] for i in infos))}
+                     "true_relres_max": float(max(i["true_relres"] for i in infos)),
+                     "unknown_iterations": work, "ps_per_unknown_iteration": 1e12 * dt / work,
+                     "slowest_instance": {"iterations": int(max(its)), "unknowns": int(ops[int(np.argmax(its))].N)}}
         print(f"[bench] batched sweep {prec}: {dt:.2f} s", file=sys.stderr, flush=True)
     out["mixed_over_fp64"] = out["mixed"]["solves_per_s"] / out["fp64"]["solves_per_s"]
     for op in ops:

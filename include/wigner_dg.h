@@ -351,17 +351,27 @@ int wdg_theta_method(const wdg_ctx* ctx);
  * when it is not positive and finite, the call returns WDG_ERR_LINE_MEMORY. */
 int wdg_set_memory_limits(wdg_ctx* ctx, double gmres_max_basis_bytes, double line_max_factor_bytes);
 
-/* Workgroup shape of the FP32 (mixed inner) chirp-z Theta kernel:
- * WDG_FFT_BLOCK_256 runs 256 threads that loop over the element's nodes;
- * WDG_FFT_BLOCK_WIDE runs ~1024 threads, four or more transforms at once,
- * for n_k_chi <= 5; AUTO (the default) picks the one that keeps more waves
- * resident per CU.  Both compute the same operator (the choice changes
- * FP32 rounding order only).  No counterpart in the reference: an
- * implementation choice, exposed so that every variant can be tested.
- * Returns WDG_ERR_ARG when the grid has no FFT plan or the wide block does
- * not fit it.  wdg_fft_block gives the threads per workgroup in use (-1: no
- * FFT plan). */
-enum { WDG_FFT_BLOCK_AUTO = 0, WDG_FFT_BLOCK_256 = 1, WDG_FFT_BLOCK_WIDE = 2 };
+/* Kernel of the FP32 (mixed inner) FFT Theta apply:
+ * WDG_FFT_BLOCK_256 runs the chirp-z kernel with 256 threads that loop over
+ * the element's nodes; WDG_FFT_BLOCK_WIDE runs it with ~1024 threads, four
+ * or more transforms at once, for n_k_chi <= 5; WDG_FFT_BLOCK_NDFT runs the
+ * near-DFT kernel (square grids n_xi = nk = 64 * 2^a 3^b, n_k_chi <= 5):
+ * transforms of length nk instead of >= 2 nk - 1, the residual phase
+ * exp(i s ubar vbar), |s| = pi/(2 nk), by a Taylor series truncated below
+ * 5e-6 relative (two terms for nk >= 512).  AUTO (the default) runs the
+ * chirp-z kernel in the shape an autotuner picks: before the first mixed
+ * solve of the context it times both workgroup shapes on the context's grid
+ * (cached per grid shape for the process; until then the shape that keeps
+ * more waves resident per CU).  NDFT is never chosen
+ * by AUTO (its truncation, ~3e-7 relative at nk = 1024, costs the mixed
+ * solver more refinement cycles than its shorter transforms save).  All
+ * compute the operator of
+ * get_Drift_accel_trial_permutedShell.m:95-151 to FP32 accuracy.  No
+ * counterpart in the reference: an implementation choice, exposed so that
+ * every variant can be tested.  Returns WDG_ERR_ARG when the grid has no
+ * FFT plan or the requested kernel does not fit it.  wdg_fft_block gives
+ * the threads per workgroup in use (-1: no FFT plan). */
+enum { WDG_FFT_BLOCK_AUTO = 0, WDG_FFT_BLOCK_256 = 1, WDG_FFT_BLOCK_WIDE = 2, WDG_FFT_BLOCK_NDFT = 3 };
 int wdg_set_fft_block(wdg_ctx* ctx, int32_t mode);
 int wdg_fft_block(const wdg_ctx* ctx);
 

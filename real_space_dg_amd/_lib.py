@@ -24,6 +24,7 @@ WDG_THETA_GEMM = 2
 WDG_FFT_BLOCK_AUTO = 0
 WDG_FFT_BLOCK_256 = 1
 WDG_FFT_BLOCK_WIDE = 2
+WDG_FFT_BLOCK_NDFT = 3
 WDG_PART_TRANSPORT = 1
 WDG_PART_POTENTIAL = 2
 WDG_PART_ALL = 3

@@ -391,6 +391,27 @@ cplx chirp(double s, double t) {
   return cplx(std::cos(M_PI * ph), std::sin(M_PI * ph));
 }
 
+// Radix sequence (8, the radices of P/64, 8) and the Stockham base twiddles
+// of the device transform of length P = 64 m.
+void fft_plan_tables(int P, std::vector<int>& rad, std::vector<int>& tw_off, std::vector<cplx>& tw) {
+  rad.clear();
+  tw_off.clear();
+  tw.clear();
+  rad.push_back(8);
+  for (int r : fft_radices(P / 64)) rad.push_back(r);
+  rad.push_back(8);
+  if (P / 64 == 1) rad = {8, 8};
+  int Ns = 1;
+  for (int R : rad) {
+    tw_off.push_back((int)tw.size());
+    for (int jm = 0; jm < Ns; ++jm) {
+      const double ang = -2.0 * M_PI * (double)jm / ((double)Ns * R);
+      tw.emplace_back(std::cos(ang), std::sin(ang));
+    }
+    Ns *= R;
+  }
+}
+
 }  // namespace
 
 void fft_host(std::vector<cplx>& x, bool inverse) {
@@ -440,21 +461,7 @@ void build_chirp(const Geometry& g, ChirpPlan& cp) {
   int P = nk + nx - 1;
   while (P % 64 != 0 || !smooth23(P / 64)) ++P;
   cp.P = P;
-  cp.rad.push_back(8);
-  for (int r : fft_radices(P / 64)) cp.rad.push_back(r);
-  cp.rad.push_back(8);
-  if (P / 64 == 1) cp.rad = {8, 8};
-  {
-    int Ns = 1;
-    for (int R : cp.rad) {
-      cp.tw_off.push_back((int)cp.tw.size());
-      for (int jm = 0; jm < Ns; ++jm) {
-        const double ang = -2.0 * M_PI * (double)jm / ((double)Ns * R);
-        cp.tw.emplace_back(std::cos(ang), std::sin(ang));
-      }
-      Ns *= R;
-    }
-  }
+  fft_plan_tables(P, cp.rad, cp.tw_off, cp.tw);
   const double norm = 1.0 / (std::sqrt(L) * P);
   cp.a_pre.resize(nk);
   cp.a_post.resize(nk);
@@ -478,6 +485,56 @@ void build_chirp(const Geometry& g, ChirpPlan& cp) {
   fft_host(cp.g1, false);
   fft_host(cp.g2, false);
   cp.ok = true;
+}
+
+void build_ndft(const Geometry& g, double trunc_tol, NdftPlan& np) {
+  np = NdftPlan();
+  const int N = g.nk;
+  if (g.n_xi != N || N < 64 || N % 64 != 0 || !smooth23(N / 64)) return;
+  const double L = g.l_xi;
+  const double hp = (g.l_xi - g.delta_xi) / (N - 1);
+  const double c = 0.5 * (N - 1);
+  for (int j = 0; j < N; ++j)
+    if (std::abs(g.k[j] + 2.0 * M_PI / L * (j - c)) > 1e-12 * 2.0 * M_PI / L * N) return;
+  for (int l = 0; l < N; ++l)
+    if (std::abs(g.y[l] - hp * (l - c)) > 1e-12 * L) return;
+  // k_j y_l = -2 pi beta u v, beta = 1/N + delta
+  const double delta = hp / L - 1.0 / N;
+  const double s = -2.0 * M_PI * delta * c * c;
+  int terms = 1;
+  double r = std::abs(s);   // |s|^terms / terms!
+  while (r > trunc_tol && terms < 4) {
+    ++terms;
+    r *= std::abs(s) / terms;
+  }
+  if (r > trunc_tol) return;
+  np.P = N;
+  np.terms = terms;
+  np.s = s;
+  np.c = c;
+  fft_plan_tables(N, np.rad, np.tw_off, np.tw);
+  np.fpre.resize(N);
+  np.fpost.resize(N);
+  np.bpre.resize(N);
+  np.bpost.resize(N);
+  // phases 2 pi c j / N and 2 pi c^2 / N reduced modulo 2 pi through
+  // c j = (N - 1) j / 2 and c^2 = (N - 1)^2 / 4 in integer arithmetic
+  const long long Nl = N, two_c = N - 1;
+  auto ph = [&](long long num, long long den) {   // exp(2 pi i num / den)
+    const long long q = ((num % den) + den) % den;
+    const double a = 2.0 * M_PI * (double)q / (double)den;
+    return cplx(std::cos(a), std::sin(a));
+  };
+  const cplx cc = ph(-two_c * two_c, 4 * Nl);   // exp(-2 pi i c^2 / N)
+  const double isl = 1.0 / std::sqrt(L);
+  for (int j = 0; j < N; ++j) {
+    const cplx pj = ph(two_c * j, 2 * Nl);        // exp(2 pi i c j / N)
+    np.fpre[j] = pj;
+    np.fpost[j] = cc * pj * isl;
+    np.bpre[j] = std::conj(pj);
+    np.bpost[j] = cplx(0.0, 1.0) * std::conj(cc) * std::conj(pj) * isl;
+  }
+  np.ok = true;
 }
 
 void inflow(const Geometry& g, const wdg_potential& pot, std::vector<cplx>& fnl,

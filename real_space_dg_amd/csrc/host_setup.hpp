@@ -90,6 +90,23 @@ void build_geometry(const wdg_desc& d, Geometry& g);
 // Fills cp (cp.ok = false when the grids are not of the chirp form).
 void build_chirp(const Geometry& g, ChirpPlan& cp);
 
+// Near-DFT form of the same two products for square grids (nk = n_xi = N,
+// N = 64 m, m = 2^a 3^b).  With c = (N-1)/2, ubar = (j - c)/c, vbar = (l - c)/c
+// and s = -2 pi (beta - 1/N) c^2:
+//   phi U     = fpost o sum_{m<terms} (i s)^m/m!  vbar^m o FFT_N(fpre o ubar^m o U)
+//   i phi^H Z = bpost o sum_{m<terms} (-i s)^m/m! ubar^m o IFFT_N(bpre o vbar^m o Z)
+// (unnormalised FFTs).  terms is the smallest count with |s|^terms/terms! <=
+// trunc_tol (at most 4; ok = false if none is).
+struct NdftPlan {
+  bool ok = false;
+  int P = 0, terms = 0;
+  double s = 0, c = 0;
+  std::vector<int> rad, tw_off;
+  std::vector<cplx> tw;
+  std::vector<cplx> fpre, fpost, bpre, bpost;   // N each
+};
+void build_ndft(const Geometry& g, double trunc_tol, NdftPlan& np);
+
 // In-place FFT of length x.size() (a 2^p 3^q length): forward
 // X_k = sum_m x_m exp(-2 pi i k m / P), or the unnormalised inverse.
 void fft_host(std::vector<cplx>& x, bool inverse);

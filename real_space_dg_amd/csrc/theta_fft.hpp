@@ -500,6 +500,84 @@ __device__ __forceinline__ void fft_any(T2 (&v)[8], LdsP<T2> buf0, LdsP<T2> buf1
   else fft_regs<INV, T2>(v, buf0, buf1, cur, tw, pl, t, TT);
 }
 
+// The Theta combine of element e in LDS: T_b (NN rows of n_xi values at Tst)
+// becomes Z_a in place; every thread handles l = threadIdx.x + k blockDim.x
+// for k < KL (KL blockDim.x >= n_xi).  Ends with Z written, before the
+// caller's barrier.
+template <int NN, typename T, typename TS, int KL>
+__device__ __forceinline__ void theta_combine(const OpConst* __restrict__ oc, const TS* __restrict__ sbar,
+                                              const double* __restrict__ cap,
+                                              LdsP<typename cvec<T>::type> Tst, int e, int nx) {
+  using T2 = typename cvec<T>::type;
+  // combine: Z_a(l) = sum_b [dg_ab(l) T_b(l) + Bbar_ab(l+1) T_b(l+1) + Bbar_ab(l) T_b(l-1)],
+  // dg = Bbar(l) + Bbar(l+1), Bbar_ab(l) = c_ab (S_ab(l) - i CAP(l) Mq_ab).  With the
+  // edge sums A_b = T_b(l) + T_b(l-1), C_b = T_b(l) + T_b(l+1):
+  //   Z_a = sum_b c_ab (S_ab(l) A_b + S_ab(l+1) C_b) - i sum_b c_ab Mq_ab (CAP(l) A_b + CAP(l+1) C_b).
+  // All a at once per l (in the kernel's precision T), then Z overwrites T.
+  constexpr int np = NN * (NN + 1) / 2;
+  const int nl = nx + 1;
+  T2 zero;
+  zero.x = 0;
+  zero.y = 0;
+  T2 zreg[KL][NN];
+  const TS* sb = sbar + (long)e * np * nl;
+#pragma unroll
+  for (int k = 0; k < KL; ++k) {
+    const int l = threadIdx.x + k * blockDim.x;
+    if (l >= nx) continue;
+    T sv0[np], sv1[np];
+    // The edges are symmetric about xi = 0 and Delta V(-xi) = -Delta V(xi),
+    // so Sbar(n_xi - l) = -Sbar(l): only the half l <= n_xi/2 is read from
+    // HBM (the mirrored reads of the other half hit the same lines in L2).
+    const int hx = nx >> 1;
+    const int l0 = l <= hx ? l : nx - l, l1 = l + 1 <= hx ? l + 1 : nx - l - 1;
+    const T s0 = l <= hx ? (T)1 : (T)-1, s1 = l + 1 <= hx ? (T)1 : (T)-1;
+#pragma unroll
+    for (int p = 0; p < np; ++p) {
+      sv0[p] = s0 * (T)sb[(long)p * nl + l0];
+      sv1[p] = s1 * (T)sb[(long)p * nl + l1];
+    }
+    const T c0 = (T)cap[l], c1 = (T)cap[l + 1];
+    T2 A[NN], Cs[NN];
+#pragma unroll
+    for (int bb = 0; bb < NN; ++bb) {
+      const T2 t1 = lds_ld<T2>(Tst, bb * nx + l);
+      const T2 t0 = (l > 0) ? lds_ld<T2>(Tst, bb * nx + l - 1) : zero;
+      const T2 t2 = (l < nx - 1) ? lds_ld<T2>(Tst, bb * nx + l + 1) : zero;
+      A[bb] = c2add(t1, t0);
+      Cs[bb] = c2add(t1, t2);
+    }
+#pragma unroll
+    for (int a = 0; a < NN; ++a) {
+      T pr = 0, pi = 0, qr = 0, qi = 0;   // potential part; CAP part before the factor -i
+#pragma unroll
+      for (int bb = 0; bb < NN; ++bb) {
+        const int lo = a < bb ? a : bb, hi = a < bb ? bb : a;
+        const int p = lo * NN - lo * (lo - 1) / 2 + (hi - lo);
+        const T c = (T)oc->cab[a * kMaxN + bb];
+        const T cm = c * (T)oc->Mq[a * kMaxN + bb];
+        const T f0 = c * sv0[p], f1 = c * sv1[p];
+        pr += f0 * A[bb].x + f1 * Cs[bb].x;
+        pi += f0 * A[bb].y + f1 * Cs[bb].y;
+        const T g0 = cm * c0, g1c = cm * c1;
+        qr += g0 * A[bb].x + g1c * Cs[bb].x;
+        qi += g0 * A[bb].y + g1c * Cs[bb].y;
+      }
+      // Z = P - i Q
+      zreg[k][a].x = pr + qi;
+      zreg[k][a].y = pi - qr;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < KL; ++k) {
+    const int l = threadIdx.x + k * blockDim.x;
+    if (l >= nx) continue;
+#pragma unroll
+    for (int a = 0; a < NN; ++a) lds_st<T2>(Tst, a * nx + l, zreg[k][a]);
+  }
+}
+
 // y = (transport + Theta) u for the elements of the grid (one workgroup per
 // element).
 // blockDim = G * TT <= NT (G groups of TT = P/8 threads, each group
@@ -517,8 +595,7 @@ __global__ __launch_bounds__(NT) void theta_fft_kernel(const OpConst* __restrict
   using T2 = typename cvec<T>::type;
   if (skip(st)) return;
   extern __shared__ __align__(16) unsigned char theta_smem[];
-  constexpr int np = NN * (NN + 1) / 2;
-  const int nk = oc->nk, nx = oc->n_xi, nl = nx + 1, P = PC > 0 ? PC : pl.P;
+  const int nk = oc->nk, nx = oc->n_xi, P = PC > 0 ? PC : pl.P;
   const int TT = P >> 3, G = blockDim.x / TT;
   const int g = threadIdx.x / TT, t = threadIdx.x % TT;
   const int e = blockIdx.x;
@@ -604,71 +681,9 @@ __global__ __launch_bounds__(NT) void theta_fft_kernel(const OpConst* __restrict
   }
   __syncthreads();
 
-  // combine: Z_a(l) = sum_b [dg_ab(l) T_b(l) + Bbar_ab(l+1) T_b(l+1) + Bbar_ab(l) T_b(l-1)],
-  // dg = Bbar(l) + Bbar(l+1), Bbar_ab(l) = c_ab (S_ab(l) - i CAP(l) Mq_ab).  With the
-  // edge sums A_b = T_b(l) + T_b(l-1), C_b = T_b(l) + T_b(l+1):
-  //   Z_a = sum_b c_ab (S_ab(l) A_b + S_ab(l+1) C_b) - i sum_b c_ab Mq_ab (CAP(l) A_b + CAP(l+1) C_b).
-  // All a at once per l (in the kernel's precision T), then Z overwrites T.
-  // l values per thread: n_xi <= (P+1)/2 means n_xi <= 4 TT, and the host
-  // launches NT = 1024 only with G >= 4 groups
-  constexpr int kMaxL = NT >= 1024 ? 1 : 1024 / NT;
-  T2 zreg[kMaxL][NN];
-  const TS* sb = sbar + (long)e * np * nl;
-#pragma unroll
-  for (int k = 0; k < kMaxL; ++k) {
-    const int l = threadIdx.x + k * blockDim.x;
-    if (l >= nx) continue;
-    T sv0[np], sv1[np];
-    // The edges are symmetric about xi = 0 and Delta V(-xi) = -Delta V(xi),
-    // so Sbar(n_xi - l) = -Sbar(l): only the half l <= n_xi/2 is read from
-    // HBM (the mirrored reads of the other half hit the same lines in L2).
-    const int hx = nx >> 1;
-    const int l0 = l <= hx ? l : nx - l, l1 = l + 1 <= hx ? l + 1 : nx - l - 1;
-    const T s0 = l <= hx ? (T)1 : (T)-1, s1 = l + 1 <= hx ? (T)1 : (T)-1;
-#pragma unroll
-    for (int p = 0; p < np; ++p) {
-      sv0[p] = s0 * (T)sb[(long)p * nl + l0];
-      sv1[p] = s1 * (T)sb[(long)p * nl + l1];
-    }
-    const T c0 = (T)cap[l], c1 = (T)cap[l + 1];
-    T2 A[NN], Cs[NN];
-#pragma unroll
-    for (int bb = 0; bb < NN; ++bb) {
-      const T2 t1 = lds_ld<T2>(Tst, bb * nx + l);
-      const T2 t0 = (l > 0) ? lds_ld<T2>(Tst, bb * nx + l - 1) : zero;
-      const T2 t2 = (l < nx - 1) ? lds_ld<T2>(Tst, bb * nx + l + 1) : zero;
-      A[bb] = c2add(t1, t0);
-      Cs[bb] = c2add(t1, t2);
-    }
-#pragma unroll
-    for (int a = 0; a < NN; ++a) {
-      T pr = 0, pi = 0, qr = 0, qi = 0;   // potential part; CAP part before the factor -i
-#pragma unroll
-      for (int bb = 0; bb < NN; ++bb) {
-        const int lo = a < bb ? a : bb, hi = a < bb ? bb : a;
-        const int p = lo * NN - lo * (lo - 1) / 2 + (hi - lo);
-        const T c = (T)oc->cab[a * kMaxN + bb];
-        const T cm = c * (T)oc->Mq[a * kMaxN + bb];
-        const T f0 = c * sv0[p], f1 = c * sv1[p];
-        pr += f0 * A[bb].x + f1 * Cs[bb].x;
-        pi += f0 * A[bb].y + f1 * Cs[bb].y;
-        const T g0 = cm * c0, g1c = cm * c1;
-        qr += g0 * A[bb].x + g1c * Cs[bb].x;
-        qi += g0 * A[bb].y + g1c * Cs[bb].y;
-      }
-      // Z = P - i Q
-      zreg[k][a].x = pr + qi;
-      zreg[k][a].y = pi - qr;
-    }
-  }
-  __syncthreads();
-#pragma unroll
-  for (int k = 0; k < kMaxL; ++k) {
-    const int l = threadIdx.x + k * blockDim.x;
-    if (l >= nx) continue;
-#pragma unroll
-    for (int a = 0; a < NN; ++a) lds_st<T2>(Tst, a * nx + l, zreg[k][a]);
-  }
+  // the Theta combine (n_xi <= (P+1)/2 <= 4 TT values; the host launches
+  // NT = 1024 only with G >= 4 groups)
+  theta_combine<NN, T, TS, (NT >= 1024 ? 1 : 1024 / NT)>(oc, sbar, cap, Tst, e, nx);
   // this thread's backward table entries
   T2 b_pre[8], g2[8], a_post[8];
   if constexpr (kPre) {
@@ -711,6 +726,155 @@ __global__ __launch_bounds__(NT) void theta_fft_kernel(const OpConst* __restrict
           yb[j] = c2add(tr, c2mul(kPre ? a_post[r] : app[j], v[r]));
         }
       }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Near-DFT form of the FP32 Theta apply for square grids (nk = n_xi = N,
+// N = 64 m).  The reference's grids give beta = h'/L = (N-2)/(N-1)^2, so
+// beta = 1/N + delta with |delta| = 1/(N (N-1)^2).  With c = (N-1)/2,
+// u_j = j - c, v_l = l - c, ubar = u/c, vbar = v/c and s = -2 pi delta c^2
+// (|s| = pi/(2N)), the phase splits into a length-N DFT and a factor
+// exp(i s ubar vbar) whose Taylor series converges like s^M/M!:
+//   phi U     = fpost o sum_m (i s)^m/m!  vbar^m o FFT_N(fpre o ubar^m o U)
+//   i phi^H Z = bpost o sum_m (-i s)^m/m! ubar^m o IFFT_N(bpre o vbar^m o Z)
+// with fpre = exp(2 pi i c j/N), fpost = exp(-2 pi i c^2/N) exp(2 pi i c l/N)
+// / sqrt(L), bpre = conj(fpre), bpost = i conj(fpost).  M terms cost M FFTs
+// of length N per transform against the chirp-z's two of length >= 2N - 1;
+// the host picks M so that s^M/M! stays below the FP32 working accuracy
+// (M = 2 for N >= 512, 3 below).  One workgroup per element with NN groups
+// of N/8 threads (one DG node per group, one round), T/Z aliased with the
+// groups' work buffers, transport rows computed at the store from u (L2).
+// ---------------------------------------------------------------------------
+struct NdftTabs {
+  const float2* tw;      // per-pass base twiddles of the length-N plan
+  const float2* fpre;    // N
+  const float2* fpost;   // N
+  const float2* bpre;    // N
+  const float2* bpost;   // N
+  float s;               // the Taylor variable's scale
+  float c;               // (N - 1) / 2
+  int terms;             // M
+};
+
+template <int NN, int PC>
+__global__ __launch_bounds__(NN * (PC > 0 ? PC : 1024) / 8) void theta_ndft_kernel(
+    const OpConst* __restrict__ oc, FftPlan pl, NdftTabs tb, const float* __restrict__ sbar,
+    const double* __restrict__ cap, const double* __restrict__ kvec, const float2* __restrict__ u,
+    float2* __restrict__ y, const GmresState* st, int with_transport) {
+  using T2 = float2;
+  if (skip(st)) return;
+  extern __shared__ __align__(16) unsigned char theta_smem[];
+  const int N = oc->nk, P = PC > 0 ? PC : pl.P;   // P == nk == n_xi
+  const int TT = P >> 3;
+  const int g = threadIdx.x / TT, t = threadIdx.x % TT;   // group g owns DG node g
+  const int e = blockIdx.x;
+  const int PL = padded_len(P);
+  LdsP<T2> Tst = (LdsP<T2>)theta_smem;   // NN * N values, aliased with the work buffers
+  LdsP<T2> buf0 = Tst + 2 * g * PL;
+  LdsP<T2> buf1 = buf0 + PL;
+  LdsP<T2> twl = Tst + 2 * NN * PL;
+  for (int i = threadIdx.x; i < pl.ntw; i += blockDim.x) lds_st<T2>(twl, i, tb.tw[i]);
+  int cur = 0;
+  T2 zero;
+  zero.x = 0;
+  zero.y = 0;
+  const float cinv = 1.0f / tb.c;
+  // u of the element, staged by one coalesced sweep
+  {
+    const T2* __restrict__ ue = u + (long)e * NN * N;
+    for (int i = threadIdx.x; i < NN * N; i += blockDim.x) lds_st<T2>(Tst, i, ue[i]);
+  }
+  __syncthreads();
+
+  // forward: T_g = fpost o sum_m (i s)^m/m! vbar^m o FFT(fpre o ubar^m o U_g)
+  {
+    T2 ur[8], acc[8];
+    float pw[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      const int j = t + TT * r;
+      ur[r] = c2mul(tb.fpre[j], lds_ld<T2>(Tst, g * N + j));
+      acc[r] = zero;
+      pw[r] = 1.0f;
+    }
+    __syncthreads();   // u is read before the work buffers overwrite it
+    float cr = 1.0f, ci = 0.0f;   // (i s)^m / m!
+    for (int m = 0; m < tb.terms; ++m) {
+      T2 v[8];
+#pragma unroll
+      for (int r = 0; r < 8; ++r) {
+        v[r].x = pw[r] * ur[r].x;
+        v[r].y = pw[r] * ur[r].y;
+      }
+      fft_any<PC, false, T2>(v, buf0, buf1, cur, twl, pl, t, TT);
+#pragma unroll
+      for (int r = 0; r < 8; ++r) {
+        const float p = pw[r];   // vbar^m at output l = t + TT r (same grid as ubar)
+        const float wr = cr * p, wi = ci * p;
+        acc[r].x += wr * v[r].x - wi * v[r].y;
+        acc[r].y += wr * v[r].y + wi * v[r].x;
+        pw[r] = p * (((float)(t + TT * r) - tb.c) * cinv);
+      }
+      const float f = tb.s / (float)(m + 1);   // times i s / (m + 1)
+      const float nr = -ci * f, ni = cr * f;
+      cr = nr;
+      ci = ni;
+    }
+    __syncthreads();   // every group's last pass has read its buffer
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      const int l = t + TT * r;
+      lds_st<T2>(Tst, g * N + l, c2mul(tb.fpost[l], acc[r]));
+    }
+  }
+  __syncthreads();
+
+  theta_combine<NN, float, float, (8 + NN - 1) / NN>(oc, sbar, cap, Tst, e, N);
+  __syncthreads();
+
+  // backward: y_g += bpost o sum_m (-i s)^m/m! ubar^m o IFFT(bpre o vbar^m o Z_g)
+  {
+    T2 zr[8], acc[8];
+    float pw[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      const int l = t + TT * r;
+      zr[r] = c2mul(tb.bpre[l], lds_ld<T2>(Tst, g * N + l));
+      acc[r] = zero;
+      pw[r] = 1.0f;
+    }
+    __syncthreads();   // Z is read before the work buffers overwrite it
+    float cr = 1.0f, ci = 0.0f;   // (-i s)^m / m!
+    for (int m = 0; m < tb.terms; ++m) {
+      T2 v[8];
+#pragma unroll
+      for (int r = 0; r < 8; ++r) {
+        v[r].x = pw[r] * zr[r].x;
+        v[r].y = pw[r] * zr[r].y;
+      }
+      fft_any<PC, true, T2>(v, buf0, buf1, cur, twl, pl, t, TT);
+#pragma unroll
+      for (int r = 0; r < 8; ++r) {
+        const float p = pw[r];
+        const float wr = cr * p, wi = ci * p;
+        acc[r].x += wr * v[r].x - wi * v[r].y;
+        acc[r].y += wr * v[r].y + wi * v[r].x;
+        pw[r] = p * (((float)(t + TT * r) - tb.c) * cinv);
+      }
+      const float f = tb.s / (float)(m + 1);   // times -i s / (m + 1)
+      const float nr = ci * f, ni = -cr * f;
+      cr = nr;
+      ci = ni;
+    }
+    T2* __restrict__ yb = y + ((long)e * NN + g) * N;
+    const T2* up = pin_lane(u);
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      const int j = t + TT * r;
+      const T2 tr = with_transport ? transport_row_g<NN>(oc, kvec, up, e, g, j) : zero;
+      yb[j] = c2add(tr, c2mul(tb.bpost[j], acc[r]));
     }
   }
 }

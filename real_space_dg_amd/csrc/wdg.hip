@@ -161,6 +161,17 @@ struct wdg_ctx {
   int theta32_base = 0, theta32_wide = 0;     // FP32 candidates: 256-thread block, wide block (0: unusable)
   size_t lds32_base = 0, lds32_wide = 0;
   bool theta32_wide_auto = false;             // the occupancy calculator prefers the wide block
+  bool theta32_auto = true;                   // FP32 chirp-z block left to the autotuner (mode AUTO)
+  bool theta32_tuned = false;                 // the autotuner has timed both blocks on this context
+  // FP32 Theta in the near-DFT form (square grids): length-N transforms
+  NdftPlan ndft;
+  bool ndft_ok = false;                       // plan exists, n <= 5 and its LDS fits
+  bool use_ndft = false;                      // the FP32 apply runs theta_ndft_kernel
+  FftPlan nplan{};
+  DevBuf<float2> d_ndft32;                    // tw, fpre, fpost, bpre, bpost
+  NdftTabs ntabs{};
+  int ndft_block = 0;                         // n * N / 8 threads
+  size_t lds_ndft = 0;
   int apply_parts = WDG_PART_ALL;             // operator parts op_apply computes (wdg_apply_parts)
   double2* scratch_dev = nullptr;             // this device's scratch array (wdg_precond_device_noalloc)
   // wdg_set_memory_limits: GMRES basis estimate (16 N (m+1) bytes, 0: no
@@ -458,6 +469,96 @@ double combine_bytes(wdg_ctx* c, int s, int ts) {
   return 2.0 * c->ncols * c->g.n_xi * s + (double)c->g.n_chi * c->oc.n_pairs * (c->g.n_xi + 1) * ts;
 }
 
+// y = (transport + Theta) u in FP32 by the near-DFT kernel (one workgroup of
+// n * N/8 threads per element).
+void op_theta_ndft(wdg_ctx* c, const float2* u, float2* y, const GmresState* st, int with_transport) {
+  const dim3 grid(c->g.n_chi), block(c->ndft_block);
+  const size_t lds = c->lds_ndft;
+#define WDG_NDFT_P(NN, PC)                                                                                   \
+  hipLaunchKernelGGL((theta_ndft_kernel<NN, PC>), grid, block, lds, c->stream, c->d_oc.p, c->nplan, c->ntabs, \
+                     c->d_sbar32.p, c->d_cap.p, c->d_k.p, u, y, st, with_transport);
+#define WDG_NDFT(NN)                                                                                          \
+  if (c->nplan.P == 1024) {                                                                                   \
+    WDG_NDFT_P(NN, 1024)                                                                                      \
+  } else if (c->nplan.P == 512) {                                                                             \
+    WDG_NDFT_P(NN, 512)                                                                                       \
+  } else if (c->nplan.P == 256) {                                                                             \
+    WDG_NDFT_P(NN, 256)                                                                                       \
+  } else {                                                                                                    \
+    WDG_NDFT_P(NN, 0)                                                                                         \
+  }                                                                                                           \
+  break;
+  switch (c->g.n) {
+    case 2: WDG_NDFT(2)
+    case 3: WDG_NDFT(3)
+    case 4: WDG_NDFT(4)
+    case 5: WDG_NDFT(5)
+    default: throw std::invalid_argument("the near-DFT Theta kernel covers n_k_chi <= 5");
+  }
+#undef WDG_NDFT
+#undef WDG_NDFT_P
+}
+
+template <int NN>
+void theta_ndft_attrs() {
+  HIPCHK(hipFuncSetAttribute((const void*)theta_ndft_kernel<NN, 0>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             kMaxLdsBytes));
+  HIPCHK(hipFuncSetAttribute((const void*)theta_ndft_kernel<NN, 256>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             kMaxLdsBytes));
+  HIPCHK(hipFuncSetAttribute((const void*)theta_ndft_kernel<NN, 512>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             kMaxLdsBytes));
+  HIPCHK(hipFuncSetAttribute((const void*)theta_ndft_kernel<NN, 1024>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             kMaxLdsBytes));
+}
+
+// Near-DFT tables of the FP32 Theta kernel (square grids, n <= 5).
+void setup_ndft(wdg_ctx* c) {
+  const Geometry& g = c->g;
+  c->ndft_ok = false;
+  c->use_ndft = false;
+  // truncation of the Taylor factor: below the FP32 transforms' own rounding
+  build_ndft(g, 5e-6, c->ndft);
+  const NdftPlan& np = c->ndft;
+  if (!np.ok || g.n > 5 || (int)np.rad.size() > kFftMaxPasses) return;
+  const int N = np.P, tt = N / 8;
+  const size_t lds = (2 * (size_t)g.n * padded_len(N) + np.tw.size()) * sizeof(float2);
+  if (lds > (size_t)kMaxLdsBytes || g.n * tt > 1024) return;
+  FftPlan& pl = c->nplan;
+  std::memset(&pl, 0, sizeof(pl));
+  pl.P = N;
+  pl.npass = (int)np.rad.size();
+  pl.ntw = (int)np.tw.size();
+  for (int i = 0; i < pl.npass; ++i) {
+    pl.rad[i] = np.rad[i];
+    pl.toff[i] = np.tw_off[i];
+  }
+  const std::vector<cplx>* parts[5] = {&np.tw, &np.fpre, &np.fpost, &np.bpre, &np.bpost};
+  size_t tot = 0, off[5];
+  for (int i = 0; i < 5; ++i) {
+    off[i] = tot;
+    tot += parts[i]->size();
+  }
+  std::vector<float2> h(tot);
+  for (int i = 0; i < 5; ++i)
+    for (size_t k = 0; k < parts[i]->size(); ++k)
+      h[off[i] + k] = make_float2((float)(*parts[i])[k].real(), (float)(*parts[i])[k].imag());
+  c->d_ndft32.alloc(tot);
+  HIPCHK(hipMemcpy(c->d_ndft32.p, h.data(), tot * sizeof(float2), hipMemcpyHostToDevice));
+  const float2* b = c->d_ndft32.p;
+  c->ntabs = NdftTabs{b + off[0], b + off[1], b + off[2], b + off[3], b + off[4],
+                      (float)np.s, (float)np.c, np.terms};
+  c->ndft_block = g.n * tt;
+  c->lds_ndft = lds;
+  switch (g.n) {
+    case 2: theta_ndft_attrs<2>(); break;
+    case 3: theta_ndft_attrs<3>(); break;
+    case 4: theta_ndft_attrs<4>(); break;
+    case 5: theta_ndft_attrs<5>(); break;
+    default: return;
+  }
+  c->ndft_ok = true;
+}
+
 // y += Theta u by the fused chirp-z kernel.
 template <typename T>
 void op_theta_fft(wdg_ctx* c, const typename cvec<T>::type* u, typename cvec<T>::type* y, const GmresState* st,
@@ -466,6 +567,12 @@ void op_theta_fft(wdg_ctx* c, const typename cvec<T>::type* u, typename cvec<T>:
   const size_t s = sizeof(typename cvec<T>::type), ts = f64 ? 8 : 4;
   // algorithmic HBM bytes: read u, write y, read the edge table
   ProfScope ps(c, P_THETA_FFT, 2.0 * c->N * s + (double)c->g.n_chi * c->oc.n_pairs * (c->g.n_xi + 1) * ts);
+  if constexpr (!f64) {
+    if (c->use_ndft) {
+      op_theta_ndft(c, u, y, st, with_transport);
+      return;
+    }
+  }
   const dim3 grid(c->g.n_chi), block(f64 ? c->theta_block64 : c->theta_block32);
   const size_t lds = f64 ? c->lds64 : c->lds32;
 #define WDG_THETA_P(NN, PC)                                                                                    \
@@ -504,6 +611,56 @@ void op_theta_fft(wdg_ctx* c, const typename cvec<T>::type* u, typename cvec<T>:
   }
 #undef WDG_THETA
 #undef WDG_THETA_P
+}
+
+// Workgroup autotuner of the FP32 chirp-z Theta kernel: times the 256-thread
+// and the wide block on this context's grid (one warm-up and five applies
+// each, HIP events on the context stream) and keeps the faster.  Results are
+// cached per grid shape for the process.  Runs once per context, before its
+// first mixed solve, while the block choice is AUTO.
+void clear_graphs(wdg_ctx* c);
+
+void autotune_theta32(wdg_ctx* c) {
+  if (!c->use_fft || c->use_ndft || !c->theta32_auto || c->theta32_tuned || c->theta32_wide == 0) return;
+  c->theta32_tuned = true;
+  static std::mutex mu;
+  static std::map<std::tuple<int, int, int, int>, bool> cache;
+  const auto key = std::make_tuple(c->g.n_chi, c->g.n, c->g.nk, c->g.n_xi);
+  int known = -1;
+  {
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = cache.find(key);
+    if (it != cache.end()) known = it->second ? 1 : 0;
+  }
+  if (known < 0) {
+    hipEvent_t t0, t1;
+    HIPCHK(hipEventCreate(&t0));
+    HIPCHK(hipEventCreate(&t1));
+    const uint32_t mask = c->prof_mask;   // the trial applies stay out of the kernel statistics
+    c->prof_mask = 0;
+    HIPCHK(hipMemsetAsync(c->d_z32.p, 0, (size_t)c->N * sizeof(float2), c->stream));
+    float ms[2] = {0, 0};
+    for (int k = 0; k < 2; ++k) {
+      c->theta_block32 = k ? c->theta32_wide : c->theta32_base;
+      c->lds32 = k ? c->lds32_wide : c->lds32_base;
+      op_theta_fft<float>(c, c->d_z32.p, c->d_w32.p, nullptr, 1);
+      HIPCHK(hipEventRecord(t0, c->stream));
+      for (int r = 0; r < 5; ++r) op_theta_fft<float>(c, c->d_z32.p, c->d_w32.p, nullptr, 1);
+      HIPCHK(hipEventRecord(t1, c->stream));
+      HIPCHK(hipEventSynchronize(t1));
+      HIPCHK(hipEventElapsedTime(&ms[k], t0, t1));
+    }
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    c->prof_mask = mask;
+    known = ms[1] < ms[0] ? 1 : 0;
+    std::lock_guard<std::mutex> lock(mu);
+    cache[key] = known != 0;
+  }
+  const bool wide = known != 0;
+  c->theta_block32 = wide ? c->theta32_wide : c->theta32_base;
+  c->lds32 = wide ? c->lds32_wide : c->lds32_base;
+  clear_graphs(c);   // captured cycles hold the launch shape
 }
 
 // The attribute is process-wide per kernel, so it is set to the hardware
@@ -1076,6 +1233,7 @@ void setup_theta(wdg_ctx* c) {
     HIPCHK(hipFuncSetAttribute((const void*)phi_fft_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                kMaxLdsBytes));
     c->use_fft = true;
+    setup_ndft(c);
   } else {
     c->use_fft = false;
     ensure_gemm(c);
@@ -1308,6 +1466,7 @@ struct SolveRun {
     c->last_its = -1;
     if (c->f2) f2_prepare_solve(c);
     else if (!c->use_fft) autotune(c, fp64);
+    else if (!fp64) autotune_theta32(c);
     precond = o.use_precond != 0 && !(c->f2 && c->f2->precond == WDG2_PRECOND_NONE);
     if (!c->h_st) HIPCHK(hipHostMalloc((void**)&c->h_st, sizeof(GmresState), hipHostMallocDefault));
     xs = (c->rhs_norm > 0) ? c->rhs_norm / c->g.scale : 1.0;  // physical x = xs * x_device
@@ -2308,15 +2467,24 @@ int wdg_set_fft_block(wdg_ctx* ctx, int32_t mode) {
   if (!ctx) return fail(nullptr, WDG_ERR_ARG, "null argument");
   if (ctx->f2) return fail(ctx, WDG_ERR_UNSUPPORTED, "not available for a full-2D context");
   return guarded(ctx, [&]() {
-    if (mode != WDG_FFT_BLOCK_AUTO && mode != WDG_FFT_BLOCK_256 && mode != WDG_FFT_BLOCK_WIDE)
+    if (mode != WDG_FFT_BLOCK_AUTO && mode != WDG_FFT_BLOCK_256 && mode != WDG_FFT_BLOCK_WIDE &&
+        mode != WDG_FFT_BLOCK_NDFT)
       throw std::invalid_argument("unknown FFT block mode");
     if (!ctx->fft_ok) throw std::invalid_argument("the chirp-z FFT formulation does not apply to this grid");
     if (mode == WDG_FFT_BLOCK_WIDE && ctx->theta32_wide == 0)
       throw std::invalid_argument("the wide FP32 Theta block does not fit this grid (n_k_chi > 5, fewer than "
                                   "four transforms per round, or LDS)");
+    if (mode == WDG_FFT_BLOCK_NDFT && !ctx->ndft_ok)
+      throw std::invalid_argument("the near-DFT FP32 Theta kernel does not apply to this grid (n_xi != nk, nk not "
+                                  "64 * 2^a 3^b, n_k_chi > 5, or LDS)");
+    const bool ndft = mode == WDG_FFT_BLOCK_NDFT;
     const bool wide = mode == WDG_FFT_BLOCK_WIDE || (mode == WDG_FFT_BLOCK_AUTO && ctx->theta32_wide_auto);
     const int blk = wide ? ctx->theta32_wide : ctx->theta32_base;
-    if (blk != ctx->theta_block32) clear_graphs(ctx);   // captured cycles hold the launch shape
+    // captured cycles hold the kernel and its launch shape
+    if (blk != ctx->theta_block32 || ndft != ctx->use_ndft) clear_graphs(ctx);
+    ctx->use_ndft = ndft;
+    ctx->theta32_auto = mode == WDG_FFT_BLOCK_AUTO;
+    ctx->theta32_tuned = false;
     ctx->theta_block32 = blk;
     ctx->lds32 = wide ? ctx->lds32_wide : ctx->lds32_base;
     return WDG_OK;
@@ -2325,7 +2493,7 @@ int wdg_set_fft_block(wdg_ctx* ctx, int32_t mode) {
 
 int wdg_fft_block(const wdg_ctx* ctx) {
   if (!ctx || !ctx->fft_ok) return -1;
-  return ctx->theta_block32;
+  return ctx->use_ndft ? ctx->ndft_block : ctx->theta_block32;
 }
 
 int wdg_theta_method(const wdg_ctx* ctx) {

@@ -267,17 +267,20 @@ class WignerOperator:
     def theta_method(self) -> str:
         return {_lib.WDG_THETA_FFT: "fft", _lib.WDG_THETA_GEMM: "gemm"}[self._lib.wdg_theta_method(self._h)]
 
-    FFT_BLOCKS = {"auto": _lib.WDG_FFT_BLOCK_AUTO, "256": _lib.WDG_FFT_BLOCK_256, "wide": _lib.WDG_FFT_BLOCK_WIDE}
+    FFT_BLOCKS = {"auto": _lib.WDG_FFT_BLOCK_AUTO, "256": _lib.WDG_FFT_BLOCK_256, "wide": _lib.WDG_FFT_BLOCK_WIDE,
+                  "ndft": _lib.WDG_FFT_BLOCK_NDFT}
 
     def set_fft_block(self, mode: str):
-        """Workgroup shape of the FP32 chirp-z kernel: 'auto', '256' or 'wide'."""
+        """Kernel of the FP32 FFT Theta apply: 'auto', the chirp-z kernel with
+        '256' or 'wide' workgroups, or 'ndft' (length-nk transforms with the
+        truncated Taylor factor, square grids)."""
         if mode not in self.FFT_BLOCKS:
             raise ValueError(f"mode must be one of {sorted(self.FFT_BLOCKS)}")
         check(self._lib.wdg_set_fft_block(self._h, self.FFT_BLOCKS[mode]), self._h)
 
     @property
     def fft_block(self) -> int:
-        """Threads per workgroup of the FP32 chirp-z kernel (-1: no FFT plan)."""
+        """Threads per workgroup of the FP32 FFT Theta kernel in use (-1: no FFT plan)."""
         return self._lib.wdg_fft_block(self._h)
 
     # ---- profiling ----

@@ -8,8 +8,12 @@ Each shape selects a different chirp-z plan of the fused Theta kernel:
   256 x 4 x 512   (the self-consistent RTD)               P = 1024, compile-time plan
   128 x 4 x 768   (a bias-sweep instance)                 P = 1536, run-time plan
   128 x 4 x 256   (a bias-sweep instance)                 P = 512,  compile-time plan
-and the FP32 kernel is run in both workgroup shapes (256 threads looping over
-the nodes, and the ~1024-thread wide block with its aliased LDS buffers).
+and the FP32 apply is run in all three kernel forms: the chirp-z kernel in both
+workgroup shapes (256 threads looping over the nodes, and the ~1024-thread
+wide block with its aliased LDS buffers) and the near-DFT kernel (length-nk
+transforms, nk = 1024/512/768/256 -> compile-time/compile-time/run-time/
+compile-time plan, two Taylor terms of exp(i s ubar vbar) for nk >= 512 and
+three for nk = 256).
 Bounds: 1e-13 relative for the FP64 apply, 1e-12 for its Theta part alone,
 1e-5 for the FP32 (mixed inner) apply and its Theta part.
 """
@@ -67,14 +71,17 @@ def test_fp64_apply_matches_oracle(big):
     assert tr < 1e-13
 
 
-@pytest.mark.parametrize("block", ["256", "wide"])
+@pytest.mark.parametrize("block", ["256", "wide", "ndft"])
 def test_fp32_apply_matches_oracle(big, block):
     op = big["op"]
     op.set_theta_method("fft")
     op.set_fft_block(block)
     threads = op.fft_block
     tt = big["P"] // 8
-    assert threads == (tt * max(1, 256 // tt) if block == "256" else tt * (1024 // tt))
+    if block == "ndft":   # one group of nk/8 threads per DG node
+        assert threads == 4 * (big["p"]["Nk"] // 8)
+    else:
+        assert threads == (tt * max(1, 256 // tt) if block == "256" else tt * (1024 // tt))
     full, theta, tr = _errs(big, "mixed")
     op.set_fft_block("auto")
     print(f"fp32 apply ({threads} threads) rel err {full:.2e}, Theta part {theta:.2e}, transport part {tr:.2e}")
