@@ -594,6 +594,65 @@ int wdg2_set_line_precond(wdg_ctx* ctx, const wdg2_line_factors* factors);
 int wdg2_observables(wdg_ctx* ctx, const double* rho, const int32_t* idx, const double* w, const int32_t* counts,
                      double* out);
 
+/* ---- full-2D electrostatics on the device ----
+ * The Poisson problem of the self-consistent full-2D loop on the material
+ * grid (RTD/eval_Poisson2D_rtd.m:1-216, RTD/solve_Poisson2D_rtd.m:85-130)
+ * and the conservative DG -> dual-volume density projection
+ * (RTD/project_DG_density_RTD.m:1-69), so that the loop's density,
+ * projection and Newton update stay in HBM between transport solves.
+ * Unique material nodes u = ix + nx*iy, iy < nu = ny - periodic_y (the
+ * periodic duplicate row is the caller's to expand).  The operator A is
+ * M of eval_Poisson2D_rtd.m restricted to the unique rows with the
+ * duplicate column folded onto row 0, given as COO over unique nodes
+ * (volts, x and y in nm: the factor 1e18 of eps0/dx^2); its coupling
+ * must stay within one material column or the neighbouring ones (bandwidth
+ * nu in the device's y-fastest order), which the 5-point stencil does. */
+typedef struct wdg2_poisson_desc {
+  int32_t nx, nu;               /* material nodes along x, unique nodes along y              */
+  int32_t n_entries;            /* nonzeros of A                                             */
+  const int32_t* a_row;         /* n_entries unique-node rows                                */
+  const int32_t* a_col;         /* n_entries unique-node columns                             */
+  const double* a_val;          /* n_entries values                                          */
+  const int32_t* node_type;     /* nx*nu: 1 free, 2 Dirichlet                                */
+  const double* net_doping;     /* nx*nu: Nd - Na [m^-3]                                     */
+  double temp;                  /* [K] (Boltzmann Jacobian of the Newton update)            */
+  int32_t x_dof, y_dof;         /* centre DOFs of the full-2D contexts along X and Y        */
+  const double* proj_x;         /* nx*x_dof row-major: integral of each X basis function
+                                   over the dual interval of material node ix               */
+  const double* proj_y;         /* nu*y_dof row-major: the same along Y (periodic: the two
+                                   endpoint half intervals merged into row 0)                */
+  const double* volume;         /* nx*nu dual volumes (index ix + nx*iy)                    */
+  int32_t n_weights;            /* zero-separation density weights, <= 8                    */
+  const int32_t* w_idx;         /* relative index of each weight (get_Observables_full2D.m) */
+  const double* w_val;          /* weight values                                             */
+} wdg2_poisson_desc;
+
+/* Builds the Poisson data on the context (a full-2D context; the contexts
+ * passed to wdg2_poisson_step must share its grid).  WDG_ERR_ARG for
+ * sizes, indices or couplings outside the description above. */
+int wdg2_poisson_setup(wdg_ctx* ctx, const wdg2_poisson_desc* desc);
+
+typedef struct wdg2_poisson_info {
+  double physical_relres_before; /* ||F_free||_inf / scale before the update               */
+  double physical_relres_after;  /* the same at the (mixed) updated potential              */
+  double linear_relres;          /* ||J delta + F||_inf / ||F||_inf of the banded solve    */
+  double unmixed_step;           /* ||delta||_inf [V]                                       */
+} wdg2_poisson_info;
+
+/* One FP64 Poisson update on the device (solve_Poisson2D_rtd.m:85-130):
+ * the density is the sum over ctxs[0..n-1] of their last solutions'
+ * zero-separation densities (n = 1 for one transport instance, the
+ * transverse modes of a device otherwise), projected onto the dual
+ * volumes; v = phi_in with the Dirichlet values bc on fixed nodes,
+ * F = A v - q (n - net), J = A (newton = 0, 'direct') or
+ * A - diag(q^2 n / kT) (newton = 1), delta = -J^-1 F by a banded LU
+ * without pivoting (J is an M-matrix up to sign), phi_out = v + alpha delta
+ * on free nodes.  phi_in, bc, phi_out, cand_out (v + delta, may be NULL)
+ * and n_out (projected density [m^-3], may be NULL) hold nx*nu values.
+ * The Poisson data are ctxs[0]'s (wdg2_poisson_setup). */
+int wdg2_poisson_step(wdg_ctx* const* ctxs, int32_t n, const double* phi_in, const double* bc, int32_t newton,
+                      double alpha, double* phi_out, double* cand_out, double* n_out, wdg2_poisson_info* info);
+
 #ifdef __cplusplus
 }
 #endif

@@ -182,6 +182,20 @@ class wdg2_line_factors(C.Structure):
                 ("t_inverse", C.POINTER(C.c_double))]
 
 
+class wdg2_poisson_desc(C.Structure):
+    _fields_ = [("nx", C.c_int32), ("nu", C.c_int32), ("n_entries", C.c_int32),
+                ("a_row", C.POINTER(C.c_int32)), ("a_col", C.POINTER(C.c_int32)), ("a_val", C.POINTER(C.c_double)),
+                ("node_type", C.POINTER(C.c_int32)), ("net_doping", C.POINTER(C.c_double)), ("temp", C.c_double),
+                ("x_dof", C.c_int32), ("y_dof", C.c_int32), ("proj_x", C.POINTER(C.c_double)),
+                ("proj_y", C.POINTER(C.c_double)), ("volume", C.POINTER(C.c_double)), ("n_weights", C.c_int32),
+                ("w_idx", C.POINTER(C.c_int32)), ("w_val", C.POINTER(C.c_double))]
+
+
+class wdg2_poisson_info(C.Structure):
+    _fields_ = [("physical_relres_before", C.c_double), ("physical_relres_after", C.c_double),
+                ("linear_relres", C.c_double), ("unmixed_step", C.c_double)]
+
+
 # name -> (restype, argtypes)
 _P = C.c_void_p
 _D = C.POINTER(C.c_double)
@@ -226,6 +240,9 @@ SIGNATURES = {
     "wdg2_set_precond": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_int32)]),
     "wdg2_set_line_precond": (C.c_int, [_P, C.POINTER(wdg2_line_factors)]),
     "wdg2_observables": (C.c_int, [_P, _D, C.POINTER(C.c_int32), _D, C.POINTER(C.c_int32), _D]),
+    "wdg2_poisson_setup": (C.c_int, [_P, C.POINTER(wdg2_poisson_desc)]),
+    "wdg2_poisson_step": (C.c_int, [C.POINTER(_P), C.c_int32, _D, _D, C.c_int32, C.c_double, _D, _D, _D,
+                                    C.POINTER(wdg2_poisson_info)]),
     "wdg_set_profiling": (C.c_int, [_P, C.c_uint32]),
     "wdg_profile": (C.c_int, [_P, C.c_char_p, C.c_int32, _D, C.POINTER(C.c_int64), _D, C.c_int32,
                               C.POINTER(C.c_int32)]),

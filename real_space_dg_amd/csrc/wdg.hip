@@ -35,6 +35,7 @@
 #include "theta_fft.hpp"
 #include "full2d.hpp"
 #include "reservoir.hpp"
+#include "poisson2d.hpp"
 
 using namespace wdg;
 
@@ -124,8 +125,18 @@ const char* kProfNames[P_COUNT] = {"transport",   "theta_gemm1",    "theta_combi
 
 struct F2Data;   // full-2D operator data (full2d_host.hpp)
 
+// Device data of the full-2D Poisson update (wdg2_poisson_setup).
+struct P2Data {
+  int nx = 0, nu = 0, n = 0, b = 0, xdof = 0, ydof = 0, ncen = 0, nrel = 0;
+  double temp = 300;
+  P2W w{};
+  DevBuf<double> A, J, net, px, py, vol, ndg, nmat, v, f, chg, phi_in, bc, phi_out, cand, info;
+  DevBuf<int> type, pxr, pyr;
+};
+
 struct wdg_ctx {
   std::shared_ptr<F2Data> f2;         // set for full-2D contexts
+  std::shared_ptr<P2Data> p2;         // full-2D Poisson data (wdg2_poisson_setup)
   // shifted operator alpha I + beta A_hat (Crank-Nicolson steps)
   bool shift_on = false;
   double shift_a = 1.0, shift_b = 0.0;
@@ -2811,6 +2822,145 @@ int wdg2_observables(wdg_ctx* ctx, const double* rho, const int32_t* idx, const 
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(out, ctx->d_obs.p, (size_t)3 * f.ncen * 16, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
+    return WDG_OK;
+  });
+}
+
+
+int wdg2_poisson_setup(wdg_ctx* ctx, const wdg2_poisson_desc* d) {
+  if (!ctx || !d) return fail(ctx, WDG_ERR_ARG, "null argument");
+  if (!ctx->f2) return fail(ctx, WDG_ERR_UNSUPPORTED, "not a full-2D context");
+  return guarded(ctx, [&]() {
+    F2Data& f = *ctx->f2;
+    const int nx = d->nx, nu = d->nu;
+    if (nx < 2 || nu < 1 || d->n_entries < 1 || !d->a_row || !d->a_col || !d->a_val || !d->node_type ||
+        !d->net_doping || !d->proj_x || !d->proj_y || !d->volume || !(d->temp > 0))
+      throw std::invalid_argument("Poisson description: sizes, arrays and temperature required");
+    if (d->x_dof != f.nxd || d->y_dof != f.nyd)
+      throw std::invalid_argument("Poisson description: x_dof, y_dof must match the context's centre DOFs");
+    if (d->n_weights < 1 || d->n_weights > kP2MaxWeights || !d->w_idx || !d->w_val)
+      throw std::invalid_argument("Poisson description: 1..8 density weights required");
+    auto p = std::make_shared<P2Data>();
+    P2Data& q = *p;
+    q.nx = nx;
+    q.nu = nu;
+    q.n = nx * nu;
+    q.b = nu;
+    q.xdof = d->x_dof;
+    q.ydof = d->y_dof;
+    q.ncen = f.ncen;
+    q.nrel = f.nrel;
+    q.temp = d->temp;
+    q.w.cnt = d->n_weights;
+    for (int j = 0; j < d->n_weights; ++j) {
+      if (d->w_idx[j] < 0 || d->w_idx[j] >= f.nrel) throw std::invalid_argument("density weight index out of range");
+      q.w.idx[j] = d->w_idx[j];
+      q.w.w[j] = d->w_val[j];
+    }
+    const int n = q.n, b = q.b, W = 2 * b + 1;
+    auto korder = [&](int u) { return (u / nx) + nu * (u % nx); };   // u = ix + nx iy -> k = iy + nu ix
+    std::vector<double> band((size_t)n * W, 0.0);
+    for (int e = 0; e < d->n_entries; ++e) {
+      const int r = d->a_row[e], c = d->a_col[e];
+      if (r < 0 || r >= n || c < 0 || c >= n) throw std::invalid_argument("Poisson operator index out of range");
+      const int kr = korder(r), kc = korder(c);
+      if (std::abs(kr - kc) > b)
+        throw std::invalid_argument("Poisson operator couples nodes beyond neighbouring material columns");
+      band[(size_t)kr * W + (kc - kr + b)] += d->a_val[e];
+    }
+    std::vector<int> type(n);
+    std::vector<double> net(n);
+    for (int u = 0; u < n; ++u) {
+      const int t = d->node_type[u];
+      if (t != 1 && t != 2) throw std::invalid_argument("Poisson node types must be 1 (free) or 2 (Dirichlet)");
+      type[korder(u)] = t;
+      net[korder(u)] = d->net_doping[u];
+    }
+    // nonzero column range of each projection row
+    auto ranges = [](const double* m, int rows, int cols) {
+      std::vector<int> r(2 * rows);
+      for (int i = 0; i < rows; ++i) {
+        int a = cols, z = 0;
+        for (int j = 0; j < cols; ++j)
+          if (m[(size_t)i * cols + j] != 0.0) {
+            a = std::min(a, j);
+            z = j + 1;
+          }
+        r[2 * i] = std::min(a, z);
+        r[2 * i + 1] = z;
+      }
+      return r;
+    };
+    for (int u = 0; u < n; ++u)
+      if (!(d->volume[u] > 0)) throw std::invalid_argument("Poisson dual volumes must be positive");
+    auto up = [&](DevBuf<double>& buf, const double* v, size_t cnt) {
+      buf.alloc(cnt);
+      HIPCHK(hipMemcpy(buf.p, v, cnt * sizeof(double), hipMemcpyHostToDevice));
+    };
+    auto upi = [&](DevBuf<int>& buf, const std::vector<int>& v) {
+      buf.alloc(v.size());
+      HIPCHK(hipMemcpy(buf.p, v.data(), v.size() * sizeof(int), hipMemcpyHostToDevice));
+    };
+    up(q.A, band.data(), band.size());
+    up(q.net, net.data(), n);
+    upi(q.type, type);
+    up(q.px, d->proj_x, (size_t)nx * q.xdof);
+    up(q.py, d->proj_y, (size_t)nu * q.ydof);
+    upi(q.pxr, ranges(d->proj_x, nx, q.xdof));
+    upi(q.pyr, ranges(d->proj_y, nu, q.ydof));
+    up(q.vol, d->volume, n);
+    q.J.alloc((size_t)n * W);
+    for (DevBuf<double>* buf : {&q.nmat, &q.v, &q.f, &q.chg, &q.phi_in, &q.bc, &q.phi_out, &q.cand}) buf->alloc(n);
+    q.ndg.alloc(q.ncen);
+    q.info.alloc(4);
+    ctx->p2 = p;
+    return WDG_OK;
+  });
+}
+
+int wdg2_poisson_step(wdg_ctx* const* ctxs, int32_t n, const double* phi_in, const double* bc, int32_t newton,
+                      double alpha, double* phi_out, double* cand_out, double* n_out, wdg2_poisson_info* info) {
+  if (!ctxs || n < 1 || !ctxs[0] || !phi_in || !bc || !phi_out || !info) return fail(nullptr, WDG_ERR_ARG, "null argument");
+  wdg_ctx* c0 = ctxs[0];
+  if (!c0->p2) return fail(c0, WDG_ERR_STATE, "wdg2_poisson_setup has not been called on the first context");
+  if (!std::isfinite(alpha) || !(alpha > 0) || alpha > 1) return fail(c0, WDG_ERR_ARG, "alpha must lie in (0, 1]");
+  for (int i = 0; i < n; ++i) {
+    if (!ctxs[i] || !ctxs[i]->f2) return batch_fail(ctxs[i], i, WDG_ERR_ARG, "every context must be a full-2D context");
+    if (!ctxs[i]->have_solution) return batch_fail(ctxs[i], i, WDG_ERR_STATE, "no solution on the context yet");
+    if (ctxs[i]->f2->ncen != c0->p2->ncen || ctxs[i]->f2->nrel != c0->p2->nrel)
+      return batch_fail(ctxs[i], i, WDG_ERR_ARG, "the contexts must share the Poisson context's grid");
+  }
+  return guarded(c0, [&]() {
+    P2Data& q = *c0->p2;
+    const int m = q.n;
+    for (int u = 0; u < m; ++u)
+      if (!std::isfinite(phi_in[u]) || !std::isfinite(bc[u])) throw std::invalid_argument("phi_in and bc must be finite");
+    // the summed density of the contexts' solutions (their streams drained first)
+    for (int i = 0; i < n; ++i) {
+      HIPCHK(hipStreamSynchronize(ctxs[i]->stream));
+      hipLaunchKernelGGL(p2_density_kernel, dim3((q.ncen + 255) / 256), dim3(256), 0, c0->stream, ctxs[i]->d_x.p,
+                         q.ncen, q.nrel, q.w, ctxs[i]->sol_scale, i > 0 ? 1 : 0, q.ndg.p);
+    }
+    hipLaunchKernelGGL(p2_project_kernel, dim3((m + 255) / 256), dim3(256), 0, c0->stream, q.ndg.p, q.nx, q.nu,
+                       q.xdof, q.ydof, q.px.p, q.pxr.p, q.py.p, q.pyr.p, q.vol.p, q.nmat.p);
+    HIPCHK(hipMemcpyAsync(q.phi_in.p, phi_in, m * sizeof(double), hipMemcpyHostToDevice, c0->stream));
+    HIPCHK(hipMemcpyAsync(q.bc.p, bc, m * sizeof(double), hipMemcpyHostToDevice, c0->stream));
+    P2Args a{q.nx, q.nu, q.n, q.b, q.A.p, q.type.p, q.net.p, q.nmat.p, q.phi_in.p, q.bc.p, q.J.p, q.v.p, q.f.p,
+             q.chg.p, q.phi_out.p, q.cand.p, q.info.p,
+             newton ? 1.602176634e-19 * 1.602176634e-19 / (1.380649e-23 * q.temp) : 0.0, alpha};
+    hipLaunchKernelGGL((p2_newton_kernel<256>), dim3(1), dim3(256), 0, c0->stream, a);
+    HIPCHK(hipGetLastError());
+    double h[4];
+    HIPCHK(hipMemcpyAsync(phi_out, q.phi_out.p, m * sizeof(double), hipMemcpyDeviceToHost, c0->stream));
+    if (cand_out) HIPCHK(hipMemcpyAsync(cand_out, q.cand.p, m * sizeof(double), hipMemcpyDeviceToHost, c0->stream));
+    if (n_out) HIPCHK(hipMemcpyAsync(n_out, q.nmat.p, m * sizeof(double), hipMemcpyDeviceToHost, c0->stream));
+    HIPCHK(hipMemcpyAsync(h, q.info.p, sizeof h, hipMemcpyDeviceToHost, c0->stream));
+    HIPCHK(hipStreamSynchronize(c0->stream));
+    info->physical_relres_before = h[0];
+    info->physical_relres_after = h[1];
+    info->linear_relres = h[2];
+    info->unmixed_step = h[3];
+    if (!std::isfinite(h[2]) || !std::isfinite(h[3])) throw StatusError(WDG_ERR_NUMERIC, "non-finite Poisson update");
     return WDG_OK;
   });
 }

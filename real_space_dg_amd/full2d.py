@@ -23,6 +23,7 @@ operator on the CPU; there is no CPU fallback.
 from __future__ import annotations
 
 import ctypes as C
+import dataclasses
 import math
 from dataclasses import dataclass, field
 
@@ -1172,13 +1173,9 @@ def gmres_basis_guard(params: dict, n_total: int, restart: int, pmode: str):
     return est, limit
 
 
-def solve_transport_DG_full2D(mat: dict, Vxy=None, EfL=None, EfR=None, rho0=None,
-                              solver: SolveOptions | None = None) -> Full2DResult:
-    """Stationary full-2D solve with the reference's signature and outputs
-    (solve_transport_DG_full2D.m:1,67-129): GMRES on the device with the
-    preconditioner named by full2D_preconditioner (auto/jacobi/rowabs/none),
-    tolerance full2D_solverTol, restart full2D_gmresRestart."""
-    setup = Full2DSetup(mat, Vxy, EfL, EfR, host_reservoirs=False)
+def _transport_options(setup, solver: SolveOptions | None):
+    """Preconditioner mode and GMRES options of a full-2D solve from the
+    setup's parameters (solve_transport_DG_full2D.m:132-308)."""
     params = setup.params
     if str(read_param(params, "full2D_solver", "gmres")).lower() not in ("gmres", "auto"):
         raise NotImplementedError("DG:Full2D:Unsupported: only the GMRES solver runs on the device")
@@ -1192,30 +1189,37 @@ def solve_transport_DG_full2D(mat: dict, Vxy=None, EfL=None, EfR=None, rho0=None
         restart = int(read_param(params, "full2D_gmresRestart", read_param(params, "full2D_gpuGmresRestart", 20)))
         solver = SolveOptions(tol=float(read_param(params, "full2D_solverTol", 1e-4)), restart=restart,
                               max_iter=int(read_param(params, "full2D_solverMaxIt", 1400)) * restart)
-    # both checks come before the device operator is built, as in the reference
-    if rho0 is not None:
-        rho0 = validate_initial_rho(rho0, setup.n_total)
-    _, basis_limit = gmres_basis_guard(params, setup.n_total, max(solver.restart, solver.restart_max), pmode)
-    with Full2DOperator(setup) as op:
-        if np.isfinite(basis_limit):
-            check(op._lib.wdg_set_memory_limits(op.ctx, basis_limit,
-                                                float(read_param(params, "full2D_lineMaxFactorBytes", 4 * 1024 ** 3))),
-                  op.ctx)
-        if pmode == "line-transverse":
-            op.set_line_transverse()
-            used = pmode
-        else:
-            used = op.set_preconditioner(pmode)
-        solver.precondition = used != "none"
-        if set(setup.mode_data) == {"source", "drain"}:
-            op.upload_contact_rhs()      # contact-mode reservoirs and the inflow rhs built on the device
-        else:
-            op.upload_rhs()
-        rho, info = op.solve(solver, x0=rho0)
-        # resvec * ||rhs|| as the reference reports it (solve_transport_DG_full2D.m:324,333)
-        resvec = op.resvec() * (op.rhs_norm if op.rhs_norm > 0 else 1.0)
-        nDG, jxDG, jyDG = op.observables()
-    target_y = setup.grid_y if "Ny" in mat else setup.Y.nodes
+    return pmode, solver
+
+
+def _set_preconditioner(op, pmode: str) -> str:
+    if pmode == "line-transverse":
+        op.set_line_transverse()
+        return pmode
+    return op.set_preconditioner(pmode)
+
+
+def _configure(op, setup, pmode: str, basis_limit: float) -> str:
+    """Memory guards, preconditioner and inflow rhs of a fresh operator."""
+    params = setup.params
+    if np.isfinite(basis_limit):
+        check(op._lib.wdg_set_memory_limits(op.ctx, basis_limit,
+                                            float(read_param(params, "full2D_lineMaxFactorBytes", 4 * 1024 ** 3))),
+              op.ctx)
+    used = _set_preconditioner(op, pmode)
+    if set(setup.mode_data) == {"source", "drain"}:
+        op.upload_contact_rhs()      # contact-mode reservoirs and the inflow rhs built on the device
+    else:
+        op.upload_rhs()
+    return used
+
+
+def _result(op, setup, rho, info, warm: bool, used: str) -> Full2DResult:
+    """The reference's outputs of one solve (solve_transport_DG_full2D.m:67-129,310-334)."""
+    # resvec * ||rhs|| as the reference reports it (solve_transport_DG_full2D.m:324,333)
+    resvec = op.resvec() * (op.rhs_norm if op.rhs_norm > 0 else 1.0)
+    nDG, jxDG, jyDG = op.observables()
+    target_y = setup.grid_y if "Ny" in setup.mat else setup.Y.nodes
     n = project_to_material_y(nDG, setup.Y.nodes, target_y)
     jx = project_to_material_y(jxDG, setup.Y.nodes, target_y)
     jy = project_to_material_y(jyDG, setup.Y.nodes, target_y)
@@ -1223,6 +1227,72 @@ def solve_transport_DG_full2D(mat: dict, Vxy=None, EfL=None, EfR=None, rho0=None
     status = "solved-gmres" if info.flag == 0 else "gmres-not-converged"
     return Full2DResult(status, rho, n, jx, jy, j, nDG, jxDG, jyDG,
                         {"solve": {"flag": info.flag, "relres": info.true_relres, "trueRelres": info.true_relres,
-                                   "initialTrueRelres": info.initial_true_relres, "warmStartUsed": rho0 is not None,
+                                   "initialTrueRelres": info.initial_true_relres, "warmStartUsed": warm,
                                    "iterations": info.iterations, "resvec": resvec, "preconditioner": used,
                                    "seconds": info.seconds}}, setup)
+
+
+def solve_transport_DG_full2D(mat: dict, Vxy=None, EfL=None, EfR=None, rho0=None,
+                              solver: SolveOptions | None = None) -> Full2DResult:
+    """Stationary full-2D solve with the reference's signature and outputs
+    (solve_transport_DG_full2D.m:1,67-129): GMRES on the device with the
+    preconditioner named by full2D_preconditioner (auto/jacobi/rowabs/none),
+    tolerance full2D_solverTol, restart full2D_gmresRestart."""
+    setup = Full2DSetup(mat, Vxy, EfL, EfR, host_reservoirs=False)
+    pmode, solver = _transport_options(setup, solver)
+    # both checks come before the device operator is built, as in the reference
+    if rho0 is not None:
+        rho0 = validate_initial_rho(rho0, setup.n_total)
+    _, basis_limit = gmres_basis_guard(setup.params, setup.n_total, max(solver.restart, solver.restart_max), pmode)
+    with Full2DOperator(setup) as op:
+        used = _configure(op, setup, pmode, basis_limit)
+        solver.precondition = used != "none"
+        rho, info = op.solve(solver, x0=rho0)
+        return _result(op, setup, rho, info, rho0 is not None, used)
+
+
+class TransportSession:
+    """One device operator for the Gummel iterations of one bias point of
+    the self-consistent loop: between iterations only the potential changes
+    (set_potential and a rebuilt preconditioner), while the contact data
+    (reservoirs, inflow rhs) stay valid because the contacts hold phi = 0 and
+    the bias; each solve warm-starts from the previous device solution, as
+    the loop's rho0 = rho of the previous iteration
+    (RTD/solve2D_rtd_self_consistent.m:116-121).  Outputs per solve as
+    solve_transport_DG_full2D, without copying rho to the host."""
+
+    def __init__(self, mat: dict, Vxy, EfL, EfR, rho0=None, solver: SolveOptions | None = None):
+        self.setup = Full2DSetup(mat, Vxy, EfL, EfR, host_reservoirs=False)
+        self.pmode, self.solver = _transport_options(self.setup, solver)
+        self.rho0 = None if rho0 is None else validate_initial_rho(rho0, self.setup.n_total)
+        _, limit = gmres_basis_guard(self.setup.params, self.setup.n_total,
+                                     max(self.solver.restart, self.solver.restart_max), self.pmode)
+        self.op = Full2DOperator(self.setup)
+        self.used = _configure(self.op, self.setup, self.pmode, limit)
+        self.solves = 0
+
+    def solve(self, Vxy=None) -> Full2DResult:
+        if Vxy is not None:
+            self.op.set_potential(Vxy)
+            self.used = _set_preconditioner(self.op, self.pmode)
+        opts = dataclasses.replace(self.solver, precondition=self.used != "none", warm_device=self.solves > 0)
+        x0 = self.rho0 if self.solves == 0 else None
+        _, info = self.op.solve(opts, x0=x0, return_x=False)
+        warm = self.solves > 0 or x0 is not None
+        self.solves += 1
+        return _result(self.op, self.setup, None, info, warm, self.used)
+
+    def solution(self) -> np.ndarray:
+        """The device solution in physical units (host copy)."""
+        x = np.empty(self.op.n, np.complex128)
+        check(self.op._lib.wdg_solution(self.op.ctx, dptr(x)), self.op.ctx)
+        return x
+
+    def close(self):
+        self.op.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

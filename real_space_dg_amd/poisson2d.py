@@ -1,5 +1,4 @@
-"""2D electrostatics and the self-consistent full-2D RTD loop (host FP64
-around the device transport solve).
+"""2D electrostatics and the self-consistent full-2D RTD loop.
 
 * `eval_poisson2d` assembles M ~ div(eps grad phi) on the material grid
   with harmonic-mean face permittivities, half control volumes at free X
@@ -15,10 +14,17 @@ around the device transport solve).
 * `transient_cn_self_consistent` interleaves Crank-Nicolson steps of the
   device operator with a Poisson update every few steps.
 
-The Poisson grid holds Nx*Ny (C5: 4,693) unknowns, so its sparse LU runs on
-the host; every transport solve runs on the device.
+`DevicePoisson2D` runs the projection and the update on the device (banded
+FP64 LU of the 5-point operator in one workgroup, wdg2_poisson_step) from the
+density of the operators' device solutions, and every loop uses it: the
+stationary loop keeps one device operator per bias point
+(full2d.TransportSession), the time-dependent loops their operators, so
+between transport solves only the potential crosses the boundary.
+`project_dg_density` and `poisson_step` are the same steps in host FP64.
 """
 from __future__ import annotations
+
+import ctypes as C
 
 import numpy as np
 import scipy.sparse as sp
@@ -250,6 +256,115 @@ def poisson_step(M, types, n, mat, phi0, bias: float, method: str = "direct", al
     return phi, info
 
 
+class DevicePoisson2D:
+    """The Poisson update of `poisson_step` and the projection of
+    `project_dg_density` on the device (wdg2_poisson_setup /
+    wdg2_poisson_step): the density comes from the device solutions of
+    full-2D operators, so a self-consistent loop keeps its transport states
+    in HBM and only the potential (nx*ny values) passes the boundary.  The
+    Poisson data live on `op`, whose grid every operator of a step shares."""
+
+    def __init__(self, op, mat=None):
+        from . import _lib as L
+        from .solver import check
+        self._lib, self._check, self.op = L.load(), check, op
+        setup = op.setup
+        mat = setup.mat if mat is None else mat
+        self.mat = mat
+        M, types, periodic = eval_poisson2d(mat)
+        nx, ny = int(mat["Nx"]), int(mat["Ny"])
+        nu = ny - int(periodic)
+        self.nx, self.ny, self.nu, self.periodic = nx, ny, nu, periodic
+        neumann = x_boundary(mat) == "neumann"
+        ut = types[:, :nu].ravel(order="F")
+        free, fixed = np.nonzero(ut == 1)[0], np.nonzero(ut == 2)[0]
+        if free.size + fixed.size != nx * nu or (not neumann and fixed.size == 0) or (neumann and fixed.size):
+            raise ValueError("RTD:PoissonNodeMap: invalid or ungrounded Poisson node types")
+        self.neumann = neumann
+        mp = np.arange(nx * nu).reshape(nu, nx).T
+        if periodic:
+            mp = np.concatenate([mp, mp[:, :1]], axis=1)
+        N = nx * ny
+        expand = sp.csr_matrix((np.ones(N), (np.arange(N), mp.ravel(order="F"))), shape=(N, nx * nu))
+        A = (M[:nx * nu] @ expand).tocoo()
+        Nd, Na = doping_si(mat)
+        if abs(setup.coord_scale - 1e-9) > 1e-20:
+            raise ValueError("RTD:PoissonCoordinateUnits: the RTD Poisson material coordinates must be in nm")
+        Tx, wx = _integration_map(setup.X, np.ravel(mat["x"]) * 1e-9)
+        Ty, wy = _integration_map(setup.Y, np.ravel(mat["y"]) * 1e-9)
+        if periodic:
+            Ty[0] += Ty[-1]
+            wy = wy.copy()
+            wy[0] += wy[-1]
+            Ty, wy = Ty[:-1], wy[:-1]
+        wn = setup.observable_weights()[0]
+        if not 1 <= len(wn) <= 8:
+            raise ValueError("1..8 zero-separation density weights expected")
+        self._keep = dict(row=np.ascontiguousarray(A.row, np.int32), col=np.ascontiguousarray(A.col, np.int32),
+                          val=np.ascontiguousarray(A.data, np.float64),
+                          types=np.ascontiguousarray(ut, np.int32),
+                          net=np.ascontiguousarray((Nd - Na)[:, :nu].ravel(order="F"), np.float64),
+                          px=np.ascontiguousarray(Tx, np.float64), py=np.ascontiguousarray(Ty, np.float64),
+                          vol=np.ascontiguousarray(np.outer(wx, wy).ravel(order="F"), np.float64),
+                          widx=np.ascontiguousarray([j for j, _ in wn], np.int32),
+                          wval=np.ascontiguousarray([w for _, w in wn], np.float64))
+        k = self._keep
+        i32 = lambda a: a.ctypes.data_as(C.POINTER(C.c_int32))
+        d = L.wdg2_poisson_desc(nx, nu, int(k["row"].size), i32(k["row"]), i32(k["col"]), L.dptr(k["val"]),
+                                i32(k["types"]), L.dptr(k["net"]), float(mat["Temp"]), setup.X.ndof, setup.Y.ndof,
+                                L.dptr(k["px"]), L.dptr(k["py"]), L.dptr(k["vol"]), int(k["widx"].size),
+                                i32(k["widx"]), L.dptr(k["wval"]))
+        check(self._lib.wdg2_poisson_setup(op.ctx, C.byref(d)), op.ctx)
+
+    def _full(self, u):
+        """Unique-node values (x fastest) -> [Nx, Ny] with the periodic duplicate row."""
+        full = np.empty((self.nx, self.ny))
+        full[:, :self.nu] = np.asarray(u).reshape(self.nu, self.nx).T
+        if self.periodic:
+            full[:, -1] = full[:, 0]
+        return full
+
+    def step(self, ops, phi0, bias: float, method: str = "direct", alpha: float = 1.0):
+        """One update from the summed density of `ops`' device solutions:
+        (phi, info) with the keys of `poisson_step` plus 'density' (the
+        projected density [m^-3] on the material grid)."""
+        method = method.lower()
+        if method == "direct":
+            if self.neumann:
+                raise ValueError("RTD:NeumannNeedsNewton: Neumann X requires newton-raphson")
+            newton = 0
+        elif method in ("newton-raphson", "newton-rhapson"):
+            newton = 1
+        else:
+            raise ValueError(f"RTD:UnknownPoissonMethod: {method}")
+        mat, nu = self.mat, self.nu
+        bc = np.zeros((self.nx, self.ny))
+        bc[-1, :] = bias
+        if mat.get("Vbc") is not None and np.size(mat["Vbc"]):
+            bc = np.asarray(mat["Vbc"], float)
+        col = lambda a: np.ascontiguousarray(np.asarray(a, float)[:, :nu].ravel(order="F"))
+        phi_u, bc_u = col(phi0), col(bc)
+        m = self.nx * nu
+        out, cand, dens = np.empty(m), np.empty(m), np.empty(m)
+        info = self._lib_info()
+        ctxs = (C.c_void_p * len(ops))(*[o.ctx for o in ops])
+        from . import _lib as L
+        self._check(self._lib.wdg2_poisson_step(ctxs, len(ops), L.dptr(phi_u), L.dptr(bc_u), newton, float(alpha),
+                                                L.dptr(out), L.dptr(cand), L.dptr(dens), C.byref(info)), self.op.ctx)
+        lin = info.linear_relres
+        upd = out
+        return self._full(upd), {
+            "converged": bool(np.all(np.isfinite(upd)) and np.isfinite(lin) and lin < 1e-8),
+            "linearRelres": lin, "physicalRelresBefore": info.physical_relres_before,
+            "physicalRelresAfter": info.physical_relres_after, "unmixedStep": info.unmixed_step,
+            "candidateUnmixed": self._full(cand), "method": method, "density": self._full(dens)}
+
+    @staticmethod
+    def _lib_info():
+        from . import _lib as L
+        return L.wdg2_poisson_info()
+
+
 def solve2d_rtd_self_consistent(mat: dict, max_iterations: int = 100, mixing: float = 0.1,
                                 potential_tolerance: float = 5e-4, density_tolerance: float = 5e-4,
                                 poisson_residual_tolerance: float = 8e-2, poisson_method: str = "direct",
@@ -259,7 +374,7 @@ def solve2d_rtd_self_consistent(mat: dict, max_iterations: int = 100, mixing: fl
     mat['Vdd'] (or `biases`).  Returns the sweep (per bias: current,
     density, potential, phi, flag, relres, converged, iterations, status)
     and the per-iteration histories."""
-    from .full2d import solve_transport_DG_full2D
+    from .full2d import TransportSession
     if x_boundary(mat) != "dirichlet":
         raise NotImplementedError("RTD:Unsupported: the Neumann-X contact calibration is not built")
     if not (0 < mixing <= 1):
@@ -271,7 +386,7 @@ def solve2d_rtd_self_consistent(mat: dict, max_iterations: int = 100, mixing: fl
     if ramp_len <= 0:
         raise ValueError("RTD:ContactGeometry: contact regions must leave a positive central length")
     ramp = np.repeat(np.clip((x - float(mat["Lc1"])) / ramp_len, 0, 1)[:, None], int(mat["Ny"]), axis=1)
-    M, types, periodic = eval_poisson2d(mat)
+    eval_poisson2d(mat)   # the Poisson data's checks before any device work
     sweep = {k: [] for k in ("bias", "current", "density", "potential", "phi", "flag", "relres", "converged",
                              "iterations", "status")}
     histories = []
@@ -281,43 +396,48 @@ def solve2d_rtd_self_consistent(mat: dict, max_iterations: int = 100, mixing: fl
             phi = b * ramp if initial_phi is None else np.asarray(initial_phi, float)
         else:
             phi = prev_phi + (b - bias[ib - 1]) * ramp
+        efl, efr = Vc[0, 0] + 0.05, Vc[-1, 0] + 0.05 - b
+        if "EfL" in mat:
+            efl = float(mat["EfL"])
+        if "EfR" in mat:
+            efr = float(mat["EfR"]) - b
         prev_n, hist, status, converged = None, [], "iteration-limit", False
-        for it in range(1, max_iterations + 1):
-            V = Vc - phi
-            efl, efr = Vc[0, 0] + 0.05, Vc[-1, 0] + 0.05 - b
-            if "EfL" in mat:
-                efl = float(mat["EfL"])
-            if "EfR" in mat:
-                efr = float(mat["EfR"]) - b
-            tr = solve_transport_DG_full2D(mat, V, efl, efr, rho0=rho_guess, solver=solver)
-            s = tr.info["solve"]
-            h = {"iteration": it, "transportFlag": s["flag"], "transportRelres": s["relres"],
-                 "warmStartUsed": bool(s["warmStartUsed"]), "initialTrueRelres": s["initialTrueRelres"],
-                 "gmresIterations": s["iterations"], "densityChange": np.inf}
-            if s["flag"] != 0 or not np.isfinite(s["relres"]):
-                status = "transport-not-converged"
+        # one device operator per bias: the iterations change its potential,
+        # warm-start from its device solution and update phi on the device
+        with TransportSession(mat, Vc - phi, efl, efr, rho0=rho_guess, solver=solver) as ses:
+            dev = DevicePoisson2D(ses.op)
+            for it in range(1, max_iterations + 1):
+                V = Vc - phi
+                tr = ses.solve(V if it > 1 else None)
+                s = tr.info["solve"]
+                h = {"iteration": it, "transportFlag": s["flag"], "transportRelres": s["relres"],
+                     "warmStartUsed": bool(s["warmStartUsed"]), "initialTrueRelres": s["initialTrueRelres"],
+                     "gmresIterations": s["iterations"], "densityChange": np.inf}
+                if s["flag"] != 0 or not np.isfinite(s["relres"]):
+                    status = "transport-not-converged"
+                    hist.append(h)
+                    break
+                if prev_n is not None:
+                    h["densityChange"] = np.linalg.norm(tr.n - prev_n) / max(np.linalg.norm(tr.n),
+                                                                             np.linalg.norm(prev_n),
+                                                                             np.finfo(float).tiny)
+                cand, pinfo = dev.step([ses.op], phi, b, poisson_method)
+                h.update({"potentialDefect": float(np.max(np.abs(cand - phi))),
+                          "poissonConverged": pinfo["converged"],
+                          "poissonPhysicalRelres": pinfo["physicalRelresBefore"]})
                 hist.append(h)
-                break
-            if prev_n is not None:
-                h["densityChange"] = np.linalg.norm(tr.n - prev_n) / max(np.linalg.norm(tr.n), np.linalg.norm(prev_n),
-                                                                         np.finfo(float).tiny)
-            n_pois, _ = project_dg_density(np.real(tr.nDG), tr.setup, periodic)
-            cand, pinfo = poisson_step(M, types, n_pois, mat, phi, b, poisson_method)
-            h.update({"potentialDefect": float(np.max(np.abs(cand - phi))), "poissonConverged": pinfo["converged"],
-                      "poissonPhysicalRelres": pinfo["physicalRelresBefore"]})
-            hist.append(h)
-            if not pinfo["converged"]:
-                status = "poisson-not-converged"
-                break
-            if (h["potentialDefect"] <= potential_tolerance and h["densityChange"] <= density_tolerance
-                    and h["poissonPhysicalRelres"] <= poisson_residual_tolerance):
-                status, converged = "fixed-point-converged", True
-                break
-            if it == max_iterations:
-                break
-            prev_n = tr.n
-            rho_guess = tr.rho
-            phi = phi + mixing * (cand - phi)
+                if not pinfo["converged"]:
+                    status = "poisson-not-converged"
+                    break
+                if (h["potentialDefect"] <= potential_tolerance and h["densityChange"] <= density_tolerance
+                        and h["poissonPhysicalRelres"] <= poisson_residual_tolerance):
+                    status, converged = "fixed-point-converged", True
+                    break
+                if it == max_iterations:
+                    break
+                prev_n = tr.n
+                phi = phi + mixing * (cand - phi)
+            rho_guess = ses.solution()
         for k, v in (("bias", b), ("current", tr.j), ("density", tr.n), ("potential", V), ("phi", phi),
                      ("flag", s["flag"]), ("relres", s["relres"]), ("converged", converged),
                      ("iterations", len(hist)), ("status", status)):
@@ -326,7 +446,6 @@ def solve2d_rtd_self_consistent(mat: dict, max_iterations: int = 100, mixing: fl
         if not converged:
             break
         prev_phi = phi
-        rho_guess = tr.rho
     return {"sweep": sweep, "histories": histories, "converged": all(sweep["converged"]) and len(hist) > 0}
 
 
@@ -336,14 +455,15 @@ def transient_cn_self_consistent(op, phi0, bias: float, dt: float, steps: int, p
     """Crank-Nicolson steps of a full-2D operator (`Full2DOperator`) with a
     2D Poisson update of the potential every `poisson_every` steps: after
     each block of steps the device density is projected onto the Poisson
-    dual volumes, one Poisson step from the current phi gives the new phi,
-    and the operator's potential becomes Vc - phi (the contacts keep
+    dual volumes, one Poisson step from the current phi gives the new phi
+    (DevicePoisson2D: density, projection and update on the device), and
+    the operator's potential becomes Vc - phi (the contacts keep
     phi = 0 and bias, so the inflow data stay valid).  rho0=None continues
     from the operator's device state (e.g. a stationary solve).  Returns
     (phi, history) with one entry per block."""
     mat = op.setup.mat
     Vc = np.asarray(mat["Vc"], float)
-    M, types, periodic = eval_poisson2d(mat)
+    dev = DevicePoisson2D(op)
     phi = np.asarray(phi0, float).copy()
     done, hist = 0, []
     while done < steps:
@@ -351,13 +471,10 @@ def transient_cn_self_consistent(op, phi0, bias: float, dt: float, steps: int, p
         _, info = op.transient_cn(dt, k, tol=tol, restart=restart, precision=precision,
                                   rho0=rho0 if done == 0 else None, return_rho=False, use_graph=use_graph)
         done += k
-        nDG, _, _ = op.observables()
-        n, tinfo = project_dg_density(np.real(nDG), op.setup, periodic)
-        new, pinfo = poisson_step(M, types, n, mat, phi, bias, poisson_method)
+        new, pinfo = dev.step([op], phi, bias, poisson_method)
         hist.append({"steps": done, "gmresIterations": info["iterations"], "maxStepRelres": info["max_relres"],
                      "flag": info["flag"], "potentialUpdate": float(np.max(np.abs(new - phi))),
-                     "poissonPhysicalRelres": pinfo["physicalRelresBefore"], "poissonConverged": pinfo["converged"],
-                     "chargeTransferError": tinfo["relativeConservationError"]})
+                     "poissonPhysicalRelres": pinfo["physicalRelresBefore"], "poissonConverged": pinfo["converged"]})
         phi = new
         op.set_potential(Vc - phi)
     return phi, hist
@@ -383,7 +500,7 @@ def transient_cn_batch_self_consistent(ops, offsets, phi0, bias: float, dt: floa
         raise ValueError("one potential offset per instance")
     mat = ops[0].setup.mat
     Vc = np.asarray(mat["Vc"], float)
-    M, types, periodic = eval_poisson2d(mat)
+    dev = DevicePoisson2D(ops[0])
     phi = np.asarray(phi0, float).copy()
     for op, off in zip(ops, offsets):
         op.set_potential(Vc - phi + float(off))
@@ -393,14 +510,12 @@ def transient_cn_batch_self_consistent(ops, offsets, phi0, bias: float, dt: floa
         _, infos = transient_cn_batch(ops, dt, k, tol=tol, restart=restart, precision=precision,
                                       rho0=rho0 if done == 0 else None, return_rho=False, use_graph=use_graph)
         done += k
-        nDG = sum(np.real(op.observables()[0]) for op in ops)
-        n, tinfo = project_dg_density(nDG, ops[0].setup, periodic)
-        new, pinfo = poisson_step(M, types, n, mat, phi, bias, poisson_method)
+        # the summed density, its projection and the update stay on the device
+        new, pinfo = dev.step(ops, phi, bias, poisson_method)
         hist.append({"steps": done, "phi": phi, "gmresIterations": [i["iterations"] for i in infos],
                      "maxStepRelres": max(i["max_relres"] for i in infos), "flag": max(i["flag"] for i in infos),
                      "potentialUpdate": float(np.max(np.abs(new - phi))),
-                     "poissonPhysicalRelres": pinfo["physicalRelresBefore"], "poissonConverged": pinfo["converged"],
-                     "chargeTransferError": tinfo["relativeConservationError"]})
+                     "poissonPhysicalRelres": pinfo["physicalRelresBefore"], "poissonConverged": pinfo["converged"]})
         phi = new
         for op, off in zip(ops, offsets):
             op.set_potential(Vc - phi + float(off))

@@ -80,3 +80,12 @@ def test_invalid_input_is_rejected_with_a_message(lib):
     assert lib.wdg_host_setup(C.byref(bad2), C.byref(pot), None, None) == -1
     assert lib.wdg_host_setup(None, None, None, None) == -1
     assert lib.wdg_theta_method(None) == -1
+
+
+def test_full2d_poisson_entry_points_reject_null_arguments(lib):
+    """wdg2_poisson_setup / wdg2_poisson_step validate their arguments before
+    touching a device."""
+    assert lib.wdg2_poisson_setup(None, None) == _lib.WDG_ERR_ARG
+    info = _lib.wdg2_poisson_info()
+    assert lib.wdg2_poisson_step(None, 1, None, None, 0, 1.0, None, None, None, C.byref(info)) == _lib.WDG_ERR_ARG
+    assert "null" in lib.wdg_last_error(None).decode()
