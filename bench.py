@@ -67,8 +67,9 @@ def parse():
     ap.add_argument("--full2d-instances", type=int, default=8,
                     help="transverse-mode instances of the batched full-2D transient (--only transient_full2d_batch)")
     ap.add_argument("--full2d-precisions", default="mixed,fp64", help="precisions of the full-2D transient (--only)")
-    ap.add_argument("--full2d-dv", choices=["interp", "table"], default="interp",
-                    help="Delta V source of the full-2D apply: interpolated in the kernel, or a stored table")
+    ap.add_argument("--full2d-dv", choices=["interp", "table"], default="table",
+                    help="Delta V source of the full-2D apply: a stored table (the library default), or "
+                         "interpolated in the kernel")
     ap.add_argument("--calibrate-copy", type=float, default=0.0, metavar="BYTES",
                     help="first run one plain device copy of BYTES bytes (HBM-counter calibration)")
     return ap.parse_args()
@@ -267,13 +268,16 @@ def self_consistent(n_bias: int = 16) -> dict:
     return out
 
 
-def transient_full2d(steps: int = 2, dt: float = 1e-15, precisions=("mixed", "fp64"), dv: str = "interp") -> dict:
+def transient_full2d(steps: int = 2, dt: float = 1e-15, precisions=("mixed", "fp64"), dv: str = "table",
+                     compare_interp: bool = True) -> dict:
     """Full-2D FET slab of the transient workload: 40x20 nm, 64x32 DG elements
     p=1, 64x64 relative cells (N = 33,554,432), Gaussian 0.2 eV gate barrier
     (sigma 5 nm); Crank-Nicolson steps of dt from the zero state with one
     row-scaled GMRES(30) solve to 1e-8 per step, in FP64 and mixed precision.
     Reports steps/s, iterations per step, the density difference of the two
-    precisions and the full-2D apply kernel against the HBM roofline."""
+    precisions and the full-2D apply kernel against the HBM roofline; with
+    compare_interp the mixed steps run once more with Delta V interpolated
+    in the apply instead of streamed from the table."""
     from real_space_dg_amd import full2d as F2
     from real_space_dg_amd.devices import fet_slab
     from real_space_dg_amd.solver import PROFILE_BITS
@@ -321,6 +325,21 @@ def transient_full2d(steps: int = 2, dt: float = 1e-15, precisions=("mixed", "fp
             if d["launches"]:
                 out[prec]["f2_dense_avg_ms"] = d["avg_ms"]
             print(f"[bench] transient full2D {prec}: {wall:.2f} s", file=sys.stderr, flush=True)
+        if compare_interp and dv == "table" and "mixed" in precisions:
+            op.set_dv_mode("interp")
+            op.transient_cn(dt, 1, tol=1e-8, precision="mixed", rho0=np.zeros(setup.n_total, complex),
+                            return_rho=False)
+            t0 = time.perf_counter()
+            op.transient_cn(dt, steps, tol=1e-8, precision="mixed", rho0=np.zeros(setup.n_total, complex),
+                            return_rho=False)
+            wall = time.perf_counter() - t0
+            op.set_profiling(mask)
+            op.transient_cn(dt, 1, tol=1e-8, precision="mixed", rho0=np.zeros(setup.n_total, complex),
+                            return_rho=False)
+            k = op.profile()["f2_apply"]
+            op.set_profiling(0)
+            out["mixed_delta_v_interp"] = {"steps_per_s": steps / wall, "f2_apply_avg_ms": k["avg_ms"]}
+            print(f"[bench] transient full2D mixed, Delta V interpolated: {wall:.2f} s", file=sys.stderr, flush=True)
     if "mixed" in out and "fp64" in out:
         out["mixed_over_fp64"] = out["mixed"]["steps_per_s"] / out["fp64"]["steps_per_s"]
         out["mixed_vs_fp64_density_rel_diff"] = float(np.max(np.abs(dens["mixed"] - dens["fp64"])) /

@@ -483,14 +483,18 @@ typedef struct wdg2_potential {
 } wdg2_potential;
 int wdg2_set_potential(wdg_ctx* ctx, const wdg2_potential* pot);
 
-/* Source of Delta V in the FV-consistent apply.  WDG2_DV_INTERP (the
- * default) interpolates the material grid inside the apply kernel (the
- * grid and the per-sample cell tables sit in LDS), so an apply streams only
- * u and y; WDG2_DV_TABLE stores Delta V per (element, quadrature point,
- * relative vertex) in HBM (12 bytes per entry for both precisions) and
- * streams it.  Both compute the same values (the table is filled by the
- * same interpolation).  Takes effect at once (or at wdg2_set_potential).
- * No counterpart in the reference: an implementation choice. */
+/* Source of Delta V in the FV-consistent apply.  WDG2_DV_TABLE (the
+ * default) stores Delta V per (element, quadrature point, relative vertex)
+ * in HBM (12 bytes per entry for both precisions) and streams it;
+ * WDG2_DV_INTERP interpolates the material grid inside the apply kernel
+ * (the grid and the per-sample cell tables sit in LDS), so an apply streams
+ * only u and y but evaluates two bilinear interpolations per sample.  The
+ * apply is bound by the latency of its dependent load rounds, not by HBM
+ * bytes, and the table form measured 2.5x faster (FET slab, 33.5M
+ * unknowns: 0.64 vs 1.60 ms per FP32 apply).  Both compute the same values
+ * (the table is filled by the same interpolation); the choice depends on
+ * the mode alone.  Takes effect at once (or at wdg2_set_potential).  No
+ * counterpart in the reference: an implementation choice. */
 enum { WDG2_DV_INTERP = 0, WDG2_DV_TABLE = 1 };
 int wdg2_set_dv_mode(wdg_ctx* ctx, int32_t mode);
 

@@ -517,9 +517,10 @@ void f2_sample_table(const double* centre, int nc, const double* rel, int nr, co
 
 // Where the FV-consistent apply gets Delta V from (c->f2->dv_mode): a table
 // per (element, quadrature point, relative vertex) in HBM that the apply
-// streams (TABLE), or the material grid, interpolated twice per sample
-// inside the apply from LDS (INTERP, the default: nothing streamed beyond u
-// and y).  The choice depends on the mode alone, never on free memory.
+// streams (TABLE, the default: one load per sample), or the material grid,
+// interpolated twice per sample inside the apply from LDS (INTERP: nothing
+// streamed beyond u and y, more work per sample).  The choice depends on the
+// mode alone, never on free memory.
 // Sets the apply tile (rows of rho_y per workgroup, LDS) for that choice and
 // uploads the constants.
 void f2_configure_dv(wdg_ctx* c) {

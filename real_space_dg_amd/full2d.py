@@ -893,8 +893,8 @@ class Full2DOperator:
         return d
 
     def set_dv_mode(self, mode: str):
-        """'interp' (Delta V interpolated in the apply, the default) or 'table'
-        (Delta V streamed from a stored table): wdg2_set_dv_mode."""
+        """'table' (Delta V streamed from a stored table, the default) or
+        'interp' (Delta V interpolated in the apply): wdg2_set_dv_mode."""
         modes = {"interp": _lib.WDG2_DV_INTERP, "table": _lib.WDG2_DV_TABLE}
         if mode not in modes:
             raise ValueError(f"mode must be one of {sorted(modes)}")
