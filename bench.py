@@ -45,6 +45,7 @@ M0 = 9.109e-31
 
 WORKLOAD = dict(n_elem=512, degree=3, nk=1024, bias_v=0.2)
 MFMA_KERNELS = {"theta_gemm1", "theta_gemm2"}   # the GEMM formulation of Theta (not the default)
+NO_GRAPHS = False   # --no-graphs: full-2D transients launch their cycles kernel by kernel
 
 
 def parse():
@@ -70,6 +71,11 @@ def parse():
     ap.add_argument("--full2d-dv", choices=["interp", "table"], default="table",
                     help="Delta V source of the full-2D apply: a stored table (the library default), or "
                          "interpolated in the kernel")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="launch the full-2D transients' restart cycles kernel by kernel instead of replaying "
+                         "captured graphs (same kernels; for kernel tracers)")
+    ap.add_argument("--theta", choices=["auto", "fft", "gemm"], default="auto",
+                    help="formulation of Theta[V] in the headline solves (wdg_set_theta_method)")
     ap.add_argument("--calibrate-copy", type=float, default=0.0, metavar="BYTES",
                     help="first run one plain device copy of BYTES bytes (HBM-counter calibration)")
     return ap.parse_args()
@@ -293,12 +299,12 @@ def transient_full2d(steps: int = 2, dt: float = 1e-15, precisions=("mixed", "fp
         op.upload_rhs()
         op.set_preconditioner("rowabs")
         op.transient_cn(dt, 1, tol=1e-8, precision="mixed", rho0=np.zeros(setup.n_total, complex),
-                        return_rho=False)   # graphs / tables warm
+                        return_rho=False, use_graph=not NO_GRAPHS)   # graphs / tables warm
         mask = (1 << PROFILE_BITS["f2_apply"]) | (1 << PROFILE_BITS["f2_dense"])
         for prec in precisions:
             # the library launches FP64 full-2D Crank-Nicolson cycles kernel by
             # kernel (graph replay of them is off, wdg_cn_opts)
-            use_graph = prec != "fp64"
+            use_graph = prec != "fp64" and not NO_GRAPHS
             op.set_profiling(0)
             t0 = time.perf_counter()
             _, info = op.transient_cn(dt, steps, tol=1e-8, precision=prec, rho0=np.zeros(setup.n_total, complex),
@@ -328,10 +334,10 @@ def transient_full2d(steps: int = 2, dt: float = 1e-15, precisions=("mixed", "fp
         if compare_interp and dv == "table" and "mixed" in precisions:
             op.set_dv_mode("interp")
             op.transient_cn(dt, 1, tol=1e-8, precision="mixed", rho0=np.zeros(setup.n_total, complex),
-                            return_rho=False)
+                            return_rho=False, use_graph=not NO_GRAPHS)
             t0 = time.perf_counter()
             op.transient_cn(dt, steps, tol=1e-8, precision="mixed", rho0=np.zeros(setup.n_total, complex),
-                            return_rho=False)
+                            return_rho=False, use_graph=not NO_GRAPHS)
             wall = time.perf_counter() - t0
             op.set_profiling(mask)
             op.transient_cn(dt, 1, tol=1e-8, precision="mixed", rho0=np.zeros(setup.n_total, complex),
@@ -448,6 +454,8 @@ def transient_full2d_batch(instances: int = 8, steps: int = 1, dt: float = 1e-15
 
 def main():
     args = parse()
+    global NO_GRAPHS
+    NO_GRAPHS = args.no_graphs
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (--gpus {args.gpus} requested); "
               "multi-GPU sharding is design-only.", file=sys.stderr)
@@ -476,6 +484,7 @@ def main():
     m_nodes = np.concatenate([m[e * WORKLOAD["degree"]: e * WORKLOAD["degree"] + WORKLOAD["degree"] + 1]
                               for e in range(WORKLOAD["n_elem"])])
     op = WignerOperator.from_mat(mat)
+    op.set_theta_method(args.theta)
 
     def step(prec: str, tol: float = RTD_BENCH_TOL):
         op.set_potential(mat["x"] * 1e-9, E[0, :, 0], efl, efr, m[0], m[-1], temp=mat["Temp"])
