@@ -551,6 +551,16 @@ def main():
             err[f"{name}_density_rel_err_vs_oracle"] = float(np.max(np.abs(nn - gn)) / np.max(np.abs(gn)))
             err[f"{name}_current_rel_err_vs_oracle"] = float(np.max(np.abs(jj - gj)) / np.max(np.abs(gj)))
         err["oracle"] = f"FP64 numpy oracle, GMRES(30) + line preconditioner, true relres {float(gold['true_relres']):.1e}"
+    # the transport apply alone (the HBM-bound "Wigner apply" of the north star:
+    # transport + CAP diagonal, 2 s N bytes), through wdg_apply_parts
+    u_probe = np.random.default_rng(0).standard_normal(op.N) + 1j * np.random.default_rng(1).standard_normal(op.N)
+    transport_roof = {}
+    for prec in ("mixed", "fp64"):
+        op.set_profiling(1 << PROFILE_BITS["transport"])
+        for _ in range(20):
+            op.apply(u_probe, prec, parts="transport")
+        transport_roof[prec] = kernel_roofline(op.profile(), "transport", "fp32", prec)
+        op.set_profiling(0)
     tot = {k: v["avg_ms"] * v["launches"] for k, v in prof_mixed.items() if v["launches"]}
     dominant = max(tot, key=tot.get)
     roof = kernel_roofline(prof_mixed, dominant, "bf16")
@@ -584,6 +594,8 @@ def main():
         # the chirp-z kernel is the whole operator apply (transport rows fused in)
         "wigner_apply_roofline": kernel_roofline(prof_mixed, "theta_fft", "fp32"),
         "wigner_apply_roofline_fp64": kernel_roofline(prof_fp64, "theta_fft", "fp64", "fp64"),
+        # the transport apply alone, 2 s N bytes per launch (wdg_apply_parts, transport part)
+        "transport_apply_roofline": transport_roof,
         "kernels_mixed": kernels,
         "kernel_timing": (f"HIP event pairs on the context stream around every kernel, over a profiled pass of "
                           f"{args.profile_steps} further solves per precision; that pass launches kernel by "

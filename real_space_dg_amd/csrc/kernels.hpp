@@ -174,59 +174,67 @@ __global__ __launch_bounds__(256) void transport_kernel(const OpConst* __restric
   using T2 = typename cvec<T>::type;
   if (skip(st)) return;
   // One thread per (e, pair of wave numbers j, j+1): the n nodes of the
-  // element and the upstream node are read once, 2 complex values per load.
+  // element and the upstream node are read once, the pair as one 16-byte
+  // packet in FP32 (two in FP64); arithmetic in the working precision T.
   constexpr int n = NN;
   const int nk = oc->nk, ne = oc->n_chi;
   const int nk2 = nk >> 1;
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long)ne * nk2) return;
   const int e = (int)(t / nk2), j = 2 * (int)(t % nk2);
-  const double tq = oc->tq;
-  double k[2] = {kvec[j], kvec[j + 1]};
-  double ak[2] = {fabs(k[0]), fabs(k[1])};
-  double ur[NN][2], ui[NN][2];
+  const T tq = (T)oc->tq;
+  const T k[2] = {(T)kvec[j], (T)kvec[j + 1]};
+  const T ak[2] = {k[0] < 0 ? -k[0] : k[0], k[1] < 0 ? -k[1] : k[1]};
+  T2 uv[NN][2];
   const T2* ue = u + (long)e * n * nk + j;
 #pragma unroll
   for (int b = 0; b < NN; ++b) {
-      const T2 v0 = ue[(long)b * nk], v1 = ue[(long)b * nk + 1];
-      ur[b][0] = v0.x; ui[b][0] = v0.y;
-      ur[b][1] = v1.x; ui[b][1] = v1.y;
+    if constexpr (sizeof(T2) == 8) {   // FP32: the pair in one 16-byte load
+      const float4 v = *reinterpret_cast<const float4*>(ue + (long)b * nk);
+      uv[b][0].x = v.x; uv[b][0].y = v.y;
+      uv[b][1].x = v.z; uv[b][1].y = v.w;
+    } else {
+      uv[b][0] = ue[(long)b * nk];
+      uv[b][1] = ue[(long)b * nk + 1];
     }
+  }
   // upstream inflow values (k > 0: last node of e-1; k < 0: first node of e+1)
-  double cr[2] = {0, 0}, ci[2] = {0, 0};
+  T2 cv[2];
 #pragma unroll
   for (int q = 0; q < 2; ++q) {
-    if (k[q] > 0 && e > 0) {
-      const T2 v = u[((long)(e - 1) * n + (n - 1)) * nk + j + q];
-      cr[q] = v.x; ci[q] = v.y;
-    } else if (k[q] < 0 && e < ne - 1) {
-      const T2 v = u[((long)(e + 1) * n) * nk + j + q];
-      cr[q] = v.x; ci[q] = v.y;
-    }
+    cv[q].x = 0;
+    cv[q].y = 0;
+    if (k[q] > 0 && e > 0) cv[q] = u[((long)(e - 1) * n + (n - 1)) * nk + j + q];
+    else if (k[q] < 0 && e < ne - 1) cv[q] = u[((long)(e + 1) * n) * nk + j + q];
   }
   T2* ye = y + (long)e * n * nk + j;
 #pragma unroll
   for (int a = 0; a < NN; ++a) {
-      T2 o[2];
+    T2 o[2];
 #pragma unroll
-      for (int q = 0; q < 2; ++q) {
-        double sr = 0, si = 0;
+    for (int q = 0; q < 2; ++q) {
+      T sr = 0, si = 0;
 #pragma unroll
-        for (int b = 0; b < NN; ++b) {
-          const double c = k[q] * oc->K1[a * kMaxN + b] + ak[q] * oc->K2[a * kMaxN + b];
-          sr += c * ur[b][q];
-          si += c * ui[b][q];
-        }
-        // -(k+|k|)/2 invM(:,0) u_{e-1,n-1}  or  (k-|k|)/2 invM(:,n-1) u_{e+1,0}
-        const double cf = (k[q] > 0) ? (-0.5 * (k[q] + ak[q]) * oc->invM0[a]) : (0.5 * (k[q] - ak[q]) * oc->invMn[a]);
-        sr += cf * cr[q];
-        si += cf * ci[q];
-        o[q].x = (T)(tq * sr);
-        o[q].y = (T)(tq * si);
+      for (int b = 0; b < NN; ++b) {
+        const T c = k[q] * (T)oc->K1[a * kMaxN + b] + ak[q] * (T)oc->K2[a * kMaxN + b];
+        sr += c * uv[b][q].x;
+        si += c * uv[b][q].y;
       }
+      // -(k+|k|)/2 invM(:,0) u_{e-1,n-1}  or  (k-|k|)/2 invM(:,n-1) u_{e+1,0}
+      const T cf = (k[q] > 0) ? ((T)-0.5 * (k[q] + ak[q]) * (T)oc->invM0[a])
+                              : ((T)0.5 * (k[q] - ak[q]) * (T)oc->invMn[a]);
+      sr += cf * cv[q].x;
+      si += cf * cv[q].y;
+      o[q].x = tq * sr;
+      o[q].y = tq * si;
+    }
+    if constexpr (sizeof(T2) == 8) {
+      *reinterpret_cast<float4*>(ye + (long)a * nk) = make_float4(o[0].x, o[0].y, o[1].x, o[1].y);
+    } else {
       ye[(long)a * nk] = o[0];
       ye[(long)a * nk + 1] = o[1];
     }
+  }
 }
 
 // ---------------------------------------------------------------------------

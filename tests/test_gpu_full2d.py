@@ -72,6 +72,24 @@ def test_apply_diagonal_rowabs(name, dv):
         assert rel(op.row_abs_sum(), s_or) < 1e-12
 
 
+@pytest.mark.parametrize("name", list(ALL))
+def test_observables_of_the_same_state(name):
+    """wdg2_observables on a given state against the oracle's
+    get_Observables_full2D restatement of the same state: the density and
+    both current rows at zero separation, every configuration (periodic Y,
+    several rho_y tiles, p = 1 and 2)."""
+    m, EfL, EfR = ALL[name]
+    p = F.init_params_full2d(m, m["V"], EfL, EfR)
+    rng = np.random.default_rng(7)
+    u = rng.standard_normal(p["nTotal"]) + 1j * rng.standard_normal(p["nTotal"])
+    with P.Full2DOperator(P.Full2DSetup(m, m["V"], EfL, EfR)) as op:
+        nDG, jxDG, jyDG = op.observables(rho=u)
+    n_or, jx_or, jy_or = F.observables_full2d(u, p, m)
+    print(f"{name}: n {rel(nDG, n_or):.1e} jx {rel(jxDG, jx_or):.1e} jy {rel(jyDG, jy_or):.1e}")
+    assert rel(nDG, n_or) < 1e-12
+    assert rel(jxDG, jx_or) < 1e-12 and rel(jyDG, jy_or) < 1e-12
+
+
 @pytest.mark.parametrize("name", ["rusanov-specular-fv", "upwind-zero-inflow", "central-closed-faces-collocated",
                                   "p1-even-rho", "periodic-rusanov-fv"])
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
