@@ -10,7 +10,8 @@ Errors are max-norm, relative to the largest |n| and |j|.
     density and current (the observables move ~250-500x the residual: a
     solve to 1e-12 lands at 2-5e-10);
   * at the benchmark's stopping tolerance (solver.RTD_BENCH_TOL) both the
-    mixed-precision and the FP64 solve are within 1e-8 of the oracle.
+    mixed-precision (with either FP32 Theta block) and the FP64 solve are
+    within 1e-8 of the oracle.
 """
 from pathlib import Path
 
@@ -62,9 +63,16 @@ def test_fp64_solve_matches_oracle(rtd):
     assert ej < 1e-10
 
 
-@pytest.mark.parametrize("prec", ["mixed", "fp64"])
+@pytest.mark.parametrize("prec", ["mixed", "mixed-256", "mixed-wide", "fp64"])
 def test_bench_tolerance_meets_observable_target(rtd, prec):
-    en, ej = _solve_errors(rtd, prec, RTD_BENCH_TOL)
+    """mixed-256 / mixed-wide force the FP32 Theta block: the wide block
+    runs the Gram-Schmidt multi-dot inside the apply (theta_fused_dot)."""
+    prec, _, block = prec.partition("-")
+    rtd["op"].set_fft_block(block or "auto")
+    try:
+        en, ej = _solve_errors(rtd, prec, RTD_BENCH_TOL)
+    finally:
+        rtd["op"].set_fft_block("auto")
     assert en < 1e-8
     assert ej < 1e-8
 

@@ -43,13 +43,18 @@ def test_fp64_matches_direct(solved):
     assert ej < 1e-8
 
 
-@pytest.mark.parametrize("method", ["fft", "gemm"])
+@pytest.mark.parametrize("method", ["fft-256", "fft-wide", "gemm"])
 def test_mixed_reaches_tolerance(solved, method):
-    solved["op"].set_theta_method(method)
+    """fft-wide runs the Gram-Schmidt multi-dot inside the Theta apply
+    (theta_fused_dot), fft-256 and gemm as a kernel of its own."""
+    solved["op"].set_theta_method(method.split("-")[0])
+    if method.startswith("fft"):
+        solved["op"].set_fft_block(method.split("-")[1])
     try:
         _mixed_reaches_tolerance(solved)
     finally:
         solved["op"].set_theta_method("auto")
+        solved["op"].set_fft_block("auto")
 
 
 def _mixed_reaches_tolerance(solved):
@@ -71,12 +76,18 @@ def _mixed_reaches_tolerance(solved):
     assert ej < 1e-7
 
 
+@pytest.mark.parametrize("block", ["256", "wide"])
 @pytest.mark.parametrize("basis", ["fp32", "bf16", "fp16"])
-def test_mixed_krylov_basis_storage(solved, basis):
+def test_mixed_krylov_basis_storage(solved, basis, block):
     """The inner basis may be stored narrower than FP32; the FP64 refinement
-    still reaches the tolerance and lands on the oracle solution."""
-    x, info = solved["op"].solve(SolveOptions(tol=1e-12, restart=30, precision="mixed", krylov_basis=basis))
-    print(basis, info)
+    still reaches the tolerance and lands on the oracle solution.  With the
+    wide Theta block a 16-bit basis is dotted inside the apply."""
+    solved["op"].set_fft_block(block)
+    try:
+        x, info = solved["op"].solve(SolveOptions(tol=1e-12, restart=30, precision="mixed", krylov_basis=basis))
+    finally:
+        solved["op"].set_fft_block("auto")
+    print(basis, block, info)
     assert info["flag"] == 0 and info["true_relres"] <= 1e-12
     err = np.linalg.norm(x - solved["x_ref"]) / np.linalg.norm(solved["x_ref"])
     assert err < 1e-9
