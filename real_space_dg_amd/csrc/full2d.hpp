@@ -783,6 +783,44 @@ __global__ void f2_scale_real_kernel(const T* __restrict__ p, const typename F2V
   }
 }
 
+// The scaling preconditioners of the FP32 solver applied to a 16-bit Krylov
+// basis vector read in place: z = p r with a real scale (REAL) or z = p o r
+// with a complex one.  Two entries per thread and step: one 8-byte basis
+// packet (FP16 pairs stored times 256, or BF16 pairs), the two scales, one
+// 16-byte store.
+template <typename VP>
+__device__ __forceinline__ void f2_basis_pair(VP p, float2& a, float2& b);
+template <>
+__device__ __forceinline__ void f2_basis_pair<ushort4>(ushort4 p, float2& a, float2& b) {
+  const float s = 1.0f / 256.0f;
+  a = make_float2((float)__builtin_bit_cast(_Float16, p.x) * s, (float)__builtin_bit_cast(_Float16, p.y) * s);
+  b = make_float2((float)__builtin_bit_cast(_Float16, p.z) * s, (float)__builtin_bit_cast(_Float16, p.w) * s);
+}
+template <>
+__device__ __forceinline__ void f2_basis_pair<uint2>(uint2 p, float2& a, float2& b) {
+  a = lv_unpack_bf16x2(p.x);
+  b = lv_unpack_bf16x2(p.y);
+}
+
+template <typename VP, bool REAL>
+__global__ void f2_scale_basis_kernel(const void* __restrict__ scale, const VP* __restrict__ v,
+                                      float4* __restrict__ z, long npairs, const GmresState* st) {
+  if (skip(st)) return;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < npairs; i += (long)gridDim.x * blockDim.x) {
+    float2 a, b;
+    f2_basis_pair<VP>(v[i], a, b);
+    float4 o;
+    if constexpr (REAL) {
+      const float2 p = reinterpret_cast<const float2*>(scale)[i];
+      o = make_float4(p.x * a.x, p.x * a.y, p.y * b.x, p.y * b.y);
+    } else {
+      const float4 p = reinterpret_cast<const float4*>(scale)[i];
+      o = make_float4(p.x * a.x - p.y * a.y, p.x * a.y + p.y * a.x, p.z * b.x - p.w * b.y, p.z * b.y + p.w * b.x);
+    }
+    z[i] = o;
+  }
+}
+
 // max |d| and the number of entries with |d| < floor (preconditioner floors).
 __global__ void f2_absmax_kernel(const double2* __restrict__ d, long n, double* __restrict__ part) {
   __shared__ double sm[256];

@@ -245,6 +245,27 @@ void f2_precond(wdg_ctx* c, const typename F2Vec<T>::T2* r, typename F2Vec<T>::T
                      (const typename F2Vec<T>::T2*)p, r, z, c->N, st);
 }
 
+// Whether the full-2D preconditioner can read a 16-bit Krylov basis vector in
+// place (the elementwise scalings), so that the FP32 solver keeps no FP32
+// copy of the newest basis vector.
+bool f2_precond_reads_basis(const wdg_ctx* c) {
+  return c->f2 && c->N % 2 == 0 &&
+         (c->f2->precond == WDG2_PRECOND_ROWABS || c->f2->precond == WDG2_PRECOND_JACOBI);
+}
+
+template <typename VP>
+void f2_precond_basis(wdg_ctx* c, const VP* v, float2* z, const GmresState* st) {
+  F2Data& f = *c->f2;
+  // a basis packet holds two entries (f2_precond_reads_basis requires an even N)
+  const long npairs = c->N / 2;
+  if (f.precond == WDG2_PRECOND_ROWABS)
+    hipLaunchKernelGGL((f2_scale_basis_kernel<VP, true>), dim3(c->red_blocks), dim3(256), 0, c->stream,
+                       (const void*)f.d_p32.p, v, reinterpret_cast<float4*>(z), npairs, st);
+  else
+    hipLaunchKernelGGL((f2_scale_basis_kernel<VP, false>), dim3(c->red_blocks), dim3(256), 0, c->stream,
+                       (const void*)f.d_p32.p, v, reinterpret_cast<float4*>(z), npairs, st);
+}
+
 // ---- setup ----
 std::vector<double> f2_merge_banded(const std::vector<const wdg2_part*>& ps, int nl, int ne, double sA) {
   const int sb = nl * nl + 2 * nl;

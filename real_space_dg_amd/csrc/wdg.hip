@@ -953,9 +953,9 @@ void gmres_cycle(wdg_ctx* c, int m, bool precond, bool single_pass) {
   const long NP = N * (long)sizeof(T2) / 16;        // working packets per vector
   const double vbytes = (double)NP * sizeof(VP);    // bytes of one stored basis vector
   const bool wide = sizeof(VP) == 8 && NP % 2 == 0;
-  // 1D contexts with the line preconditioner read a 16-bit basis vector in
-  // place, so the newest vector needs no FP32 copy
-  const bool direct16 = narrow && precond && !c->f2;
+  // the 1D line preconditioner and the full-2D scalings read a 16-bit basis
+  // vector in place, so the newest vector needs no FP32 copy
+  const bool direct16 = narrow && precond && (!c->f2 || f2_precond_reads_basis(c));
   PT* wp = reinterpret_cast<PT*>(w);
   PT* vcp = direct16 ? nullptr : reinterpret_cast<PT*>(vcur);
   GmresBufs gb = c->gb();
@@ -970,7 +970,8 @@ void gmres_cycle(wdg_ctx* c, int m, bool precond, bool single_pass) {
       bool done = false;
       if constexpr (narrow && std::is_same<T, float>::value) {
         if (direct16) {
-          op_precond_basis<VP>(c, V + (size_t)j * NP, z, st);
+          if (c->f2) f2_precond_basis<VP>(c, V + (size_t)j * NP, z, st);
+          else op_precond_basis<VP>(c, V + (size_t)j * NP, z, st);
           done = true;
         }
       }
