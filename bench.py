@@ -57,7 +57,8 @@ def parse():
     ap.add_argument("--profile-steps", type=int, default=2,
                     help="solves per precision in the profiled pass that times each kernel")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sample-iters", type=int, default=6)
+    ap.add_argument("--cpu-sample-iters", type=int, default=30,
+                    help="GMRES iterations of the CPU baseline's sample at the headline workload (one restart cycle)")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the batched-sweep and self-consistent workloads reported beside the metric")
     ap.add_argument("--only", choices=["batched_sweep", "self_consistent", "transient_full2d", "transient_full2d_batch",
@@ -191,6 +192,49 @@ def cpu_baseline(mat, E, efl, efr, its_per_solve: int, sample_iters: int) -> dic
                        f"{t_iter:.3f} s each on the same workload, scaled to {its_per_solve} "
                        f"iterations per solve (the FP64 GPU solver's count)"),
             "seconds_per_iteration": t_iter, "setup_seconds": t_setup}
+
+
+def c1_timed_solves(threads: int) -> dict:
+    """The plain-method workload (64 elements p=2 x 128 k, zero bias, GMRES(30)
+    + line preconditioner to 1e-10) solved completely on the host by the
+    oracle and on the GPU in both precisions: wall times of whole solves, no
+    extrapolation."""
+    from oracle import krylov as K
+    from oracle import wigner1d as W
+    from real_space_dg_amd.devices import rtd_1d
+    from real_space_dg_amd.solver import SolveOptions, WignerOperator
+    mat, E, _, efl, efr = rtd_1d(64, 2, 128, bias_v=0.0)
+    m = mat["me_x"][0, :, 0] * M0
+    t0 = time.perf_counter()
+    p = W.init_params(mat, efl, efr, E[0, :, 0], m, 1.0)
+    op = W.MatrixFreeOperator.build(mat, p, E[0, :, 0])
+    Bbar = op.edge_coefficients()
+    pre = K.LinePreconditioner(op, Bbar)
+    _, rhs = W.get_diff(p)
+    _, cinfo = K.gmres(lambda u: op.apply(u, Bbar), pre.apply, rhs, restart=30, tol=1e-10, maxit=20000)
+    t_cpu = time.perf_counter() - t0
+    out = {"workload": "RTD 60 nm, zero bias, 64 DG elements p=2 x 128 k, GMRES(30) + line preconditioner "
+                       "to true relres 1e-10 (setup included on both sides)",
+           "cpu": {"seconds": t_cpu, "iterations": int(cinfo["iterations"]), "threads": threads,
+                   "true_relres": float(cinfo["true_relres"]) if "true_relres" in cinfo else None,
+                   "kind": "oracle FP64 restatement (numpy), one complete solve"}}
+    g = WignerOperator.from_mat(mat)
+    for prec in ("fp64", "mixed"):
+        def solve():
+            g.set_potential(mat["x"] * 1e-9, E[0, :, 0], efl, efr, m[0], m[-1], temp=mat["Temp"])
+            return g.solve(SolveOptions(tol=1e-10, restart=30, precision=prec), return_x=False)[1]
+        solve()
+        reps = 10
+        t0 = time.perf_counter()
+        infos = [solve() for _ in range(reps)]
+        dt = (time.perf_counter() - t0) / reps
+        out["gpu_" + prec] = {"seconds": dt, "iterations": int(infos[-1]["iterations"]),
+                              "true_relres": float(infos[-1]["true_relres"])}
+    g.close()
+    out["gpu_fp64_over_cpu"] = out["cpu"]["seconds"] / out["gpu_fp64"]["seconds"]
+    print(f"[bench] C1 complete solves: CPU {t_cpu:.2f} s, GPU FP64 {1e3 * out['gpu_fp64']['seconds']:.2f} ms",
+          file=sys.stderr, flush=True)
+    return out
 
 
 def copy_calibration(nbytes: float):
@@ -610,6 +654,7 @@ def main():
     if not args.no_cpu_baseline:
         print("[bench] CPU baseline sample", file=sys.stderr, flush=True)
         line["cpu_baseline"] = cpu_baseline(mat, E, efl, efr, info_d["iterations"], args.cpu_sample_iters)
+        line["c1_timed_solves"] = c1_timed_solves(line["cpu_baseline"]["cores"])
     else:
         line["cpu_baseline"] = None
     print(json.dumps(line))
