@@ -45,7 +45,7 @@ M0 = 9.109e-31
 
 WORKLOAD = dict(n_elem=512, degree=3, nk=1024, bias_v=0.2)
 MFMA_KERNELS = {"theta_gemm1", "theta_gemm2"}   # the GEMM formulation of Theta (not the default)
-NO_GRAPHS = False   # --no-graphs: full-2D transients launch their cycles kernel by kernel
+NO_GRAPHS = False   # --no-graphs: solves and transients launch their restart cycles kernel by kernel
 
 
 def parse():
@@ -73,8 +73,8 @@ def parse():
                     help="Delta V source of the full-2D apply: a stored table (the library default), or "
                          "interpolated in the kernel")
     ap.add_argument("--no-graphs", action="store_true",
-                    help="launch the full-2D transients' restart cycles kernel by kernel instead of replaying "
-                         "captured graphs (same kernels; for kernel tracers)")
+                    help="launch the solves' and transients' restart cycles kernel by kernel instead of "
+                         "replaying captured graphs (same kernels; for kernel tracers)")
     ap.add_argument("--theta", choices=["auto", "fft", "gemm"], default="auto",
                     help="formulation of Theta[V] in the headline solves (wdg_set_theta_method)")
     ap.add_argument("--calibrate-copy", type=float, default=0.0, metavar="BYTES",
@@ -532,7 +532,8 @@ def main():
 
     def step(prec: str, tol: float = RTD_BENCH_TOL):
         op.set_potential(mat["x"] * 1e-9, E[0, :, 0], efl, efr, m[0], m[-1], temp=mat["Temp"])
-        _, info = op.solve(SolveOptions(tol=tol, restart=30, precision=prec, max_iter=20000), return_x=False)
+        _, info = op.solve(SolveOptions(tol=tol, restart=30, precision=prec, max_iter=20000,
+                                        use_graph=not NO_GRAPHS), return_x=False)
         n, j, _ = op.observables(m_nodes)
         return info, n, j
 
