@@ -932,15 +932,16 @@ class Full2DOperator:
 
     def transient_cn(self, dt: float, nsteps: int, a_scale: float = 1.0, b_scale: float = 1.0, tol: float = 1e-10,
                      restart: int = 30, precision: str = "fp64", rho0=None, return_rho: bool = True,
-                     use_graph: bool = True):
+                     use_graph: bool = True, krylov_basis: str = "fp16"):
         """Crank-Nicolson steps of d rho/dt = b_scale*rhs - a_scale*A rho
         (wdg_transient_cn); the Jacobi / row-sum preconditioner in use is
         rebuilt for the shifted operator.  use_graph=False launches the restart
-        cycles kernel by kernel instead of replaying captured graphs.
-        Returns (rho, info)."""
+        cycles kernel by kernel instead of replaying captured graphs;
+        krylov_basis is the mixed solver's basis storage ('fp16', 'bf16',
+        'fp32').  Returns (rho, info)."""
         from .solver import transient_cn
         return transient_cn(self._lib, self.ctx, self.n, dt, nsteps, a_scale, b_scale, tol, restart, precision,
-                            rho0, return_rho, use_graph=use_graph)
+                            rho0, return_rho, use_graph=use_graph, krylov_basis=krylov_basis)
 
     def set_profiling(self, mask: int):
         check(self._lib.wdg_set_profiling(self.ctx, int(mask)), self.ctx)

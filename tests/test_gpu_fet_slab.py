@@ -73,6 +73,27 @@ def test_fet_slab_crank_nicolson_matches_oracle(slab, precision):
         assert rel(jx, jx_ref) < 1e-7
 
 
+@pytest.mark.parametrize("basis", ["fp16", "bf16"])
+@pytest.mark.parametrize("precond", ["rowabs", "jacobi"])
+def test_mixed_basis_read_in_place(slab, precond, basis):
+    """The FP32 solver's scaling preconditioners read the 16-bit basis vector
+    in place (f2_scale_basis_kernel: real row-sum scale, complex Jacobi scale;
+    FP16 and BF16 packets): the same three steps against the oracle."""
+    m, V, system = slab
+    p = system[0]
+    n = p["nTotal"]
+    ref = _cn(system, 3)
+    with P.Full2DOperator(P.Full2DSetup(m, V, EFL, EFR)) as op:
+        op.upload_rhs()
+        used = op.set_preconditioner(precond)
+        print(f"preconditioner {precond} -> {used}, basis {basis}", flush=True)
+        rho, info = op.transient_cn(DT, 3, tol=1e-10, restart=30, precision="mixed", rho0=np.zeros(n, complex),
+                                    krylov_basis=basis)
+        print(f"FET slab CN mixed {used} {basis}: {info}", flush=True)
+        assert info["flag"] == 0 and info["steps"] == 3
+        assert rel(rho, ref) < 1e-8, rel(rho, ref)
+
+
 def test_fet_slab_eight_mode_batch_matches_oracle(slab):
     from real_space_dg_amd.solver import transient_cn_batch
     m, V, system0 = slab
