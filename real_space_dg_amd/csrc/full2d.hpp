@@ -606,19 +606,41 @@ __global__ void __launch_bounds__(kF2Block) f2_dense_kernel(const F2Const* __res
     if (d.mask && !d.mask[t]) return;
   }
   if constexpr (!RG) {
-    for (int idx = threadIdx.x; idx < n * n; idx += blockDim.x) {
-      const int m = idx / n, k = idx - m * n;
-      const double2 v = d.R[idx];
-      T2 w;
-      w.x = (T)(d.coef * v.x);
-      w.y = (T)(d.coef * v.y);
-      sR[(size_t)m * (n + 1) + k] = w;
+    // eight loads in flight per thread before their conversions and stores
+    const int nn = n * n, bs = blockDim.x;
+    for (int i0 = threadIdx.x; i0 < nn; i0 += 8 * bs) {
+      double2 v[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) v[q] = (i0 + q * bs < nn) ? d.R[i0 + q * bs] : make_double2(0.0, 0.0);
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int idx = i0 + q * bs;
+        if (idx < nn) {
+          const int m = idx / n, k = idx - m * n;
+          T2 w;
+          w.x = (T)(d.coef * v[q].x);
+          w.y = (T)(d.coef * v[q].y);
+          sR[(size_t)m * (n + 1) + k] = w;
+        }
+      }
     }
   }
   const int nw = n * cn;
+  // staging order: the index that is contiguous in memory runs fastest
+  // (along the axis for axis 0, across it for axis 1)
+  auto widx = [&](int idx, int& k, int& cc) {
+    if (d.axis == 0) {
+      k = idx % n;
+      cc = idx / n;
+    } else {
+      cc = idx % cn;
+      k = idx / cn;
+    }
+  };
   if (plain) {
     for (int idx = threadIdx.x; idx < nw; idx += blockDim.x) {
-      const int k = idx % n, cc = idx / n;
+      int k, cc;
+      widx(idx, k, cc);
       sW[(size_t)k * d.chunk + cc] = u[(long)line * nrel + rel(k, c0 + cc)];
     }
   } else if (d.face < 0) {
@@ -626,7 +648,8 @@ __global__ void __launch_bounds__(kF2Block) f2_dense_kernel(const F2Const* __res
     const int sb = d.nl * d.nl + 2 * d.nl;
     const double* B = d.blocks + (size_t)e * sb;
     for (int idx = threadIdx.x; idx < nw; idx += blockDim.x) {
-      const int k = idx % n, cc = idx / n;
+      int k, cc;
+      widx(idx, k, cc);
       const long r = rel(k, c0 + cc);
       T2 w = f2_zero<T2>();
       for (int b = 0; b < d.nl; ++b) f2_fma(w, (T)B[a * d.nl + b], u[dof(e * d.nl + b, t) * nrel + r]);
@@ -640,7 +663,8 @@ __global__ void __launch_bounds__(kF2Block) f2_dense_kernel(const F2Const* __res
     const int ktr = d.face == 0 ? 0 : nalong - 1;
     const long base = dof(ktr, t) * nrel;
     for (int idx = threadIdx.x; idx < nw; idx += blockDim.x) {
-      const int k = idx % n, cc = idx / n;
+      int k, cc;
+      widx(idx, k, cc);
       sW[(size_t)k * d.chunk + cc] = u[base + rel(k, c0 + cc)];
     }
   }
@@ -683,29 +707,53 @@ __global__ void __launch_bounds__(kF2Block) f2_dense_kernel(const F2Const* __res
           }
       }
     }
+    // y (+)= the outputs.  Every (output, node) address belongs to this
+    // thread alone, so all the loads of the read-modify-writes are issued
+    // before the stores (the compiler cannot prove that they do not alias)
+    bool ok[4];
+    long r[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-      if (cc[j] >= cn || mm[j] >= n) continue;
-      const long r = rel(mm[j], c0 + cc[j]);
-      if (plain) {
-        y[(long)line * nrel + r] = acc[j];
-      } else if (d.face < 0) {
-        const long o = dof(k_out, t) * nrel + r;
-        T2 v = y[o];
-        v.x += acc[j].x;
-        v.y += acc[j].y;
-        y[o] = v;
-      } else {
-        for (int a = 0; a < d.nl; ++a) {
-          const int k = d.face == 0 ? a : nalong - d.nl + a;
-          const T l = (T)d.lift[a];
-          const long o = dof(k, t) * nrel + r;
-          T2 v = y[o];
-          v.x += l * acc[j].x;
-          v.y += l * acc[j].y;
-          y[o] = v;
+      ok[j] = cc[j] < cn && mm[j] < n;
+      r[j] = ok[j] ? rel(mm[j], c0 + cc[j]) : 0;
+    }
+    if (plain) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        if (ok[j]) y[(long)line * nrel + r[j]] = acc[j];
+    } else if (d.face < 0) {
+      const long ob = dof(k_out, t) * nrel;
+      T2 v[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) v[j] = ok[j] ? y[ob + r[j]] : f2_zero<T2>();
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        if (ok[j]) {
+          v[j].x += acc[j].x;
+          v[j].y += acc[j].y;
+          y[ob + r[j]] = v[j];
         }
+    } else {
+      T2 v[kF2MaxNl][4];
+      long ob[kF2MaxNl];
+      T l[kF2MaxNl];
+#pragma unroll
+      for (int a = 0; a < kF2MaxNl; ++a) {
+        const int k = d.face == 0 ? a : nalong - d.nl + a;
+        ob[a] = a < d.nl ? dof(k, t) * nrel : 0;
+        l[a] = a < d.nl ? (T)d.lift[a] : (T)0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) v[a][j] = (a < d.nl && ok[j]) ? y[ob[a] + r[j]] : f2_zero<T2>();
       }
+#pragma unroll
+      for (int a = 0; a < kF2MaxNl; ++a)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          if (a < d.nl && ok[j]) {
+            v[a][j].x += l[a] * acc[j].x;
+            v[a][j].y += l[a] * acc[j].y;
+            y[ob[a] + r[j]] = v[a][j];
+          }
     }
   }
 }

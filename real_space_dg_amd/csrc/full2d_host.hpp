@@ -121,14 +121,19 @@ void f2_launch_apply(wdg_ctx* c, const typename F2Vec<T>::T2* u, typename F2Vec<
   }
 }
 
-// Cells across the axis per workgroup of a dense part: enough workgroups to
-// fill the chip, within the LDS budget.
-// Relative cells across the axis per workgroup; *rglobal = 1 when the n x n
-// factor does not fit in LDS beside one column and is read from global memory.
-int f2_dense_chunk(int n, int ncol, int lines, int* rglobal) {
+// Relative cells across the axis per workgroup of a dense part: enough
+// workgroups to fill the chip, but at least enough columns that the product
+// phase gives every thread of the workgroup an item (each workgroup stages the
+// whole n x n factor, so thin chunks re-read it for little work), within the
+// LDS budget.  *rglobal = 1 when the factor does not fit in LDS beside one
+// column and is read from global memory.
+int f2_dense_chunk(int n, int ncol, int lines, int axis, int* rglobal) {
   const int want = std::max(1, (2048 + lines - 1) / lines);
   int chunk = std::max(1, (ncol + want - 1) / want);
-  chunk = std::max(chunk, std::min(ncol, 4));
+  // product items: axis 0 has n rows x (chunk/4) column quads, axis 1 chunk
+  // columns x (n/4) row quads
+  const int fill = axis == 0 ? 4 * ((kF2Block + n - 1) / n) : (kF2Block + (n + 3) / 4 - 1) / ((n + 3) / 4);
+  chunk = std::max(chunk, std::min(ncol, std::max(4, fill)));
   *rglobal = f2_dense_lds(n, 1, sizeof(double2)) > kF2MaxLds ? 1 : 0;
   while (chunk > 1 && f2_dense_lds(n, chunk, sizeof(double2), *rglobal) > kF2MaxLds) chunk = (chunk + 1) / 2;
   if (f2_dense_lds(n, chunk, sizeof(double2), *rglobal) > kF2MaxLds)
@@ -419,7 +424,7 @@ void f2_create_impl(wdg_ctx* c, const wdg2_desc& d, const wdg2_part* parts, int 
         upd(dp->blocks, b);
         dp->k.blocks = dp->blocks.p;
         dp->lines = ncen;
-        dp->k.chunk = f2_dense_chunk(nr, ax == 0 ? nry : nrx, dp->lines, &dp->k.rglobal);
+        dp->k.chunk = f2_dense_chunk(nr, ax == 0 ? nry : nrx, dp->lines, ax, &dp->k.rglobal);
       } else {
         upd(dp->lift, std::vector<double>(p.c_diag, p.c_diag + nl));
         dp->k.lift = dp->lift.p;
@@ -430,7 +435,7 @@ void f2_create_impl(wdg_ctx* c, const wdg2_desc& d, const wdg2_part* parts, int 
           dp->k.mask = dp->mask.p;
         }
         dp->lines = ntr;
-        dp->k.chunk = f2_dense_chunk(nr, ax == 0 ? nry : nrx, dp->lines, &dp->k.rglobal);
+        dp->k.chunk = f2_dense_chunk(nr, ax == 0 ? nry : nrx, dp->lines, ax, &dp->k.rglobal);
       }
       HIPCHK(hipStreamSynchronize(c->stream));   // host vectors above go out of scope
       f->dense.push_back(std::move(dp));
@@ -798,7 +803,7 @@ void f2_set_line_impl(wdg_ctx* c, const wdg2_line_factors& lf) {
     dp.n = nrx;
     dp.lines = f.ncen;
     dp.k = F2Dense{};
-    dp.k.chunk = f2_dense_chunk(nrx, f.d.n_rho_y, dp.lines, &dp.k.rglobal);
+    dp.k.chunk = f2_dense_chunk(nrx, f.d.n_rho_y, dp.lines, 0, &dp.k.rglobal);
     dp.k.axis = 0;
     dp.k.face = -2;
     dp.k.nl = nl;
