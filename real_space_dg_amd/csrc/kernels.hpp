@@ -866,22 +866,43 @@ __global__ __launch_bounds__(256) void line_aggregate_inv_kernel(const OpConst* 
   const int out = fwd ? NN - 1 : 0;
   const int e_lo = c * S, e_hi = min(ne, e_lo + S);
   double Ar = 0, Ai = 0, Br = 1, Bi = 0;
-  for (int step = 0; step < e_hi - e_lo; ++step) {
+  // the next element's inputs are loaded while this one is composed (only
+  // the running map crosses elements)
+  const int ns = e_hi - e_lo;
+  uint32_t mc[NN], mn[NN];
+  float2 vc[NN], vn[NN], bc, bn;
+  auto load = [&](int step, uint32_t (&m)[NN], float2 (&v)[NN], float2& be) {
     const int e = fwd ? e_lo + step : e_hi - 1 - step;
     const uint32_t* di = dinv + (long)e * NN * NN * nk + j;
     const long re = (long)e * NN * nk + j;
+#pragma unroll
+    for (int b = 0; b < NN; ++b) {
+      m[b] = di[(long)(out * NN + b) * nk];
+      v[b] = line_in(r, re + (long)b * nk);
+    }
+    be = dcpl[((long)e * NN + out) * nk + j];
+  };
+  if (ns > 0) load(0, mc, vc, bc);
+  for (int step = 0; step < ns; ++step) {
+    if (step + 1 < ns) load(step + 1, mn, vn, bn);
     float al_r = 0, al_i = 0;
 #pragma unroll
     for (int b = 0; b < NN; ++b) {
-      const float2 m = lv_unpack_bf16x2(di[(long)(out * NN + b) * nk]), v = line_in(r, re + (long)b * nk);
+      const float2 m = lv_unpack_bf16x2(mc[b]), v = vc[b];
       al_r += m.x * v.x - m.y * v.y;
       al_i += m.x * v.y + m.y * v.x;
     }
-    const float2 be = dcpl[((long)e * NN + out) * nk + j];
+    const float2 be = bc;
     // s_out = alpha - beta * s_in, composed with the running map s_in = A + B s0
     const double nAr = al_r - (be.x * Ar - be.y * Ai), nAi = al_i - (be.x * Ai + be.y * Ar);
     const double nBr = -(be.x * Br - be.y * Bi), nBi = -(be.x * Bi + be.y * Br);
     Ar = nAr; Ai = nAi; Br = nBr; Bi = nBi;
+#pragma unroll
+    for (int b = 0; b < NN; ++b) {
+      mc[b] = mn[b];
+      vc[b] = vn[b];
+    }
+    bc = bn;
   }
   agg[((long)c * nk + j) * 2 + 0] = mk(Ar, Ai);
   agg[((long)c * nk + j) * 2 + 1] = mk(Br, Bi);
@@ -906,24 +927,38 @@ __global__ __launch_bounds__(256) void line_apply_inv_kernel(const OpConst* __re
   const int e_lo = c * S, e_hi = min(ne, e_lo + S);
   const double2 s_in = agg[((long)c * nk + j) * 2 + 0];
   float sr = (float)s_in.x, si = (float)s_in.y;
-  for (int step = 0; step < e_hi - e_lo; ++step) {
+  // the next element's inputs are loaded while this one is swept (only the
+  // outflow value crosses elements)
+  const int ns = e_hi - e_lo;
+  uint32_t mc[NN * NN], mn[NN * NN];
+  float2 vc[NN], vn[NN], bcc[NN], bcn[NN];
+  auto load = [&](int step, uint32_t (&m)[NN * NN], float2 (&v)[NN], float2 (&bcv)[NN]) {
     const int e = fwd ? e_lo + step : e_hi - 1 - step;
     const uint32_t* di = dinv + (long)e * NN * NN * nk + j;
     const float2* dc = dcpl + (long)e * NN * nk + j;
     const long re = (long)e * NN * nk + j;
-    float2 v[NN];
 #pragma unroll
-    for (int b = 0; b < NN; ++b) v[b] = line_in(r, re + (long)b * nk);
+    for (int b = 0; b < NN; ++b) {
+      v[b] = line_in(r, re + (long)b * nk);
+      bcv[b] = dc[(long)b * nk];
+    }
+#pragma unroll
+    for (int q = 0; q < NN * NN; ++q) m[q] = di[(long)q * nk];
+  };
+  if (ns > 0) load(0, mc, vc, bcc);
+  for (int step = 0; step < ns; ++step) {
+    const int e = fwd ? e_lo + step : e_hi - 1 - step;
+    if (step + 1 < ns) load(step + 1, mn, vn, bcn);
     float2 zo[NN];
 #pragma unroll
     for (int a = 0; a < NN; ++a) {
-      const float2 bc = dc[(long)a * nk];
+      const float2 bc = bcc[a];
       float zr = -(bc.x * sr - bc.y * si), zi = -(bc.x * si + bc.y * sr);
 #pragma unroll
       for (int b = 0; b < NN; ++b) {
-        const float2 m = lv_unpack_bf16x2(di[(long)(a * NN + b) * nk]);
-        zr += m.x * v[b].x - m.y * v[b].y;
-        zi += m.x * v[b].y + m.y * v[b].x;
+        const float2 m = lv_unpack_bf16x2(mc[a * NN + b]);
+        zr += m.x * vc[b].x - m.y * vc[b].y;
+        zi += m.x * vc[b].y + m.y * vc[b].x;
       }
       zo[a] = make_float2(zr, zi);
     }
@@ -931,6 +966,13 @@ __global__ __launch_bounds__(256) void line_apply_inv_kernel(const OpConst* __re
     for (int a = 0; a < NN; ++a) {
       z[((long)e * NN + a) * nk + j] = zo[a];
       if (a == out) { sr = zo[a].x; si = zo[a].y; }
+    }
+#pragma unroll
+    for (int q = 0; q < NN * NN; ++q) mc[q] = mn[q];
+#pragma unroll
+    for (int b = 0; b < NN; ++b) {
+      vc[b] = vn[b];
+      bcc[b] = bcn[b];
     }
   }
 }
