@@ -242,6 +242,14 @@ int wdg_poisson_step(wdg_ctx* ctx, const double* phi_in, double bias, double* ph
 int wdg_apply_device(wdg_ctx* ctx, const void* d_u, void* d_y, int32_t precision);
 int wdg_precond_device(wdg_ctx* ctx, const void* d_r, void* d_z, int32_t precision);
 
+/* Average device time of one operator apply (wdg_apply_device's kernels) in
+ * the given precision: nrep applies enqueued back to back on the context
+ * stream between one HIP event pair, on the context's own work buffers (the
+ * apply's cost does not depend on the values).  A measurement helper with
+ * no counterpart in the reference: per-kernel event pairs around single
+ * launches also time the host's submission gaps. */
+int wdg_time_apply(wdg_ctx* ctx, int32_t precision, int32_t nrep, double* ms_per_apply);
+
 /* wdg_precond_device that allocates nothing: for callers that cannot call
  * hipMalloc (code under HIP graph capture, tight serving loops, harnesses
  * that forbid allocation).  No counterpart in the reference.  1D contexts

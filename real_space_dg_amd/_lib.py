@@ -215,6 +215,7 @@ SIGNATURES = {
     "wdg_poisson_setup": (C.c_int, [_P, C.POINTER(wdg_poisson_desc)]),
     "wdg_poisson_step": (C.c_int, [_P, _D, C.c_double, _D, _D, C.POINTER(wdg_poisson_info)]),
     "wdg_apply_device": (C.c_int, [_P, _P, _P, C.c_int32]),
+    "wdg_time_apply": (C.c_int, [_P, C.c_int32, C.c_int32, C.POINTER(C.c_double)]),
     "wdg_precond_device": (C.c_int, [_P, _P, _P, C.c_int32]),
     "wdg_precond_device_noalloc": (C.c_int, [_P, _P, _P, C.c_int32]),
     "wdg_host_setup": (C.c_int, [C.POINTER(wdg_desc), C.POINTER(wdg_potential), _D, _D]),

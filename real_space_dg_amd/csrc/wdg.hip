@@ -1817,6 +1817,55 @@ int wdg_apply_device(wdg_ctx* ctx, const void* d_u, void* d_y, int32_t precision
   });
 }
 
+int wdg_time_apply(wdg_ctx* ctx, int32_t precision, int32_t nrep, double* ms_per_apply) {
+  if (!ctx || !ms_per_apply) return fail(ctx, WDG_ERR_ARG, "null argument");
+  if (!ctx->have_potential) return fail(ctx, WDG_ERR_STATE, "wdg_set_potential has not been called");
+  if (precision != WDG_PREC_FP64 && precision != WDG_PREC_MIXED) return fail(ctx, WDG_ERR_ARG, "unknown precision");
+  if (nrep < 1) return fail(ctx, WDG_ERR_ARG, "nrep must be at least 1");
+  if (ctx->shift_on) return fail(ctx, WDG_ERR_STATE, "a Crank-Nicolson shift is active");
+  return guarded(ctx, [&]() {
+    const long N = ctx->N;
+    const bool f64 = precision == WDG_PREC_FP64;
+    if (f64) {
+      ctx->d_u64.alloc(N);
+      ctx->d_v64.alloc(N);
+      HIPCHK(hipMemsetAsync(ctx->d_u64.p, 0, (size_t)N * 16, ctx->stream));
+    } else {
+      ctx->d_u32.alloc(N);
+      ctx->d_v32.alloc(N);
+      HIPCHK(hipMemsetAsync(ctx->d_u32.p, 0, (size_t)N * 8, ctx->stream));
+    }
+    const uint32_t mask = ctx->prof_mask;   // no per-kernel event pairs inside the timed group
+    ctx->prof_mask = 0;
+    auto apply = [&]() {
+      if (f64) op_apply<double>(ctx, ctx->d_u64.p, ctx->d_v64.p, nullptr);
+      else op_apply<float>(ctx, ctx->d_u32.p, ctx->d_v32.p, nullptr);
+    };
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    float ms = 0;
+    try {
+      HIPCHK(hipEventCreate(&t0));
+      HIPCHK(hipEventCreate(&t1));
+      apply();   // warm-up
+      HIPCHK(hipEventRecord(t0, ctx->stream));
+      for (int r = 0; r < nrep; ++r) apply();
+      HIPCHK(hipEventRecord(t1, ctx->stream));
+      HIPCHK(hipEventSynchronize(t1));
+      HIPCHK(hipEventElapsedTime(&ms, t0, t1));
+    } catch (...) {
+      ctx->prof_mask = mask;
+      if (t0) (void)hipEventDestroy(t0);
+      if (t1) (void)hipEventDestroy(t1);
+      throw;
+    }
+    ctx->prof_mask = mask;
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    *ms_per_apply = (double)ms / nrep;
+    return WDG_OK;
+  });
+}
+
 int wdg_precond(wdg_ctx* ctx, const double* r, double* z) {
   if (!ctx || !r || !z) return fail(ctx, WDG_ERR_ARG, "null argument");
   if (!ctx->have_potential) return fail(ctx, WDG_ERR_STATE, "wdg_set_potential has not been called");

@@ -161,6 +161,13 @@ class WignerOperator:
                   self._h)
         return y
 
+    def time_apply(self, precision: str = "mixed", nrep: int = 50) -> float:
+        """Average device time (ms) of one operator apply: nrep applies back
+        to back between one HIP event pair (wdg_time_apply)."""
+        ms = C.c_double(0.0)
+        check(self._lib.wdg_time_apply(self._h, PRECISIONS[precision], int(nrep), C.byref(ms)), self._h)
+        return ms.value
+
     def precond(self, r) -> np.ndarray:
         r = _as_c128(r, self.N)
         z = np.empty(self.N, np.complex128)

@@ -89,3 +89,10 @@ def test_full2d_poisson_entry_points_reject_null_arguments(lib):
     info = _lib.wdg2_poisson_info()
     assert lib.wdg2_poisson_step(None, 1, None, None, 0, 1.0, None, None, None, C.byref(info)) == _lib.WDG_ERR_ARG
     assert "null" in lib.wdg_last_error(None).decode()
+
+
+def test_time_apply_rejects_null_arguments(lib):
+    """wdg_time_apply validates its arguments before touching a device."""
+    ms = C.c_double(0.0)
+    assert lib.wdg_time_apply(None, _lib.WDG_PREC_MIXED, 10, C.byref(ms)) == _lib.WDG_ERR_ARG
+    assert "null" in lib.wdg_last_error(None).decode()
