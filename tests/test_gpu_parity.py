@@ -9,8 +9,11 @@ pytestmark = pytest.mark.gpu
 # (elements, degree, nk, bias, inflow model).  The negative bias makes the
 # potential rise towards x = L, so min(EM) != EM(end) and the x >= L clamp of
 # V_transient (to min(EM)) is exercised; DG_MSA_2D is the k_z-integrated inflow.
+# Degrees 1-4 run the Theta kernels' compile-time transform plans (n <= 5;
+# nk = 64 -> P = 128), degree 5 and nk = 96 the run-time plans.
 CASES = [(16, 2, 64, 0.0, "DG_MSA_3D"), (8, 3, 64, 0.2, "DG_MSA_3D"), (12, 1, 96, 0.1, "DG_MSA_3D"),
-         (6, 5, 64, 0.1, "DG_MSA_3D"), (8, 2, 64, -0.15, "DG_MSA_3D"), (10, 3, 64, 0.1, "DG_MSA_2D")]
+         (6, 5, 64, 0.1, "DG_MSA_3D"), (8, 2, 64, -0.15, "DG_MSA_3D"), (10, 3, 64, 0.1, "DG_MSA_2D"),
+         (8, 4, 64, 0.1, "DG_MSA_3D")]
 
 
 @pytest.fixture(scope="module", params=CASES, ids=lambda c: "ne%d_p%d_nk%d_b%.2f_%s" % c)
