@@ -897,12 +897,14 @@ __global__ __launch_bounds__(256) void line_aggregate_inv_kernel(const OpConst* 
     const double nAr = al_r - (be.x * Ar - be.y * Ai), nAi = al_i - (be.x * Ai + be.y * Ar);
     const double nBr = -(be.x * Br - be.y * Bi), nBi = -(be.x * Bi + be.y * Br);
     Ar = nAr; Ai = nAi; Br = nBr; Bi = nBi;
+    if (step + 1 < ns) {
 #pragma unroll
-    for (int b = 0; b < NN; ++b) {
-      mc[b] = mn[b];
-      vc[b] = vn[b];
+      for (int b = 0; b < NN; ++b) {
+        mc[b] = mn[b];
+        vc[b] = vn[b];
+      }
+      bc = bn;
     }
-    bc = bn;
   }
   agg[((long)c * nk + j) * 2 + 0] = mk(Ar, Ai);
   agg[((long)c * nk + j) * 2 + 1] = mk(Br, Bi);
@@ -967,12 +969,14 @@ __global__ __launch_bounds__(256) void line_apply_inv_kernel(const OpConst* __re
       z[((long)e * NN + a) * nk + j] = zo[a];
       if (a == out) { sr = zo[a].x; si = zo[a].y; }
     }
+    if (step + 1 < ns) {
 #pragma unroll
-    for (int q = 0; q < NN * NN; ++q) mc[q] = mn[q];
+      for (int q = 0; q < NN * NN; ++q) mc[q] = mn[q];
 #pragma unroll
-    for (int b = 0; b < NN; ++b) {
-      vc[b] = vn[b];
-      bcc[b] = bcn[b];
+      for (int b = 0; b < NN; ++b) {
+        vc[b] = vn[b];
+        bcc[b] = bcn[b];
+      }
     }
   }
 }
