@@ -434,11 +434,16 @@ def transient_full2d_sc(instances: int = 8, steps: int = 20, poisson_every: int 
                 op.upload_contact_rhs()
                 op.set_preconditioner("rowabs")
                 ops.append(op)
+                print(f"[bench] self-consistent FET slab {prec}: instance {len(ops)} set up", file=sys.stderr,
+                      flush=True)
             n = ops[0].n
             t0 = time.perf_counter()
             phi, hist = PS.transient_cn_batch_self_consistent(
                 ops, offsets, phi0, bias, dt, steps, poisson_every=poisson_every, precision=prec, tol=1e-8,
-                rho0=[np.zeros(n, complex)] * instances, use_graph=prec != "fp64")
+                rho0=[np.zeros(n, complex)] * instances, use_graph=prec != "fp64",
+                on_block=lambda h: print(f"[bench] self-consistent FET slab {prec}: {h['steps']} steps, "
+                                         f"{time.perf_counter() - t0:.1f} s, Poisson update "
+                                         f"{h['potentialUpdate']:.2e} V", file=sys.stderr, flush=True))
             wall = time.perf_counter() - t0
         finally:
             for op in ops:

@@ -475,6 +475,8 @@ def transient_cn_self_consistent(op, phi0, bias: float, dt: float, steps: int, p
         hist.append({"steps": done, "gmresIterations": info["iterations"], "maxStepRelres": info["max_relres"],
                      "flag": info["flag"], "potentialUpdate": float(np.max(np.abs(new - phi))),
                      "poissonPhysicalRelres": pinfo["physicalRelresBefore"], "poissonConverged": pinfo["converged"]})
+        if on_block is not None:
+            on_block(hist[-1])
         phi = new
         op.set_potential(Vc - phi)
     return phi, hist
@@ -483,7 +485,7 @@ def transient_cn_self_consistent(op, phi0, bias: float, dt: float, steps: int, p
 def transient_cn_batch_self_consistent(ops, offsets, phi0, bias: float, dt: float, steps: int,
                                        poisson_every: int = 10, precision: str = "mixed", tol: float = 1e-8,
                                        restart: int = 30, poisson_method: str = "newton-raphson", rho0=None,
-                                       use_graph: bool = True):
+                                       use_graph: bool = True, on_block=None):
     """Crank-Nicolson steps of several full-2D instances of one device (the
     transverse modes: instance i sees the potential Vc - phi + offsets[i]),
     all stepped in one batched call (wdg_transient_cn_batch), with a 2D
@@ -492,8 +494,9 @@ def transient_cn_batch_self_consistent(ops, offsets, phi0, bias: float, dt: floa
     volumes, one Poisson step from the current phi, and every instance's
     potential becomes Vc - phi_new + offset (the contacts keep phi = 0 and
     bias, so the inflow data stay valid).  rho0: None (continue from the
-    device states) or one initial state per instance.  Returns (phi, history)
-    with one entry per block, holding the phi the block ran with."""
+    device states) or one initial state per instance.  on_block: called with
+    each block's history entry as soon as the block is done.  Returns (phi,
+    history) with one entry per block, holding the phi the block ran with."""
     from .solver import transient_cn_batch
     ops = list(ops)
     if len(offsets) != len(ops):
@@ -516,6 +519,8 @@ def transient_cn_batch_self_consistent(ops, offsets, phi0, bias: float, dt: floa
                      "maxStepRelres": max(i["max_relres"] for i in infos), "flag": max(i["flag"] for i in infos),
                      "potentialUpdate": float(np.max(np.abs(new - phi))),
                      "poissonPhysicalRelres": pinfo["physicalRelresBefore"], "poissonConverged": pinfo["converged"]})
+        if on_block is not None:
+            on_block(hist[-1])
         phi = new
         for op, off in zip(ops, offsets):
             op.set_potential(Vc - phi + float(off))
