@@ -513,7 +513,7 @@ def main():
     global NO_GRAPHS
     NO_GRAPHS = args.no_graphs
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (--gpus {args.gpus} requested); "
+        print(f"bench.py: runs on one GPU only (--gpus {args.gpus} requested); "
               "multi-GPU sharding is design-only.", file=sys.stderr)
         sys.exit(2)
     if args.calibrate_copy > 0:
@@ -614,7 +614,7 @@ def main():
             err[f"{name}_density_rel_err_vs_oracle"] = float(np.max(np.abs(nn - gn)) / np.max(np.abs(gn)))
             err[f"{name}_current_rel_err_vs_oracle"] = float(np.max(np.abs(jj - gj)) / np.max(np.abs(gj)))
         err["oracle"] = f"FP64 numpy oracle, GMRES(30) + line preconditioner, true relres {float(gold['true_relres']):.1e}"
-    # the transport apply alone (the HBM-bound "Wigner apply" of the north star:
+    # the transport apply alone (the HBM-bound part of the Wigner apply:
     # transport + CAP diagonal, 2 s N bytes), through wdg_apply_parts
     u_probe = np.random.default_rng(0).standard_normal(op.N) + 1j * np.random.default_rng(1).standard_normal(op.N)
     transport_roof = {}

@@ -29,8 +29,8 @@ Pinning (what checks each function beyond retyping; see tests/test_oracle_*.py):
                               no k=0 for even Nk.
   verteilung                  DG_MSA_3D closed form; cosine transform checked by
                               brute force; phi\\f checked as a residual.
-                              TSU_ESAKI (transverse-integrated contacts, an
-                              addition of this build): flat-band zero-bias
+                              TSU_ESAKI (transverse-integrated contacts, not
+                              in the reference): flat-band zero-bias
                               density equals the 3D Fermi-Dirac integral.
   get_diff                    upwind structure (k>0 couples only e-1, k<0 only
                               e+1); constant state with matching inflow is a
