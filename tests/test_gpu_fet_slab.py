@@ -149,12 +149,14 @@ def test_self_consistent_batch_matches_oracle_blocks():
             op.set_preconditioner("rowabs")
             ops.append(op)
         n = ops[0].n
+        seen = []
         phi, hist = PS.transient_cn_batch_self_consistent(ops, offsets, phi0, bias, DT, 6, poisson_every=3,
                                                           precision="fp64", tol=1e-12,
-                                                          rho0=[np.zeros(n, complex)] * 2)
+                                                          rho0=[np.zeros(n, complex)] * 2, on_block=seen.append)
         print("SC blocks:", [(h["steps"], h["gmresIterations"], round(h["potentialUpdate"], 6)) for h in hist],
               flush=True)
         assert len(hist) == 2 and all(h["flag"] == 0 for h in hist)
+        assert [h["steps"] for h in seen] == [3, 6] and all(a is b for a, b in zip(seen, hist))
         # the oracle, block by block, with the reservoirs of each instance's setup
         M, types, periodic = PS.eval_poisson2d(mat)
         states = [np.zeros(n, complex)] * 2
