@@ -451,7 +451,8 @@ def solve2d_rtd_self_consistent(mat: dict, max_iterations: int = 100, mixing: fl
 
 def transient_cn_self_consistent(op, phi0, bias: float, dt: float, steps: int, poisson_every: int = 10,
                                  precision: str = "mixed", tol: float = 1e-8, restart: int = 30,
-                                 poisson_method: str = "newton-raphson", rho0=None, use_graph: bool = True):
+                                 poisson_method: str = "newton-raphson", rho0=None, use_graph: bool = True,
+                                 on_block=None):
     """Crank-Nicolson steps of a full-2D operator (`Full2DOperator`) with a
     2D Poisson update of the potential every `poisson_every` steps: after
     each block of steps the device density is projected onto the Poisson
@@ -459,8 +460,9 @@ def transient_cn_self_consistent(op, phi0, bias: float, dt: float, steps: int, p
     (DevicePoisson2D: density, projection and update on the device), and
     the operator's potential becomes Vc - phi (the contacts keep
     phi = 0 and bias, so the inflow data stay valid).  rho0=None continues
-    from the operator's device state (e.g. a stationary solve).  Returns
-    (phi, history) with one entry per block."""
+    from the operator's device state (e.g. a stationary solve).  on_block:
+    called with each block's history entry as soon as the block is done.
+    Returns (phi, history) with one entry per block."""
     mat = op.setup.mat
     Vc = np.asarray(mat["Vc"], float)
     dev = DevicePoisson2D(op)
